@@ -1,0 +1,483 @@
+"""Python binding of libhcamd.so, the gfx950 graph epoch library.
+
+Thin ctypes layer over the C ABI declared in include/hc_capi.h: host CSR
+graphs (generator, reference binary files, SourcePicker, equal-edge
+segments), device-resident graphs in HBM, the epoch kernels and the full
+BFS / SSSP / PageRank runs.  There is no fallback path: if the shared
+library is missing or a call fails, an exception is raised.
+
+Reference counterparts (chrisliu/hetero-compute): src/graph.cuh (CSR, loader,
+SourcePicker, compute_equal_edge_ranges), src/kernels/gpu/{sssp,bfs,pr}.cuh
+(drivers and epoch kernels), src/kernels/cpu/bfs.cuh:48-114 (bfs_do_cpu).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhcamd.so")
+
+INF_DIST = 0x3FFFFFFF
+INVALID_NODE = -1
+
+MAP_THREAD, MAP_WAVE, MAP_BLOCK, MAP_BUCKETED = 0, 1, 2, 3
+
+HC_OK = 0
+ERR_NAMES = {
+    -1: "HC_ERR_INVALID", -2: "HC_ERR_HIP", -3: "HC_ERR_OOM", -4: "HC_ERR_IO",
+    -5: "HC_ERR_UNSUPPORTED", -6: "HC_ERR_BUSY", -7: "HC_ERR_NOT_FOUND",
+    -8: "HC_ERR_NO_DEVICE",
+}
+
+
+class HcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class HostGraphStruct(C.Structure):
+    _fields_ = [("num_nodes", C.c_int32), ("num_edges", C.c_int64),
+                ("index", C.POINTER(C.c_int64)),
+                ("neighbors", C.POINTER(C.c_int32)),
+                ("weights", C.POINTER(C.c_int32))]
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("num_nodes", C.c_int32), ("num_edges", C.c_int64),
+                ("start_id", C.c_int32), ("end_id", C.c_int32),
+                ("degree_sorted", C.c_int32), ("weighted", C.c_int32),
+                ("offset_bytes", C.c_int32), ("num_nonzero", C.c_int32),
+                ("max_degree", C.c_int64), ("device_bytes", C.c_size_t)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 128), ("arch", C.c_char * 64),
+                ("compute_units", C.c_int32), ("wavefront_size", C.c_int32),
+                ("hbm_bytes", C.c_size_t), ("lds_bytes_per_block", C.c_int32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("epochs", C.c_int32),
+                ("push_epochs", C.c_int32), ("pull_epochs", C.c_int32),
+                ("launches", C.c_int32), ("reached", C.c_int64),
+                ("residual", C.c_double), ("alg_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SsspOpts(C.Structure):
+    _fields_ = [("direction", C.c_int32), ("alpha", C.c_int32),
+                ("max_epochs", C.c_int32)]
+
+
+class BfsOpts(C.Structure):
+    _fields_ = [("alpha", C.c_int32), ("beta", C.c_int32),
+                ("direction", C.c_int32)]
+
+
+class PrOpts(C.Structure):
+    _fields_ = [("damping", C.c_float), ("tolerance", C.c_double),
+                ("max_iters", C.c_int32)]
+
+
+_lib = None
+
+_P = C.c_void_p
+_SIGS = {
+    "hc_last_error": ([], C.c_char_p),
+    "hc_version": ([], C.c_int),
+    "hc_init": ([C.c_int], C.c_int),
+    "hc_get_device_info": ([C.POINTER(DeviceInfo)], C.c_int),
+    "hc_device_synchronize": ([], C.c_int),
+    "hc_dmalloc": ([C.POINTER(_P), C.c_size_t], C.c_int),
+    "hc_dfree": ([_P], C.c_int),
+    "hc_hmalloc_pinned": ([C.POINTER(_P), C.c_size_t], C.c_int),
+    "hc_hfree_pinned": ([_P], C.c_int),
+    "hc_memcpy": ([_P, _P, C.c_size_t, C.c_int, _P], C.c_int),
+    "hc_memset_d32": ([_P, C.c_int32, C.c_size_t, _P], C.c_int),
+    "hc_stream_create": ([C.POINTER(_P)], C.c_int),
+    "hc_stream_destroy": ([_P], C.c_int),
+    "hc_stream_synchronize": ([_P], C.c_int),
+    "hc_host_graph_generate": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64,
+                                C.POINTER(HostGraphStruct)], C.c_int),
+    "hc_host_graph_from_edges": ([_P, _P, _P, C.c_int64, C.c_int32, C.c_int,
+                                  C.c_int, C.POINTER(HostGraphStruct)], C.c_int),
+    "hc_host_graph_read": ([C.c_char_p, C.c_int, C.POINTER(HostGraphStruct)], C.c_int),
+    "hc_host_graph_write": ([C.c_char_p, C.POINTER(HostGraphStruct), C.c_int], C.c_int),
+    "hc_host_graph_free": ([C.POINTER(HostGraphStruct)], None),
+    "hc_pick_sources": ([C.POINTER(HostGraphStruct), C.c_int64, C.c_int32, _P], C.c_int),
+    "hc_equal_edge_ranges": ([C.POINTER(HostGraphStruct), C.c_int32, C.c_int32, _P], C.c_int),
+    "hc_graph_upload": ([C.POINTER(HostGraphStruct), C.POINTER(_P)], C.c_int),
+    "hc_graph_upload_range": ([C.POINTER(HostGraphStruct), C.c_int32, C.c_int32,
+                               C.POINTER(_P)], C.c_int),
+    "hc_graph_free": ([_P], C.c_int),
+    "hc_graph_get_info": ([_P, C.POINTER(GraphInfo)], C.c_int),
+    "hc_sssp_pull_epoch": ([_P, C.c_int, C.c_int32, C.c_int32, _P, _P, C.c_int,
+                            C.c_int, _P], C.c_int),
+    "hc_bfs_pull_epoch": ([_P, C.c_int, C.c_int32, C.c_int32, _P, _P, _P, _P,
+                           C.c_int, C.c_int, _P], C.c_int),
+    "hc_bfs_push_epoch": ([_P, _P, C.c_int32, _P, _P, _P, _P, _P], C.c_int),
+    "hc_pr_pull_epoch": ([_P, C.c_int32, C.c_int32, C.c_float, _P, _P, _P, _P,
+                          _P, _P, _P], C.c_int),
+    "hc_sssp": ([_P, C.c_int32, _P, C.POINTER(SsspOpts), C.POINTER(RunStats), _P], C.c_int),
+    "hc_sssp_pull": ([_P, C.c_int, _P, _P, C.c_int, C.c_int, C.POINTER(RunStats),
+                      _P], C.c_int),
+    "hc_bfs": ([_P, C.c_int32, _P, _P, C.POINTER(BfsOpts), C.POINTER(RunStats), _P], C.c_int),
+    "hc_pagerank": ([_P, _P, C.POINTER(PrOpts), C.POINTER(RunStats), _P], C.c_int),
+}
+
+
+def lib():
+    """Loads libhcamd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first "
+                               "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != HC_OK:
+        raise HcError(rc, lib().hc_last_error().decode(errors="replace"))
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_info() -> dict:
+    d = DeviceInfo()
+    check(lib().hc_get_device_info(C.byref(d)))
+    return {"name": d.name.decode(), "arch": d.arch.decode(),
+            "compute_units": d.compute_units, "wavefront_size": d.wavefront_size,
+            "hbm_bytes": d.hbm_bytes, "lds_bytes_per_block": d.lds_bytes_per_block}
+
+
+def synchronize():
+    check(lib().hc_device_synchronize())
+
+
+# ---------------------------------------------------------------------------
+# Host graphs
+# ---------------------------------------------------------------------------
+
+class HostGraph:
+    """CSR in host memory: index int64[N+1], neighbors/weights int32[m]."""
+
+    def __init__(self, struct: HostGraphStruct, owned: bool, keep=None):
+        self._s = struct
+        self._owned = owned
+        self._keep = keep
+
+    # -- constructors --------------------------------------------------------
+    @classmethod
+    def generate(cls, kind: str, scale: int, degree: int, weighted: bool = True,
+                 seed: int = 27491095) -> "HostGraph":
+        if kind not in ("kron", "uniform"):
+            raise ValueError("kind must be 'kron' or 'uniform'")
+        s = HostGraphStruct()
+        check(lib().hc_host_graph_generate(int(kind == "uniform"), scale, degree,
+                                           int(weighted), seed, C.byref(s)))
+        return cls(s, True)
+
+    @classmethod
+    def from_edges(cls, src, dst, num_nodes: int, weights=None,
+                   symmetrize: bool = True, relabel: bool = False) -> "HostGraph":
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.int32)
+        if src.shape != dst.shape or (w is not None and w.shape != src.shape):
+            raise ValueError("edge arrays differ in length")
+        s = HostGraphStruct()
+        check(lib().hc_host_graph_from_edges(_ptr(src), _ptr(dst), _ptr(w), src.size,
+                                             num_nodes, int(symmetrize), int(relabel),
+                                             C.byref(s)))
+        return cls(s, True)
+
+    @classmethod
+    def from_csr(cls, index, neighbors, weights=None) -> "HostGraph":
+        """Wraps numpy CSR arrays (kept alive by the object, not copied)."""
+        index = np.ascontiguousarray(index, dtype=np.int64)
+        neighbors = np.ascontiguousarray(neighbors, dtype=np.int32)
+        weights = None if weights is None else np.ascontiguousarray(weights, dtype=np.int32)
+        s = HostGraphStruct()
+        s.num_nodes = index.size - 1
+        s.num_edges = neighbors.size
+        s.index = index.ctypes.data_as(C.POINTER(C.c_int64))
+        s.neighbors = neighbors.ctypes.data_as(C.POINTER(C.c_int32))
+        s.weights = (weights.ctypes.data_as(C.POINTER(C.c_int32))
+                     if weights is not None else C.POINTER(C.c_int32)())
+        return cls(s, False, keep=(index, neighbors, weights))
+
+    @classmethod
+    def read(cls, path: str, weighted: bool) -> "HostGraph":
+        s = HostGraphStruct()
+        check(lib().hc_host_graph_read(path.encode(), int(weighted), C.byref(s)))
+        return cls(s, True)
+
+    def write(self, path: str, weighted: bool):
+        check(lib().hc_host_graph_write(path.encode(), C.byref(self._s), int(weighted)))
+
+    # -- accessors -----------------------------------------------------------
+    @property
+    def num_nodes(self) -> int:
+        return self._s.num_nodes
+
+    @property
+    def num_edges(self) -> int:
+        return self._s.num_edges
+
+    @property
+    def weighted(self) -> bool:
+        return bool(self._s.weights)
+
+    @property
+    def index(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._s.index, shape=(self.num_nodes + 1,))
+
+    @property
+    def neighbors(self) -> np.ndarray:
+        if self.num_edges == 0:
+            return np.zeros(0, np.int32)
+        return np.ctypeslib.as_array(self._s.neighbors, shape=(self.num_edges,))
+
+    @property
+    def weights(self):
+        if not self.weighted:
+            return None
+        if self.num_edges == 0:
+            return np.zeros(0, np.int32)
+        return np.ctypeslib.as_array(self._s.weights, shape=(self.num_edges,))
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.index)
+
+    def pick_sources(self, count: int, seed: int = 37525) -> np.ndarray:
+        out = np.zeros(count, np.int32)
+        check(lib().hc_pick_sources(C.byref(self._s), seed, count, _ptr(out)))
+        return out
+
+    def equal_edge_ranges(self, segments: int, chunk: int = 1) -> np.ndarray:
+        out = np.zeros(segments + 1, np.int32)
+        check(lib().hc_equal_edge_ranges(C.byref(self._s), segments, chunk, _ptr(out)))
+        return out
+
+    def struct_ref(self):
+        return C.byref(self._s)
+
+    def free(self):
+        if self._owned and self._s.index:
+            lib().hc_host_graph_free(C.byref(self._s))
+        self._keep = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Device memory and graphs
+# ---------------------------------------------------------------------------
+
+class DeviceArray:
+    """Raw device allocation with numpy-typed host transfers."""
+
+    def __init__(self, count: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        p = C.c_void_p()
+        check(lib().hc_dmalloc(C.byref(p), max(1, self.count) * self.dtype.itemsize))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, arr) -> "DeviceArray":
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.size, arr.dtype)
+        d.upload(arr)
+        return d
+
+    @property
+    def nbytes(self):
+        return self.count * self.dtype.itemsize
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.size == self.count
+        check(lib().hc_memcpy(self.ptr, _ptr(arr), self.nbytes, 1, None))
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.count, self.dtype)
+        check(lib().hc_memcpy(_ptr(out), self.ptr, self.nbytes, 2, None))
+        return out
+
+    def fill32(self, value: int):
+        assert self.dtype.itemsize == 4
+        v = int(np.array([value & 0xFFFFFFFF], np.uint32).view(np.int32)[0])
+        check(lib().hc_memset_d32(self.ptr, v, self.count, None))
+
+    def free(self):
+        if self.ptr:
+            lib().hc_dfree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    """CSR resident in HBM (whole graph or a row range)."""
+
+    def __init__(self, host: HostGraph, start: int | None = None, end: int | None = None):
+        h = C.c_void_p()
+        if start is None and end is None:
+            check(lib().hc_graph_upload(host.struct_ref(), C.byref(h)))
+        else:
+            start = 0 if start is None else start
+            end = host.num_nodes if end is None else end
+            check(lib().hc_graph_upload_range(host.struct_ref(), start, end, C.byref(h)))
+        self.h = h
+        self.num_nodes = host.num_nodes
+
+    def info(self) -> dict:
+        i = GraphInfo()
+        check(lib().hc_graph_get_info(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def free(self):
+        if self.h:
+            lib().hc_graph_free(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Full runs
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Result:
+    values: np.ndarray
+    stats: dict
+    parents: np.ndarray | None = None
+
+
+def sssp(g: DeviceGraph, source: int, direction: int = 0, alpha: int = 0,
+         max_epochs: int = 0, out: DeviceArray | None = None,
+         fetch: bool = True) -> Result:
+    d = out if out is not None else DeviceArray(g.num_nodes, np.int32)
+    st = RunStats()
+    o = SsspOpts(direction, alpha, max_epochs)
+    check(lib().hc_sssp(g.h, source, d.ptr, C.byref(o), C.byref(st), None))
+    return Result(d.get() if fetch else None, st.as_dict())
+
+
+def sssp_pull(g: DeviceGraph, init: np.ndarray, mapping: int = MAP_BUCKETED,
+              blocks: int = 0, threads: int = 0) -> Result:
+    di = DeviceArray.from_host(np.asarray(init, np.int32))
+    d = DeviceArray(g.num_nodes, np.int32)
+    st = RunStats()
+    check(lib().hc_sssp_pull(g.h, mapping, di.ptr, d.ptr, blocks, threads,
+                             C.byref(st), None))
+    return Result(d.get(), st.as_dict())
+
+
+def bfs(g: DeviceGraph, source: int, alpha: int = 15, beta: int = 18,
+        direction: int = 0, want_parents: bool = True,
+        depth_out: DeviceArray | None = None, fetch: bool = True) -> Result:
+    dd = depth_out if depth_out is not None else DeviceArray(g.num_nodes, np.int32)
+    dp = DeviceArray(g.num_nodes, np.int32) if want_parents else None
+    st = RunStats()
+    o = BfsOpts(alpha, beta, direction)
+    check(lib().hc_bfs(g.h, source, dd.ptr, dp.ptr if dp else None, C.byref(o),
+                       C.byref(st), None))
+    return Result(dd.get() if fetch else None, st.as_dict(),
+                  dp.get() if (dp is not None and fetch) else None)
+
+
+def pagerank(g: DeviceGraph, damping: float = 0.85, tolerance: float = 1e-4,
+             max_iters: int = 20) -> Result:
+    d = DeviceArray(g.num_nodes, np.float32)
+    st = RunStats()
+    o = PrOpts(damping, tolerance, max_iters)
+    check(lib().hc_pagerank(g.h, d.ptr, C.byref(o), C.byref(st), None))
+    return Result(d.get(), st.as_dict())
+
+
+# ---------------------------------------------------------------------------
+# Epoch API
+# ---------------------------------------------------------------------------
+
+def sssp_pull_epoch(g: DeviceGraph, dist: DeviceArray, start: int, end: int,
+                    mapping: int = MAP_BUCKETED, blocks: int = 0,
+                    threads: int = 0) -> int:
+    cnt = DeviceArray.from_host(np.zeros(1, np.int32))
+    check(lib().hc_sssp_pull_epoch(g.h, mapping, start, end, dist.ptr, cnt.ptr,
+                                   blocks, threads, None))
+    return int(cnt.get()[0])
+
+
+def bfs_pull_epoch(g: DeviceGraph, parents: DeviceArray, frontier: DeviceArray,
+                   next_frontier: DeviceArray, start: int, end: int,
+                   mapping: int = MAP_BUCKETED, blocks: int = 0,
+                   threads: int = 0) -> int:
+    cnt = DeviceArray.from_host(np.zeros(1, np.int32))
+    check(lib().hc_bfs_pull_epoch(g.h, mapping, start, end, parents.ptr,
+                                  frontier.ptr, next_frontier.ptr, cnt.ptr,
+                                  blocks, threads, None))
+    return int(cnt.get()[0])
+
+
+def bfs_push_epoch(g: DeviceGraph, frontier: np.ndarray, parents: DeviceArray):
+    f = DeviceArray.from_host(np.asarray(frontier, np.int32))
+    nxt = DeviceArray(g.num_nodes, np.int32)
+    nsz = DeviceArray.from_host(np.zeros(1, np.int32))
+    ne = DeviceArray.from_host(np.zeros(1, np.int64))
+    check(lib().hc_bfs_push_epoch(g.h, f.ptr, len(frontier), parents.ptr, nxt.ptr,
+                                  nsz.ptr, ne.ptr, None))
+    k = int(nsz.get()[0])
+    return nxt.get()[:k], int(ne.get()[0])
+
+
+def pr_pull_epoch(g: DeviceGraph, start: int, end: int, damping: float,
+                  score_in: DeviceArray, contrib: DeviceArray, inv_degree: DeviceArray,
+                  score_out: DeviceArray, contrib_out: DeviceArray) -> float:
+    r = DeviceArray.from_host(np.zeros(1, np.float64))
+    check(lib().hc_pr_pull_epoch(g.h, start, end, damping, score_in.ptr, contrib.ptr,
+                                 inv_degree.ptr, score_out.ptr, contrib_out.ptr,
+                                 r.ptr, None))
+    return float(r.get()[0])
+
+
+def gteps(num_edges_directed: int, ms: float) -> float:
+    """Reference GTEPS: m / 2 / t (src/benchmarks/gpu_benchmark.cuh:590)."""
+    return num_edges_directed / 2.0 / (ms / 1e3) / 1e9

@@ -1,0 +1,669 @@
+// BFS epoch kernels for gfx950.
+//
+//  * Bottom-up (pull) epoch, semantics of epoch_bfs_pull_cpu
+//    (src/kernels/cpu/bfs.cuh:305-335): every unvisited vertex takes the
+//    first neighbour (CSR order) found in the frontier bitmap as its parent.
+//    Lane groups of 1/4/16/64 test consecutive neighbours and stop at the
+//    first ballot hit, so the parent chosen is the same as the sequential
+//    scan's.
+//  * Top-down (push) epoch, semantics of epoch_bfs_push_cpu_by_node
+//    (src/kernels/cpu/bfs.cuh:212-244): frontier vertices claim unvisited
+//    neighbours with compare-and-swap; winners are appended to the next
+//    frontier (wavefront-aggregated) and their degrees summed (scout count).
+//  * Direction-optimizing loop, switching rule of bfs_do_cpu<alpha,beta>
+//    (src/kernels/cpu/bfs.cuh:48-114), evaluated on the device by the last
+//    workgroup of every epoch launch.  Both epochs emit the next frontier as a
+//    bitmap and as degree-classed queues, so no conversion pass is needed
+//    when the direction changes.
+#include <algorithm>
+#include <cstring>
+
+#include "frontier.h"
+
+namespace hc {
+
+__device__ __forceinline__ bool bit_test(const uint32_t *__restrict__ bm, int32_t v) {
+    return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+// Scans row [b, e) of u with G lanes from lane `lane` (group base lane gbase
+// within the wavefront); returns the first neighbour in the frontier.
+template <int G, typename OffT>
+__device__ __forceinline__ bool bu_scan(const GraphView<OffT> &g,
+                                        const uint32_t *__restrict__ frontier,
+                                        OffT b, OffT e, int lane, int gbase,
+                                        int32_t &parent, uint32_t &scanned) {
+    if (G == 1) {
+        for (OffT i = b; i < e; i++) {
+            int32_t v = g.adj[i];
+            scanned++;
+            if (bit_test(frontier, v)) {
+                parent = v;
+                return true;
+            }
+        }
+        return false;
+    } else {
+        for (OffT base = b; base < e; base += OffT(G)) {
+            OffT i = base + OffT(lane);
+            bool hit = false;
+            int32_t v = 0;
+            if (i < e) {
+                v = g.adj[i];
+                scanned++;
+                hit = bit_test(frontier, v);
+            }
+            uint64_t m = __ballot(hit);
+            uint64_t gm = G == 64 ? m : ((m >> gbase) & ((uint64_t(1) << G) - 1));
+            if (gm) {
+                int first = __ffsll((unsigned long long)gm) - 1;
+                parent = __shfl(v, gbase + first);
+                return true;
+            }
+        }
+        return false;
+    }
+}
+
+template <int G, typename OffT>
+__device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
+                                         const GraphView<OffT> &g,
+                                         const int32_t *__restrict__ visited,
+                                         const uint32_t *__restrict__ frontier,
+                                         bool &found, int32_t &u, int32_t &par,
+                                         uint32_t &scanned) {
+    constexpr int PER = kBlock / G;
+    const int gid = threadIdx.x / G, lane = threadIdx.x % G;
+    const int gbase = lane_id() - lane;
+    const int idx = k * PER + gid;
+    if (idx < p.cnt[c]) {
+        int32_t uu = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
+        if (visited[uu] < 0) {
+            int32_t pr = -1;
+            bool f = bu_scan<G>(g, frontier, g.rbeg(uu), g.rend(uu), lane, gbase,
+                                pr, scanned);
+            if (lane == 0 && f) {
+                found = true;
+                u = uu;
+                par = pr;
+            }
+        }
+    }
+}
+
+// One bottom-up sweep over the plan (classes 2..5; the BFS plan keeps classes
+// 0 and 1 empty).  visited is depth (loop) or parents (epoch API); LOOP also
+// writes depth/parents and the queue form of the next frontier.
+template <bool LOOP, typename OffT>
+__device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
+                         int32_t *__restrict__ visited, int32_t *__restrict__ parents,
+                         int32_t level, const uint32_t *__restrict__ frontier,
+                         uint32_t *__restrict__ next, const FrontierBufs &fb,
+                         StepCtl *ctl, int nxt, uint32_t &local_new,
+                         uint32_t &scanned, unsigned long long &edges_acc) {
+    const int total = p.unit_start[kNumClasses];
+    for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
+        int c = 0;
+        while (unit >= p.unit_start[c + 1]) c++;
+        const int k = unit - p.unit_start[c];
+        bool found = false;
+        int32_t u = 0, par = -1;
+        if (c == 2) bu_group<64>(p, c, k, g, visited, frontier, found, u, par, scanned);
+        else if (c == 3) bu_group<16>(p, c, k, g, visited, frontier, found, u, par, scanned);
+        else if (c == 4) bu_group<4>(p, c, k, g, visited, frontier, found, u, par, scanned);
+        else if (c == 5) bu_group<1>(p, c, k, g, visited, frontier, found, u, par, scanned);
+        if (found) {
+            if (LOOP) {
+                visited[u] = level;
+                if (parents) parents[u] = par;
+            } else {
+                visited[u] = par;
+            }
+            atomicOr(&next[u >> 5], 1u << (u & 31));
+            local_new++;
+        }
+        if (LOOP) frontier_append(g, fb, ctl, nxt, found, u, edges_acc);
+    }
+}
+
+// Top-down expansion of one frontier row slice.
+template <int G, typename OffT>
+__device__ __forceinline__ void td_range(const GraphView<OffT> &g,
+                                         int32_t *__restrict__ depth,
+                                         int32_t *__restrict__ parents,
+                                         int32_t level, uint32_t *__restrict__ next,
+                                         const FrontierBufs &fb, StepCtl *ctl,
+                                         int nxt, int32_t u, OffT b, OffT e,
+                                         int lane, uint32_t &scanned,
+                                         unsigned long long &edges_acc) {
+    for (OffT i = b + OffT(lane); i < e; i += OffT(G)) {
+        int32_t v = g.adj[i];
+        scanned++;
+        bool app = false;
+        if (depth[v] < 0 && atomicCAS(&depth[v], -1, level) == -1) {
+            app = true;
+            if (parents) parents[v] = u;
+            atomicOr(&next[v >> 5], 1u << (v & 31));
+        }
+        frontier_append(g, fb, ctl, nxt, app, v, edges_acc);
+    }
+}
+
+template <typename OffT>
+__device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
+                         int32_t *__restrict__ parents, int32_t level,
+                         uint32_t *__restrict__ next, const FrontierBufs &fb,
+                         StepCtl *ctl, int cur, int nxt, uint32_t &scanned,
+                         unsigned long long &edges_acc) {
+    const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
+                   n1 = ctl->cnt[cur][1], n2 = ctl->cnt[cur][2];
+    const uint32_t u1 = nch + n0;
+    const uint32_t u2 = u1 + (n1 + kWavesPerBlock - 1) / kWavesPerBlock;
+    const uint32_t total = u2 + (n2 + kSmallPerUnit - 1) / kSmallPerUnit;
+    for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
+        if (unit < nch) {
+            HubChunk h = fb.ch[cur][unit];
+            td_range<kBlock>(g, depth, parents, level, next, fb, ctl, nxt, h.u,
+                             OffT(h.b), OffT(h.e), threadIdx.x, scanned, edges_acc);
+        } else if (unit < u1) {
+            int32_t u = fb.q[cur][0][unit - nch];
+            td_range<kBlock>(g, depth, parents, level, next, fb, ctl, nxt, u,
+                             g.rbeg(u), g.rend(u), threadIdx.x, scanned, edges_acc);
+        } else if (unit < u2) {
+            uint32_t idx = (unit - u1) * kWavesPerBlock + threadIdx.x / kWave;
+            if (idx < n1) {
+                int32_t u = fb.q[cur][1][idx];
+                td_range<kWave>(g, depth, parents, level, next, fb, ctl, nxt, u,
+                                g.rbeg(u), g.rend(u), lane_id(), scanned, edges_acc);
+            }
+        } else {
+            uint32_t idx = (unit - u2) * kSmallPerUnit + threadIdx.x / kSmallLanes;
+            if (idx < n2) {
+                int32_t u = fb.q[cur][2][idx];
+                td_range<kSmallLanes>(g, depth, parents, level, next, fb, ctl, nxt,
+                                      u, g.rbeg(u), g.rend(u),
+                                      threadIdx.x % kSmallLanes, scanned, edges_acc);
+            }
+        }
+    }
+}
+
+// Direction-optimizing controller: the decisions of bfs_do_cpu
+// (src/kernels/cpu/bfs.cuh:68-103) applied between device epochs.
+__device__ void bfs_controller(StepCtl *c, int ran_mode, int offb) {
+    const int k = c->epoch, cur = k & 1, nxt = cur ^ 1;
+    uint32_t q = 0;
+    for (int j : {0, 1, 2, 4}) q += load_acquire_u32(&c->cnt[nxt][j]);
+    const long long sc = (long long)load_acquire_u64(
+        reinterpret_cast<const uint64_t *>(&c->edges[nxt]));
+    const unsigned long long scanned = load_acquire_u64(
+        reinterpret_cast<const uint64_t *>(&c->scanned));
+    c->reached += q;
+    if (ran_mode == kPush) {
+        uint32_t in_n = c->cnt[cur][0] + c->cnt[cur][1] + c->cnt[cur][2] + c->cnt[cur][4];
+        // queue entry + offsets per frontier vertex; id + depth probe per
+        // edge; depth, parent, bitmap and queue writes per new vertex.
+        c->alg_bytes += (unsigned long long)(4 + 2 * offb) * in_n + 8ull * scanned + 16ull * q;
+        c->push_epochs++;
+    } else {
+        // visited flag per swept row, offsets of unvisited rows (bounded by
+        // the sweep), id + bitmap bit per examined edge, writes per new vertex.
+        c->alg_bytes += 4ull * c->plan_rows + 4ull * scanned + scanned / 8 +
+                        16ull * q + 4ull * ((c->n + 31) / 32);
+        c->pull_epochs++;
+    }
+    for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
+    c->edges[cur] = 0;
+    c->scanned = 0;
+    c->epoch = k + 1;
+    const long long n = c->n;
+    if (c->direction != 0) {
+        c->mode = c->direction == 1 ? kPull : kPush;
+        if (q == 0) c->done = 1;
+    } else if (ran_mode == kPush) {
+        c->scout = sc;
+        if (q == 0) {
+            c->done = 1;
+        } else if (c->scout > c->edges_to_check / c->alpha) {
+            c->mode = kPull;
+            c->num_nodes = (int32_t)q;
+        } else {
+            c->mode = kPush;
+            c->edges_to_check -= c->scout;
+            c->scout = 0;
+        }
+    } else {
+        c->prev_nodes = c->num_nodes;
+        c->num_nodes = (int32_t)q;
+        if ((long long)q > c->prev_nodes || (long long)q > n / c->beta) {
+            c->mode = kPull;
+        } else {
+            c->scout = 1;
+            if (q == 0) {
+                c->done = 1;
+            } else if (c->scout > c->edges_to_check / c->alpha) {
+                c->mode = kPull;
+                c->num_nodes = (int32_t)q;
+            } else {
+                c->mode = kPush;
+                c->edges_to_check -= c->scout;
+                c->scout = 0;
+            }
+        }
+    }
+    if (c->max_epochs > 0 && k + 1 >= c->max_epochs) c->done = 1;
+    c->blocks_done = 0;
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void bfs_step_kernel(
+    PlanDev p, GraphView<OffT> g, int32_t *__restrict__ depth,
+    int32_t *__restrict__ parents, FrontierBufs fb, StepCtl *ctl) {
+    if (ctl->done) return;
+    const int mode = ctl->mode;
+    const int k = ctl->epoch, cur = k & 1, nxt = cur ^ 1;
+    const uint32_t *frontier = fb.bm[k % 3];
+    uint32_t *next = fb.bm[(k + 1) % 3];
+    uint32_t *stale = fb.bm[(k + 2) % 3];
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words;
+         w += gridDim.x * kBlock)
+        stale[w] = 0;
+    const int32_t level = k + 1;
+    uint32_t scanned = 0, nnew = 0;
+    unsigned long long eacc = 0;
+    if (mode == kPull)
+        bu_sweep<true>(p, g, depth, parents, level, frontier, next, fb, ctl, nxt,
+                       nnew, scanned, eacc);
+    else
+        td_sweep(g, depth, parents, level, next, fb, ctl, cur, nxt, scanned, eacc);
+    unsigned long long es = block_sum_u64(eacc);
+    if (threadIdx.x == 0 && es) atomicAdd(&ctl->edges[nxt], es);
+    unsigned long long ss = block_sum_u64(scanned);
+    if (threadIdx.x == 0 && ss) atomicAdd(&ctl->scanned, ss);
+    if (last_block_done(&ctl->blocks_done) && threadIdx.x == 0)
+        bfs_controller(ctl, mode, sizeof(OffT));
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void bfs_init_kernel(
+    GraphView<OffT> g, int32_t *__restrict__ depth, int32_t *__restrict__ parents,
+    FrontierBufs fb, StepCtl *ctl, int32_t src, int direction, int alpha,
+    int beta, int max_epochs, long long m, int32_t plan_rows) {
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride) {
+        depth[i] = i == src ? 0 : -1;
+        if (parents) parents[i] = i == src ? src : -1;
+    }
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words; w += stride) {
+        fb.bm[0][w] = (w == (src >> 5)) ? (1u << (src & 31)) : 0u;
+        fb.bm[1][w] = 0;
+        fb.bm[2][w] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int p = 0; p < 2; p++) {
+            for (int j = 0; j < 5; j++) ctl->cnt[p][j] = 0;
+            ctl->edges[p] = 0;
+        }
+        ctl->blocks_done = 0;
+        ctl->done = 0;
+        ctl->epoch = 0;
+        ctl->max_epochs = max_epochs;
+        ctl->direction = direction;
+        ctl->alpha = alpha;
+        ctl->beta = beta;
+        ctl->m = m;
+        ctl->n = g.n;
+        ctl->push_epochs = ctl->pull_epochs = 0;
+        ctl->reached = 1;
+        ctl->alg_bytes = 0;
+        ctl->scanned = 0;
+        ctl->plan_rows = plan_rows;
+        OffT b = g.rbeg(src), e = g.rend(src);
+        int64_t d = int64_t(e - b);
+        if (d > kHubDeg) {
+            uint32_t nch = 0;
+            for (int64_t x = b; x < int64_t(e); x += kChunkEdges, nch++) {
+                int64_t ce = x + kChunkEdges < int64_t(e) ? x + kChunkEdges : int64_t(e);
+                fb.ch[0][nch] = HubChunk{src, 0, x, ce};
+            }
+            ctl->cnt[0][3] = nch;
+            ctl->cnt[0][4] = 1;
+        } else {
+            int cls = d > kBigDeg ? 0 : (d > kMidDeg ? 1 : 2);
+            fb.q[0][cls][0] = src;
+            ctl->cnt[0][cls] = 1;
+        }
+        ctl->edges[0] = (unsigned long long)d;
+        // Initial decision of bfs_do_cpu: num_edges = deg(src),
+        // edges_to_check = m.
+        ctl->edges_to_check = m;
+        ctl->scout = d;
+        if (direction == 1) {
+            ctl->mode = kPull;
+            ctl->num_nodes = 1;
+        } else if (direction == 2) {
+            ctl->mode = kPush;
+        } else if (ctl->scout > ctl->edges_to_check / alpha) {
+            ctl->mode = kPull;
+            ctl->num_nodes = 1;
+        } else {
+            ctl->mode = kPush;
+            ctl->edges_to_check -= ctl->scout;
+            ctl->scout = 0;
+        }
+    }
+}
+
+// ---- epoch-API kernels (reference semantics: parents marks visited) ----
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void bfs_pull_fused_kernel(
+    PlanDev p, GraphView<OffT> g, int32_t *__restrict__ parents,
+    const uint32_t *__restrict__ frontier, uint32_t *__restrict__ next,
+    int32_t *num_nodes) {
+    uint32_t nnew = 0, scanned = 0;
+    unsigned long long eacc = 0;
+    FrontierBufs fb{};
+    bu_sweep<false>(p, g, parents, nullptr, 0, frontier, next, fb, nullptr, 0,
+                    nnew, scanned, eacc);
+    unsigned long long s = block_sum_u64(nnew);
+    if (threadIdx.x == 0 && s) atomicAdd(num_nodes, (int32_t)s);
+}
+
+template <typename OffT>
+__global__ void bfs_pull_thread_kernel(GraphView<OffT> g,
+                                       int32_t *__restrict__ parents,
+                                       int32_t start, int32_t end,
+                                       const uint32_t *__restrict__ frontier,
+                                       uint32_t *__restrict__ next,
+                                       int32_t *num_nodes) {
+    uint32_t nnew = 0, scanned = 0;
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t u = start + blockIdx.x * blockDim.x + threadIdx.x; u < end;
+         u += stride) {
+        if (parents[u] != -1) continue;
+        int32_t par = -1;
+        if (bu_scan<1>(g, frontier, g.rbeg(u), g.rend(u), 0, 0, par, scanned)) {
+            parents[u] = par;
+            atomicOr(&next[u >> 5], 1u << (u & 31));
+            nnew++;
+        }
+    }
+    for (int s = 32; s > 0; s >>= 1) nnew += __shfl_xor(nnew, s, 64);
+    if (lane_id() == 0 && nnew) atomicAdd(num_nodes, (int32_t)nnew);
+}
+
+template <typename OffT>
+__global__ void bfs_pull_wave_kernel(GraphView<OffT> g,
+                                     int32_t *__restrict__ parents,
+                                     int32_t start, int32_t end,
+                                     const uint32_t *__restrict__ frontier,
+                                     uint32_t *__restrict__ next,
+                                     int32_t *num_nodes) {
+    uint32_t nnew = 0, scanned = 0;
+    const int32_t waves = gridDim.x * (blockDim.x / kWave);
+    const int32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    for (int32_t u = start + wid; u < end; u += waves) {
+        if (parents[u] != -1) continue;
+        int32_t par = -1;
+        if (bu_scan<64>(g, frontier, g.rbeg(u), g.rend(u), lane_id(), 0, par,
+                        scanned) &&
+            lane_id() == 0) {
+            parents[u] = par;
+            atomicOr(&next[u >> 5], 1u << (u & 31));
+            nnew++;
+        }
+    }
+    if (lane_id() == 0 && nnew) atomicAdd(num_nodes, (int32_t)nnew);
+}
+
+// Top-down epoch of the epoch API: wavefront per frontier vertex,
+// compare-and-swap on parents, queue output, scout count.
+template <typename OffT>
+__global__ void bfs_push_epoch_kernel(GraphView<OffT> g,
+                                      const int32_t *__restrict__ frontier,
+                                      int32_t fsize, int32_t *__restrict__ parents,
+                                      int32_t *__restrict__ next_q,
+                                      int32_t *next_size, long long *num_edges) {
+    const int32_t waves = gridDim.x * (blockDim.x / kWave);
+    const int32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    unsigned long long eacc = 0;
+    for (int32_t f = wid; f < fsize; f += waves) {
+        const int32_t u = frontier[f];
+        const OffT b = g.rbeg(u), e = g.rend(u);
+        for (OffT i = b + OffT(lane_id()); i < e; i += OffT(kWave)) {
+            int32_t v = g.adj[i];
+            bool won = parents[v] == -1 && atomicCAS(&parents[v], -1, u) == -1;
+            if (won) eacc += (unsigned long long)(g.rend(v) - g.rbeg(v));
+            wave_append(next_q, reinterpret_cast<uint32_t *>(next_size), won, v);
+        }
+    }
+    for (int s = 32; s > 0; s >>= 1) eacc += __shfl_xor(eacc, s, 64);
+    if (lane_id() == 0 && eacc)
+        atomicAdd(reinterpret_cast<unsigned long long *>(num_edges), eacc);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct BfsWs {
+    FrontierWs f;
+    int32_t *depth = nullptr;  // used when the caller passes no depth array
+};
+
+void free_bfs_ws(hc_graph_s *g) {
+    if (!g->ws_bfs) return;
+    auto *ws = static_cast<BfsWs *>(g->ws_bfs);
+    frontier_ws_free(&ws->f);
+    if (ws->depth) hipFree(ws->depth);
+    delete ws;
+    g->ws_bfs = nullptr;
+}
+
+static int get_bfs_ws(hc_graph_s *g, BfsWs **out) {
+    if (!g->ws_bfs) {
+        auto *ws = new BfsWs();
+        int rc = frontier_ws_alloc(g, &ws->f);
+        if (rc != HC_OK) {
+            delete ws;
+            return rc;
+        }
+        if (hipMalloc(&ws->depth, sizeof(int32_t) * std::max(g->n, 1)) != hipSuccess) {
+            frontier_ws_free(&ws->f);
+            delete ws;
+            return fail(HC_ERR_OOM, "BFS depth workspace allocation failed");
+        }
+        g->ws_bfs = ws;
+    }
+    *out = static_cast<BfsWs *>(g->ws_bfs);
+    return HC_OK;
+}
+
+// Bottom-up plan: no chunk or workgroup classes (early exit makes them
+// wasteful); wavefront for degree > 256, 16 lanes > 32, 4 lanes > 8.
+static PlanConfig bu_config() {
+    PlanConfig c;
+    c.hub_threshold = INT64_MAX;
+    c.block_threshold = INT64_MAX;
+    c.wave_threshold = 256;
+    c.g16_threshold = 32;
+    c.g4_threshold = 8;
+    return c;
+}
+
+template <typename OffT>
+static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
+                        int32_t *parents, int direction, int alpha, int beta,
+                        hc_run_stats_t *stats, hipStream_t s) {
+    BfsWs *ws = nullptr;
+    HC_TRY(get_bfs_ws(g, &ws));
+    const PlanHost *ph = nullptr;
+    HC_TRY(get_plan(g, 0, g->n, bu_config(), &ph));
+    int32_t plan_rows = 0;
+    for (int c = 0; c < kNumClasses; c++) plan_rows += c == 0 ? 0 : ph->dev.cnt[c];
+    if (!depth) depth = ws->depth;
+    GraphView<OffT> v = view<OffT>(g);
+    const int grid = grid_blocks(8);
+    const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
+    bfs_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
+        v, depth, parents, ws->f.fb, ws->f.ctl, src, direction, alpha, beta, 0,
+        (long long)g->m, plan_rows);
+    HC_HIP(hipGetLastError());
+    hipEvent_t ev0, ev1, evb[2];
+    HC_HIP(hipEventCreate(&ev0));
+    HC_HIP(hipEventCreate(&ev1));
+    HC_HIP(hipEventCreateWithFlags(&evb[0], hipEventDisableTiming));
+    HC_HIP(hipEventCreateWithFlags(&evb[1], hipEventDisableTiming));
+    StepCtl *hslot = ws->f.h_ctl;
+    hslot->done = 0;
+    HC_HIP(hipEventRecord(ev0, s));
+    const int batch = 4;
+    const long long cap = (long long)g->n + 2 * batch + 2;  // levels <= n
+    long long launches = 0;
+    int rc = HC_OK;
+    bool done = false;
+    int b = 0;
+    while (!done && launches < cap) {
+        for (int i = 0; i < batch; i++) {
+            bfs_step_kernel<OffT><<<grid, kBlock, 0, s>>>(ph->dev, v, depth, parents,
+                                                          ws->f.fb, ws->f.ctl);
+            launches++;
+        }
+        if (hipGetLastError() != hipSuccess) {
+            rc = fail(HC_ERR_HIP, "step launch failed");
+            break;
+        }
+        hipMemcpyAsync(&hslot->done, &ws->f.ctl->done, sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s);
+        hipEventRecord(evb[b & 1], s);
+        if (b > 0) {
+            hipEventSynchronize(evb[(b - 1) & 1]);
+            done = __atomic_load_n(&hslot->done, __ATOMIC_ACQUIRE) != 0;
+        }
+        b++;
+    }
+    HC_HIP(hipEventRecord(ev1, s));
+    HC_HIP(hipEventSynchronize(ev1));
+    float ms = 0;
+    HC_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    StepCtl fc;
+    HC_HIP(hipMemcpy(&fc, ws->f.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    hipEventDestroy(evb[0]);
+    hipEventDestroy(evb[1]);
+    if (rc != HC_OK) return rc;
+    if (!fc.done) return fail(HC_ERR_HIP, "BFS loop did not finish in %lld launches", launches);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->ms = ms;
+        stats->epochs = fc.epoch;
+        stats->push_epochs = fc.push_epochs;
+        stats->pull_epochs = fc.pull_epochs;
+        stats->launches = (int32_t)launches;
+        stats->reached = (int64_t)fc.reached;
+        stats->alg_bytes = (double)fc.alg_bytes;
+    }
+    return HC_OK;
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int hc_bfs(hc_graph_t g, int32_t source, int32_t *depth, int32_t *parents,
+           const hc_bfs_opts_t *opts, hc_run_stats_t *stats,
+           hc_stream_t stream) {
+    HC_CHECK_ARG(g, "graph is NULL");
+    HC_CHECK_ARG(g->row0 == 0 && g->rows == g->n,
+                 "full runs need a whole-graph handle");
+    HC_CHECK_ARG(source >= 0 && source < g->n, "source %d outside [0,%d)",
+                 source, g->n);
+    int alpha = opts && opts->alpha > 0 ? opts->alpha : 15;
+    int beta = opts && opts->beta > 0 ? opts->beta : 18;
+    int direction = opts ? opts->direction : 0;
+    HC_CHECK_ARG(direction >= 0 && direction <= 2, "direction must be 0, 1 or 2");
+    HC_TRY(ensure_init());
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    return g->off64 ? run_bfs_loop<uint64_t>(g, source, depth, parents, direction, alpha, beta, stats, s)
+                    : run_bfs_loop<uint32_t>(g, source, depth, parents, direction, alpha, beta, stats, s);
+}
+
+int hc_bfs_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
+                      int32_t end_id, int32_t *parents,
+                      const uint32_t *frontier, uint32_t *next_frontier,
+                      int32_t *num_nodes, int blocks, int threads,
+                      hc_stream_t stream) {
+    HC_CHECK_ARG(g && parents && frontier && next_frontier && num_nodes,
+                 "NULL argument");
+    HC_CHECK_ARG(g->row0 <= start_id && start_id <= end_id &&
+                     end_id <= g->row0 + g->rows,
+                 "range [%d,%d) outside the rows held [%d,%d)", start_id,
+                 end_id, g->row0, g->row0 + g->rows);
+    HC_CHECK_ARG(mapping == HC_MAP_THREAD || mapping == HC_MAP_WAVE ||
+                     mapping == HC_MAP_BUCKETED,
+                 "BFS pull epochs support THREAD, WAVE and BUCKETED mappings");
+    HC_CHECK_ARG(blocks >= 0 && threads >= 0 && threads <= 1024, "bad launch shape");
+    if (mapping == HC_MAP_WAVE)
+        HC_CHECK_ARG(threads % kWave == 0, "thread count must be a multiple of 64");
+    HC_TRY(ensure_init());
+    if (start_id == end_id) return HC_OK;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    if (!threads) threads = 256;
+    if (!blocks) blocks = grid_blocks(8);
+    auto go = [&](auto tag) -> int {
+        using OffT = decltype(tag);
+        GraphView<OffT> v = view<OffT>(g);
+        if (mapping == HC_MAP_THREAD) {
+            bfs_pull_thread_kernel<OffT><<<blocks, threads, 0, s>>>(
+                v, parents, start_id, end_id, frontier, next_frontier, num_nodes);
+        } else if (mapping == HC_MAP_WAVE) {
+            bfs_pull_wave_kernel<OffT><<<blocks, threads, 0, s>>>(
+                v, parents, start_id, end_id, frontier, next_frontier, num_nodes);
+        } else {
+            const PlanHost *ph = nullptr;
+            HC_TRY(get_plan(g, start_id, end_id, bu_config(), &ph));
+            if (ph->total_units == 0) return HC_OK;
+            int nb = std::max(1, std::min(ph->total_units, grid_blocks(8)));
+            bfs_pull_fused_kernel<OffT><<<nb, kBlock, 0, s>>>(
+                ph->dev, v, parents, frontier, next_frontier, num_nodes);
+        }
+        HC_HIP(hipGetLastError());
+        return HC_OK;
+    };
+    return g->off64 ? go(uint64_t(0)) : go(uint32_t(0));
+}
+
+int hc_bfs_push_epoch(hc_graph_t g, const int32_t *frontier,
+                      int32_t frontier_size, int32_t *parents,
+                      int32_t *next_frontier, int32_t *next_size,
+                      int64_t *num_edges, hc_stream_t stream) {
+    HC_CHECK_ARG(g && parents && next_frontier && next_size && num_edges,
+                 "NULL argument");
+    HC_CHECK_ARG(frontier_size >= 0, "frontier_size < 0");
+    HC_CHECK_ARG(frontier_size == 0 || frontier, "frontier is NULL");
+    HC_CHECK_ARG(g->row0 == 0 && g->rows == g->n,
+                 "push epochs need a whole-graph handle");
+    HC_TRY(ensure_init());
+    if (frontier_size == 0) return HC_OK;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    int blocks = std::max(1, std::min(grid_blocks(8), (frontier_size + 3) / 4));
+    if (g->off64)
+        bfs_push_epoch_kernel<uint64_t><<<blocks, 256, 0, s>>>(
+            view<uint64_t>(g), frontier, frontier_size, parents, next_frontier,
+            next_size, reinterpret_cast<long long *>(num_edges));
+    else
+        bfs_push_epoch_kernel<uint32_t><<<blocks, 256, 0, s>>>(
+            view<uint32_t>(g), frontier, frontier_size, parents, next_frontier,
+            next_size, reinterpret_cast<long long *>(num_edges));
+    HC_HIP(hipGetLastError());
+    return HC_OK;
+}
+
+}  // extern "C"

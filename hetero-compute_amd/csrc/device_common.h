@@ -1,0 +1,220 @@
+// Device-side building blocks shared by the epoch kernels (hipcc only).
+//
+// Wavefronts are 64 lanes on gfx950; every group reduction, ballot and
+// aggregated append below is written for that width.  Offsets are templated
+// on uint32_t / uint64_t so graphs under 2^32 edges use 4-byte row offsets.
+#ifndef HC_DEVICE_COMMON_H
+#define HC_DEVICE_COMMON_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "hc_internal.h"
+
+#define HC_HIP(expr)                                                        \
+    do {                                                                    \
+        hipError_t hc_e_ = (expr);                                          \
+        if (hc_e_ != hipSuccess)                                            \
+            return hc::fail(HC_ERR_HIP, "%s failed: %s (%s:%d)", #expr,     \
+                            hipGetErrorString(hc_e_), __FILE__, __LINE__); \
+    } while (0)
+
+namespace hc {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;          // threads per workgroup of the fused kernels
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+// ------------------------------------------------------------------------
+// Graph view passed by value to kernels.
+// ------------------------------------------------------------------------
+template <typename OffT>
+struct GraphView {
+    const OffT *off;     // rows+1 offsets, local to the handle's first row
+    const int32_t *adj;  // neighbour ids (global)
+    const int32_t *wt;   // weights or nullptr
+    int32_t row0;        // global id of local row 0
+    int32_t n;           // global vertex count
+    __device__ __forceinline__ OffT rbeg(int32_t u) const { return off[u - row0]; }
+    __device__ __forceinline__ OffT rend(int32_t u) const { return off[u - row0 + 1]; }
+};
+
+// ------------------------------------------------------------------------
+// Degree-bucketed plan of a vertex range.
+// Classes (in unit order): 0 hub chunks (workgroup per edge chunk),
+// 1 workgroup per vertex, 2 wavefront per vertex, 3 16-lane group per vertex,
+// 4 4-lane group per vertex, 5 thread per vertex.  Vertices of degree 0 are in
+// no class (the zero tail of a degree-sorted graph is skipped).
+// ------------------------------------------------------------------------
+constexpr int kNumClasses = 6;
+constexpr int kClassGroup[kNumClasses] = {kBlock, kBlock, 64, 16, 4, 1};
+constexpr int kClassPerUnit[kNumClasses] = {1, 1, 4, 16, 64, 256};
+
+struct HubChunk {
+    int32_t u;      // global vertex id
+    int32_t hub;    // index of the hub vertex within the plan
+    int64_t b, e;   // local edge range of this chunk
+};
+
+struct PlanDev {
+    // Per class: either a contiguous id range [beg, beg+cnt) or a list.
+    int32_t beg[kNumClasses];
+    int32_t cnt[kNumClasses];           // vertices (class 0: chunks)
+    const int32_t *list[kNumClasses];   // nullptr => range
+    int32_t unit_start[kNumClasses + 1];
+    const HubChunk *chunks;             // class 0 descriptors
+    int32_t num_hubs;                   // distinct hub vertices
+    const int32_t *hub_chunk_start;     // num_hubs+1 prefix of chunks per hub
+    // Degree-0 vertices of the range (not in any unit): range or list.
+    int32_t zero_beg;
+    int32_t zero_cnt;
+    const int32_t *zero_list;
+};
+
+struct PlanConfig {
+    int64_t hub_threshold = 16384;  // deg > this -> chunks
+    int64_t hub_chunk = 4096;       // edges per chunk
+    int64_t block_threshold = 1024; // deg > this -> workgroup per vertex
+    int64_t wave_threshold = 96;    // deg > this -> wavefront per vertex
+    int64_t g16_threshold = 24;     // deg > this -> 16 lanes
+    int64_t g4_threshold = 5;       // deg > this -> 4 lanes, else 1
+    bool operator<(const PlanConfig &o) const {
+        return std::tie(hub_threshold, hub_chunk, block_threshold,
+                        wave_threshold, g16_threshold, g4_threshold) <
+               std::tie(o.hub_threshold, o.hub_chunk, o.block_threshold,
+                        o.wave_threshold, o.g16_threshold, o.g4_threshold);
+    }
+};
+
+// Host-side owner of a plan's device arrays.
+struct PlanHost {
+    PlanDev dev{};
+    void *dmem = nullptr;  // single allocation holding lists and chunks
+    int32_t total_units = 0;
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    return (uint64_t(1) << lane_id()) - 1;
+}
+
+template <int G>
+__device__ __forceinline__ int32_t group_min(int32_t v) {
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1) v = min(v, __shfl_xor(v, s, G));
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ int64_t group_sum64(int64_t v) {
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1) v += __shfl_xor(v, s, G);
+    return v;
+}
+
+// Saturating distance candidate: d <= HC_INF_DIST and w < 2^31, so the
+// unsigned sum cannot wrap; results are clamped to the sentinel.
+__device__ __forceinline__ int32_t relax(int32_t d, int32_t w) {
+    uint32_t c = uint32_t(d) + uint32_t(w);
+    return int32_t(c < uint32_t(HC_INF_DIST) ? c : uint32_t(HC_INF_DIST));
+}
+
+// Wavefront-aggregated append: every active lane with pred set gets a
+// distinct slot of q; one atomic per wavefront.
+__device__ __forceinline__ void wave_append(int32_t *q, uint32_t *count,
+                                            bool pred, int32_t value) {
+    uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (pred) q[base + __popcll(m & lanemask_lt())] = value;
+}
+
+__device__ __forceinline__ uint32_t load_acquire_u32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t load_acquire_u64(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Last-workgroup detection: returns true in every thread of the workgroup
+// that finished last.  Every thread releases its own global writes before the
+// ticket is taken, and the last workgroup acquires, so all writes (stores and
+// atomics) of every workgroup are visible to it afterwards.
+__device__ __forceinline__ bool last_block_done(uint32_t *blocks_done) {
+    __shared__ int s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = atomicAdd(blocks_done, 1u);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return s_last != 0;
+}
+
+// ------------------------------------------------------------------------
+// Runtime state (host)
+// ------------------------------------------------------------------------
+struct DeviceState {
+    int device = -1;
+    int cus = 0;
+    size_t hbm = 0;
+    hipStream_t stream = nullptr;  // library default stream
+    bool ready = false;
+};
+DeviceState &device_state();
+int ensure_init();
+int grid_blocks(int per_cu = 8);
+hipStream_t pick_stream(hc_stream_t s);
+
+}  // namespace hc
+
+// Host-side graph handle.
+struct hc_graph_s {
+    int32_t n = 0;          // global vertex count
+    int32_t row0 = 0;       // first row held
+    int32_t rows = 0;       // rows held
+    int64_t m = 0;          // edges held
+    bool weighted = false;
+    bool sorted = false;    // degrees non-increasing over the held rows
+    bool off64 = false;
+    int32_t num_nonzero = 0;
+    int64_t max_degree = 0;
+    void *d_off = nullptr;
+    int32_t *d_adj = nullptr;
+    int32_t *d_wt = nullptr;
+    size_t bytes = 0;
+    std::vector<int64_t> h_off;  // local offsets on the host (planning)
+    std::map<std::tuple<int32_t, int32_t, hc::PlanConfig>, hc::PlanHost> plans;
+    std::mutex mu;
+    // Per-algorithm workspaces (allocated on first use, freed with the graph).
+    void *ws_sssp = nullptr;
+    void *ws_bfs = nullptr;
+    void *ws_pr = nullptr;
+};
+
+namespace hc {
+// Plan for rows [start, end) under cfg (cached on the graph).
+int get_plan(hc_graph_s *g, int32_t start, int32_t end, const PlanConfig &cfg,
+             const PlanHost **out);
+template <typename OffT>
+inline GraphView<OffT> view(const hc_graph_s *g) {
+    return GraphView<OffT>{static_cast<const OffT *>(g->d_off), g->d_adj,
+                           g->d_wt, g->row0, g->n};
+}
+void free_sssp_ws(hc_graph_s *g);
+void free_bfs_ws(hc_graph_s *g);
+void free_pr_ws(hc_graph_s *g);
+}  // namespace hc
+
+#endif

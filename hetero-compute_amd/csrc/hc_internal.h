@@ -1,0 +1,33 @@
+// Internal helpers shared by the host (g++) and device (hipcc) halves of the
+// library: error state, status codes, small checked-call macros.
+#ifndef HC_INTERNAL_H
+#define HC_INTERNAL_H
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/hc_capi.h"
+
+namespace hc {
+
+// Records a formatted message for hc_last_error() on this thread and returns
+// the status code so call sites can `return fail(...)`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+}  // namespace hc
+
+#define HC_CHECK_ARG(cond, ...)                                   \
+    do {                                                          \
+        if (!(cond)) return hc::fail(HC_ERR_INVALID, __VA_ARGS__); \
+    } while (0)
+
+#define HC_TRY(expr)               \
+    do {                           \
+        int hc_rc_ = (expr);       \
+        if (hc_rc_ != HC_OK) return hc_rc_; \
+    } while (0)
+
+#endif

@@ -1,0 +1,488 @@
+// PageRank pull epochs for gfx950 (Jacobi form).
+//
+// score'[u] = (1-d)/N + d * sum_{v in N(u)} contrib[v], contrib[v] =
+// score[v]/deg(v): the per-vertex formula of epoch_pr_pull_cpu_one_to_one
+// (src/kernels/cpu/pr.cuh:165-175) with
+//   * the neighbour sum in FP32 with Kahan compensation per lane and
+//     TwoSum-combined lane and chunk partials (fixed order: deterministic);
+//   * contrib computed once per vertex as (float)((double)score * inv_deg)
+//     with inv_deg = 1/deg in FP64, fused into the epoch that writes score;
+//   * the L1 residual sum |score' - score| in FP64: one partial per
+//     workgroup, summed in workgroup order by the last workgroup, which also
+//     takes the stopping decision (residual < tol or max_iters reached).
+// Scores start at 1/N, as pr_pull_cpu does (src/kernels/cpu/pr.cuh:38-42).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "frontier.h"
+
+namespace hc {
+
+struct KSum {
+    float hi, lo;  // value ~ hi + lo
+};
+
+__device__ __forceinline__ KSum two_sum_combine(KSum a, KSum b) {
+    float s = a.hi + b.hi;
+    float bp = s - a.hi;
+    float err = (a.hi - (s - bp)) + (b.hi - bp);
+    return KSum{s, a.lo + b.lo + err};
+}
+
+template <int G>
+__device__ __forceinline__ KSum group_ksum(KSum k) {
+#pragma unroll
+    for (int s = G / 2; s > 0; s >>= 1) {
+        KSum o{__shfl_xor(k.hi, s, G), __shfl_xor(k.lo, s, G)};
+        // Combine in a lane-order-independent way: the lower lane's value is
+        // the left operand on both sides of the exchange.
+        bool low = (lane_id() & s) == 0;
+        k = low ? two_sum_combine(k, o) : two_sum_combine(o, k);
+    }
+    return k;
+}
+
+// Kahan-compensated sum of contrib over edges [b, e) strided by G.
+template <int G, typename OffT>
+__device__ __forceinline__ KSum row_ksum(const GraphView<OffT> &g,
+                                         const float *__restrict__ contrib,
+                                         OffT b, OffT e, int lane) {
+    float sum = 0.0f, comp = 0.0f;
+    OffT i = b + OffT(lane);
+    for (; i + OffT(3 * G) < e; i += OffT(4 * G)) {
+        float x0 = contrib[g.adj[i]], x1 = contrib[g.adj[i + G]],
+              x2 = contrib[g.adj[i + 2 * G]], x3 = contrib[g.adj[i + 3 * G]];
+        float xs[4] = {x0, x1, x2, x3};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float y = xs[j] - comp;
+            float t = sum + y;
+            comp = (t - sum) - y;
+            sum = t;
+        }
+    }
+    for (; i < e; i += OffT(G)) {
+        float y = contrib[g.adj[i]] - comp;
+        float t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    }
+    return KSum{sum, -comp};
+}
+
+__device__ __forceinline__ KSum block_ksum(KSum k) {
+    __shared__ float s_hi[16], s_lo[16];
+    k = group_ksum<64>(k);
+    __syncthreads();
+    if (lane_id() == 0) {
+        s_hi[threadIdx.x / kWave] = k.hi;
+        s_lo[threadIdx.x / kWave] = k.lo;
+    }
+    __syncthreads();
+    KSum r{s_hi[0], s_lo[0]};
+    const int nw = blockDim.x / kWave;
+    for (int i = 1; i < nw; i++) r = two_sum_combine(r, KSum{s_hi[i], s_lo[i]});
+    return r;
+}
+
+__device__ __forceinline__ double block_sum_f64(double v) {
+    __shared__ double s_part[16];
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+    __syncthreads();
+    if (lane_id() == 0) s_part[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x / kWave); i++) t += s_part[i];
+    return t;
+}
+
+struct PrBufs {
+    const double *inv_deg;
+    float *hub_hi;         // per chunk partials
+    float *hub_lo;
+    uint32_t *hub_cnt;     // per hub: chunks finished this epoch
+    double *hub_resid;     // per hub: |score' - score|
+    double *partials;      // per workgroup residual partial
+};
+
+struct PrCtl {
+    uint32_t blocks_done;
+    int32_t done;
+    int32_t iter;
+    int32_t max_iters;
+    double tol;
+    double residual;
+    double alg_bytes;
+    double epoch_bytes;
+};
+
+struct PrEpoch {
+    const float *s_in;
+    const float *c_in;
+    float *s_out;
+    float *c_out;
+    float base;
+    float damp;
+};
+
+__device__ __forceinline__ double pr_finalize(const PrEpoch &ep, const double *inv_deg,
+                                              int32_t u, KSum k) {
+    float sum = k.hi + k.lo;
+    float s_new = __fadd_rn(ep.base, __fmul_rn(ep.damp, sum));
+    ep.s_out[u] = s_new;
+    ep.c_out[u] = (float)((double)s_new * inv_deg[u]);
+    return fabs((double)s_new - (double)ep.s_in[u]);
+}
+
+template <int G, typename OffT>
+__device__ __forceinline__ void pr_group(const PlanDev &p, int c, int k,
+                                         const GraphView<OffT> &g,
+                                         const PrEpoch &ep, const PrBufs &b,
+                                         double &racc) {
+    constexpr int PER = kBlock / G;
+    const int gid = threadIdx.x / G, lane = threadIdx.x % G;
+    const int idx = k * PER + gid;
+    if (idx < p.cnt[c]) {
+        int32_t u = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
+        KSum s = row_ksum<G>(g, ep.c_in, g.rbeg(u), g.rend(u), lane);
+        if (G > 1) s = group_ksum<G>(s);
+        if (lane == 0) racc += pr_finalize(ep, b.inv_deg, u, s);
+    }
+}
+
+// One Jacobi sweep over the plan units and the zero-degree vertices.
+// Returns this thread's residual contribution (hub residuals go to
+// b.hub_resid instead, so the total does not depend on which workgroup
+// finishes a hub).
+template <typename OffT>
+__device__ double pr_sweep(const PlanDev &p, const GraphView<OffT> &g,
+                           const PrEpoch &ep, const PrBufs &b) {
+    double racc = 0.0;
+    const int gstride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.zero_cnt; i += gstride) {
+        int32_t u = p.zero_list ? p.zero_list[i] : p.zero_beg + i;
+        racc += pr_finalize(ep, b.inv_deg, u, KSum{0.0f, 0.0f});
+    }
+    const int total = p.unit_start[kNumClasses];
+    for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
+        int c = 0;
+        while (unit >= p.unit_start[c + 1]) c++;
+        const int k = unit - p.unit_start[c];
+        if (c == 0) {
+            HubChunk h = p.chunks[k];
+            KSum s = block_ksum(row_ksum<kBlock>(g, ep.c_in, OffT(h.b), OffT(h.e),
+                                                 threadIdx.x));
+            if (threadIdx.x == 0) {
+                b.hub_hi[k] = s.hi;
+                b.hub_lo[k] = s.lo;
+                __threadfence();
+                const int32_t c0 = p.hub_chunk_start[h.hub];
+                const int32_t c1 = p.hub_chunk_start[h.hub + 1];
+                uint32_t t = atomicAdd(&b.hub_cnt[h.hub], 1u);
+                if (t == uint32_t(c1 - c0 - 1)) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    KSum tot{0.0f, 0.0f};
+                    for (int32_t j = c0; j < c1; j++) {
+                        KSum part{__hip_atomic_load(&b.hub_hi[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                                  __hip_atomic_load(&b.hub_lo[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+                        tot = two_sum_combine(tot, part);
+                    }
+                    b.hub_resid[h.hub] = pr_finalize(ep, b.inv_deg, h.u, tot);
+                    b.hub_cnt[h.hub] = 0;
+                }
+            }
+        } else if (c == 1) {
+            int32_t u = p.list[1] ? p.list[1][k] : p.beg[1] + k;
+            KSum s = block_ksum(row_ksum<kBlock>(g, ep.c_in, g.rbeg(u), g.rend(u),
+                                                 threadIdx.x));
+            if (threadIdx.x == 0) racc += pr_finalize(ep, b.inv_deg, u, s);
+        } else if (c == 2) {
+            pr_group<64>(p, c, k, g, ep, b, racc);
+        } else if (c == 3) {
+            pr_group<16>(p, c, k, g, ep, b, racc);
+        } else if (c == 4) {
+            pr_group<4>(p, c, k, g, ep, b, racc);
+        } else {
+            pr_group<1>(p, c, k, g, ep, b, racc);
+        }
+    }
+    return racc;
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void pr_step_kernel(
+    PlanDev p, GraphView<OffT> g, float *s0, float *s1, float *c0, float *c1,
+    PrBufs b, PrCtl *ctl, float base, float damp) {
+    if (ctl->done) return;
+    const int it = ctl->iter;
+    const bool odd = it & 1;
+    PrEpoch ep{odd ? s1 : s0, odd ? c1 : c0, odd ? s0 : s1, odd ? c0 : c1, base, damp};
+    double r = block_sum_f64(pr_sweep(p, g, ep, b));
+    if (threadIdx.x == 0) b.partials[blockIdx.x] = r;
+    if (last_block_done(&ctl->blocks_done)) {
+        // Fixed-order total: workgroup partials, then hub residuals.
+        __shared__ double s_tot[kBlock];
+        double t = 0.0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) t += b.partials[i];
+        for (int i = threadIdx.x; i < p.num_hubs; i += kBlock) t += b.hub_resid[i];
+        s_tot[threadIdx.x] = t;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) s_tot[threadIdx.x] += s_tot[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            ctl->residual = s_tot[0];
+            ctl->iter = it + 1;
+            ctl->alg_bytes += ctl->epoch_bytes;
+            if (s_tot[0] < ctl->tol || it + 1 >= ctl->max_iters) ctl->done = 1;
+            ctl->blocks_done = 0;
+        }
+    }
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void pr_init_kernel(
+    GraphView<OffT> g, float *s0, float *c0, double *inv_deg, uint32_t *hub_cnt,
+    int32_t num_hubs, PrCtl *ctl, int max_iters, double tol, double epoch_bytes) {
+    const int stride = gridDim.x * kBlock;
+    const float init = 1.0f / (float)g.n;
+    for (int u = blockIdx.x * kBlock + threadIdx.x; u < g.n; u += stride) {
+        int64_t d = int64_t(g.rend(u) - g.rbeg(u));
+        double inv = d > 0 ? 1.0 / (double)d : 0.0;
+        inv_deg[u] = inv;
+        s0[u] = init;
+        c0[u] = (float)((double)init * inv);
+    }
+    for (int h = blockIdx.x * kBlock + threadIdx.x; h < num_hubs; h += stride) hub_cnt[h] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->blocks_done = 0;
+        ctl->done = max_iters <= 0 ? 1 : 0;
+        ctl->iter = 0;
+        ctl->max_iters = max_iters;
+        ctl->tol = tol;
+        ctl->residual = 0.0;
+        ctl->alg_bytes = 0.0;
+        ctl->epoch_bytes = epoch_bytes;
+    }
+}
+
+// Epoch API: one sweep over a plan, residual accumulated atomically.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void pr_epoch_kernel(
+    PlanDev p, GraphView<OffT> g, PrEpoch ep, PrBufs b, double *residual_out) {
+    double r = block_sum_f64(pr_sweep(p, g, ep, b));
+    if (threadIdx.x == 0 && residual_out) atomicAdd(residual_out, r);
+}
+
+template <typename OffT>
+__global__ void pr_hub_resid_kernel(const double *hub_resid, int32_t n, double *out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && out) {
+        double t = 0;
+        for (int i = 0; i < n; i++) t += hub_resid[i];
+        atomicAdd(out, t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct PrWs {
+    float *s[2] = {nullptr, nullptr};
+    float *c[2] = {nullptr, nullptr};
+    double *inv_deg = nullptr;
+    float *hub_hi = nullptr, *hub_lo = nullptr;
+    uint32_t *hub_cnt = nullptr;
+    double *hub_resid = nullptr;
+    double *partials = nullptr;
+    PrCtl *ctl = nullptr;
+    int32_t chunk_cap = 0, hub_cap = 0, part_cap = 0;
+    void *dmem = nullptr;
+};
+
+void free_pr_ws(hc_graph_s *g) {
+    if (!g->ws_pr) return;
+    auto *ws = static_cast<PrWs *>(g->ws_pr);
+    if (ws->dmem) hipFree(ws->dmem);
+    delete ws;
+    g->ws_pr = nullptr;
+}
+
+static int get_pr_ws(hc_graph_s *g, int32_t chunks, int32_t hubs, PrWs **out) {
+    auto *ws = static_cast<PrWs *>(g->ws_pr);
+    const int32_t parts = grid_blocks(8);
+    if (ws && ws->chunk_cap >= chunks && ws->hub_cap >= hubs && ws->part_cap >= parts) {
+        *out = ws;
+        return HC_OK;
+    }
+    free_pr_ws(g);
+    ws = new PrWs();
+    const size_t n = std::max(g->n, 1);
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t sz_f = al(sizeof(float) * n), sz_d = al(sizeof(double) * n);
+    size_t sz_ch = al(sizeof(float) * std::max(chunks, 1));
+    size_t sz_h = al(sizeof(double) * std::max(hubs, 1));
+    size_t sz_p = al(sizeof(double) * parts);
+    size_t total = 4 * sz_f + sz_d + 2 * sz_ch + 2 * sz_h + sz_p + al(sizeof(PrCtl));
+    char *base = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&base), total) != hipSuccess) {
+        delete ws;
+        return fail(HC_ERR_OOM, "PageRank workspace of %zu bytes failed", total);
+    }
+    size_t o = 0;
+    for (int i = 0; i < 2; i++) {
+        ws->s[i] = reinterpret_cast<float *>(base + o);
+        o += sz_f;
+        ws->c[i] = reinterpret_cast<float *>(base + o);
+        o += sz_f;
+    }
+    ws->inv_deg = reinterpret_cast<double *>(base + o);
+    o += sz_d;
+    ws->hub_hi = reinterpret_cast<float *>(base + o);
+    o += sz_ch;
+    ws->hub_lo = reinterpret_cast<float *>(base + o);
+    o += sz_ch;
+    ws->hub_cnt = reinterpret_cast<uint32_t *>(base + o);
+    o += sz_h;
+    ws->hub_resid = reinterpret_cast<double *>(base + o);
+    o += sz_h;
+    ws->partials = reinterpret_cast<double *>(base + o);
+    o += sz_p;
+    ws->ctl = reinterpret_cast<PrCtl *>(base + o);
+    ws->dmem = base;
+    ws->chunk_cap = std::max(chunks, 1);
+    ws->hub_cap = std::max(hubs, 1);
+    ws->part_cap = parts;
+    g->ws_pr = ws;
+    *out = ws;
+    return HC_OK;
+}
+
+static long long pr_epoch_bytes(const hc_graph_s *g, const PlanHost *ph) {
+    const PlanDev &p = ph->dev;
+    long long rows = p.num_hubs;
+    for (int c = 1; c < kNumClasses; c++) rows += p.cnt[c];
+    const long long ob = g->off64 ? 8 : 4;
+    // offsets; id + gathered contrib per edge; score in/out, contrib out and
+    // FP64 inverse degree per row; zero-degree rows: score in/out + contrib.
+    return ob * (rows + 1) + 8ll * g->m + 20ll * rows + 12ll * p.zero_cnt;
+}
+
+template <typename OffT>
+static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
+                  int max_iters, hc_run_stats_t *stats, hipStream_t s) {
+    const PlanHost *ph = nullptr;
+    HC_TRY(get_plan(g, 0, g->n, PlanConfig{}, &ph));
+    PrWs *ws = nullptr;
+    HC_TRY(get_pr_ws(g, ph->dev.cnt[0], ph->dev.num_hubs, &ws));
+    GraphView<OffT> v = view<OffT>(g);
+    const int grid = grid_blocks(8);
+    const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
+    const float base = (1.0f - damp) / (float)g->n;
+    pr_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
+        v, ws->s[0], ws->c[0], ws->inv_deg, ws->hub_cnt, ph->dev.num_hubs, ws->ctl,
+        max_iters, tol, (double)pr_epoch_bytes(g, ph));
+    HC_HIP(hipGetLastError());
+    PrBufs b{ws->inv_deg, ws->hub_hi, ws->hub_lo, ws->hub_cnt, ws->hub_resid, ws->partials};
+    hipEvent_t e0, e1;
+    HC_HIP(hipEventCreate(&e0));
+    HC_HIP(hipEventCreate(&e1));
+    HC_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < max_iters; i++)
+        pr_step_kernel<OffT><<<grid, kBlock, 0, s>>>(ph->dev, v, ws->s[0], ws->s[1],
+                                                     ws->c[0], ws->c[1], b, ws->ctl,
+                                                     base, damp);
+    HC_HIP(hipGetLastError());
+    HC_HIP(hipEventRecord(e1, s));
+    HC_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    HC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    PrCtl c;
+    HC_HIP(hipMemcpy(&c, ws->ctl, sizeof(PrCtl), hipMemcpyDeviceToHost));
+    const float *final_s = (c.iter & 1) ? ws->s[1] : ws->s[0];
+    HC_HIP(hipMemcpyAsync(scores, final_s, sizeof(float) * g->n,
+                          hipMemcpyDeviceToDevice, s));
+    HC_HIP(hipStreamSynchronize(s));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->ms = ms;
+        stats->epochs = c.iter;
+        stats->pull_epochs = c.iter;
+        stats->launches = max_iters;
+        stats->reached = -1;
+        stats->residual = c.residual;
+        stats->alg_bytes = c.alg_bytes;
+    }
+    return HC_OK;
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int hc_pagerank(hc_graph_t g, float *scores, const hc_pr_opts_t *opts,
+                hc_run_stats_t *stats, hc_stream_t stream) {
+    HC_CHECK_ARG(g && scores, "NULL argument");
+    HC_CHECK_ARG(g->row0 == 0 && g->rows == g->n,
+                 "full runs need a whole-graph handle");
+    HC_CHECK_ARG(g->n > 0, "empty graph");
+    float damp = opts && opts->damping > 0 ? opts->damping : 0.85f;
+    // tolerance 0 selects the default; a negative tolerance disables the
+    // residual test (exactly max_iters iterations).
+    double tol = opts && opts->tolerance != 0 ? opts->tolerance : 1e-4;
+    int max_iters = opts && opts->max_iters > 0 ? opts->max_iters : 20;
+    HC_CHECK_ARG(damp < 1.0f, "damping must be in (0,1)");
+    HC_CHECK_ARG(max_iters <= 100000, "max_iters too large");
+    HC_TRY(ensure_init());
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    return g->off64 ? run_pr<uint64_t>(g, scores, damp, tol, max_iters, stats, s)
+                    : run_pr<uint32_t>(g, scores, damp, tol, max_iters, stats, s);
+}
+
+int hc_pr_pull_epoch(hc_graph_t g, int32_t start_id, int32_t end_id,
+                     float damping, const float *score_in,
+                     const float *contrib, const double *inv_degree,
+                     float *score_out, float *contrib_out,
+                     double *residual_out, hc_stream_t stream) {
+    HC_CHECK_ARG(g && score_in && contrib && inv_degree && score_out && contrib_out,
+                 "NULL argument");
+    HC_CHECK_ARG(g->row0 <= start_id && start_id <= end_id &&
+                     end_id <= g->row0 + g->rows,
+                 "range [%d,%d) outside the rows held [%d,%d)", start_id,
+                 end_id, g->row0, g->row0 + g->rows);
+    HC_CHECK_ARG(damping > 0 && damping < 1, "damping must be in (0,1)");
+    HC_TRY(ensure_init());
+    if (start_id == end_id) return HC_OK;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    const PlanHost *ph = nullptr;
+    HC_TRY(get_plan(g, start_id, end_id, PlanConfig{}, &ph));
+    PrWs *ws = nullptr;
+    HC_TRY(get_pr_ws(g, ph->dev.cnt[0], ph->dev.num_hubs, &ws));
+    if (ph->dev.num_hubs > 0)
+        HC_HIP(hipMemsetAsync(ws->hub_cnt, 0, sizeof(uint32_t) * ph->dev.num_hubs, s));
+    PrBufs b{inv_degree, ws->hub_hi, ws->hub_lo, ws->hub_cnt, ws->hub_resid, ws->partials};
+    PrEpoch ep{score_in, contrib, score_out, contrib_out,
+               (1.0f - damping) / (float)g->n, damping};
+    const int nb = std::max(1, std::min(std::max(ph->total_units, 1), grid_blocks(8)));
+    if (g->off64) {
+        pr_epoch_kernel<uint64_t><<<nb, kBlock, 0, s>>>(ph->dev, view<uint64_t>(g), ep, b, residual_out);
+        pr_hub_resid_kernel<uint64_t><<<1, 64, 0, s>>>(ws->hub_resid, ph->dev.num_hubs, residual_out);
+    } else {
+        pr_epoch_kernel<uint32_t><<<nb, kBlock, 0, s>>>(ph->dev, view<uint32_t>(g), ep, b, residual_out);
+        pr_hub_resid_kernel<uint32_t><<<1, 64, 0, s>>>(ws->hub_resid, ph->dev.num_hubs, residual_out);
+    }
+    HC_HIP(hipGetLastError());
+    return HC_OK;
+}
+
+}  // extern "C"

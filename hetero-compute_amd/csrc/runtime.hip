@@ -1,0 +1,445 @@
+// Runtime: device selection, memory and stream helpers, CSR upload into HBM
+// (replacing copy_subgraph_to_device, src/graph.cuh:250-286) and the
+// degree-bucket plans the fused epoch kernels run from.
+#include <algorithm>
+#include <cstring>
+
+#include "device_common.h"
+
+namespace hc {
+
+DeviceState &device_state() {
+    static DeviceState s;
+    return s;
+}
+
+int ensure_init() {
+    DeviceState &s = device_state();
+    if (s.ready) {
+        HC_HIP(hipSetDevice(s.device));
+        return HC_OK;
+    }
+    return hc_init(0);
+}
+
+int grid_blocks(int per_cu) { return std::max(1, device_state().cus * per_cu); }
+
+hipStream_t pick_stream(hc_stream_t s) {
+    return s ? static_cast<hipStream_t>(s) : device_state().stream;
+}
+
+// Builds (or returns the cached) plan for rows [start, end).
+int get_plan(hc_graph_s *g, int32_t start, int32_t end, const PlanConfig &cfg,
+             const PlanHost **out) {
+    auto key = std::make_tuple(start, end, cfg);
+    auto it = g->plans.find(key);
+    if (it != g->plans.end()) {
+        *out = &it->second;
+        return HC_OK;
+    }
+    auto deg = [&](int32_t u) {
+        return g->h_off[u - g->row0 + 1] - g->h_off[u - g->row0];
+    };
+    auto cls_of = [&](int64_t d) {
+        if (d == 0) return -1;
+        if (d > cfg.hub_threshold) return 0;
+        if (d > cfg.block_threshold) return 1;
+        if (d > cfg.wave_threshold) return 2;
+        if (d > cfg.g16_threshold) return 3;
+        if (d > cfg.g4_threshold) return 4;
+        return 5;
+    };
+    PlanHost ph;
+    PlanDev &p = ph.dev;
+    std::vector<int32_t> lists[kNumClasses];
+    std::vector<HubChunk> chunks;
+    std::vector<int32_t> hub_start;
+    std::vector<int32_t> hubs;
+    std::vector<int32_t> zeros;
+    if (g->sorted) {
+        // Contiguous classes: class c is [first u with deg <= T_{c-1},
+        // first u with deg <= T_c).
+        const int64_t thr[kNumClasses] = {cfg.hub_threshold, cfg.block_threshold,
+                                          cfg.wave_threshold, cfg.g16_threshold,
+                                          cfg.g4_threshold, 0};
+        int32_t lo = start;
+        for (int c = 0; c < kNumClasses; c++) {
+            // first index in [lo, end) with deg <= thr[c]
+            int32_t a = lo, b = end;
+            while (a < b) {
+                int32_t mid = a + (b - a) / 2;
+                if (deg(mid) <= thr[c]) b = mid; else a = mid + 1;
+            }
+            p.beg[c] = lo;
+            p.cnt[c] = a - lo;
+            lo = a;
+        }
+        p.zero_beg = lo;
+        p.zero_cnt = end - lo;
+        for (int32_t u = p.beg[0]; u < p.beg[0] + p.cnt[0]; u++) hubs.push_back(u);
+    } else {
+        for (int32_t u = start; u < end; u++) {
+            int c = cls_of(deg(u));
+            if (c < 0) {
+                zeros.push_back(u);
+                continue;
+            }
+            if (c == 0) hubs.push_back(u);
+            else lists[c].push_back(u);
+        }
+        for (int c = 1; c < kNumClasses; c++) {
+            p.beg[c] = 0;
+            p.cnt[c] = (int32_t)lists[c].size();
+        }
+        p.zero_beg = 0;
+        p.zero_cnt = (int32_t)zeros.size();
+    }
+    // Hub chunks.
+    for (size_t h = 0; h < hubs.size(); h++) {
+        int32_t u = hubs[h];
+        int64_t b = g->h_off[u - g->row0], e = g->h_off[u - g->row0 + 1];
+        hub_start.push_back((int32_t)chunks.size());
+        for (int64_t x = b; x < e; x += cfg.hub_chunk)
+            chunks.push_back(HubChunk{u, (int32_t)h, x, std::min(e, x + cfg.hub_chunk)});
+    }
+    hub_start.push_back((int32_t)chunks.size());
+    p.beg[0] = 0;
+    p.cnt[0] = (int32_t)chunks.size();
+    p.num_hubs = (int32_t)hubs.size();
+    p.unit_start[0] = 0;
+    for (int c = 0; c < kNumClasses; c++) {
+        int32_t units = (p.cnt[c] + kClassPerUnit[c] - 1) / kClassPerUnit[c];
+        p.unit_start[c + 1] = p.unit_start[c] + units;
+    }
+    ph.total_units = p.unit_start[kNumClasses];
+    // One device allocation: chunks | hub_start | lists...
+    size_t bytes = chunks.size() * sizeof(HubChunk);
+    size_t off_hub = bytes;
+    bytes += hub_start.size() * sizeof(int32_t);
+    size_t off_list[kNumClasses] = {0};
+    for (int c = 1; c < kNumClasses; c++) {
+        bytes = (bytes + 7) & ~size_t(7);
+        off_list[c] = bytes;
+        bytes += lists[c].size() * sizeof(int32_t);
+    }
+    bytes = (bytes + 7) & ~size_t(7);
+    size_t off_zero = bytes;
+    bytes += zeros.size() * sizeof(int32_t);
+    bytes = std::max<size_t>(bytes, 8);
+    HC_HIP(hipMalloc(&ph.dmem, bytes));
+    char *base = static_cast<char *>(ph.dmem);
+    if (!chunks.empty())
+        HC_HIP(hipMemcpy(base, chunks.data(), chunks.size() * sizeof(HubChunk),
+                         hipMemcpyHostToDevice));
+    HC_HIP(hipMemcpy(base + off_hub, hub_start.data(),
+                     hub_start.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    p.chunks = reinterpret_cast<const HubChunk *>(base);
+    p.hub_chunk_start = reinterpret_cast<const int32_t *>(base + off_hub);
+    p.list[0] = nullptr;
+    for (int c = 1; c < kNumClasses; c++) {
+        if (g->sorted) {
+            p.list[c] = nullptr;
+        } else {
+            if (!lists[c].empty())
+                HC_HIP(hipMemcpy(base + off_list[c], lists[c].data(),
+                                 lists[c].size() * sizeof(int32_t),
+                                 hipMemcpyHostToDevice));
+            p.list[c] = reinterpret_cast<const int32_t *>(base + off_list[c]);
+        }
+    }
+    if (g->sorted) {
+        p.zero_list = nullptr;
+    } else {
+        if (!zeros.empty())
+            HC_HIP(hipMemcpy(base + off_zero, zeros.data(),
+                             zeros.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        p.zero_list = reinterpret_cast<const int32_t *>(base + off_zero);
+    }
+    auto res = g->plans.emplace(key, ph);
+    *out = &res.first->second;
+    return HC_OK;
+}
+
+static int upload_rows(const hc_host_graph_t *hg, int32_t start, int32_t end,
+                       hc_graph_t *out) {
+    HC_CHECK_ARG(hg && out, "NULL argument");
+    HC_CHECK_ARG(hg->index, "graph index is NULL");
+    HC_CHECK_ARG(hg->num_nodes >= 0, "num_nodes < 0");
+    HC_CHECK_ARG(0 <= start && start <= end && end <= hg->num_nodes,
+                 "row range [%d,%d) outside [0,%d)", start, end, hg->num_nodes);
+    HC_CHECK_ARG(hg->num_edges == 0 || hg->neighbors,
+                 "graph neighbors are NULL");
+    HC_TRY(ensure_init());
+    *out = nullptr;
+    const int64_t *idx = hg->index;
+    HC_CHECK_ARG(idx[0] == 0 && idx[hg->num_nodes] == hg->num_edges,
+                 "index[0] must be 0 and index[N] must equal num_edges");
+    const int32_t rows = end - start;
+    const int64_t e0 = idx[start], e1 = idx[end];
+    const int64_t m = e1 - e0;
+    auto *g = new (std::nothrow) hc_graph_s();
+    if (!g) return fail(HC_ERR_OOM, "host allocation failed");
+    g->n = hg->num_nodes;
+    g->row0 = start;
+    g->rows = rows;
+    g->m = m;
+    g->weighted = hg->weights != nullptr;
+    // 4-byte offsets while every offset plus an unrolled stride stays far
+    // from 2^32; HC_FORCE_OFF64=1 selects 8-byte offsets for testing.
+    const char *force64 = getenv("HC_FORCE_OFF64");
+    g->off64 = m >= (int64_t(1) << 31) || (force64 && force64[0] == '1');
+    g->h_off.resize(size_t(rows) + 1);
+    bool sorted = true;
+    int64_t maxd = 0;
+    int32_t nz = 0;
+    for (int32_t i = 0; i <= rows; i++) {
+        g->h_off[i] = idx[start + i] - e0;
+        if (i > 0) {
+            int64_t d = g->h_off[i] - g->h_off[i - 1];
+            if (d < 0) {
+                delete g;
+                return fail(HC_ERR_INVALID, "index not monotone at row %d",
+                            start + i - 1);
+            }
+            if (i > 1 && d > g->h_off[i - 1] - g->h_off[i - 2]) sorted = false;
+            maxd = std::max(maxd, d);
+            if (d > 0) nz = i;
+        }
+    }
+    g->sorted = sorted;
+    g->max_degree = maxd;
+    // For sorted rows, [row0, row0+nz) holds every non-zero-degree row.
+    g->num_nonzero = sorted ? nz : rows;
+    // Validate neighbour ids and weights (the reference reads them unchecked).
+    const int32_t *nb = hg->neighbors + e0;
+    for (int64_t e = 0; e < m; e++)
+        if (nb[e] < 0 || nb[e] >= hg->num_nodes) {
+            delete g;
+            return fail(HC_ERR_INVALID, "neighbour %lld = %d outside [0,%d)",
+                        (long long)(e0 + e), nb[e], hg->num_nodes);
+        }
+    if (g->weighted) {
+        const int32_t *w = hg->weights + e0;
+        for (int64_t e = 0; e < m; e++)
+            if (w[e] < 0 || w[e] >= HC_INF_DIST) {
+                delete g;
+                return fail(HC_ERR_INVALID,
+                            "weight %lld = %d outside [0, 2^30)",
+                            (long long)(e0 + e), w[e]);
+            }
+    }
+    size_t ob = g->off64 ? 8 : 4;
+    size_t off_bytes = ob * (size_t(rows) + 1);
+    size_t adj_bytes = sizeof(int32_t) * size_t(std::max<int64_t>(m, 1));
+    hipError_t e1r = hipMalloc(&g->d_off, off_bytes);
+    hipError_t e2r = hipMalloc(reinterpret_cast<void **>(&g->d_adj), adj_bytes);
+    hipError_t e3r = hipSuccess;
+    if (g->weighted)
+        e3r = hipMalloc(reinterpret_cast<void **>(&g->d_wt), adj_bytes);
+    if (e1r != hipSuccess || e2r != hipSuccess || e3r != hipSuccess) {
+        hipFree(g->d_off);
+        hipFree(g->d_adj);
+        hipFree(g->d_wt);
+        delete g;
+        return fail(HC_ERR_OOM, "device allocation of the CSR (%zu bytes) failed",
+                    off_bytes + adj_bytes * (hg->weights ? 2 : 1));
+    }
+    g->bytes = off_bytes + adj_bytes * (g->weighted ? 2 : 1);
+    int rc = HC_OK;
+    if (g->off64) {
+        std::vector<uint64_t> o(g->h_off.begin(), g->h_off.end());
+        if (hipMemcpy(g->d_off, o.data(), off_bytes, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(HC_ERR_HIP, "offset upload failed");
+    } else {
+        std::vector<uint32_t> o(g->h_off.begin(), g->h_off.end());
+        if (hipMemcpy(g->d_off, o.data(), off_bytes, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(HC_ERR_HIP, "offset upload failed");
+    }
+    if (rc == HC_OK && m > 0 &&
+        hipMemcpy(g->d_adj, nb, sizeof(int32_t) * m, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(HC_ERR_HIP, "neighbour upload failed");
+    if (rc == HC_OK && g->weighted && m > 0 &&
+        hipMemcpy(g->d_wt, hg->weights + e0, sizeof(int32_t) * m,
+                  hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(HC_ERR_HIP, "weight upload failed");
+    if (rc != HC_OK) {
+        hc_graph_free(g);
+        return rc;
+    }
+    *out = g;
+    return HC_OK;
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int hc_init(int device) {
+    DeviceState &s = device_state();
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return fail(HC_ERR_NO_DEVICE, "no HIP device available (%s)",
+                    e != hipSuccess ? hipGetErrorString(e) : "count 0");
+    HC_CHECK_ARG(device >= 0 && device < count, "device %d of %d", device, count);
+    if (s.ready && s.device == device) {
+        HC_HIP(hipSetDevice(device));
+        return HC_OK;
+    }
+    HC_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HC_HIP(hipGetDeviceProperties(&prop, device));
+    s.device = device;
+    s.cus = prop.multiProcessorCount;
+    s.hbm = prop.totalGlobalMem;
+    if (!s.stream) HC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    s.ready = true;
+    return HC_OK;
+}
+
+int hc_get_device_info(hc_device_info_t *out) {
+    HC_CHECK_ARG(out, "out is NULL");
+    HC_TRY(ensure_init());
+    hipDeviceProp_t prop;
+    HC_HIP(hipGetDeviceProperties(&prop, device_state().device));
+    std::memset(out, 0, sizeof(*out));
+    std::snprintf(out->name, sizeof(out->name), "%s", prop.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
+    out->compute_units = prop.multiProcessorCount;
+    out->wavefront_size = prop.warpSize;
+    out->hbm_bytes = prop.totalGlobalMem;
+    out->lds_bytes_per_block = (int32_t)prop.sharedMemPerBlock;
+    return HC_OK;
+}
+
+int hc_device_synchronize(void) {
+    HC_TRY(ensure_init());
+    HC_HIP(hipDeviceSynchronize());
+    return HC_OK;
+}
+
+int hc_dmalloc(void **ptr, size_t bytes) {
+    HC_CHECK_ARG(ptr, "ptr is NULL");
+    HC_TRY(ensure_init());
+    *ptr = nullptr;
+    if (hipMalloc(ptr, std::max<size_t>(bytes, 4)) != hipSuccess)
+        return fail(HC_ERR_OOM, "hipMalloc(%zu) failed", bytes);
+    return HC_OK;
+}
+
+int hc_dfree(void *ptr) {
+    if (!ptr) return HC_OK;
+    HC_HIP(hipFree(ptr));
+    return HC_OK;
+}
+
+int hc_hmalloc_pinned(void **ptr, size_t bytes) {
+    HC_CHECK_ARG(ptr, "ptr is NULL");
+    HC_TRY(ensure_init());
+    if (hipHostMalloc(ptr, std::max<size_t>(bytes, 4), hipHostMallocDefault) != hipSuccess)
+        return fail(HC_ERR_OOM, "hipHostMalloc(%zu) failed", bytes);
+    return HC_OK;
+}
+
+int hc_hfree_pinned(void *ptr) {
+    if (!ptr) return HC_OK;
+    HC_HIP(hipHostFree(ptr));
+    return HC_OK;
+}
+
+int hc_memcpy(void *dst, const void *src, size_t bytes, int kind,
+              hc_stream_t stream) {
+    HC_CHECK_ARG(kind >= 1 && kind <= 3, "kind must be 1, 2 or 3");
+    if (bytes == 0) return HC_OK;
+    HC_CHECK_ARG(dst && src, "NULL pointer");
+    HC_TRY(ensure_init());
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                      : kind == 2 ? hipMemcpyDeviceToHost
+                                  : hipMemcpyDeviceToDevice;
+    hipStream_t s = pick_stream(stream);
+    HC_HIP(hipMemcpyAsync(dst, src, bytes, k, s));
+    HC_HIP(hipStreamSynchronize(s));
+    return HC_OK;
+}
+
+__global__ void fill_i32_kernel(int32_t *p, int32_t v, size_t n) {
+    size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+int hc_memset_d32(void *dst, int32_t value, size_t count, hc_stream_t stream) {
+    if (count == 0) return HC_OK;
+    HC_CHECK_ARG(dst, "dst is NULL");
+    HC_TRY(ensure_init());
+    hipStream_t s = pick_stream(stream);
+    int blocks = (int)std::min<size_t>((count + 255) / 256, size_t(grid_blocks(8)));
+    fill_i32_kernel<<<blocks, 256, 0, s>>>(static_cast<int32_t *>(dst), value, count);
+    HC_HIP(hipGetLastError());
+    HC_HIP(hipStreamSynchronize(s));
+    return HC_OK;
+}
+
+int hc_stream_create(hc_stream_t *stream) {
+    HC_CHECK_ARG(stream, "stream is NULL");
+    HC_TRY(ensure_init());
+    hipStream_t s;
+    HC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return HC_OK;
+}
+
+int hc_stream_destroy(hc_stream_t stream) {
+    if (!stream) return HC_OK;
+    HC_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+    return HC_OK;
+}
+
+int hc_stream_synchronize(hc_stream_t stream) {
+    HC_TRY(ensure_init());
+    HC_HIP(hipStreamSynchronize(pick_stream(stream)));
+    return HC_OK;
+}
+
+int hc_graph_upload(const hc_host_graph_t *g, hc_graph_t *out) {
+    HC_CHECK_ARG(g, "graph is NULL");
+    return upload_rows(g, 0, g->num_nodes, out);
+}
+
+int hc_graph_upload_range(const hc_host_graph_t *g, int32_t start_id,
+                          int32_t end_id, hc_graph_t *out) {
+    HC_CHECK_ARG(g, "graph is NULL");
+    return upload_rows(g, start_id, end_id, out);
+}
+
+int hc_graph_free(hc_graph_t g) {
+    if (!g) return HC_OK;
+    free_sssp_ws(g);
+    free_bfs_ws(g);
+    free_pr_ws(g);
+    for (auto &kv : g->plans) hipFree(kv.second.dmem);
+    hipFree(g->d_off);
+    hipFree(g->d_adj);
+    hipFree(g->d_wt);
+    delete g;
+    return HC_OK;
+}
+
+int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out) {
+    HC_CHECK_ARG(g && out, "NULL argument");
+    out->num_nodes = g->n;
+    out->num_edges = g->m;
+    out->start_id = g->row0;
+    out->end_id = g->row0 + g->rows;
+    out->degree_sorted = g->sorted ? 1 : 0;
+    out->weighted = g->weighted ? 1 : 0;
+    out->offset_bytes = g->off64 ? 8 : 4;
+    out->num_nonzero = g->num_nonzero;
+    out->max_degree = g->max_degree;
+    out->device_bytes = g->bytes;
+    return HC_OK;
+}
+
+}  // extern "C"

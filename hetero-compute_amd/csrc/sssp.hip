@@ -1,0 +1,679 @@
+// SSSP epoch kernels for gfx950.
+//
+//  * Pull epoch (Bellman-Ford, in place): dist[u] = min(dist[u],
+//    min_{(v,w)} dist[v]+w), the arithmetic of epoch_sssp_pull_cpu_one_to_one
+//    (src/kernels/cpu/sssp.cuh:155-182) with int32 weights and a saturating
+//    sentinel.  Vertex mappings: thread / wavefront / workgroup per vertex
+//    (counterparts of src/kernels/gpu/sssp.cuh:120-281) and the fused
+//    degree-bucketed plan (hub chunks with atomicMin, workgroup, wavefront,
+//    16-lane, 4-lane and 1-lane groups in one launch).
+//  * Push epoch: relax the out-edges of the active frontier with atomicMin and
+//    append improved vertices (deduplicated through a bitmap) to the next
+//    frontier, wavefront-aggregated.
+//  * Device-driven loop: one launch per epoch; the last workgroup of each
+//    launch picks the next direction (pull when the frontier's edges exceed
+//    m/alpha) and stops the loop when nothing changed, so the host only
+//    enqueues launches.
+#include <algorithm>
+#include <cstring>
+
+#include "frontier.h"
+
+namespace hc {
+
+// ---------------------------------------------------------------------------
+// Row reductions
+// ---------------------------------------------------------------------------
+
+// Minimum candidate over edges [b, e) taken G apart starting at b+lane; four
+// independent gathers per trip keep several HBM requests in flight per lane.
+template <int G, typename OffT>
+__device__ __forceinline__ int32_t row_min(const GraphView<OffT> &g,
+                                           const int32_t *__restrict__ dist,
+                                           OffT b, OffT e, int lane) {
+    int32_t m = HC_INF_DIST;
+    OffT i = b + OffT(lane);
+    for (; i + OffT(3 * G) < e; i += OffT(4 * G)) {
+        int32_t v0 = g.adj[i], v1 = g.adj[i + G], v2 = g.adj[i + 2 * G],
+                v3 = g.adj[i + 3 * G];
+        int32_t w0 = g.wt[i], w1 = g.wt[i + G], w2 = g.wt[i + 2 * G],
+                w3 = g.wt[i + 3 * G];
+        int32_t d0 = dist[v0], d1 = dist[v1], d2 = dist[v2], d3 = dist[v3];
+        m = min(m, min(min(relax(d0, w0), relax(d1, w1)),
+                       min(relax(d2, w2), relax(d3, w3))));
+    }
+    for (; i < e; i += OffT(G)) m = min(m, relax(dist[g.adj[i]], g.wt[i]));
+    return m;
+}
+
+// Workgroup minimum for any blockDim that is a multiple of 64 (<= 1024).
+// The leading barrier keeps the LDS slots from being overwritten while a
+// previous call is still reading them (the reuse race of
+// src/kernels/gpu/sssp.cuh:259-275).
+__device__ __forceinline__ int32_t block_min(int32_t m) {
+    __shared__ int32_t s_min[16];
+    m = group_min<64>(m);
+    __syncthreads();
+    if (lane_id() == 0) s_min[threadIdx.x / kWave] = m;
+    __syncthreads();
+    int32_t r = s_min[0];
+    const int nw = blockDim.x / kWave;
+    for (int i = 1; i < nw; i++) r = min(r, s_min[i]);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Fused degree-bucketed pull sweep
+// ---------------------------------------------------------------------------
+
+template <int G, typename OffT>
+__device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
+                                           const GraphView<OffT> &g,
+                                           int32_t *__restrict__ dist,
+                                           bool &chg, int32_t &u) {
+    constexpr int PER = kBlock / G;
+    const int gid = threadIdx.x / G, lane = threadIdx.x % G;
+    const int idx = k * PER + gid;
+    if (idx < p.cnt[c]) {
+        int32_t uu = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
+        int32_t m = row_min<G>(g, dist, g.rbeg(uu), g.rend(uu), lane);
+        if (G > 1) m = group_min<G>(m);
+        if (lane == 0) {
+            int32_t cur = dist[uu];
+            if (m < cur) {
+                dist[uu] = m;
+                chg = true;
+            }
+            u = uu;
+        }
+    }
+}
+
+// One pull sweep over the plan's units (grid-stride).  LOOP selects frontier
+// output (device-driven loop) or an update count (epoch API).
+template <bool LOOP, typename OffT>
+__device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
+                                int32_t *__restrict__ dist,
+                                const FrontierBufs &fb, StepCtl *ctl, int nxt,
+                                uint32_t &local_upd,
+                                unsigned long long &edges_acc) {
+    const int total = p.unit_start[kNumClasses];
+    for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
+        int c = 0;
+        while (unit >= p.unit_start[c + 1]) c++;
+        const int k = unit - p.unit_start[c];
+        bool chg = false;
+        int32_t u = 0;
+        if (c <= 1) {
+            int32_t uu;
+            OffT b, e;
+            if (c == 0) {
+                HubChunk h = p.chunks[k];
+                uu = h.u;
+                b = OffT(h.b);
+                e = OffT(h.e);
+            } else {
+                uu = p.list[1] ? p.list[1][k] : p.beg[1] + k;
+                b = g.rbeg(uu);
+                e = g.rend(uu);
+            }
+            int32_t m = block_min(row_min<kBlock>(g, dist, b, e, threadIdx.x));
+            if (threadIdx.x == 0) {
+                if (c == 0) {
+                    if (m < dist[uu]) {
+                        int32_t old = atomicMin(&dist[uu], m);
+                        chg = m < old;
+                    }
+                } else {
+                    int32_t cur = dist[uu];
+                    if (m < cur) {
+                        dist[uu] = m;
+                        chg = true;
+                    }
+                }
+                u = uu;
+            }
+        } else if (c == 2) {
+            pull_group<64>(p, c, k, g, dist, chg, u);
+        } else if (c == 3) {
+            pull_group<16>(p, c, k, g, dist, chg, u);
+        } else if (c == 4) {
+            pull_group<4>(p, c, k, g, dist, chg, u);
+        } else {
+            pull_group<1>(p, c, k, g, dist, chg, u);
+        }
+        if (LOOP) {
+            bool first = false;
+            if (chg) {
+                uint32_t bit = 1u << (u & 31);
+                first = !(atomicOr(&fb.bm[nxt][u >> 5], bit) & bit);
+            }
+            frontier_append(g, fb, ctl, nxt, first, u, edges_acc);
+        } else {
+            local_upd += chg ? 1u : 0u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Push expansion of the frontier
+// ---------------------------------------------------------------------------
+
+template <int G, typename OffT>
+__device__ __forceinline__ void push_range(const GraphView<OffT> &g,
+                                           int32_t *__restrict__ dist,
+                                           const FrontierBufs &fb, StepCtl *ctl,
+                                           int nxt, int32_t u, OffT b, OffT e,
+                                           int lane,
+                                           unsigned long long &edges_acc) {
+    const int32_t du = dist[u];
+    if (du >= HC_INF_DIST) return;  // group-uniform
+    for (OffT i = b + OffT(lane); i < e; i += OffT(G)) {
+        int32_t v = g.adj[i];
+        int32_t nd = relax(du, g.wt[i]);
+        bool app = false;
+        if (nd < dist[v]) {
+            int32_t old = atomicMin(&dist[v], nd);
+            if (nd < old) {
+                uint32_t bit = 1u << (v & 31);
+                app = !(atomicOr(&fb.bm[nxt][v >> 5], bit) & bit);
+            }
+        }
+        frontier_append(g, fb, ctl, nxt, app, v, edges_acc);
+    }
+}
+
+template <typename OffT>
+__device__ void sssp_push_sweep(const GraphView<OffT> &g,
+                                int32_t *__restrict__ dist,
+                                const FrontierBufs &fb, StepCtl *ctl, int cur,
+                                int nxt, unsigned long long &edges_acc) {
+    const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
+                   n1 = ctl->cnt[cur][1], n2 = ctl->cnt[cur][2];
+    const uint32_t u1 = nch + n0;
+    const uint32_t u2 = u1 + (n1 + kWavesPerBlock - 1) / kWavesPerBlock;
+    const uint32_t total = u2 + (n2 + kSmallPerUnit - 1) / kSmallPerUnit;
+    for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
+        if (unit < nch) {
+            HubChunk h = fb.ch[cur][unit];
+            push_range<kBlock>(g, dist, fb, ctl, nxt, h.u, OffT(h.b), OffT(h.e),
+                               threadIdx.x, edges_acc);
+        } else if (unit < u1) {
+            int32_t u = fb.q[cur][0][unit - nch];
+            push_range<kBlock>(g, dist, fb, ctl, nxt, u, g.rbeg(u), g.rend(u),
+                               threadIdx.x, edges_acc);
+        } else if (unit < u2) {
+            uint32_t idx = (unit - u1) * kWavesPerBlock + threadIdx.x / kWave;
+            if (idx < n1) {
+                int32_t u = fb.q[cur][1][idx];
+                push_range<kWave>(g, dist, fb, ctl, nxt, u, g.rbeg(u), g.rend(u),
+                                  lane_id(), edges_acc);
+            }
+        } else {
+            uint32_t idx = (unit - u2) * kSmallPerUnit + threadIdx.x / kSmallLanes;
+            if (idx < n2) {
+                int32_t u = fb.q[cur][2][idx];
+                push_range<kSmallLanes>(g, dist, fb, ctl, nxt, u, g.rbeg(u),
+                                        g.rend(u), threadIdx.x % kSmallLanes,
+                                        edges_acc);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Controller (runs in thread 0 of the last workgroup of a step)
+// ---------------------------------------------------------------------------
+
+__device__ void sssp_controller(StepCtl *c, int ran_mode, long long sweep_bytes) {
+    const int k = c->epoch, cur = k & 1, nxt = cur ^ 1;
+    uint32_t out_n = 0;
+    for (int j : {0, 1, 2, 4}) out_n += load_acquire_u32(&c->cnt[nxt][j]);
+    const unsigned long long out_e = load_acquire_u64(
+        reinterpret_cast<const uint64_t *>(&c->edges[nxt]));
+    if (ran_mode == kPush) {
+        uint32_t in_n = c->cnt[cur][0] + c->cnt[cur][1] + c->cnt[cur][2] + c->cnt[cur][4];
+        // offsets + dist[u] per frontier vertex; idx + weight + dist[v] per
+        // edge; dist write + queue entry per improved vertex.
+        c->alg_bytes += 12ull * in_n + 12ull * c->edges[cur] + 8ull * out_n;
+        c->push_epochs++;
+    } else {
+        c->alg_bytes += (unsigned long long)sweep_bytes + 8ull * out_n;
+        c->pull_epochs++;
+    }
+    for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
+    c->edges[cur] = 0;
+    c->epoch = k + 1;
+    if (out_n == 0 || (c->max_epochs > 0 && k + 1 >= c->max_epochs)) {
+        c->done = 1;
+    } else if (c->direction == 1) {
+        c->mode = kPull;
+    } else if (c->direction == 2) {
+        c->mode = kPush;
+    } else {
+        c->mode = (long long)out_e > c->m / c->alpha ? kPull : kPush;
+    }
+    c->blocks_done = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void sssp_step_kernel(
+    PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, FrontierBufs fb,
+    StepCtl *ctl, long long sweep_bytes) {
+    if (ctl->done) return;
+    const int mode = ctl->mode;
+    const int cur = ctl->epoch & 1, nxt = cur ^ 1;
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words;
+         w += gridDim.x * kBlock)
+        fb.bm[cur][w] = 0;
+    unsigned long long eacc = 0;
+    uint32_t upd = 0;
+    if (mode == kPull)
+        sssp_pull_sweep<true>(p, g, dist, fb, ctl, nxt, upd, eacc);
+    else
+        sssp_push_sweep(g, dist, fb, ctl, cur, nxt, eacc);
+    unsigned long long es = block_sum_u64(eacc);
+    if (threadIdx.x == 0 && es) atomicAdd(&ctl->edges[nxt], es);
+    if (last_block_done(&ctl->blocks_done) && threadIdx.x == 0)
+        sssp_controller(ctl, mode, sweep_bytes);
+}
+
+// Initialises distances, bitmaps and the control block for one run.
+// init_dist == nullptr: single source `src`; otherwise copy init_dist and
+// start in pull mode with an empty frontier (sssp_pull_gpu semantics).
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void sssp_init_kernel(
+    GraphView<OffT> g, int32_t *__restrict__ dist,
+    const int32_t *__restrict__ init_dist, FrontierBufs fb, StepCtl *ctl,
+    int32_t src, int direction, int alpha, int max_epochs, long long m) {
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride)
+        dist[i] = init_dist ? init_dist[i] : (i == src ? 0 : HC_INF_DIST);
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words; w += stride) {
+        fb.bm[0][w] = 0;
+        fb.bm[1][w] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int p = 0; p < 2; p++) {
+            for (int j = 0; j < 5; j++) ctl->cnt[p][j] = 0;
+            ctl->edges[p] = 0;
+        }
+        ctl->blocks_done = 0;
+        ctl->done = 0;
+        ctl->epoch = 0;
+        ctl->max_epochs = max_epochs;
+        ctl->direction = direction;
+        ctl->alpha = alpha;
+        ctl->m = m;
+        ctl->n = g.n;
+        ctl->push_epochs = ctl->pull_epochs = 0;
+        ctl->reached = 0;
+        ctl->alg_bytes = 0;
+        if (init_dist) {
+            ctl->mode = kPull;
+        } else {
+            OffT b = g.rbeg(src), e = g.rend(src);
+            int64_t d = int64_t(e - b);
+            if (d > kHubDeg) {
+                uint32_t nch = 0;
+                for (int64_t x = b; x < int64_t(e); x += kChunkEdges, nch++) {
+                    int64_t ce = x + kChunkEdges < int64_t(e) ? x + kChunkEdges : int64_t(e);
+                    fb.ch[0][nch] = HubChunk{src, 0, x, ce};
+                }
+                ctl->cnt[0][3] = nch;
+                ctl->cnt[0][4] = 1;
+            } else {
+                int cls = d > kBigDeg ? 0 : (d > kMidDeg ? 1 : 2);
+                fb.q[0][cls][0] = src;
+                ctl->cnt[0][cls] = 1;
+            }
+            ctl->edges[0] = (unsigned long long)d;
+            ctl->mode = direction == 1 ? kPull
+                        : direction == 2 ? kPush
+                        : (d > m / alpha ? kPull : kPush);
+        }
+    }
+}
+
+// Epoch-API kernels with the reference's launch shapes (thread, wavefront
+// and workgroup per vertex, grid-stride over [start, end)).
+template <typename OffT>
+__global__ void sssp_pull_thread_kernel(GraphView<OffT> g,
+                                        int32_t *__restrict__ dist,
+                                        int32_t start, int32_t end,
+                                        int32_t *updated) {
+    uint32_t upd = 0;
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t u = start + blockIdx.x * blockDim.x + threadIdx.x; u < end;
+         u += stride) {
+        int32_t m = row_min<1>(g, dist, g.rbeg(u), g.rend(u), 0);
+        if (m < dist[u]) {
+            dist[u] = m;
+            upd++;
+        }
+    }
+    for (int s = 32; s > 0; s >>= 1) upd += __shfl_xor(upd, s, 64);
+    if (lane_id() == 0 && upd) atomicAdd(updated, (int32_t)upd);
+}
+
+template <typename OffT>
+__global__ void sssp_pull_wave_kernel(GraphView<OffT> g,
+                                      int32_t *__restrict__ dist,
+                                      int32_t start, int32_t end,
+                                      int32_t *updated) {
+    uint32_t upd = 0;
+    const int32_t waves = gridDim.x * (blockDim.x / kWave);
+    const int32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    for (int32_t u = start + wid; u < end; u += waves) {
+        int32_t m = group_min<64>(row_min<64>(g, dist, g.rbeg(u), g.rend(u), lane_id()));
+        if (lane_id() == 0 && m < dist[u]) {
+            dist[u] = m;
+            upd++;
+        }
+    }
+    if (lane_id() == 0 && upd) atomicAdd(updated, (int32_t)upd);
+}
+
+template <typename OffT>
+__global__ void sssp_pull_block_kernel(GraphView<OffT> g,
+                                       int32_t *__restrict__ dist,
+                                       int32_t start, int32_t end,
+                                       int32_t *updated) {
+    uint32_t upd = 0;
+    for (int32_t u = start + blockIdx.x; u < end; u += gridDim.x) {
+        const OffT b = g.rbeg(u), e = g.rend(u);
+        int32_t m = HC_INF_DIST;
+        for (OffT i = b + threadIdx.x; i < e; i += blockDim.x)
+            m = min(m, relax(dist[g.adj[i]], g.wt[i]));
+        m = block_min(m);
+        if (threadIdx.x == 0 && m < dist[u]) {
+            dist[u] = m;
+            upd++;
+        }
+    }
+    if (threadIdx.x == 0 && upd) atomicAdd(updated, (int32_t)upd);
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
+    PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist,
+    int32_t *updated) {
+    uint32_t upd = 0;
+    unsigned long long eacc = 0;
+    FrontierBufs fb{};
+    sssp_pull_sweep<false>(p, g, dist, fb, nullptr, 0, upd, eacc);
+    unsigned long long s = block_sum_u64(upd);
+    if (threadIdx.x == 0 && s) atomicAdd(updated, (int32_t)s);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct SsspWs {
+    FrontierWs f;
+};
+
+void free_sssp_ws(hc_graph_s *g) {
+    if (!g->ws_sssp) return;
+    auto *ws = static_cast<SsspWs *>(g->ws_sssp);
+    frontier_ws_free(&ws->f);
+    delete ws;
+    g->ws_sssp = nullptr;
+}
+
+static int get_sssp_ws(hc_graph_s *g, SsspWs **out) {
+    if (!g->ws_sssp) {
+        auto *ws = new SsspWs();
+        int rc = frontier_ws_alloc(g, &ws->f);
+        if (rc != HC_OK) {
+            delete ws;
+            return rc;
+        }
+        g->ws_sssp = ws;
+    }
+    *out = static_cast<SsspWs *>(g->ws_sssp);
+    return HC_OK;
+}
+
+// Bytes a bucketed pull sweep reads and writes per the algorithm: row
+// offsets, neighbour id + weight + gathered distance per edge, own distance.
+long long sssp_sweep_bytes(const hc_graph_s *g, const PlanHost *ph) {
+    const PlanDev &p = ph->dev;
+    long long rows = 0;
+    for (int c = 1; c < kNumClasses; c++) rows += p.cnt[c];
+    rows += p.num_hubs;
+    const long long ob = g->off64 ? 8 : 4;
+    return ob * (rows + 1) + 12ll * g->m + 4ll * rows;
+}
+
+template <typename OffT>
+static int launch_pull_epoch(hc_graph_s *g, int mapping, int32_t start,
+                             int32_t end, int32_t *dist, int32_t *updated,
+                             int blocks, int threads, hipStream_t s) {
+    GraphView<OffT> v = view<OffT>(g);
+    switch (mapping) {
+        case HC_MAP_THREAD:
+            if (!threads) threads = 256;
+            if (!blocks) blocks = std::max(1, std::min(grid_blocks(8), (end - start + threads - 1) / threads));
+            sssp_pull_thread_kernel<OffT><<<blocks, threads, 0, s>>>(v, dist, start, end, updated);
+            break;
+        case HC_MAP_WAVE:
+            if (!threads) threads = 256;
+            if (!blocks) blocks = grid_blocks(8);
+            sssp_pull_wave_kernel<OffT><<<blocks, threads, 0, s>>>(v, dist, start, end, updated);
+            break;
+        case HC_MAP_BLOCK:
+            if (!threads) threads = 256;
+            if (!blocks) blocks = grid_blocks(8);
+            sssp_pull_block_kernel<OffT><<<blocks, threads, 0, s>>>(v, dist, start, end, updated);
+            break;
+        default: {
+            const PlanHost *ph = nullptr;
+            HC_TRY(get_plan(g, start, end, PlanConfig{}, &ph));
+            if (ph->total_units == 0) return HC_OK;
+            int nb = std::max(1, std::min(ph->total_units, grid_blocks(8)));
+            sssp_pull_fused_kernel<OffT><<<nb, kBlock, 0, s>>>(ph->dev, v, dist, updated);
+        }
+    }
+    HC_HIP(hipGetLastError());
+    return HC_OK;
+}
+
+static int check_launch(int mapping, int blocks, int threads) {
+    HC_CHECK_ARG(mapping >= HC_MAP_THREAD && mapping <= HC_MAP_BUCKETED,
+                 "unknown mapping %d", mapping);
+    HC_CHECK_ARG(blocks >= 0 && threads >= 0, "negative launch shape");
+    HC_CHECK_ARG(threads <= 1024, "threads must be <= 1024");
+    if (mapping == HC_MAP_WAVE || mapping == HC_MAP_BLOCK)
+        HC_CHECK_ARG(threads % kWave == 0,
+                     "thread count must be a multiple of 64 for this mapping");
+    return HC_OK;
+}
+
+template <typename OffT>
+static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
+                         int32_t *dist, int direction, int alpha,
+                         int max_epochs, hc_run_stats_t *stats, hipStream_t s) {
+    SsspWs *ws = nullptr;
+    HC_TRY(get_sssp_ws(g, &ws));
+    const PlanHost *ph = nullptr;
+    HC_TRY(get_plan(g, 0, g->n, PlanConfig{}, &ph));
+    GraphView<OffT> v = view<OffT>(g);
+    const long long sweep = sssp_sweep_bytes(g, ph);
+    const int grid = grid_blocks(8);
+    const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
+    sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
+        v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
+        max_epochs, (long long)g->m);
+    HC_HIP(hipGetLastError());
+    hipEvent_t ev0, ev1, evb[2];
+    HC_HIP(hipEventCreate(&ev0));
+    HC_HIP(hipEventCreate(&ev1));
+    HC_HIP(hipEventCreateWithFlags(&evb[0], hipEventDisableTiming));
+    HC_HIP(hipEventCreateWithFlags(&evb[1], hipEventDisableTiming));
+    StepCtl *hslot = ws->f.h_ctl;
+    hslot->done = 0;
+    HC_HIP(hipEventRecord(ev0, s));
+    // Enqueue batches of steps; the host checks batch b-1's control block
+    // while batch b is already queued, so the device never idles on a poll.
+    // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
+    const int batch = 8;
+    const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2 * batch + 2;
+    long long launches = 0;
+    int rc = HC_OK;
+    bool done = false;
+    int b = 0;
+    while (!done && launches < cap) {
+        for (int i = 0; i < batch; i++) {
+            sssp_step_kernel<OffT><<<grid, kBlock, 0, s>>>(ph->dev, v, dist, ws->f.fb,
+                                                           ws->f.ctl, sweep);
+            launches++;
+        }
+        if (hipGetLastError() != hipSuccess) {
+            rc = fail(HC_ERR_HIP, "step launch failed");
+            break;
+        }
+        hipMemcpyAsync(&hslot->done, &ws->f.ctl->done, sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s);
+        hipEventRecord(evb[b & 1], s);
+        if (b > 0) {
+            hipEventSynchronize(evb[(b - 1) & 1]);
+            done = __atomic_load_n(&hslot->done, __ATOMIC_ACQUIRE) != 0;
+        }
+        b++;
+    }
+    HC_HIP(hipEventRecord(ev1, s));
+    HC_HIP(hipEventSynchronize(ev1));
+    float ms = 0;
+    HC_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    StepCtl final_ctl;
+    HC_HIP(hipMemcpy(&final_ctl, ws->f.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    hipEventDestroy(evb[0]);
+    hipEventDestroy(evb[1]);
+    if (rc != HC_OK) return rc;
+    if (!final_ctl.done && max_epochs <= 0)
+        return fail(HC_ERR_HIP, "SSSP loop did not converge in %lld launches", launches);
+    if (stats) {
+        stats->ms = ms;
+        stats->epochs = final_ctl.epoch;
+        stats->push_epochs = final_ctl.push_epochs;
+        stats->pull_epochs = final_ctl.pull_epochs;
+        stats->launches = (int32_t)launches;
+        stats->reached = -1;
+        stats->residual = 0;
+        stats->alg_bytes = (double)final_ctl.alg_bytes;
+    }
+    return HC_OK;
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int hc_sssp_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
+                       int32_t end_id, int32_t *dist, int32_t *updated,
+                       int blocks, int threads, hc_stream_t stream) {
+    HC_CHECK_ARG(g && dist && updated, "NULL argument");
+    HC_CHECK_ARG(g->weighted, "SSSP needs a weighted graph");
+    HC_CHECK_ARG(g->row0 <= start_id && start_id <= end_id &&
+                     end_id <= g->row0 + g->rows,
+                 "range [%d,%d) outside the rows held [%d,%d)", start_id,
+                 end_id, g->row0, g->row0 + g->rows);
+    HC_TRY(check_launch(mapping, blocks, threads));
+    HC_TRY(ensure_init());
+    if (start_id == end_id) return HC_OK;
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    return g->off64 ? launch_pull_epoch<uint64_t>(g, mapping, start_id, end_id, dist, updated, blocks, threads, s)
+                    : launch_pull_epoch<uint32_t>(g, mapping, start_id, end_id, dist, updated, blocks, threads, s);
+}
+
+int hc_sssp(hc_graph_t g, int32_t source, int32_t *dist,
+            const hc_sssp_opts_t *opts, hc_run_stats_t *stats,
+            hc_stream_t stream) {
+    HC_CHECK_ARG(g && dist, "NULL argument");
+    HC_CHECK_ARG(g->weighted, "SSSP needs a weighted graph");
+    HC_CHECK_ARG(g->row0 == 0 && g->rows == g->n,
+                 "full runs need a whole-graph handle");
+    HC_CHECK_ARG(source >= 0 && source < g->n, "source %d outside [0,%d)",
+                 source, g->n);
+    int direction = opts ? opts->direction : 0;
+    int alpha = opts && opts->alpha > 0 ? opts->alpha : 8;
+    int max_epochs = opts ? opts->max_epochs : 0;
+    HC_CHECK_ARG(direction >= 0 && direction <= 2, "direction must be 0, 1 or 2");
+    HC_TRY(ensure_init());
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    return g->off64 ? run_sssp_loop<uint64_t>(g, source, nullptr, dist, direction, alpha, max_epochs, stats, s)
+                    : run_sssp_loop<uint32_t>(g, source, nullptr, dist, direction, alpha, max_epochs, stats, s);
+}
+
+int hc_sssp_pull(hc_graph_t g, int mapping, const int32_t *init_dist,
+                 int32_t *dist, int blocks, int threads,
+                 hc_run_stats_t *stats, hc_stream_t stream) {
+    HC_CHECK_ARG(g && init_dist && dist, "NULL argument");
+    HC_CHECK_ARG(g->weighted, "SSSP needs a weighted graph");
+    HC_CHECK_ARG(g->row0 == 0 && g->rows == g->n,
+                 "full runs need a whole-graph handle");
+    HC_TRY(check_launch(mapping, blocks, threads));
+    HC_TRY(ensure_init());
+    hipStream_t s = pick_stream(stream);
+    if (mapping == HC_MAP_BUCKETED) {
+        std::lock_guard<std::mutex> lk(g->mu);
+        return g->off64 ? run_sssp_loop<uint64_t>(g, 0, init_dist, dist, 1, 8, 0, stats, s)
+                        : run_sssp_loop<uint32_t>(g, 0, init_dist, dist, 1, 8, 0, stats, s);
+    }
+    // Host-driven loop of sssp_pull_gpu (src/kernels/gpu/sssp.cuh:76-85):
+    // zero the counter, run one epoch, read the counter back.
+    int32_t *d_upd = nullptr;
+    HC_HIP(hipMalloc(&d_upd, sizeof(int32_t)));
+    int rc = HC_OK;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    if (hipMemcpyAsync(dist, init_dist, sizeof(int32_t) * g->n,
+                       hipMemcpyDeviceToDevice, s) != hipSuccess)
+        rc = fail(HC_ERR_HIP, "copy of init_dist failed");
+    int epochs = 0;
+    hipEventRecord(e0, s);
+    int32_t h_upd = 1;
+    while (rc == HC_OK && h_upd != 0) {
+        hipMemsetAsync(d_upd, 0, sizeof(int32_t), s);
+        {
+            std::lock_guard<std::mutex> lk(g->mu);
+            rc = g->off64 ? launch_pull_epoch<uint64_t>(g, mapping, 0, g->n, dist, d_upd, blocks, threads, s)
+                          : launch_pull_epoch<uint32_t>(g, mapping, 0, g->n, dist, d_upd, blocks, threads, s);
+        }
+        if (rc != HC_OK) break;
+        if (hipMemcpyAsync(&h_upd, d_upd, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = fail(HC_ERR_HIP, "epoch failed");
+        epochs++;
+    }
+    hipEventRecord(e1, s);
+    hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(d_upd);
+    if (rc == HC_OK && stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->ms = ms;
+        stats->epochs = epochs;
+        stats->pull_epochs = epochs;
+        stats->launches = epochs;
+    }
+    return rc;
+}
+
+}  // extern "C"

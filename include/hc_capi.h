@@ -1,0 +1,295 @@
+/*
+ * hc_capi.h -- C ABI of the MI355X (gfx950) graph epoch library.
+ *
+ * Every entry point returns an int status (HC_OK == 0, negative on error) and
+ * never calls exit(); the message of the last error on the calling thread is
+ * returned by hc_last_error().  No C++ or HIP types appear in the signatures:
+ * device memory is passed as plain pointers, streams as opaque hc_stream_t
+ * (a hipStream_t; NULL selects the library's own stream).
+ *
+ * Reference boundary this replaces (chrisliu/hetero-compute):
+ *   - epoch-kernel signatures            src/kernels/kernel_types.cuh:32-63
+ *   - single-device drivers              src/kernels/gpu/sssp.cuh:42-100,
+ *                                        src/kernels/gpu/bfs.cuh:21-95,
+ *                                        src/kernels/gpu/pr.cuh:42-118,
+ *                                        src/kernels/cpu/bfs.cuh:48-114
+ *   - CSR container / loader / H2D copy  src/graph.cuh:26-100, 207-286
+ *   - segmentation and source picking    src/graph.cuh:106-201
+ *   - error convention CUDA_ERRCHK/CONDCHK (print + exit) src/cuda.cuh:14-21,
+ *     src/util.h:16-22 -> status codes here.
+ */
+#ifndef HC_CAPI_H
+#define HC_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Types and constants                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Integer distance sentinel.  The reference uses float infinity
+ * (src/graph.cuh:32); with int32 weights an explicit sentinel is needed
+ * (numeric_limits<int>::infinity() is 0).  Candidate distances saturate at
+ * HC_INF_DIST, so INF + w stays INF. */
+#define HC_INF_DIST 0x3FFFFFFF
+/* src/graph.cuh:35 */
+#define HC_INVALID_NODE (-1)
+
+enum hc_status {
+    HC_OK = 0,
+    HC_ERR_INVALID = -1,     /* bad argument / malformed input          */
+    HC_ERR_HIP = -2,         /* HIP runtime error                       */
+    HC_ERR_OOM = -3,         /* device or host allocation failed        */
+    HC_ERR_IO = -4,          /* file open/read/write failure            */
+    HC_ERR_UNSUPPORTED = -5, /* valid request this build cannot serve   */
+    HC_ERR_BUSY = -6,        /* admission refused (serving)             */
+    HC_ERR_NOT_FOUND = -7,   /* unknown name / id                       */
+    HC_ERR_NO_DEVICE = -8    /* no usable GPU                           */
+};
+
+typedef void *hc_stream_t;              /* hipStream_t                      */
+typedef struct hc_graph_s *hc_graph_t;  /* device-resident CSR (HBM)        */
+
+/* Vertex-to-lane mappings of an epoch kernel.  THREAD/WAVE/BLOCK are the
+ * reference's one_to_one / warp_red / block_red (src/kernels/gpu/sssp.cuh:
+ * 120-281) re-thought for 64-lane wavefronts; BUCKETED is the degree-bucketed
+ * plan (hub chunks / workgroup / wavefront / lane-group / thread per vertex)
+ * computed from the degree-sorted vertex order. */
+enum hc_mapping {
+    HC_MAP_THREAD = 0,
+    HC_MAP_WAVE = 1,
+    HC_MAP_BLOCK = 2,
+    HC_MAP_BUCKETED = 3
+};
+
+/* Host-resident CSR.  index has num_nodes+1 int64 entries (reference
+ * offset_t, src/graph.cuh:28); neighbors/weights have num_edges int32 entries
+ * (structure-of-arrays instead of the reference's 8-byte wnode_t AoS,
+ * src/graph.cuh:38-41).  weights may be NULL for an unweighted graph. */
+typedef struct hc_host_graph {
+    int32_t num_nodes;
+    int64_t num_edges;
+    int64_t *index;
+    int32_t *neighbors;
+    int32_t *weights;
+} hc_host_graph_t;
+
+typedef struct hc_graph_info {
+    int32_t num_nodes;     /* vertices of the whole graph                    */
+    int64_t num_edges;     /* directed edges held by this handle             */
+    int32_t start_id;      /* first row held (range handles)                 */
+    int32_t end_id;        /* one past the last row held                     */
+    int32_t degree_sorted; /* 1 if degrees are non-increasing over rows      */
+    int32_t weighted;
+    int32_t offset_bytes;  /* 4 or 8: width of the device row offsets        */
+    int32_t num_nonzero;   /* rows [start_id, start_id+num_nonzero) have deg>0
+                              (sorted graphs); == rows otherwise             */
+    int64_t max_degree;
+    size_t device_bytes;   /* HBM held by the CSR arrays                     */
+} hc_graph_info_t;
+
+typedef struct hc_device_info {
+    char name[128];
+    char arch[64];
+    int32_t compute_units;
+    int32_t wavefront_size;
+    size_t hbm_bytes;
+    int32_t lds_bytes_per_block;
+} hc_device_info_t;
+
+/* Statistics of a full traversal / iteration run. */
+typedef struct hc_run_stats {
+    double ms;             /* device time of the epoch loop (HIP events)      */
+    int32_t epochs;        /* epochs executed                                 */
+    int32_t push_epochs;   /* top-down / push epochs                          */
+    int32_t pull_epochs;   /* bottom-up / pull epochs                         */
+    int32_t launches;      /* kernel launches issued for the loop             */
+    int64_t reached;       /* vertices reached (BFS), -1 if not tracked       */
+    double residual;       /* PageRank: final L1 residual (FP64)              */
+    double alg_bytes;      /* algorithmic HBM bytes of the epochs run          */
+} hc_run_stats_t;
+
+/* ------------------------------------------------------------------------ */
+/* Runtime                                                                  */
+/* ------------------------------------------------------------------------ */
+
+const char *hc_last_error(void);
+int hc_version(void);
+/* Select the device and cache its properties (CU count for grid sizing). */
+int hc_init(int device);
+int hc_get_device_info(hc_device_info_t *out);
+int hc_device_synchronize(void);
+int hc_dmalloc(void **ptr, size_t bytes);
+int hc_dfree(void *ptr);
+int hc_hmalloc_pinned(void **ptr, size_t bytes);
+int hc_hfree_pinned(void *ptr);
+/* kind: 1 host->device, 2 device->host, 3 device->device */
+int hc_memcpy(void *dst, const void *src, size_t bytes, int kind,
+              hc_stream_t stream);
+int hc_memset_d32(void *dst, int32_t value, size_t count, hc_stream_t stream);
+int hc_stream_create(hc_stream_t *stream);
+int hc_stream_destroy(hc_stream_t stream);
+int hc_stream_synchronize(hc_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Host graphs: generator/builder, file I/O, sources, segments              */
+/* ------------------------------------------------------------------------ */
+
+/* gapbs-style synthetic graph (src/gapbs/generator.h:34-172,
+ * src/gapbs/builder.h:39-407, src/gapbs/gapbs.cuh:121-125): R-MAT
+ * (A=.57,B=.19,C=.19) with an ID permutation, or uniform; 2^18-edge blocks
+ * reseeded with seed+block; symmetrized; self-loops and duplicate edges
+ * removed (the lightest duplicate is kept); relabelled by descending degree
+ * (ties: larger old id first); neighbour lists sorted.  Weighted graphs get
+ * int32 weights uniform in [1,255], equal in both directions. */
+int hc_host_graph_generate(int uniform, int scale, int degree, int weighted,
+                           int64_t seed, hc_host_graph_t *out);
+/* Build a CSR from an edge list with the same builder steps. */
+int hc_host_graph_from_edges(const int32_t *src, const int32_t *dst,
+                             const int32_t *weights, int64_t num_edges,
+                             int32_t num_nodes, int symmetrize, int relabel,
+                             hc_host_graph_t *out);
+/* Reference binary format (src/graph.cuh:207-239, src/gapbs/gapbs.cuh:
+ * 58-112): int32 N, int64 m, int64 index[N+1], then m 8-byte {int32 v;
+ * float w} (.wsg, weighted=1) or m int32 ids (.sg, weighted=0).  Weights read
+ * from a .wsg must be integral and in [0, 2^30). */
+int hc_host_graph_read(const char *path, int weighted, hc_host_graph_t *out);
+int hc_host_graph_write(const char *path, const hc_host_graph_t *g,
+                        int weighted);
+void hc_host_graph_free(hc_host_graph_t *g);
+/* SourcePicker (src/graph.cuh:106-141): mt19937(seed), uniform ids, rejects
+ * degree-0 vertices and lone self-loops.  seed 37525 is the reference's. */
+int hc_pick_sources(const hc_host_graph_t *g, int64_t seed, int32_t count,
+                    int32_t *out);
+/* compute_equal_edge_ranges (src/graph.cuh:156-201): out has
+ * num_segments+1 entries. */
+int hc_equal_edge_ranges(const hc_host_graph_t *g, int32_t num_segments,
+                         int32_t chunk_size, int32_t *out);
+
+/* ------------------------------------------------------------------------ */
+/* Device graphs (CSR in HBM)                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Whole graph.  Offsets are stored as uint32 when the edge count allows,
+ * otherwise uint64; ids and weights as int32 structure-of-arrays. */
+int hc_graph_upload(const hc_host_graph_t *g, hc_graph_t *out);
+/* Rows [start_id, end_id) only, with down-shifted offsets
+ * (copy_subgraph_to_device, src/graph.cuh:250-286).  Epoch calls on a range
+ * handle take global vertex ids; per-vertex state stays global-length. */
+int hc_graph_upload_range(const hc_host_graph_t *g, int32_t start_id,
+                          int32_t end_id, hc_graph_t *out);
+int hc_graph_free(hc_graph_t g);
+int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
+
+/* ------------------------------------------------------------------------ */
+/* Epoch kernels (one bulk-synchronous sweep of [start_id, end_id))         */
+/* ------------------------------------------------------------------------ */
+/* All state pointers are device pointers of global length.  Counters are
+ * accumulated (add), so the caller zeroes them, as the reference drivers do
+ * (src/kernels/gpu/sssp.cuh:78).  blocks/threads apply to THREAD/WAVE/BLOCK
+ * (threads % 64 == 0 for WAVE and BLOCK; 0 selects a default); BUCKETED
+ * ignores them. */
+
+/* epoch_sssp_pull_gpu_{one_to_one,warp_red,block_red}
+ * (src/kernels/gpu/sssp.cuh:120-281; signature kernel_types.cuh:36-37):
+ * dist[u] = min(dist[u], min_{(v,w) in N(u)} dist[v] + w) in place;
+ * *updated += number of vertices whose distance changed. */
+int hc_sssp_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
+                       int32_t end_id, int32_t *dist, int32_t *updated,
+                       int blocks, int threads, hc_stream_t stream);
+
+/* epoch_bfs_pull_gpu_* (src/kernels/gpu/bfs.cuh:116-288, signature
+ * kernel_types.cuh:60-63) / epoch_bfs_pull_cpu (src/kernels/cpu/bfs.cuh:
+ * 305-335): every unvisited u (parents[u] == -1) whose neighbour list holds a
+ * vertex of the frontier bitmap takes the first such neighbour (CSR order) as
+ * parent, is set in next_frontier, and counts into *num_nodes.  Bitmaps are
+ * 32-bit words, bit (v & 31) of word v >> 5 (src/bitmap.cuh:19-34). */
+int hc_bfs_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
+                      int32_t end_id, int32_t *parents,
+                      const uint32_t *frontier, uint32_t *next_frontier,
+                      int32_t *num_nodes, int blocks, int threads,
+                      hc_stream_t stream);
+
+/* epoch_bfs_push_cpu_by_node (src/kernels/cpu/bfs.cuh:212-244), on the GPU:
+ * for each u of frontier[0..frontier_size), each unvisited neighbour v gets
+ * parents[v] = u (compare-and-swap from -1), is appended to next_frontier
+ * (*next_size += 1) and adds deg(v) to *num_edges (the scout count). */
+int hc_bfs_push_epoch(hc_graph_t g, const int32_t *frontier,
+                      int32_t frontier_size, int32_t *parents,
+                      int32_t *next_frontier, int32_t *next_size,
+                      int64_t *num_edges, hc_stream_t stream);
+
+/* PageRank pull epoch, Jacobi form of epoch_pr_pull_{cpu,gpu}_one_to_one
+ * (src/kernels/cpu/pr.cuh:157-187, src/kernels/gpu/pr.cuh:140-174):
+ * score_out[u] = (1-d)/N + d * sum_{v in N(u)} contrib[v], the sum in FP32
+ * with Kahan compensation, and contrib_out[u] = (float)(score_out[u] *
+ * inv_degree[u]) with inv_degree in FP64.  residual_partials receives one
+ * FP64 partial of sum |score_out - score_in| per workgroup (may be NULL). */
+int hc_pr_pull_epoch(hc_graph_t g, int32_t start_id, int32_t end_id,
+                     float damping, const float *score_in,
+                     const float *contrib, const double *inv_degree,
+                     float *score_out, float *contrib_out,
+                     double *residual_out, hc_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Full runs with the graph resident (timer scope = epoch loop only,        */
+/* src/kernels/gpu/sssp.cuh:76-86)                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hc_sssp_opts {
+    int32_t direction;  /* 0 auto push/pull, 1 pull only, 2 push only        */
+    int32_t alpha;      /* go pull when frontier edges > m/alpha (0 = default)*/
+    int32_t max_epochs; /* 0 = unbounded                                     */
+} hc_sssp_opts_t;
+
+/* Single-source SSSP, int32 weights >= 0: dist (device, num_nodes) receives
+ * the distances, HC_INF_DIST where unreachable.  Epoch loop of Bellman-Ford
+ * pull sweeps (degree-bucketed) and push epochs over the active frontier,
+ * chosen per epoch; the fixpoint equals sssp_pull_cpu's
+ * (src/kernels/cpu/sssp.cuh:33-65). */
+int hc_sssp(hc_graph_t g, int32_t source, int32_t *dist,
+            const hc_sssp_opts_t *opts, hc_run_stats_t *stats,
+            hc_stream_t stream);
+
+/* sssp_pull_gpu (src/kernels/gpu/sssp.cuh:42-100): pull epochs with the given
+ * mapping from init_dist (device) until no vertex changes. */
+int hc_sssp_pull(hc_graph_t g, int mapping, const int32_t *init_dist,
+                 int32_t *dist, int blocks, int threads,
+                 hc_run_stats_t *stats, hc_stream_t stream);
+
+typedef struct hc_bfs_opts {
+    int32_t alpha;     /* push -> pull when scout > edges_to_check/alpha (15) */
+    int32_t beta;      /* pull -> push when n <= prev and n <= N/beta (18)    */
+    int32_t direction; /* 0 direction-optimizing, 1 pull only, 2 push only   */
+} hc_bfs_opts_t;
+
+/* bfs_do_cpu<alpha,beta> (src/kernels/cpu/bfs.cuh:48-114) on the GPU with the
+ * same switching rule.  depth (device, may be NULL) gets hop counts (-1 if
+ * unreached); parents (device, may be NULL) a valid BFS tree with
+ * parents[source] == source. */
+int hc_bfs(hc_graph_t g, int32_t source, int32_t *depth, int32_t *parents,
+           const hc_bfs_opts_t *opts, hc_run_stats_t *stats,
+           hc_stream_t stream);
+
+typedef struct hc_pr_opts {
+    float damping;     /* 0.85 (src/kernels/cpu/pr.cuh:139)                  */
+    double tolerance;  /* stop when the FP64 L1 residual < tolerance (1e-4);
+                          0 = default, < 0 = run exactly max_iters           */
+    int32_t max_iters; /* 20                                                 */
+} hc_pr_opts_t;
+
+/* PageRank pull, Jacobi iteration from 1/N (src/kernels/cpu/pr.cuh:33-65
+ * start value); scores (device, num_nodes) in FP32. */
+int hc_pagerank(hc_graph_t g, float *scores, const hc_pr_opts_t *opts,
+                hc_run_stats_t *stats, hc_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HC_CAPI_H */
