@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Benchmark: single-source BFS and SSSP GTEPS on one MI355X.
+
+Workload (the single-GPU configuration the metric is quoted on): a
+gapbs-style Kronecker graph at scale 23, edge factor 16, symmetrized, int32
+weights uniform in [1,255], plus the uniform random graph of the same size;
+BFS is direction-optimizing (alpha=15, beta=18), SSSP is the Bellman-Ford
+push/pull epoch loop.  Sources are the reference SourcePicker's first draws
+(mt19937 seed 37525).  One step = one source: a BFS and an SSSP run on each
+graph with the graph resident in HBM.
+
+GTEPS follows the reference: m_directed / 2 / t, t = mean epoch-loop time
+over the sources (src/benchmarks/gpu_benchmark.cuh:577-590).  `value` is the
+SSSP GTEPS on the Kronecker graph; every other figure is in `gteps`.
+The CPU baseline is the oracle's restatement of the reference OpenMP
+sssp_pull_cpu / bfs_do_cpu, timed on this host's cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 64] [--warmup 2] [--scale 23]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("GTEPS (BFS/SSSP/PR, Kronecker scale 23) + batched-query GTEPS and "
+          "p99 latency")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_graph(hc, kind, scale, degree, steps, warmup, seed):
+    t0 = time.time()
+    g = hc.HostGraph.generate(kind, scale, degree, weighted=True, seed=seed)
+    gen_s = time.time() - t0
+    t0 = time.time()
+    dg = hc.DeviceGraph(g)
+    upload_s = time.time() - t0
+    n, m = g.num_nodes, g.num_edges
+    log(f"[{kind}{scale}] n={n} m={m} generate {gen_s:.1f}s upload {upload_s:.2f}s")
+    sources = g.pick_sources(max(steps, warmup))
+    dist = hc.DeviceArray(n, np.int32)
+    depth = hc.DeviceArray(n, np.int32)
+    for s in sources[:warmup]:
+        hc.bfs(dg, int(s), depth_out=depth, want_parents=False, fetch=False)
+        hc.sssp(dg, int(s), out=dist, fetch=False)
+    hc.synchronize()
+    bfs_ms, sssp_ms, sssp_bytes, sssp_launch, bfs_bytes, bfs_launch = [], [], [], [], [], []
+    sssp_ep, bfs_ep = [], []
+    wall0 = time.time()
+    for s in sources[:steps]:
+        b = hc.bfs(dg, int(s), depth_out=depth, want_parents=False, fetch=False)
+        r = hc.sssp(dg, int(s), out=dist, fetch=False)
+        bfs_ms.append(b.stats["ms"])
+        bfs_bytes.append(b.stats["alg_bytes"])
+        bfs_launch.append(b.stats["launches"])
+        bfs_ep.append(b.stats["epochs"])
+        sssp_ms.append(r.stats["ms"])
+        sssp_bytes.append(r.stats["alg_bytes"])
+        sssp_launch.append(r.stats["launches"])
+        sssp_ep.append(r.stats["epochs"])
+    hc.synchronize()
+    wall = time.time() - wall0
+    out = {
+        "g": g, "dg": dg, "n": n, "m": m, "sources": sources, "gen_s": gen_s,
+        "upload_s": upload_s, "wall_s": wall,
+        "bfs_ms": np.array(bfs_ms), "sssp_ms": np.array(sssp_ms),
+        "sssp_bytes": np.array(sssp_bytes), "sssp_launches": np.array(sssp_launch),
+        "bfs_bytes": np.array(bfs_bytes), "bfs_launches": np.array(bfs_launch),
+        "sssp_epochs": np.array(sssp_ep), "bfs_epochs": np.array(bfs_ep),
+    }
+    return out
+
+
+def check_exact(hc, oracle, res, count):
+    """Bit-exactness of the first `count` sources against the oracle."""
+    g, dg = res["g"], res["dg"]
+    ok_sssp = ok_bfs = True
+    for s in map(int, res["sources"][:count]):
+        d = hc.sssp(dg, s).values
+        ok_sssp &= bool(np.array_equal(d, oracle.sssp_dijkstra(g.index, g.neighbors,
+                                                                g.weights, s)))
+        b = hc.bfs(dg, s)
+        depth = oracle.bfs_depth(g.index, g.neighbors, s)
+        ok_bfs &= bool(np.array_equal(b.values, depth))
+        ok_bfs &= oracle.verify_parents(depth, b.parents, s) == 0
+    return ok_sssp, ok_bfs
+
+
+def cpu_baseline(oracle, res, budget_s, max_sources):
+    """Reference OpenMP SSSP pull and DO-BFS on this host, bounded sample."""
+    g = res["g"]
+    idx, nb, w = g.index, g.neighbors, g.weights
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    sssp_ms, bfs_ms, used = [], [], []
+    t0 = time.time()
+    for s in map(int, res["sources"][:max_sources]):
+        init = np.full(g.num_nodes, oracle.INF, np.int32)
+        init[s] = 0
+        _, ms, _ = oracle.sssp_pull_omp(idx, nb, w, init, threads)
+        sssp_ms.append(ms)
+        _, bms, _ = oracle.bfs_do(idx, nb, s, 15, 18, threads)
+        bfs_ms.append(bms)
+        used.append(s)
+        if time.time() - t0 > budget_s:
+            break
+    return {"sssp_ms": np.array(sssp_ms), "bfs_ms": np.array(bfs_ms),
+            "sources": used, "threads": threads, "wall_s": time.time() - t0}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=23)
+    ap.add_argument("--degree", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=27491095)
+    ap.add_argument("--graphs", default="kron,uniform")
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of CPU-baseline work (bounded sample)")
+    ap.add_argument("--cpu-max-sources", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", type=int, default=1,
+                    help="sources checked bit-exact against the oracle")
+    ap.add_argument("--pagerank", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the epoch kernels on a "
+              "single MI355X (multi-GPU sharding is a design note only).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    hc = ge.package()
+    hc.check(hc.lib().hc_init(0))
+    dev = hc.device_info()
+    log(f"device {dev['name']} {dev['arch']} CUs={dev['compute_units']}")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    kinds = [k for k in args.graphs.split(",") if k]
+    results = {}
+    for kind in kinds:
+        results[kind] = run_graph(hc, kind, args.scale, args.degree, args.steps,
+                                  args.warmup, args.seed)
+
+    gteps = {}
+    for kind, r in results.items():
+        gteps[f"bfs_{kind}{args.scale}"] = hc.gteps(r["m"], r["bfs_ms"].mean())
+        gteps[f"sssp_{kind}{args.scale}"] = hc.gteps(r["m"], r["sssp_ms"].mean())
+
+    main_kind = kinds[0]
+    R = results[main_kind]
+    pr = None
+    if args.pagerank:
+        p = hc.pagerank(R["dg"])
+        p2 = hc.pagerank(R["dg"])
+        pr = {"iters": p2.stats["epochs"], "ms": p2.stats["ms"],
+              "residual": p2.stats["residual"],
+              "gteps": hc.gteps(R["m"], p2.stats["ms"]),
+              "alg_gbs": p2.stats["alg_bytes"] / (p2.stats["ms"] / 1e3) / 1e9}
+        gteps[f"pr_{main_kind}{args.scale}"] = pr["gteps"]
+
+    exact = {}
+    for kind, r in results.items():
+        if args.check > 0:
+            s_ok, b_ok = check_exact(hc, oracle, r, args.check)
+            exact[kind] = {"sssp": s_ok, "bfs": b_ok, "sources_checked": args.check}
+
+    # Dominant kernel over the timed region: the device-driven epoch kernel of
+    # the algorithm with the larger share of device time.
+    tot_sssp = sum(r["sssp_ms"].sum() for r in results.values())
+    tot_bfs = sum(r["bfs_ms"].sum() for r in results.values())
+    if tot_sssp >= tot_bfs:
+        kname = "sssp_step_kernel"
+        byts = sum(r["sssp_bytes"].sum() for r in results.values())
+        launches = sum(r["sssp_launches"].sum() for r in results.values())
+        ms = tot_sssp
+    else:
+        kname = "bfs_step_kernel"
+        byts = sum(r["bfs_bytes"].sum() for r in results.values())
+        launches = sum(r["bfs_launches"].sum() for r in results.values())
+        ms = tot_bfs
+    achieved = byts / (ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "kernel": kname,
+                "avg_launch_us": round(ms * 1e3 / launches, 2),
+                "alg_bytes_per_launch": round(byts / launches, 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        cb = cpu_baseline(oracle, R, args.cpu_budget, args.cpu_max_sources)
+        cpu_sssp = hc.gteps(R["m"], cb["sssp_ms"].mean())
+        cpu_bfs = hc.gteps(R["m"], cb["bfs_ms"].mean())
+        cpu = {"value": round(cpu_sssp, 4), "unit": "GTEPS", "cores": cb["threads"],
+               "kind": "port",
+               "sample": (f"oracle restatement of sssp_pull_cpu (OpenMP, cyclic) on "
+                          f"{main_kind}{args.scale}, first {len(cb['sources'])} "
+                          f"SourcePicker sources ({cb['wall_s']:.1f}s)"),
+               "bfs_do_gteps": round(cpu_bfs, 3),
+               "sssp_ms_mean": round(float(cb["sssp_ms"].mean()), 2),
+               "bfs_ms_mean": round(float(cb["bfs_ms"].mean()), 2),
+               "gpu_over_cpu": {
+                   "sssp": round(gteps[f"sssp_{main_kind}{args.scale}"] / cpu_sssp, 1),
+                   "bfs": round(gteps[f"bfs_{main_kind}{args.scale}"] / cpu_bfs, 1)}}
+
+    value = gteps[f"sssp_{main_kind}{args.scale}"]
+    line = {
+        "metric": METRIC, "value": round(value, 4), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(float(R["sssp_ms"].mean()), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32",
+        "data": f"synthetic gapbs-style generator (seed {args.seed}), weights U[1,255]",
+        "config": {
+            "workload": (f"single-source SSSP (Bellman-Ford push/pull) and BFS "
+                         f"(direction-optimizing a=15 b=18) on Kronecker scale "
+                         f"{args.scale} ef{args.degree} and uniform scale {args.scale} "
+                         f"ef{args.degree}, {args.steps} SourcePicker sources"),
+            "graphs": {k: {"n": r["n"], "m": int(r["m"])} for k, r in results.items()},
+        },
+        "gteps": {k: round(v, 3) for k, v in gteps.items()},
+        "ms_mean": {**{f"bfs_{k}": round(float(r["bfs_ms"].mean()), 4)
+                       for k, r in results.items()},
+                    **{f"sssp_{k}": round(float(r["sssp_ms"].mean()), 4)
+                       for k, r in results.items()}},
+        "epochs_mean": {**{f"bfs_{k}": float(r["bfs_epochs"].mean()) for k, r in results.items()},
+                        **{f"sssp_{k}": float(r["sssp_epochs"].mean())
+                           for k, r in results.items()}},
+        "pagerank": pr,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "bit_exact": exact,
+        "device": dev["name"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

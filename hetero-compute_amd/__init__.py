@@ -71,6 +71,12 @@ class RunStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class EpochTrace(C.Structure):
+    _fields_ = [("count", C.c_int32), ("mode", C.c_int8 * 128),
+                ("frontier", C.c_uint32 * 128), ("frontier_edges", C.c_uint64 * 128),
+                ("scanned", C.c_uint64 * 128)]
+
+
 class SsspOpts(C.Structure):
     _fields_ = [("direction", C.c_int32), ("alpha", C.c_int32),
                 ("max_epochs", C.c_int32)]
@@ -130,7 +136,34 @@ _SIGS = {
                       _P], C.c_int),
     "hc_bfs": ([_P, C.c_int32, _P, _P, C.POINTER(BfsOpts), C.POINTER(RunStats), _P], C.c_int),
     "hc_pagerank": ([_P, _P, C.POINTER(PrOpts), C.POINTER(RunStats), _P], C.c_int),
+    "hc_get_epoch_trace": ([_P, C.c_int, C.POINTER(EpochTrace)], C.c_int),
+    "hc_set_tuning": ([C.c_char_p, C.c_int64], C.c_int),
+    "hc_get_tuning": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
+    "hc_reset_tuning": ([], C.c_int),
 }
+
+TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "pull.hub_threshold", "pull.hub_chunk",
+               "pull.block_threshold", "pull.wave_threshold", "pull.g16_threshold",
+               "pull.g4_threshold", "bu.wave_threshold", "bu.g16_threshold",
+               "bu.g4_threshold")
+
+
+def set_tuning(key: str, value: int):
+    check(lib().hc_set_tuning(key.encode(), int(value)))
+
+
+def get_tuning(key: str) -> int:
+    v = C.c_int64()
+    check(lib().hc_get_tuning(key.encode(), C.byref(v)))
+    return v.value
+
+
+def tuning() -> dict:
+    return {k: get_tuning(k) for k in TUNING_KEYS}
+
+
+def reset_tuning():
+    check(lib().hc_reset_tuning())
 
 
 def lib():
@@ -476,6 +509,15 @@ def pr_pull_epoch(g: DeviceGraph, start: int, end: int, damping: float,
                                  inv_degree.ptr, score_out.ptr, contrib_out.ptr,
                                  r.ptr, None))
     return float(r.get()[0])
+
+
+def epoch_trace(g: DeviceGraph, algorithm: str) -> list[dict]:
+    """Per-epoch record of the last sssp/bfs run on g."""
+    t = EpochTrace()
+    check(lib().hc_get_epoch_trace(g.h, 0 if algorithm == "sssp" else 1, C.byref(t)))
+    return [{"mode": "pull" if t.mode[i] else "push", "frontier": t.frontier[i],
+             "frontier_edges": t.frontier_edges[i], "scanned": t.scanned[i]}
+            for i in range(t.count)]
 
 
 def gteps(num_edges_directed: int, ms: float) -> float:

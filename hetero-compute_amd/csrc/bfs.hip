@@ -12,9 +12,10 @@
 //    frontier (wavefront-aggregated) and their degrees summed (scout count).
 //  * Direction-optimizing loop, switching rule of bfs_do_cpu<alpha,beta>
 //    (src/kernels/cpu/bfs.cuh:48-114), evaluated on the device by the last
-//    workgroup of every epoch launch.  Both epochs emit the next frontier as a
-//    bitmap and as degree-classed queues, so no conversion pass is needed
-//    when the direction changes.
+//    controller kernel between epochs.  Bottom-up epochs emit the next
+//    frontier as a bitmap only; top-down epochs emit bitmap and queues; the
+//    compaction kernel builds queues from the bitmap when the direction turns
+//    from bottom-up to top-down.
 #include <algorithm>
 #include <cstring>
 
@@ -35,7 +36,7 @@ __device__ __forceinline__ bool bu_scan(const GraphView<OffT> &g,
                                         int32_t &parent, uint32_t &scanned) {
     if (G == 1) {
         for (OffT i = b; i < e; i++) {
-            int32_t v = g.adj[i];
+            int32_t v = ldnt(&g.adj[i]);
             scanned++;
             if (bit_test(frontier, v)) {
                 parent = v;
@@ -49,7 +50,7 @@ __device__ __forceinline__ bool bu_scan(const GraphView<OffT> &g,
             bool hit = false;
             int32_t v = 0;
             if (i < e) {
-                v = g.adj[i];
+                v = ldnt(&g.adj[i]);
                 scanned++;
                 hit = bit_test(frontier, v);
             }
@@ -92,15 +93,15 @@ __device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
 }
 
 // One bottom-up sweep over the plan (classes 2..5; the BFS plan keeps classes
-// 0 and 1 empty).  visited is depth (loop) or parents (epoch API); LOOP also
-// writes depth/parents and the queue form of the next frontier.
+// 0 and 1 empty).  LOOP: visited is depth; found vertices get depth/parent
+// and a bit in the next-frontier bitmap, counted with their degree.
+// Epoch API: visited is parents.
 template <bool LOOP, typename OffT>
 __device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
                          int32_t *__restrict__ visited, int32_t *__restrict__ parents,
                          int32_t level, const uint32_t *__restrict__ frontier,
-                         uint32_t *__restrict__ next, const FrontierBufs &fb,
-                         StepCtl *ctl, int nxt, uint32_t &local_new,
-                         uint32_t &scanned, unsigned long long &edges_acc) {
+                         uint32_t *__restrict__ next, uint32_t &cnt,
+                         unsigned long long &edges_acc, uint32_t &scanned) {
     const int total = p.unit_start[kNumClasses];
     for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
         int c = 0;
@@ -112,17 +113,20 @@ __device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
         else if (c == 3) bu_group<16>(p, c, k, g, visited, frontier, found, u, par, scanned);
         else if (c == 4) bu_group<4>(p, c, k, g, visited, frontier, found, u, par, scanned);
         else if (c == 5) bu_group<1>(p, c, k, g, visited, frontier, found, u, par, scanned);
-        if (found) {
-            if (LOOP) {
+        if (LOOP) {
+            if (found) {
                 visited[u] = level;
                 if (parents) parents[u] = par;
-            } else {
-                visited[u] = par;
             }
+            if (wave_set_bit(next, u, found)) {
+                cnt++;
+                edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
+            }
+        } else if (found) {
+            visited[u] = par;
             atomicOr(&next[u >> 5], 1u << (u & 31));
-            local_new++;
+            cnt++;
         }
-        if (LOOP) frontier_append(g, fb, ctl, nxt, found, u, edges_acc);
     }
 }
 
@@ -134,8 +138,9 @@ __device__ __forceinline__ void td_range(const GraphView<OffT> &g,
                                          int32_t level, uint32_t *__restrict__ next,
                                          const FrontierBufs &fb, StepCtl *ctl,
                                          int nxt, int32_t u, OffT b, OffT e,
-                                         int lane, uint32_t &scanned,
-                                         unsigned long long &edges_acc) {
+                                         int lane, uint32_t &cnt,
+                                         unsigned long long &edges_acc,
+                                         uint32_t &scanned) {
     for (OffT i = b + OffT(lane); i < e; i += OffT(G)) {
         int32_t v = g.adj[i];
         scanned++;
@@ -144,8 +149,10 @@ __device__ __forceinline__ void td_range(const GraphView<OffT> &g,
             app = true;
             if (parents) parents[v] = u;
             atomicOr(&next[v >> 5], 1u << (v & 31));
+            cnt++;
+            edges_acc += (unsigned long long)(g.rend(v) - g.rbeg(v));
         }
-        frontier_append(g, fb, ctl, nxt, app, v, edges_acc);
+        frontier_append(g, fb, ctl, nxt, app, v);
     }
 }
 
@@ -153,8 +160,8 @@ template <typename OffT>
 __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
                          int32_t *__restrict__ parents, int32_t level,
                          uint32_t *__restrict__ next, const FrontierBufs &fb,
-                         StepCtl *ctl, int cur, int nxt, uint32_t &scanned,
-                         unsigned long long &edges_acc) {
+                         StepCtl *ctl, int cur, int nxt, uint32_t &cnt,
+                         unsigned long long &edges_acc, uint32_t &scanned) {
     const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
                    n1 = ctl->cnt[cur][1], n2 = ctl->cnt[cur][2];
     const uint32_t u1 = nch + n0;
@@ -164,17 +171,17 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
             td_range<kBlock>(g, depth, parents, level, next, fb, ctl, nxt, h.u,
-                             OffT(h.b), OffT(h.e), threadIdx.x, scanned, edges_acc);
+                             OffT(h.b), OffT(h.e), threadIdx.x, cnt, edges_acc, scanned);
         } else if (unit < u1) {
             int32_t u = fb.q[cur][0][unit - nch];
             td_range<kBlock>(g, depth, parents, level, next, fb, ctl, nxt, u,
-                             g.rbeg(u), g.rend(u), threadIdx.x, scanned, edges_acc);
+                             g.rbeg(u), g.rend(u), threadIdx.x, cnt, edges_acc, scanned);
         } else if (unit < u2) {
             uint32_t idx = (unit - u1) * kWavesPerBlock + threadIdx.x / kWave;
             if (idx < n1) {
                 int32_t u = fb.q[cur][1][idx];
                 td_range<kWave>(g, depth, parents, level, next, fb, ctl, nxt, u,
-                                g.rbeg(u), g.rend(u), lane_id(), scanned, edges_acc);
+                                g.rbeg(u), g.rend(u), lane_id(), cnt, edges_acc, scanned);
             }
         } else {
             uint32_t idx = (unit - u2) * kSmallPerUnit + threadIdx.x / kSmallLanes;
@@ -182,77 +189,10 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
                 int32_t u = fb.q[cur][2][idx];
                 td_range<kSmallLanes>(g, depth, parents, level, next, fb, ctl, nxt,
                                       u, g.rbeg(u), g.rend(u),
-                                      threadIdx.x % kSmallLanes, scanned, edges_acc);
+                                      threadIdx.x % kSmallLanes, cnt, edges_acc, scanned);
             }
         }
     }
-}
-
-// Direction-optimizing controller: the decisions of bfs_do_cpu
-// (src/kernels/cpu/bfs.cuh:68-103) applied between device epochs.
-__device__ void bfs_controller(StepCtl *c, int ran_mode, int offb) {
-    const int k = c->epoch, cur = k & 1, nxt = cur ^ 1;
-    uint32_t q = 0;
-    for (int j : {0, 1, 2, 4}) q += load_acquire_u32(&c->cnt[nxt][j]);
-    const long long sc = (long long)load_acquire_u64(
-        reinterpret_cast<const uint64_t *>(&c->edges[nxt]));
-    const unsigned long long scanned = load_acquire_u64(
-        reinterpret_cast<const uint64_t *>(&c->scanned));
-    c->reached += q;
-    if (ran_mode == kPush) {
-        uint32_t in_n = c->cnt[cur][0] + c->cnt[cur][1] + c->cnt[cur][2] + c->cnt[cur][4];
-        // queue entry + offsets per frontier vertex; id + depth probe per
-        // edge; depth, parent, bitmap and queue writes per new vertex.
-        c->alg_bytes += (unsigned long long)(4 + 2 * offb) * in_n + 8ull * scanned + 16ull * q;
-        c->push_epochs++;
-    } else {
-        // visited flag per swept row, offsets of unvisited rows (bounded by
-        // the sweep), id + bitmap bit per examined edge, writes per new vertex.
-        c->alg_bytes += 4ull * c->plan_rows + 4ull * scanned + scanned / 8 +
-                        16ull * q + 4ull * ((c->n + 31) / 32);
-        c->pull_epochs++;
-    }
-    for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
-    c->edges[cur] = 0;
-    c->scanned = 0;
-    c->epoch = k + 1;
-    const long long n = c->n;
-    if (c->direction != 0) {
-        c->mode = c->direction == 1 ? kPull : kPush;
-        if (q == 0) c->done = 1;
-    } else if (ran_mode == kPush) {
-        c->scout = sc;
-        if (q == 0) {
-            c->done = 1;
-        } else if (c->scout > c->edges_to_check / c->alpha) {
-            c->mode = kPull;
-            c->num_nodes = (int32_t)q;
-        } else {
-            c->mode = kPush;
-            c->edges_to_check -= c->scout;
-            c->scout = 0;
-        }
-    } else {
-        c->prev_nodes = c->num_nodes;
-        c->num_nodes = (int32_t)q;
-        if ((long long)q > c->prev_nodes || (long long)q > n / c->beta) {
-            c->mode = kPull;
-        } else {
-            c->scout = 1;
-            if (q == 0) {
-                c->done = 1;
-            } else if (c->scout > c->edges_to_check / c->alpha) {
-                c->mode = kPull;
-                c->num_nodes = (int32_t)q;
-            } else {
-                c->mode = kPush;
-                c->edges_to_check -= c->scout;
-                c->scout = 0;
-            }
-        }
-    }
-    if (c->max_epochs > 0 && k + 1 >= c->max_epochs) c->done = 1;
-    c->blocks_done = 0;
 }
 
 template <typename OffT>
@@ -269,26 +209,89 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
          w += gridDim.x * kBlock)
         stale[w] = 0;
     const int32_t level = k + 1;
-    uint32_t scanned = 0, nnew = 0;
+    uint32_t cnt = 0, scanned = 0;
     unsigned long long eacc = 0;
     if (mode == kPull)
-        bu_sweep<true>(p, g, depth, parents, level, frontier, next, fb, ctl, nxt,
-                       nnew, scanned, eacc);
+        bu_sweep<true>(p, g, depth, parents, level, frontier, next, cnt, eacc, scanned);
     else
-        td_sweep(g, depth, parents, level, next, fb, ctl, cur, nxt, scanned, eacc);
-    unsigned long long es = block_sum_u64(eacc);
-    if (threadIdx.x == 0 && es) atomicAdd(&ctl->edges[nxt], es);
-    unsigned long long ss = block_sum_u64(scanned);
-    if (threadIdx.x == 0 && ss) atomicAdd(&ctl->scanned, ss);
-    if (last_block_done(&ctl->blocks_done) && threadIdx.x == 0)
-        bfs_controller(ctl, mode, sizeof(OffT));
+        td_sweep(g, depth, parents, level, next, fb, ctl, cur, nxt, cnt, eacc, scanned);
+    write_partials(fb.part, cnt, eacc, scanned);
+}
+
+// Direction-optimizing controller (one workgroup): the decisions of
+// bfs_do_cpu (src/kernels/cpu/bfs.cuh:68-103) applied between epochs.
+__global__ __launch_bounds__(kBlock) void bfs_ctl_kernel(FrontierBufs fb,
+                                                         StepCtl *c, int offb) {
+    if (c->done) return;
+    unsigned long long q, sc, scanned;
+    sum_partials(fb.part, c->grid, q, sc, scanned);
+    if (threadIdx.x != 0) return;
+    const int k = c->epoch, cur = k & 1;
+    const int ran = c->mode;
+    c->reached += q;
+    trace_epoch(c, k, ran, q, sc, scanned);
+    if (ran == kPush) {
+        // queue entry + offsets per frontier vertex; id + depth probe per
+        // edge; depth, parent, bitmap and queue writes per new vertex.
+        c->alg_bytes += (unsigned long long)(4 + 2 * offb) * c->in_n + 8ull * scanned + 16ull * q;
+        c->push_epochs++;
+    } else {
+        // visited flag per swept row, id + bitmap bit per examined edge,
+        // writes per new vertex, bitmap clear.
+        c->alg_bytes += 4ull * c->plan_rows + 4ull * scanned + scanned / 8 +
+                        12ull * q + 4ull * ((c->n + 31) / 32);
+        c->pull_epochs++;
+    }
+    for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
+    c->epoch = k + 1;
+    c->in_n = (long long)q;
+    c->in_e = (long long)sc;
+    const long long n = c->n;
+    int next = ran;
+    if (c->direction != 0) {
+        next = c->direction == 1 ? kPull : kPush;
+        if (q == 0) c->done = 1;
+    } else if (ran == kPush) {
+        c->scout = (long long)sc;
+        if (q == 0) {
+            c->done = 1;
+        } else if (c->scout > c->edges_to_check / c->alpha) {
+            next = kPull;
+            c->num_nodes = (long long)q;
+        } else {
+            next = kPush;
+            c->edges_to_check -= c->scout;
+            c->scout = 0;
+        }
+    } else {
+        c->prev_nodes = c->num_nodes;
+        c->num_nodes = (long long)q;
+        if ((long long)q > c->prev_nodes || (long long)q > n / c->beta) {
+            next = kPull;
+        } else {
+            c->scout = 1;
+            if (q == 0) {
+                c->done = 1;
+            } else if (c->scout > c->edges_to_check / c->alpha) {
+                next = kPull;
+                c->num_nodes = (long long)q;
+            } else {
+                next = kPush;
+                c->edges_to_check -= c->scout;
+                c->scout = 0;
+            }
+        }
+    }
+    if (c->max_epochs > 0 && k + 1 >= c->max_epochs) c->done = 1;
+    c->need_compact = (!c->done && ran == kPull && next == kPush) ? 1 : 0;
+    c->mode = next;
 }
 
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
     GraphView<OffT> g, int32_t *__restrict__ depth, int32_t *__restrict__ parents,
     FrontierBufs fb, StepCtl *ctl, int32_t src, int direction, int alpha,
-    int beta, int max_epochs, long long m, int32_t plan_rows) {
+    int beta, int max_epochs, long long m, int32_t plan_rows, int grid) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride) {
         depth[i] = i == src ? 0 : -1;
@@ -300,27 +303,28 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
         fb.bm[2][w] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < 2; p++)
             for (int j = 0; j < 5; j++) ctl->cnt[p][j] = 0;
-            ctl->edges[p] = 0;
-        }
-        ctl->blocks_done = 0;
         ctl->done = 0;
         ctl->epoch = 0;
         ctl->max_epochs = max_epochs;
         ctl->direction = direction;
         ctl->alpha = alpha;
         ctl->beta = beta;
-        ctl->m = m;
+        ctl->need_compact = 0;
         ctl->n = g.n;
+        ctl->grid = grid;
+        ctl->m = m;
         ctl->push_epochs = ctl->pull_epochs = 0;
         ctl->reached = 1;
         ctl->alg_bytes = 0;
-        ctl->scanned = 0;
+        ctl->t_start = wall_clock64();
+        ctl->t_end = ctl->t_start;
         ctl->plan_rows = plan_rows;
         OffT b = g.rbeg(src), e = g.rend(src);
         int64_t d = int64_t(e - b);
-        if (d > kHubDeg) {
+        int cls = degree_class(d);
+        if (cls == 3) {
             uint32_t nch = 0;
             for (int64_t x = b; x < int64_t(e); x += kChunkEdges, nch++) {
                 int64_t ce = x + kChunkEdges < int64_t(e) ? x + kChunkEdges : int64_t(e);
@@ -329,23 +333,23 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
             ctl->cnt[0][3] = nch;
             ctl->cnt[0][4] = 1;
         } else {
-            int cls = d > kBigDeg ? 0 : (d > kMidDeg ? 1 : 2);
             fb.q[0][cls][0] = src;
             ctl->cnt[0][cls] = 1;
         }
-        ctl->edges[0] = (unsigned long long)d;
+        ctl->in_n = 1;
+        ctl->in_e = d;
         // Initial decision of bfs_do_cpu: num_edges = deg(src),
         // edges_to_check = m.
         ctl->edges_to_check = m;
         ctl->scout = d;
+        ctl->num_nodes = 1;
+        ctl->prev_nodes = 0;
         if (direction == 1) {
             ctl->mode = kPull;
-            ctl->num_nodes = 1;
         } else if (direction == 2) {
             ctl->mode = kPush;
         } else if (ctl->scout > ctl->edges_to_check / alpha) {
             ctl->mode = kPull;
-            ctl->num_nodes = 1;
         } else {
             ctl->mode = kPush;
             ctl->edges_to_check -= ctl->scout;
@@ -363,9 +367,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_fused_kernel(
     int32_t *num_nodes) {
     uint32_t nnew = 0, scanned = 0;
     unsigned long long eacc = 0;
-    FrontierBufs fb{};
-    bu_sweep<false>(p, g, parents, nullptr, 0, frontier, next, fb, nullptr, 0,
-                    nnew, scanned, eacc);
+    bu_sweep<false>(p, g, parents, nullptr, 0, frontier, next, nnew, eacc, scanned);
     unsigned long long s = block_sum_u64(nnew);
     if (threadIdx.x == 0 && s) atomicAdd(num_nodes, (int32_t)s);
 }
@@ -462,6 +464,9 @@ void free_bfs_ws(hc_graph_s *g) {
 }
 
 static int get_bfs_ws(hc_graph_s *g, BfsWs **out) {
+    // The partial slots are sized for the launch grid; re-allocate when the
+    // tuned grid changes.
+    if (g->ws_bfs && static_cast<BfsWs *>(g->ws_bfs)->f.grid != grid_blocks()) free_bfs_ws(g);
     if (!g->ws_bfs) {
         auto *ws = new BfsWs();
         int rc = frontier_ws_alloc(g, &ws->f);
@@ -480,17 +485,26 @@ static int get_bfs_ws(hc_graph_s *g, BfsWs **out) {
     return HC_OK;
 }
 
-// Bottom-up plan: no chunk or workgroup classes (early exit makes them
-// wasteful); wavefront for degree > 256, 16 lanes > 32, 4 lanes > 8.
-static PlanConfig bu_config() {
-    PlanConfig c;
-    c.hub_threshold = INT64_MAX;
-    c.block_threshold = INT64_MAX;
-    c.wave_threshold = 256;
-    c.g16_threshold = 32;
-    c.g4_threshold = 8;
-    return c;
-}
+template <typename OffT>
+struct BfsLaunch {
+    PlanDev plan;
+    GraphView<OffT> g;
+    int32_t *depth;
+    int32_t *parents;
+    FrontierBufs fb;
+    StepCtl *ctl;
+    int grid;
+    hipStream_t s;
+    static int launch(void *arg) {
+        auto *a = static_cast<BfsLaunch *>(arg);
+        bfs_step_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->plan, a->g, a->depth,
+                                                            a->parents, a->fb, a->ctl);
+        bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
+        compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
+        if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "BFS epoch launch failed");
+        return HC_OK;
+    }
+};
 
 template <typename OffT>
 static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
@@ -499,69 +513,34 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     BfsWs *ws = nullptr;
     HC_TRY(get_bfs_ws(g, &ws));
     const PlanHost *ph = nullptr;
-    HC_TRY(get_plan(g, 0, g->n, bu_config(), &ph));
+    HC_TRY(get_plan(g, 0, g->n, tuning().bu, &ph));
     int32_t plan_rows = 0;
-    for (int c = 0; c < kNumClasses; c++) plan_rows += c == 0 ? 0 : ph->dev.cnt[c];
+    for (int c = 1; c < kNumClasses; c++) plan_rows += ph->dev.cnt[c];
     if (!depth) depth = ws->depth;
     GraphView<OffT> v = view<OffT>(g);
-    const int grid = grid_blocks(8);
+    const int grid = ws->f.grid;
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     bfs_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, depth, parents, ws->f.fb, ws->f.ctl, src, direction, alpha, beta, 0,
-        (long long)g->m, plan_rows);
+        (long long)g->m, plan_rows, grid);
     HC_HIP(hipGetLastError());
-    hipEvent_t ev0, ev1, evb[2];
-    HC_HIP(hipEventCreate(&ev0));
-    HC_HIP(hipEventCreate(&ev1));
-    HC_HIP(hipEventCreateWithFlags(&evb[0], hipEventDisableTiming));
-    HC_HIP(hipEventCreateWithFlags(&evb[1], hipEventDisableTiming));
-    StepCtl *hslot = ws->f.h_ctl;
-    hslot->done = 0;
-    HC_HIP(hipEventRecord(ev0, s));
-    const int batch = 4;
-    const long long cap = (long long)g->n + 2 * batch + 2;  // levels <= n
-    long long launches = 0;
-    int rc = HC_OK;
-    bool done = false;
-    int b = 0;
-    while (!done && launches < cap) {
-        for (int i = 0; i < batch; i++) {
-            bfs_step_kernel<OffT><<<grid, kBlock, 0, s>>>(ph->dev, v, depth, parents,
-                                                          ws->f.fb, ws->f.ctl);
-            launches++;
-        }
-        if (hipGetLastError() != hipSuccess) {
-            rc = fail(HC_ERR_HIP, "step launch failed");
-            break;
-        }
-        hipMemcpyAsync(&hslot->done, &ws->f.ctl->done, sizeof(int32_t),
-                       hipMemcpyDeviceToHost, s);
-        hipEventRecord(evb[b & 1], s);
-        if (b > 0) {
-            hipEventSynchronize(evb[(b - 1) & 1]);
-            done = __atomic_load_n(&hslot->done, __ATOMIC_ACQUIRE) != 0;
-        }
-        b++;
-    }
-    HC_HIP(hipEventRecord(ev1, s));
-    HC_HIP(hipEventSynchronize(ev1));
+    BfsLaunch<OffT> L{ph->dev, v, depth, parents, ws->f.fb, ws->f.ctl, grid, s};
+    const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
-    HC_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    long long launches = 0;
     StepCtl fc;
-    HC_HIP(hipMemcpy(&fc, ws->f.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
-    hipEventDestroy(evb[0]);
-    hipEventDestroy(evb[1]);
-    if (rc != HC_OK) return rc;
-    if (!fc.done) return fail(HC_ERR_HIP, "BFS loop did not finish in %lld launches", launches);
+    HC_TRY(run_epoch_loop(&ws->f, s, cap, &BfsLaunch<OffT>::launch, &L, &ms,
+                          &launches, &fc));
+    if (!fc.done) return fail(HC_ERR_HIP, "BFS loop did not finish in %lld epochs", launches);
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
-        stats->ms = ms;
+        // Device wall clock from the init kernel to the controller of the
+        // last epoch (launches queued past it are not part of the loop).
+        stats->ms = double(fc.t_end - fc.t_start) / device_state().clock_khz;
         stats->epochs = fc.epoch;
         stats->push_epochs = fc.push_epochs;
         stats->pull_epochs = fc.pull_epochs;
-        stats->launches = (int32_t)launches;
+        stats->launches = (int32_t)(3 * launches);
         stats->reached = (int64_t)fc.reached;
         stats->alg_bytes = (double)fc.alg_bytes;
     }
@@ -615,7 +594,7 @@ int hc_bfs_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
     std::lock_guard<std::mutex> lk(g->mu);
     hipStream_t s = pick_stream(stream);
     if (!threads) threads = 256;
-    if (!blocks) blocks = grid_blocks(8);
+    if (!blocks) blocks = grid_blocks();
     auto go = [&](auto tag) -> int {
         using OffT = decltype(tag);
         GraphView<OffT> v = view<OffT>(g);
@@ -627,9 +606,9 @@ int hc_bfs_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
                 v, parents, start_id, end_id, frontier, next_frontier, num_nodes);
         } else {
             const PlanHost *ph = nullptr;
-            HC_TRY(get_plan(g, start_id, end_id, bu_config(), &ph));
+            HC_TRY(get_plan(g, start_id, end_id, tuning().bu, &ph));
             if (ph->total_units == 0) return HC_OK;
-            int nb = std::max(1, std::min(ph->total_units, grid_blocks(8)));
+            int nb = std::max(1, std::min(ph->total_units, grid_blocks()));
             bfs_pull_fused_kernel<OffT><<<nb, kBlock, 0, s>>>(
                 ph->dev, v, parents, frontier, next_frontier, num_nodes);
         }
@@ -653,7 +632,7 @@ int hc_bfs_push_epoch(hc_graph_t g, const int32_t *frontier,
     if (frontier_size == 0) return HC_OK;
     std::lock_guard<std::mutex> lk(g->mu);
     hipStream_t s = pick_stream(stream);
-    int blocks = std::max(1, std::min(grid_blocks(8), (frontier_size + 3) / 4));
+    int blocks = std::max(1, std::min(grid_blocks(), (frontier_size + 3) / 4));
     if (g->off64)
         bfs_push_epoch_kernel<uint64_t><<<blocks, 256, 0, s>>>(
             view<uint64_t>(g), frontier, frontier_size, parents, next_frontier,

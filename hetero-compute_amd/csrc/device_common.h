@@ -91,6 +91,23 @@ struct PlanConfig {
     }
 };
 
+// Process-wide tuning table (set through hc_set_tuning by the autotuner or a
+// caller; read when plans and workspaces are built).
+struct Tuning {
+    int grid_per_cu = 8;   // workgroups per CU of the fused / step launches
+    int sssp_alpha = 8;    // SSSP goes pull when frontier edges > m / alpha
+    PlanConfig pull;       // SSSP and PageRank pull sweeps
+    PlanConfig bu;         // BFS bottom-up sweeps (no chunk / WG classes)
+    Tuning() {
+        bu.hub_threshold = INT64_MAX;
+        bu.block_threshold = INT64_MAX;
+        bu.wave_threshold = 256;
+        bu.g16_threshold = 32;
+        bu.g4_threshold = 8;
+    }
+};
+Tuning &tuning();
+
 // Host-side owner of a plan's device arrays.
 struct PlanHost {
     PlanDev dev{};
@@ -99,6 +116,14 @@ struct PlanHost {
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Streaming load of the adjacency / weight arrays: each element is read once
+// per sweep, so it is marked non-temporal to leave L2 to the reused
+// per-vertex state (distances, scores, frontier bits).
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     return (uint64_t(1) << lane_id()) - 1;
@@ -145,6 +170,45 @@ __device__ __forceinline__ uint64_t load_acquire_u64(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Sets bit u of bm for every lane with pred; lanes that share a word are
+// merged into one atomicOr (the degree-sorted classes hand a wavefront
+// consecutive vertices).  Returns whether this lane's bit was newly set.
+__device__ __forceinline__ bool wave_set_bit(uint32_t *bm, int32_t u, bool pred) {
+    const uint32_t word = pred ? uint32_t(u) >> 5 : 0u;
+    const uint32_t bit = pred ? (1u << (u & 31)) : 0u;
+    bool first = false;
+    uint64_t todo = __ballot(pred);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t w = __shfl(word, leader);
+        const bool mine = pred && word == w;
+        const uint64_t grp = __ballot(mine);
+        uint32_t bits = mine ? bit : 0u;
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) bits |= __shfl_xor(bits, s, 64);
+        uint32_t old = 0;
+        if (lane_id() == leader) old = atomicOr(&bm[w], bits);
+        old = __shfl(old, leader);
+        if (mine) first = !(old & bit);
+        todo &= ~grp;
+    }
+    return first;
+}
+
+// Sum of a per-thread value over a 256-thread workgroup; valid in thread 0.
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
+    __shared__ unsigned long long s_part[kWavesPerBlock];
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+    __syncthreads();
+    if (lane_id() == 0) s_part[threadIdx.x / kWave] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kWavesPerBlock; i++) t += s_part[i];
+    return t;
+}
+
 // Last-workgroup detection: returns true in every thread of the workgroup
 // that finished last.  Every thread releases its own global writes before the
 // ticket is taken, and the last workgroup acquires, so all writes (stores and
@@ -168,13 +232,15 @@ __device__ __forceinline__ bool last_block_done(uint32_t *blocks_done) {
 struct DeviceState {
     int device = -1;
     int cus = 0;
+    double clock_khz = 100000.0;  // device wall clock rate
     size_t hbm = 0;
     hipStream_t stream = nullptr;  // library default stream
     bool ready = false;
 };
 DeviceState &device_state();
 int ensure_init();
-int grid_blocks(int per_cu = 8);
+// Workgroups for a full-device launch: per_cu per CU (0 = the tuned value).
+int grid_blocks(int per_cu = 0);
 hipStream_t pick_stream(hc_stream_t s);
 
 }  // namespace hc

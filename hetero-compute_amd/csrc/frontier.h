@@ -1,15 +1,27 @@
 // Device-resident frontier and epoch-control state shared by the
 // device-driven epoch loops (SSSP push/pull and BFS top-down/bottom-up).
 //
-// An epoch k reads the frontier of parity k&1 and writes parity (k+1)&1.
-// Appended vertices are routed by degree into classes that map to different
-// lane counts when the frontier is expanded:
+// Every epoch is three launches on one stream:
+//   step      (full grid)  runs the epoch; each workgroup writes its partial
+//                          counts (frontier size, degree sum, edges examined)
+//                          to its own slot -- no same-address atomics;
+//   control   (1 workgroup) sums the partials in a fixed order, records the
+//                          epoch, picks the next direction or stops;
+//   compact   (full grid)  only after a pull epoch that is followed by a push
+//                          epoch: turns the frontier bitmap into the
+//                          degree-classed queues the push expansion reads.
+// All cross-workgroup data flows through kernel boundaries, so no device-
+// scope fences are needed inside a kernel, and the host only enqueues.
+//
+// Push frontiers are routed by degree into classes that map to different
+// lane counts when expanded:
 //   chunk: deg > kHubDeg      -> one workgroup per kChunkEdges-edge chunk
 //   q[0] : deg > kBigDeg      -> one workgroup per vertex
 //   q[1] : deg > kMidDeg      -> one wavefront per vertex
 //   q[2] : otherwise          -> kSmallLanes lanes per vertex
-// The last workgroup of every step launch runs the controller, so the host
-// can enqueue many steps without reading anything back.
+// Chunks are small because a push epoch is latency-bound (each edge is a
+// dependent gather + atomic): many short chunks keep all CUs busy when the
+// frontier holds only a few hubs.
 #ifndef HC_FRONTIER_H
 #define HC_FRONTIER_H
 
@@ -17,19 +29,18 @@
 
 namespace hc {
 
-constexpr int64_t kHubDeg = 16384;
-constexpr int64_t kChunkEdges = 4096;
-constexpr int64_t kBigDeg = 1024;
-constexpr int64_t kMidDeg = 48;
+constexpr int64_t kHubDeg = 1024;
+constexpr int64_t kChunkEdges = 512;
+constexpr int64_t kBigDeg = 256;
+constexpr int64_t kMidDeg = 16;
 constexpr int kSmallLanes = 8;
 constexpr int kSmallPerUnit = kBlock / kSmallLanes;
+constexpr int kTraceLen = 128;
 
 enum StepMode : int32_t { kPush = 0, kPull = 1 };
 
 struct StepCtl {
-    uint32_t cnt[2][5];            // [parity][q0,q1,q2,chunks,hub vertices]
-    unsigned long long edges[2];   // sum of degrees appended per parity
-    uint32_t blocks_done;
+    uint32_t cnt[2][5];            // queue fill [parity][q0,q1,q2,chunks,hubs]
     int32_t done;
     int32_t mode;
     int32_t epoch;
@@ -37,85 +48,244 @@ struct StepCtl {
     int32_t direction;             // 0 auto, 1 pull only, 2 push only
     int32_t alpha;
     int32_t beta;
+    int32_t need_compact;          // next push epoch needs queues from bitmap
+    int32_t n;                     // vertices
+    int32_t grid;                  // workgroups of the step launches
+    int32_t push_epochs;
+    int32_t pull_epochs;
     long long m;                   // edges of the graph
     long long edges_to_check;      // BFS direction-optimizing state
     long long scout;
-    int32_t num_nodes;             // BFS pull phase: current frontier size
-    int32_t prev_nodes;
-    int32_t n;                     // vertices
-    int32_t push_epochs;
-    int32_t pull_epochs;
-    int32_t pad;
+    long long num_nodes;           // BFS pull phase: current frontier size
+    long long prev_nodes;
+    long long in_n;                // size of the frontier the next epoch reads
+    long long in_e;                // its degree sum
+    long long sweep_bytes;         // algorithmic bytes of one pull sweep
     unsigned long long reached;
     unsigned long long alg_bytes;  // algorithmic bytes moved by the epochs
-    unsigned long long scanned;    // edges examined by the running epoch
-    int32_t plan_rows;             // rows swept by a pull epoch
-    int32_t pad2;
+    int32_t plan_rows;
+    int32_t pad;
+    // Device wall clock (constant rate): start of the loop, end of the last
+    // epoch, and end of each traced epoch.
+    long long t_start;
+    long long t_end;
+    long long tr_t[kTraceLen];
+    // Per-epoch trace of the first kTraceLen epochs.
+    int8_t tr_mode[kTraceLen];
+    uint32_t tr_n[kTraceLen];
+    unsigned long long tr_e[kTraceLen];
+    unsigned long long tr_scan[kTraceLen];
+};
+
+struct Partials {
+    uint32_t *n;               // frontier vertices produced, per workgroup
+    unsigned long long *e;     // their degree sum
+    unsigned long long *s;     // edges examined
 };
 
 struct FrontierBufs {
     int32_t *q[2][3];
     HubChunk *ch[2];
     uint32_t *bm[3];
-    int32_t words;                 // bitmap words
+    int32_t words;             // bitmap words
+    Partials part;
 };
 
 // Workspace layout for one graph (SSSP and BFS share the shape).
 struct FrontierWs {
     FrontierBufs fb{};
     StepCtl *ctl = nullptr;
-    StepCtl *h_ctl = nullptr;      // pinned mirror for polling
+    StepCtl *h_ctl = nullptr;  // pinned mirror for polling
     void *dmem = nullptr;
     int64_t chunk_cap = 0;
+    int32_t grid = 0;
 };
 
 int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws);
 void frontier_ws_free(FrontierWs *ws);
 
+__device__ __forceinline__ int degree_class(int64_t d) {
+    return d > kHubDeg ? 3 : (d > kBigDeg ? 0 : (d > kMidDeg ? 1 : 2));
+}
+
+// Appends v (when pred) to the degree-classed queues of parity p.
+// Wavefront-aggregated; used by push epochs, whose output is bounded by the
+// direction heuristic, so the per-wavefront atomics stay few.
 template <typename OffT>
 __device__ __forceinline__ void frontier_append(const GraphView<OffT> &g,
                                                 const FrontierBufs &fb,
-                                                StepCtl *ctl, int nxt,
-                                                bool pred, int32_t v,
-                                                unsigned long long &edges_acc) {
+                                                StepCtl *ctl, int p, bool pred,
+                                                int32_t v) {
     int cls = -1;
     OffT b = 0, e = 0;
     if (pred) {
         b = g.rbeg(v);
         e = g.rend(v);
-        int64_t d = int64_t(e - b);
-        edges_acc += (unsigned long long)d;
-        cls = d > kHubDeg ? 3 : (d > kBigDeg ? 0 : (d > kMidDeg ? 1 : 2));
+        cls = degree_class(int64_t(e - b));
     }
-    wave_append(fb.q[nxt][0], &ctl->cnt[nxt][0], cls == 0, v);
-    wave_append(fb.q[nxt][1], &ctl->cnt[nxt][1], cls == 1, v);
-    wave_append(fb.q[nxt][2], &ctl->cnt[nxt][2], cls == 2, v);
+    wave_append(fb.q[p][0], &ctl->cnt[p][0], cls == 0, v);
+    wave_append(fb.q[p][1], &ctl->cnt[p][1], cls == 1, v);
+    wave_append(fb.q[p][2], &ctl->cnt[p][2], cls == 2, v);
     if (cls == 3) {
         int64_t d = int64_t(e - b);
         uint32_t nch = (uint32_t)((d + kChunkEdges - 1) / kChunkEdges);
-        uint32_t base = atomicAdd(&ctl->cnt[nxt][3], nch);
-        atomicAdd(&ctl->cnt[nxt][4], 1u);
+        uint32_t base = atomicAdd(&ctl->cnt[p][3], nch);
+        atomicAdd(&ctl->cnt[p][4], 1u);
         for (uint32_t j = 0; j < nch; j++) {
             int64_t cb = int64_t(b) + int64_t(j) * kChunkEdges;
             int64_t ce = cb + kChunkEdges < int64_t(e) ? cb + kChunkEdges : int64_t(e);
-            fb.ch[nxt][base + j] = HubChunk{v, 0, cb, ce};
+            fb.ch[p][base + j] = HubChunk{v, 0, cb, ce};
         }
     }
 }
 
-// Sum of a per-thread value over the workgroup; result valid in thread 0.
-__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
-    __shared__ unsigned long long s_part[kWavesPerBlock];
+// Workgroup sums of three per-thread counters, written to this workgroup's
+// partial slot (thread 0).
+__device__ __forceinline__ void write_partials(const Partials &part, uint32_t n,
+                                               unsigned long long e,
+                                               unsigned long long s) {
+    __shared__ uint32_t s_n[kWavesPerBlock];
+    __shared__ unsigned long long s_e[kWavesPerBlock], s_s[kWavesPerBlock];
 #pragma unroll
-    for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+    for (int sh = 32; sh > 0; sh >>= 1) {
+        n += __shfl_xor(n, sh, 64);
+        e += __shfl_xor(e, sh, 64);
+        s += __shfl_xor(s, sh, 64);
+    }
     __syncthreads();
-    if (lane_id() == 0) s_part[threadIdx.x / kWave] = v;
+    if (lane_id() == 0) {
+        s_n[threadIdx.x / kWave] = n;
+        s_e[threadIdx.x / kWave] = e;
+        s_s[threadIdx.x / kWave] = s;
+    }
     __syncthreads();
-    unsigned long long t = 0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kWavesPerBlock; i++) t += s_part[i];
-    return t;
+    if (threadIdx.x == 0) {
+        uint32_t tn = 0;
+        unsigned long long te = 0, ts = 0;
+        for (int i = 0; i < kWavesPerBlock; i++) {
+            tn += s_n[i];
+            te += s_e[i];
+            ts += s_s[i];
+        }
+        part.n[blockIdx.x] = tn;
+        part.e[blockIdx.x] = te;
+        part.s[blockIdx.x] = ts;
+    }
 }
+
+// Fixed-order sum of the step's partials, run by one workgroup of kBlock
+// threads; results valid in every thread.
+__device__ __forceinline__ void sum_partials(const Partials &part, int grid,
+                                             unsigned long long &n,
+                                             unsigned long long &e,
+                                             unsigned long long &s) {
+    __shared__ unsigned long long r_n[kBlock], r_e[kBlock], r_s[kBlock];
+    unsigned long long a = 0, b = 0, c = 0;
+    for (int i = threadIdx.x; i < grid; i += blockDim.x) {
+        a += part.n[i];
+        b += part.e[i];
+        c += part.s[i];
+    }
+    r_n[threadIdx.x] = a;
+    r_e[threadIdx.x] = b;
+    r_s[threadIdx.x] = c;
+    __syncthreads();
+    for (int sh = kBlock / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh) {
+            r_n[threadIdx.x] += r_n[threadIdx.x + sh];
+            r_e[threadIdx.x] += r_e[threadIdx.x + sh];
+            r_s[threadIdx.x] += r_s[threadIdx.x + sh];
+        }
+        __syncthreads();
+    }
+    n = r_n[0];
+    e = r_e[0];
+    s = r_s[0];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void trace_epoch(StepCtl *c, int k, int mode,
+                                            unsigned long long out_n,
+                                            unsigned long long out_e,
+                                            unsigned long long scanned) {
+    const long long t = wall_clock64();
+    c->t_end = t;
+    if (k < kTraceLen) {
+        c->tr_t[k] = t;
+        c->tr_mode[k] = (int8_t)mode;
+        c->tr_n[k] = (uint32_t)out_n;
+        c->tr_e[k] = out_e;
+        c->tr_scan[k] = scanned;
+    }
+}
+
+// Bitmap -> degree-classed queues of parity (epoch & 1).  Each workgroup
+// takes a contiguous run of bitmap words, counts its vertices per class in
+// LDS and reserves its output with one atomic per class.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
+                                                         FrontierBufs fb,
+                                                         StepCtl *ctl,
+                                                         int three_bitmaps) {
+    if (ctl->done || !ctl->need_compact) return;
+    const int k = ctl->epoch;
+    const int p = k & 1;
+    const uint32_t *bm = three_bitmaps ? fb.bm[k % 3] : fb.bm[k & 1];
+    const int words = fb.words;
+    const int per_block = (words + gridDim.x - 1) / gridDim.x;
+    const int w0 = blockIdx.x * per_block;
+    const int w1 = min(words, w0 + per_block);
+    __shared__ uint32_t s_cnt[5], s_base[5];
+    for (int base_w = w0; base_w < w1; base_w += kBlock) {
+        if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const int w = base_w + threadIdx.x;
+        uint32_t bits = (w < w1) ? bm[w] : 0u;
+        // pass 1: count per class (class 3 counts chunks; slot 4 hubs)
+        uint32_t my[5] = {0, 0, 0, 0, 0};
+        for (uint32_t x = bits; x; x &= x - 1) {
+            int32_t v = w * 32 + __ffs(x) - 1;
+            int64_t d = int64_t(g.rend(v) - g.rbeg(v));
+            int c = degree_class(d);
+            if (c < 3) {
+                my[c]++;
+            } else {
+                my[3] += (uint32_t)((d + kChunkEdges - 1) / kChunkEdges);
+                my[4]++;
+            }
+        }
+        uint32_t off[5];
+        for (int c = 0; c < 5; c++) off[c] = my[c] ? atomicAdd(&s_cnt[c], my[c]) : 0;
+        __syncthreads();
+        if (threadIdx.x < 5)
+            s_base[threadIdx.x] = s_cnt[threadIdx.x]
+                                      ? atomicAdd(&ctl->cnt[p][threadIdx.x], s_cnt[threadIdx.x])
+                                      : 0;
+        __syncthreads();
+        // pass 2: write queue entries and chunk descriptors
+        for (uint32_t x = bits; x; x &= x - 1) {
+            int32_t v = w * 32 + __ffs(x) - 1;
+            OffT b = g.rbeg(v), e = g.rend(v);
+            int64_t d = int64_t(e - b);
+            int c = degree_class(d);
+            if (c < 3) {
+                fb.q[p][c][s_base[c] + off[c]++] = v;
+            } else {
+                for (int64_t cb = int64_t(b); cb < int64_t(e); cb += kChunkEdges) {
+                    int64_t ce = cb + kChunkEdges < int64_t(e) ? cb + kChunkEdges : int64_t(e);
+                    fb.ch[p][s_base[3] + off[3]++] = HubChunk{v, 0, cb, ce};
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Host loop shared by SSSP and BFS: enqueue `launch_epoch` batches and poll
+// the done flag of the previous batch while the next one is queued.
+int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
+                   int (*launch_epoch)(void *), void *arg, float *ms,
+                   long long *launches, StepCtl *final_ctl);
 
 }  // namespace hc
 

@@ -1,6 +1,8 @@
-// Frontier workspace allocation (device queues, chunk lists, bitmaps and the
-// epoch controller block) shared by the SSSP and BFS epoch loops.
+// Frontier workspace allocation (device queues, chunk lists, bitmaps,
+// per-workgroup partials and the control block) and the host side of the
+// device-driven epoch loop, shared by SSSP and BFS.
 #include <algorithm>
+#include <cstring>
 
 #include "frontier.h"
 
@@ -9,8 +11,9 @@ namespace hc {
 int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     const int64_t n = g->n;
     const int32_t words = (int32_t)((n + 31) / 32);
-    // Chunk capacity: every hub vertex can be appended at most once per
-    // epoch, so the sum of its chunk counts bounds one parity's list.
+    const int32_t grid = grid_blocks();
+    // Chunk capacity: every hub vertex can be in one parity's frontier at
+    // most once, so the sum of its chunk counts bounds one parity's list.
     int64_t cap = 1;
     for (int32_t i = 0; i < g->rows; i++) {
         int64_t d = g->h_off[i + 1] - g->h_off[i];
@@ -20,7 +23,8 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     size_t qbytes = al(sizeof(int32_t) * std::max<int64_t>(n, 1));
     size_t cbytes = al(sizeof(HubChunk) * cap);
     size_t bbytes = al(sizeof(uint32_t) * std::max(words, 1));
-    size_t total = 6 * qbytes + 2 * cbytes + 3 * bbytes + al(sizeof(StepCtl));
+    size_t pbytes = al(sizeof(uint32_t) * grid) + 2 * al(sizeof(unsigned long long) * grid);
+    size_t total = 6 * qbytes + 2 * cbytes + 3 * bbytes + pbytes + al(sizeof(StepCtl));
     char *base = nullptr;
     if (hipMalloc(reinterpret_cast<void **>(&base), total) != hipSuccess)
         return fail(HC_ERR_OOM, "frontier workspace of %zu bytes failed", total);
@@ -38,10 +42,17 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
         ws->fb.bm[p] = reinterpret_cast<uint32_t *>(base + o);
         o += bbytes;
     }
+    ws->fb.part.n = reinterpret_cast<uint32_t *>(base + o);
+    o += al(sizeof(uint32_t) * grid);
+    ws->fb.part.e = reinterpret_cast<unsigned long long *>(base + o);
+    o += al(sizeof(unsigned long long) * grid);
+    ws->fb.part.s = reinterpret_cast<unsigned long long *>(base + o);
+    o += al(sizeof(unsigned long long) * grid);
     ws->ctl = reinterpret_cast<StepCtl *>(base + o);
     ws->fb.words = words;
     ws->dmem = base;
     ws->chunk_cap = cap;
+    ws->grid = grid;
     if (hipHostMalloc(reinterpret_cast<void **>(&ws->h_ctl), sizeof(StepCtl),
                       hipHostMallocDefault) != hipSuccess) {
         hipFree(base);
@@ -58,4 +69,75 @@ void frontier_ws_free(FrontierWs *ws) {
     ws->h_ctl = nullptr;
 }
 
+int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
+                   int (*launch_epoch)(void *), void *arg, float *ms,
+                   long long *launches, StepCtl *final_ctl) {
+    hipEvent_t ev0, ev1, evb[2];
+    HC_HIP(hipEventCreate(&ev0));
+    HC_HIP(hipEventCreate(&ev1));
+    HC_HIP(hipEventCreateWithFlags(&evb[0], hipEventDisableTiming));
+    HC_HIP(hipEventCreateWithFlags(&evb[1], hipEventDisableTiming));
+    StepCtl *h = ws->h_ctl;
+    h->done = 0;
+    HC_HIP(hipEventRecord(ev0, s));
+    // Batches of two epochs: the host reads batch b-1's done flag while batch
+    // b is queued, so at most ~4 epochs are enqueued past the last one (they
+    // return at once) and the device never waits on a poll.
+    long long epochs = 0;
+    int rc = HC_OK;
+    bool done = false;
+    int b = 0;
+    while (!done && epochs < cap) {
+        const int batch = 2;
+        for (int i = 0; i < batch; i++) {
+            rc = launch_epoch(arg);
+            if (rc != HC_OK) break;
+            epochs++;
+        }
+        if (rc != HC_OK) break;
+        hipMemcpyAsync(&h->done, &ws->ctl->done, sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s);
+        hipEventRecord(evb[b & 1], s);
+        if (b > 0) {
+            hipEventSynchronize(evb[(b - 1) & 1]);
+            done = __atomic_load_n(&h->done, __ATOMIC_ACQUIRE) != 0;
+        }
+        b++;
+    }
+    HC_HIP(hipEventRecord(ev1, s));
+    HC_HIP(hipEventSynchronize(ev1));
+    HC_HIP(hipEventElapsedTime(ms, ev0, ev1));
+    HC_HIP(hipMemcpy(final_ctl, ws->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    hipEventDestroy(evb[0]);
+    hipEventDestroy(evb[1]);
+    *launches = epochs;
+    return rc;
+}
+
 }  // namespace hc
+
+using namespace hc;
+
+extern "C" int hc_get_epoch_trace(hc_graph_t g, int algorithm,
+                                  hc_epoch_trace_t *out) {
+    HC_CHECK_ARG(g && out, "NULL argument");
+    HC_CHECK_ARG(algorithm == 0 || algorithm == 1, "algorithm must be 0 (SSSP) or 1 (BFS)");
+    std::lock_guard<std::mutex> lk(g->mu);
+    // Both workspaces start with their FrontierWs.
+    void *ws = algorithm == 0 ? g->ws_sssp : g->ws_bfs;
+    std::memset(out, 0, sizeof(*out));
+    if (!ws) return HC_OK;
+    StepCtl c;
+    HC_HIP(hipMemcpy(&c, static_cast<FrontierWs *>(ws)->ctl, sizeof(StepCtl),
+                     hipMemcpyDeviceToHost));
+    out->count = std::min(c.epoch, kTraceLen);
+    for (int i = 0; i < out->count; i++) {
+        out->mode[i] = c.tr_mode[i];
+        out->frontier[i] = c.tr_n[i];
+        out->frontier_edges[i] = c.tr_e[i];
+        out->scanned[i] = c.tr_scan[i];
+    }
+    return HC_OK;
+}

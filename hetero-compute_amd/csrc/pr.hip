@@ -8,8 +8,9 @@
 //   * contrib computed once per vertex as (float)((double)score * inv_deg)
 //     with inv_deg = 1/deg in FP64, fused into the epoch that writes score;
 //   * the L1 residual sum |score' - score| in FP64: one partial per
-//     workgroup, summed in workgroup order by the last workgroup, which also
-//     takes the stopping decision (residual < tol or max_iters reached).
+//     workgroup, summed in workgroup order by a one-workgroup controller
+//     kernel, which also finishes the chunked hub vertices and takes the
+//     stopping decision (residual < tol or max_iters reached).
 // Scores start at 1/N, as pr_pull_cpu does (src/kernels/cpu/pr.cuh:38-42).
 #include <algorithm>
 #include <cmath>
@@ -51,8 +52,8 @@ __device__ __forceinline__ KSum row_ksum(const GraphView<OffT> &g,
     float sum = 0.0f, comp = 0.0f;
     OffT i = b + OffT(lane);
     for (; i + OffT(3 * G) < e; i += OffT(4 * G)) {
-        float x0 = contrib[g.adj[i]], x1 = contrib[g.adj[i + G]],
-              x2 = contrib[g.adj[i + 2 * G]], x3 = contrib[g.adj[i + 3 * G]];
+        float x0 = contrib[ldnt(&g.adj[i])], x1 = contrib[ldnt(&g.adj[i + G])],
+              x2 = contrib[ldnt(&g.adj[i + 2 * G])], x3 = contrib[ldnt(&g.adj[i + 3 * G])];
         float xs[4] = {x0, x1, x2, x3};
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -63,7 +64,7 @@ __device__ __forceinline__ KSum row_ksum(const GraphView<OffT> &g,
         }
     }
     for (; i < e; i += OffT(G)) {
-        float y = contrib[g.adj[i]] - comp;
+        float y = contrib[ldnt(&g.adj[i])] - comp;
         float t = sum + y;
         comp = (t - sum) - y;
         sum = t;
@@ -109,7 +110,7 @@ struct PrBufs {
 };
 
 struct PrCtl {
-    uint32_t blocks_done;
+    uint32_t unused;
     int32_t done;
     int32_t iter;
     int32_t max_iters;
@@ -117,6 +118,8 @@ struct PrCtl {
     double residual;
     double alg_bytes;
     double epoch_bytes;
+    long long t_start;  // device wall clock at init
+    long long t_end;    // and at the end of the last iteration
 };
 
 struct PrEpoch {
@@ -175,24 +178,10 @@ __device__ double pr_sweep(const PlanDev &p, const GraphView<OffT> &g,
             HubChunk h = p.chunks[k];
             KSum s = block_ksum(row_ksum<kBlock>(g, ep.c_in, OffT(h.b), OffT(h.e),
                                                  threadIdx.x));
+            // Chunk partials are combined by the controller kernel.
             if (threadIdx.x == 0) {
                 b.hub_hi[k] = s.hi;
                 b.hub_lo[k] = s.lo;
-                __threadfence();
-                const int32_t c0 = p.hub_chunk_start[h.hub];
-                const int32_t c1 = p.hub_chunk_start[h.hub + 1];
-                uint32_t t = atomicAdd(&b.hub_cnt[h.hub], 1u);
-                if (t == uint32_t(c1 - c0 - 1)) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    KSum tot{0.0f, 0.0f};
-                    for (int32_t j = c0; j < c1; j++) {
-                        KSum part{__hip_atomic_load(&b.hub_hi[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                                  __hip_atomic_load(&b.hub_lo[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
-                        tot = two_sum_combine(tot, part);
-                    }
-                    b.hub_resid[h.hub] = pr_finalize(ep, b.inv_deg, h.u, tot);
-                    b.hub_cnt[h.hub] = 0;
-                }
             }
         } else if (c == 1) {
             int32_t u = p.list[1] ? p.list[1][k] : p.beg[1] + k;
@@ -222,25 +211,51 @@ __global__ __launch_bounds__(kBlock) void pr_step_kernel(
     PrEpoch ep{odd ? s1 : s0, odd ? c1 : c0, odd ? s0 : s1, odd ? c0 : c1, base, damp};
     double r = block_sum_f64(pr_sweep(p, g, ep, b));
     if (threadIdx.x == 0) b.partials[blockIdx.x] = r;
-    if (last_block_done(&ctl->blocks_done)) {
-        // Fixed-order total: workgroup partials, then hub residuals.
-        __shared__ double s_tot[kBlock];
-        double t = 0.0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) t += b.partials[i];
-        for (int i = threadIdx.x; i < p.num_hubs; i += kBlock) t += b.hub_resid[i];
-        s_tot[threadIdx.x] = t;
+}
+
+// Hub vertices: combine their chunk partials in chunk order and finish
+// them; then the fixed-order residual total (workgroup partials, then hub
+// residuals).  One workgroup of kBlock threads; returns the total in every
+// thread.
+__device__ double pr_hubs_and_residual(const PlanDev &p, const PrEpoch &ep,
+                                       const PrBufs &b, int nparts) {
+    for (int h = threadIdx.x; h < p.num_hubs; h += kBlock) {
+        const int32_t c0 = p.hub_chunk_start[h], c1 = p.hub_chunk_start[h + 1];
+        KSum tot{0.0f, 0.0f};
+        for (int32_t j = c0; j < c1; j++) tot = two_sum_combine(tot, KSum{b.hub_hi[j], b.hub_lo[j]});
+        b.hub_resid[h] = pr_finalize(ep, b.inv_deg, p.chunks[c0].u, tot);
+    }
+    __syncthreads();
+    __shared__ double s_tot[kBlock];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) t += b.partials[i];
+    for (int i = threadIdx.x; i < p.num_hubs; i += kBlock) t += b.hub_resid[i];
+    s_tot[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) s_tot[threadIdx.x] += s_tot[threadIdx.x + s];
         __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) s_tot[threadIdx.x] += s_tot[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            ctl->residual = s_tot[0];
-            ctl->iter = it + 1;
-            ctl->alg_bytes += ctl->epoch_bytes;
-            if (s_tot[0] < ctl->tol || it + 1 >= ctl->max_iters) ctl->done = 1;
-            ctl->blocks_done = 0;
-        }
+    }
+    double r = s_tot[0];
+    __syncthreads();
+    return r;
+}
+
+// Controller of the iteration loop (one workgroup).
+__global__ __launch_bounds__(kBlock) void pr_ctl_kernel(
+    PlanDev p, float *s0, float *s1, float *c0, float *c1, PrBufs b, PrCtl *ctl,
+    float base, float damp, int nparts) {
+    if (ctl->done) return;
+    const int it = ctl->iter;
+    const bool odd = it & 1;
+    PrEpoch ep{odd ? s1 : s0, odd ? c1 : c0, odd ? s0 : s1, odd ? c0 : c1, base, damp};
+    const double r = pr_hubs_and_residual(p, ep, b, nparts);
+    if (threadIdx.x == 0) {
+        ctl->residual = r;
+        ctl->iter = it + 1;
+        ctl->alg_bytes += ctl->epoch_bytes;
+        ctl->t_end = wall_clock64();
+        if (r < ctl->tol || it + 1 >= ctl->max_iters) ctl->done = 1;
     }
 }
 
@@ -259,8 +274,9 @@ __global__ __launch_bounds__(kBlock) void pr_init_kernel(
     }
     for (int h = blockIdx.x * kBlock + threadIdx.x; h < num_hubs; h += stride) hub_cnt[h] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->blocks_done = 0;
         ctl->done = max_iters <= 0 ? 1 : 0;
+        ctl->t_start = wall_clock64();
+        ctl->t_end = ctl->t_start;
         ctl->iter = 0;
         ctl->max_iters = max_iters;
         ctl->tol = tol;
@@ -270,21 +286,20 @@ __global__ __launch_bounds__(kBlock) void pr_init_kernel(
     }
 }
 
-// Epoch API: one sweep over a plan, residual accumulated atomically.
+// Epoch API: one sweep over a plan; its finish kernel completes the hubs and
+// adds the fixed-order residual to *residual_out.
 template <typename OffT>
-__global__ __launch_bounds__(kBlock) void pr_epoch_kernel(
-    PlanDev p, GraphView<OffT> g, PrEpoch ep, PrBufs b, double *residual_out) {
+__global__ __launch_bounds__(kBlock) void pr_epoch_kernel(PlanDev p, GraphView<OffT> g,
+                                                          PrEpoch ep, PrBufs b) {
     double r = block_sum_f64(pr_sweep(p, g, ep, b));
-    if (threadIdx.x == 0 && residual_out) atomicAdd(residual_out, r);
+    if (threadIdx.x == 0) b.partials[blockIdx.x] = r;
 }
 
-template <typename OffT>
-__global__ void pr_hub_resid_kernel(const double *hub_resid, int32_t n, double *out) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && out) {
-        double t = 0;
-        for (int i = 0; i < n; i++) t += hub_resid[i];
-        atomicAdd(out, t);
-    }
+__global__ __launch_bounds__(kBlock) void pr_epoch_finish_kernel(PlanDev p, PrEpoch ep,
+                                                                 PrBufs b, int nparts,
+                                                                 double *residual_out) {
+    const double r = pr_hubs_and_residual(p, ep, b, nparts);
+    if (threadIdx.x == 0 && residual_out) *residual_out += r;
 }
 
 // ---------------------------------------------------------------------------
@@ -314,7 +329,7 @@ void free_pr_ws(hc_graph_s *g) {
 
 static int get_pr_ws(hc_graph_s *g, int32_t chunks, int32_t hubs, PrWs **out) {
     auto *ws = static_cast<PrWs *>(g->ws_pr);
-    const int32_t parts = grid_blocks(8);
+    const int32_t parts = grid_blocks();
     if (ws && ws->chunk_cap >= chunks && ws->hub_cap >= hubs && ws->part_cap >= parts) {
         *out = ws;
         return HC_OK;
@@ -376,11 +391,11 @@ template <typename OffT>
 static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
                   int max_iters, hc_run_stats_t *stats, hipStream_t s) {
     const PlanHost *ph = nullptr;
-    HC_TRY(get_plan(g, 0, g->n, PlanConfig{}, &ph));
+    HC_TRY(get_plan(g, 0, g->n, tuning().pull, &ph));
     PrWs *ws = nullptr;
     HC_TRY(get_pr_ws(g, ph->dev.cnt[0], ph->dev.num_hubs, &ws));
     GraphView<OffT> v = view<OffT>(g);
-    const int grid = grid_blocks(8);
+    const int grid = grid_blocks();
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     const float base = (1.0f - damp) / (float)g->n;
     pr_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
@@ -392,10 +407,13 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     HC_HIP(hipEventCreate(&e0));
     HC_HIP(hipEventCreate(&e1));
     HC_HIP(hipEventRecord(e0, s));
-    for (int i = 0; i < max_iters; i++)
+    for (int i = 0; i < max_iters; i++) {
         pr_step_kernel<OffT><<<grid, kBlock, 0, s>>>(ph->dev, v, ws->s[0], ws->s[1],
                                                      ws->c[0], ws->c[1], b, ws->ctl,
                                                      base, damp);
+        pr_ctl_kernel<<<1, kBlock, 0, s>>>(ph->dev, ws->s[0], ws->s[1], ws->c[0],
+                                           ws->c[1], b, ws->ctl, base, damp, grid);
+    }
     HC_HIP(hipGetLastError());
     HC_HIP(hipEventRecord(e1, s));
     HC_HIP(hipEventSynchronize(e1));
@@ -411,7 +429,8 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     HC_HIP(hipStreamSynchronize(s));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
-        stats->ms = ms;
+        (void)ms;
+        stats->ms = double(c.t_end - c.t_start) / device_state().clock_khz;
         stats->epochs = c.iter;
         stats->pull_epochs = c.iter;
         stats->launches = max_iters;
@@ -465,22 +484,18 @@ int hc_pr_pull_epoch(hc_graph_t g, int32_t start_id, int32_t end_id,
     std::lock_guard<std::mutex> lk(g->mu);
     hipStream_t s = pick_stream(stream);
     const PlanHost *ph = nullptr;
-    HC_TRY(get_plan(g, start_id, end_id, PlanConfig{}, &ph));
+    HC_TRY(get_plan(g, start_id, end_id, tuning().pull, &ph));
     PrWs *ws = nullptr;
     HC_TRY(get_pr_ws(g, ph->dev.cnt[0], ph->dev.num_hubs, &ws));
-    if (ph->dev.num_hubs > 0)
-        HC_HIP(hipMemsetAsync(ws->hub_cnt, 0, sizeof(uint32_t) * ph->dev.num_hubs, s));
     PrBufs b{inv_degree, ws->hub_hi, ws->hub_lo, ws->hub_cnt, ws->hub_resid, ws->partials};
     PrEpoch ep{score_in, contrib, score_out, contrib_out,
                (1.0f - damping) / (float)g->n, damping};
-    const int nb = std::max(1, std::min(std::max(ph->total_units, 1), grid_blocks(8)));
-    if (g->off64) {
-        pr_epoch_kernel<uint64_t><<<nb, kBlock, 0, s>>>(ph->dev, view<uint64_t>(g), ep, b, residual_out);
-        pr_hub_resid_kernel<uint64_t><<<1, 64, 0, s>>>(ws->hub_resid, ph->dev.num_hubs, residual_out);
-    } else {
-        pr_epoch_kernel<uint32_t><<<nb, kBlock, 0, s>>>(ph->dev, view<uint32_t>(g), ep, b, residual_out);
-        pr_hub_resid_kernel<uint32_t><<<1, 64, 0, s>>>(ws->hub_resid, ph->dev.num_hubs, residual_out);
-    }
+    const int nb = std::max(1, std::min(std::max(ph->total_units, 1), grid_blocks()));
+    if (g->off64)
+        pr_epoch_kernel<uint64_t><<<nb, kBlock, 0, s>>>(ph->dev, view<uint64_t>(g), ep, b);
+    else
+        pr_epoch_kernel<uint32_t><<<nb, kBlock, 0, s>>>(ph->dev, view<uint32_t>(g), ep, b);
+    pr_epoch_finish_kernel<<<1, kBlock, 0, s>>>(ph->dev, ep, b, nb, residual_out);
     HC_HIP(hipGetLastError());
     return HC_OK;
 }

@@ -22,7 +22,51 @@ int ensure_init() {
     return hc_init(0);
 }
 
-int grid_blocks(int per_cu) { return std::max(1, device_state().cus * per_cu); }
+Tuning &tuning() {
+    static Tuning t;
+    return t;
+}
+
+int grid_blocks(int per_cu) {
+    if (per_cu <= 0) per_cu = tuning().grid_per_cu;
+    return std::max(1, device_state().cus * per_cu);
+}
+
+namespace {
+int64_t g_knob_tmp;
+int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
+    Tuning &t = tuning();
+    *is_int = false;
+    *iref = nullptr;
+    struct {
+        const char *k;
+        int64_t *p;
+    } i64[] = {
+        {"pull.hub_threshold", &t.pull.hub_threshold},
+        {"pull.hub_chunk", &t.pull.hub_chunk},
+        {"pull.block_threshold", &t.pull.block_threshold},
+        {"pull.wave_threshold", &t.pull.wave_threshold},
+        {"pull.g16_threshold", &t.pull.g16_threshold},
+        {"pull.g4_threshold", &t.pull.g4_threshold},
+        {"bu.wave_threshold", &t.bu.wave_threshold},
+        {"bu.g16_threshold", &t.bu.g16_threshold},
+        {"bu.g4_threshold", &t.bu.g4_threshold},
+    };
+    for (auto &x : i64)
+        if (!strcmp(key, x.k)) return x.p;
+    struct {
+        const char *k;
+        int *p;
+    } i32[] = {{"grid_per_cu", &t.grid_per_cu}, {"sssp.alpha", &t.sssp_alpha}};
+    for (auto &x : i32)
+        if (!strcmp(key, x.k)) {
+            *is_int = true;
+            *iref = x.p;
+            return &g_knob_tmp;
+        }
+    return nullptr;
+}
+}  // namespace
 
 hipStream_t pick_stream(hc_stream_t s) {
     return s ? static_cast<hipStream_t>(s) : device_state().stream;
@@ -293,6 +337,11 @@ int hc_init(int device) {
     HC_HIP(hipGetDeviceProperties(&prop, device));
     s.device = device;
     s.cus = prop.multiProcessorCount;
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            s.clock_khz = khz;
+    }
     s.hbm = prop.totalGlobalMem;
     if (!s.stream) HC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     s.ready = true;
@@ -375,7 +424,7 @@ int hc_memset_d32(void *dst, int32_t value, size_t count, hc_stream_t stream) {
     HC_CHECK_ARG(dst, "dst is NULL");
     HC_TRY(ensure_init());
     hipStream_t s = pick_stream(stream);
-    int blocks = (int)std::min<size_t>((count + 255) / 256, size_t(grid_blocks(8)));
+    int blocks = (int)std::min<size_t>((count + 255) / 256, size_t(grid_blocks()));
     fill_i32_kernel<<<blocks, 256, 0, s>>>(static_cast<int32_t *>(dst), value, count);
     HC_HIP(hipGetLastError());
     HC_HIP(hipStreamSynchronize(s));
@@ -424,6 +473,39 @@ int hc_graph_free(hc_graph_t g) {
     hipFree(g->d_adj);
     hipFree(g->d_wt);
     delete g;
+    return HC_OK;
+}
+
+int hc_set_tuning(const char *key, int64_t value) {
+    HC_CHECK_ARG(key, "key is NULL");
+    bool is_int = false;
+    int *iref = nullptr;
+    int64_t *p = knob_ref(key, &is_int, &iref);
+    if (!p) return fail(HC_ERR_NOT_FOUND, "unknown tuning key '%s'", key);
+    if (is_int) {
+        HC_CHECK_ARG(value >= 1 && value <= 1 << 20, "%s out of range", key);
+        *iref = (int)value;
+    } else {
+        HC_CHECK_ARG(value >= 0, "%s must be >= 0", key);
+        if (!strcmp(key, "pull.hub_chunk"))
+            HC_CHECK_ARG(value >= 256, "pull.hub_chunk must be >= 256");
+        *p = value;
+    }
+    return HC_OK;
+}
+
+int hc_get_tuning(const char *key, int64_t *value) {
+    HC_CHECK_ARG(key && value, "NULL argument");
+    bool is_int = false;
+    int *iref = nullptr;
+    int64_t *p = knob_ref(key, &is_int, &iref);
+    if (!p) return fail(HC_ERR_NOT_FOUND, "unknown tuning key '%s'", key);
+    *value = is_int ? *iref : *p;
+    return HC_OK;
+}
+
+int hc_reset_tuning(void) {
+    tuning() = Tuning();
     return HC_OK;
 }
 

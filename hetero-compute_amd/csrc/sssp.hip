@@ -10,10 +10,10 @@
 //  * Push epoch: relax the out-edges of the active frontier with atomicMin and
 //    append improved vertices (deduplicated through a bitmap) to the next
 //    frontier, wavefront-aggregated.
-//  * Device-driven loop: one launch per epoch; the last workgroup of each
-//    launch picks the next direction (pull when the frontier's edges exceed
-//    m/alpha) and stops the loop when nothing changed, so the host only
-//    enqueues launches.
+//  * Device-driven loop (step / control / compact launches per epoch, see
+//    frontier.h): the controller picks the next direction (pull when the
+//    frontier's degree sum exceeds m/alpha) and stops the loop when nothing
+//    changed, so the host only enqueues launches.
 #include <algorithm>
 #include <cstring>
 
@@ -34,15 +34,15 @@ __device__ __forceinline__ int32_t row_min(const GraphView<OffT> &g,
     int32_t m = HC_INF_DIST;
     OffT i = b + OffT(lane);
     for (; i + OffT(3 * G) < e; i += OffT(4 * G)) {
-        int32_t v0 = g.adj[i], v1 = g.adj[i + G], v2 = g.adj[i + 2 * G],
-                v3 = g.adj[i + 3 * G];
-        int32_t w0 = g.wt[i], w1 = g.wt[i + G], w2 = g.wt[i + 2 * G],
-                w3 = g.wt[i + 3 * G];
+        int32_t v0 = ldnt(&g.adj[i]), v1 = ldnt(&g.adj[i + G]),
+                v2 = ldnt(&g.adj[i + 2 * G]), v3 = ldnt(&g.adj[i + 3 * G]);
+        int32_t w0 = ldnt(&g.wt[i]), w1 = ldnt(&g.wt[i + G]),
+                w2 = ldnt(&g.wt[i + 2 * G]), w3 = ldnt(&g.wt[i + 3 * G]);
         int32_t d0 = dist[v0], d1 = dist[v1], d2 = dist[v2], d3 = dist[v3];
         m = min(m, min(min(relax(d0, w0), relax(d1, w1)),
                        min(relax(d2, w2), relax(d3, w3))));
     }
-    for (; i < e; i += OffT(G)) m = min(m, relax(dist[g.adj[i]], g.wt[i]));
+    for (; i < e; i += OffT(G)) m = min(m, relax(dist[ldnt(&g.adj[i])], ldnt(&g.wt[i])));
     return m;
 }
 
@@ -89,14 +89,15 @@ __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
     }
 }
 
-// One pull sweep over the plan's units (grid-stride).  LOOP selects frontier
-// output (device-driven loop) or an update count (epoch API).
+
+// One pull sweep over the plan's units (grid-stride).  LOOP: changed
+// vertices are set in the next-frontier bitmap and counted (first setting
+// only: hub chunks may lower the same vertex twice); otherwise changes are
+// counted for the epoch API.
 template <bool LOOP, typename OffT>
 __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
-                                int32_t *__restrict__ dist,
-                                const FrontierBufs &fb, StepCtl *ctl, int nxt,
-                                uint32_t &local_upd,
-                                unsigned long long &edges_acc) {
+                                int32_t *__restrict__ dist, uint32_t *next_bm,
+                                uint32_t &cnt, unsigned long long &edges_acc) {
     const int total = p.unit_start[kNumClasses];
     for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
         int c = 0;
@@ -143,14 +144,12 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
             pull_group<1>(p, c, k, g, dist, chg, u);
         }
         if (LOOP) {
-            bool first = false;
-            if (chg) {
-                uint32_t bit = 1u << (u & 31);
-                first = !(atomicOr(&fb.bm[nxt][u >> 5], bit) & bit);
+            if (wave_set_bit(next_bm, u, chg)) {
+                cnt++;
+                edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
             }
-            frontier_append(g, fb, ctl, nxt, first, u, edges_acc);
         } else {
-            local_upd += chg ? 1u : 0u;
+            cnt += chg ? 1u : 0u;
         }
     }
 }
@@ -164,10 +163,9 @@ __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
                                            int32_t *__restrict__ dist,
                                            const FrontierBufs &fb, StepCtl *ctl,
                                            int nxt, int32_t u, OffT b, OffT e,
-                                           int lane,
+                                           int lane, uint32_t &cnt,
                                            unsigned long long &edges_acc) {
     const int32_t du = dist[u];
-    if (du >= HC_INF_DIST) return;  // group-uniform
     for (OffT i = b + OffT(lane); i < e; i += OffT(G)) {
         int32_t v = g.adj[i];
         int32_t nd = relax(du, g.wt[i]);
@@ -179,7 +177,11 @@ __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
                 app = !(atomicOr(&fb.bm[nxt][v >> 5], bit) & bit);
             }
         }
-        frontier_append(g, fb, ctl, nxt, app, v, edges_acc);
+        if (app) {
+            cnt++;
+            edges_acc += (unsigned long long)(g.rend(v) - g.rbeg(v));
+        }
+        frontier_append(g, fb, ctl, nxt, app, v);
     }
 }
 
@@ -187,7 +189,8 @@ template <typename OffT>
 __device__ void sssp_push_sweep(const GraphView<OffT> &g,
                                 int32_t *__restrict__ dist,
                                 const FrontierBufs &fb, StepCtl *ctl, int cur,
-                                int nxt, unsigned long long &edges_acc) {
+                                int nxt, uint32_t &cnt,
+                                unsigned long long &edges_acc) {
     const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
                    n1 = ctl->cnt[cur][1], n2 = ctl->cnt[cur][2];
     const uint32_t u1 = nch + n0;
@@ -197,24 +200,24 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
             push_range<kBlock>(g, dist, fb, ctl, nxt, h.u, OffT(h.b), OffT(h.e),
-                               threadIdx.x, edges_acc);
+                               threadIdx.x, cnt, edges_acc);
         } else if (unit < u1) {
             int32_t u = fb.q[cur][0][unit - nch];
             push_range<kBlock>(g, dist, fb, ctl, nxt, u, g.rbeg(u), g.rend(u),
-                               threadIdx.x, edges_acc);
+                               threadIdx.x, cnt, edges_acc);
         } else if (unit < u2) {
             uint32_t idx = (unit - u1) * kWavesPerBlock + threadIdx.x / kWave;
             if (idx < n1) {
                 int32_t u = fb.q[cur][1][idx];
                 push_range<kWave>(g, dist, fb, ctl, nxt, u, g.rbeg(u), g.rend(u),
-                                  lane_id(), edges_acc);
+                                  lane_id(), cnt, edges_acc);
             }
         } else {
             uint32_t idx = (unit - u2) * kSmallPerUnit + threadIdx.x / kSmallLanes;
             if (idx < n2) {
                 int32_t u = fb.q[cur][2][idx];
                 push_range<kSmallLanes>(g, dist, fb, ctl, nxt, u, g.rbeg(u),
-                                        g.rend(u), threadIdx.x % kSmallLanes,
+                                        g.rend(u), threadIdx.x % kSmallLanes, cnt,
                                         edges_acc);
             }
         }
@@ -222,64 +225,66 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
 }
 
 // ---------------------------------------------------------------------------
-// Controller (runs in thread 0 of the last workgroup of a step)
-// ---------------------------------------------------------------------------
-
-__device__ void sssp_controller(StepCtl *c, int ran_mode, long long sweep_bytes) {
-    const int k = c->epoch, cur = k & 1, nxt = cur ^ 1;
-    uint32_t out_n = 0;
-    for (int j : {0, 1, 2, 4}) out_n += load_acquire_u32(&c->cnt[nxt][j]);
-    const unsigned long long out_e = load_acquire_u64(
-        reinterpret_cast<const uint64_t *>(&c->edges[nxt]));
-    if (ran_mode == kPush) {
-        uint32_t in_n = c->cnt[cur][0] + c->cnt[cur][1] + c->cnt[cur][2] + c->cnt[cur][4];
-        // offsets + dist[u] per frontier vertex; idx + weight + dist[v] per
-        // edge; dist write + queue entry per improved vertex.
-        c->alg_bytes += 12ull * in_n + 12ull * c->edges[cur] + 8ull * out_n;
-        c->push_epochs++;
-    } else {
-        c->alg_bytes += (unsigned long long)sweep_bytes + 8ull * out_n;
-        c->pull_epochs++;
-    }
-    for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
-    c->edges[cur] = 0;
-    c->epoch = k + 1;
-    if (out_n == 0 || (c->max_epochs > 0 && k + 1 >= c->max_epochs)) {
-        c->done = 1;
-    } else if (c->direction == 1) {
-        c->mode = kPull;
-    } else if (c->direction == 2) {
-        c->mode = kPush;
-    } else {
-        c->mode = (long long)out_e > c->m / c->alpha ? kPull : kPush;
-    }
-    c->blocks_done = 0;
-}
-
-// ---------------------------------------------------------------------------
-// Kernels
+// Kernels of the device-driven loop
 // ---------------------------------------------------------------------------
 
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, FrontierBufs fb,
-    StepCtl *ctl, long long sweep_bytes) {
+    StepCtl *ctl) {
     if (ctl->done) return;
     const int mode = ctl->mode;
     const int cur = ctl->epoch & 1, nxt = cur ^ 1;
+    // The bitmap of parity cur deduplicated the frontier this epoch reads;
+    // it is not used again until it becomes the output of epoch k+1.
     for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words;
          w += gridDim.x * kBlock)
         fb.bm[cur][w] = 0;
-    unsigned long long eacc = 0;
-    uint32_t upd = 0;
+    uint32_t cnt = 0;
+    unsigned long long eacc = 0, scanned = 0;
     if (mode == kPull)
-        sssp_pull_sweep<true>(p, g, dist, fb, ctl, nxt, upd, eacc);
+        sssp_pull_sweep<true>(p, g, dist, fb.bm[nxt], cnt, eacc);
     else
-        sssp_push_sweep(g, dist, fb, ctl, cur, nxt, eacc);
-    unsigned long long es = block_sum_u64(eacc);
-    if (threadIdx.x == 0 && es) atomicAdd(&ctl->edges[nxt], es);
-    if (last_block_done(&ctl->blocks_done) && threadIdx.x == 0)
-        sssp_controller(ctl, mode, sweep_bytes);
+        sssp_push_sweep(g, dist, fb, ctl, cur, nxt, cnt, eacc);
+    write_partials(fb.part, cnt, eacc, scanned);
+}
+
+// Epoch controller: one workgroup.
+__global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb,
+                                                          StepCtl *c) {
+    if (c->done) return;
+    unsigned long long out_n, out_e, unused;
+    sum_partials(fb.part, c->grid, out_n, out_e, unused);
+    if (threadIdx.x != 0) return;
+    const int k = c->epoch, cur = k & 1;
+    const int ran = c->mode;
+    if (ran == kPush) {
+        // offsets + dist[u] per frontier vertex; id + weight + dist[v] per
+        // edge; dist write + bitmap/queue entry per improved vertex.
+        c->alg_bytes += 12ull * c->in_n + 12ull * c->in_e + 8ull * out_n;
+        c->push_epochs++;
+        trace_epoch(c, k, ran, out_n, out_e, c->in_e);
+    } else {
+        c->alg_bytes += (unsigned long long)c->sweep_bytes + 8ull * out_n;
+        c->pull_epochs++;
+        trace_epoch(c, k, ran, out_n, out_e, (unsigned long long)c->m);
+    }
+    for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
+    c->epoch = k + 1;
+    c->in_n = (long long)out_n;
+    c->in_e = (long long)out_e;
+    int next = ran;
+    if (out_n == 0 || (c->max_epochs > 0 && k + 1 >= c->max_epochs)) {
+        c->done = 1;
+    } else if (c->direction == 1) {
+        next = kPull;
+    } else if (c->direction == 2) {
+        next = kPush;
+    } else {
+        next = (long long)out_e > c->m / c->alpha ? kPull : kPush;
+    }
+    c->need_compact = (!c->done && ran == kPull && next == kPush) ? 1 : 0;
+    c->mode = next;
 }
 
 // Initialises distances, bitmaps and the control block for one run.
@@ -289,7 +294,8 @@ template <typename OffT>
 __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
     GraphView<OffT> g, int32_t *__restrict__ dist,
     const int32_t *__restrict__ init_dist, FrontierBufs fb, StepCtl *ctl,
-    int32_t src, int direction, int alpha, int max_epochs, long long m) {
+    int32_t src, int direction, int alpha, int max_epochs, long long m,
+    long long sweep_bytes, int grid) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride)
         dist[i] = init_dist ? init_dist[i] : (i == src ? 0 : HC_INF_DIST);
@@ -298,27 +304,32 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
         fb.bm[1][w] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < 2; p++)
             for (int j = 0; j < 5; j++) ctl->cnt[p][j] = 0;
-            ctl->edges[p] = 0;
-        }
-        ctl->blocks_done = 0;
         ctl->done = 0;
         ctl->epoch = 0;
         ctl->max_epochs = max_epochs;
         ctl->direction = direction;
         ctl->alpha = alpha;
-        ctl->m = m;
+        ctl->need_compact = 0;
         ctl->n = g.n;
+        ctl->grid = grid;
+        ctl->m = m;
+        ctl->sweep_bytes = sweep_bytes;
         ctl->push_epochs = ctl->pull_epochs = 0;
         ctl->reached = 0;
         ctl->alg_bytes = 0;
+        ctl->t_start = wall_clock64();
+        ctl->t_end = ctl->t_start;
         if (init_dist) {
             ctl->mode = kPull;
+            ctl->in_n = g.n;
+            ctl->in_e = m;
         } else {
             OffT b = g.rbeg(src), e = g.rend(src);
             int64_t d = int64_t(e - b);
-            if (d > kHubDeg) {
+            int cls = degree_class(d);
+            if (cls == 3) {
                 uint32_t nch = 0;
                 for (int64_t x = b; x < int64_t(e); x += kChunkEdges, nch++) {
                     int64_t ce = x + kChunkEdges < int64_t(e) ? x + kChunkEdges : int64_t(e);
@@ -327,11 +338,11 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
                 ctl->cnt[0][3] = nch;
                 ctl->cnt[0][4] = 1;
             } else {
-                int cls = d > kBigDeg ? 0 : (d > kMidDeg ? 1 : 2);
                 fb.q[0][cls][0] = src;
                 ctl->cnt[0][cls] = 1;
             }
-            ctl->edges[0] = (unsigned long long)d;
+            ctl->in_n = 1;
+            ctl->in_e = d;
             ctl->mode = direction == 1 ? kPull
                         : direction == 2 ? kPush
                         : (d > m / alpha ? kPull : kPush);
@@ -339,8 +350,6 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
     }
 }
 
-// Epoch-API kernels with the reference's launch shapes (thread, wavefront
-// and workgroup per vertex, grid-stride over [start, end)).
 template <typename OffT>
 __global__ void sssp_pull_thread_kernel(GraphView<OffT> g,
                                         int32_t *__restrict__ dist,
@@ -404,8 +413,7 @@ __global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
     int32_t *updated) {
     uint32_t upd = 0;
     unsigned long long eacc = 0;
-    FrontierBufs fb{};
-    sssp_pull_sweep<false>(p, g, dist, fb, nullptr, 0, upd, eacc);
+    sssp_pull_sweep<false>(p, g, dist, nullptr, upd, eacc);
     unsigned long long s = block_sum_u64(upd);
     if (threadIdx.x == 0 && s) atomicAdd(updated, (int32_t)s);
 }
@@ -413,6 +421,7 @@ __global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+
 
 struct SsspWs {
     FrontierWs f;
@@ -427,6 +436,9 @@ void free_sssp_ws(hc_graph_s *g) {
 }
 
 static int get_sssp_ws(hc_graph_s *g, SsspWs **out) {
+    // The partial slots are sized for the launch grid; re-allocate when the
+    // tuned grid changes.
+    if (g->ws_sssp && static_cast<SsspWs *>(g->ws_sssp)->f.grid != grid_blocks()) free_sssp_ws(g);
     if (!g->ws_sssp) {
         auto *ws = new SsspWs();
         int rc = frontier_ws_alloc(g, &ws->f);
@@ -459,24 +471,24 @@ static int launch_pull_epoch(hc_graph_s *g, int mapping, int32_t start,
     switch (mapping) {
         case HC_MAP_THREAD:
             if (!threads) threads = 256;
-            if (!blocks) blocks = std::max(1, std::min(grid_blocks(8), (end - start + threads - 1) / threads));
+            if (!blocks) blocks = std::max(1, std::min(grid_blocks(), (end - start + threads - 1) / threads));
             sssp_pull_thread_kernel<OffT><<<blocks, threads, 0, s>>>(v, dist, start, end, updated);
             break;
         case HC_MAP_WAVE:
             if (!threads) threads = 256;
-            if (!blocks) blocks = grid_blocks(8);
+            if (!blocks) blocks = grid_blocks();
             sssp_pull_wave_kernel<OffT><<<blocks, threads, 0, s>>>(v, dist, start, end, updated);
             break;
         case HC_MAP_BLOCK:
             if (!threads) threads = 256;
-            if (!blocks) blocks = grid_blocks(8);
+            if (!blocks) blocks = grid_blocks();
             sssp_pull_block_kernel<OffT><<<blocks, threads, 0, s>>>(v, dist, start, end, updated);
             break;
         default: {
             const PlanHost *ph = nullptr;
-            HC_TRY(get_plan(g, start, end, PlanConfig{}, &ph));
+            HC_TRY(get_plan(g, start, end, tuning().pull, &ph));
             if (ph->total_units == 0) return HC_OK;
-            int nb = std::max(1, std::min(ph->total_units, grid_blocks(8)));
+            int nb = std::max(1, std::min(ph->total_units, grid_blocks()));
             sssp_pull_fused_kernel<OffT><<<nb, kBlock, 0, s>>>(ph->dev, v, dist, updated);
         }
     }
@@ -495,6 +507,26 @@ static int check_launch(int mapping, int blocks, int threads) {
     return HC_OK;
 }
 
+
+template <typename OffT>
+struct SsspLaunch {
+    PlanDev plan;
+    GraphView<OffT> g;
+    int32_t *dist;
+    FrontierBufs fb;
+    StepCtl *ctl;
+    int grid;
+    hipStream_t s;
+    static int launch(void *arg) {
+        auto *a = static_cast<SsspLaunch *>(arg);
+        sssp_step_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->plan, a->g, a->dist, a->fb, a->ctl);
+        sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
+        compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 0);
+        if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "SSSP epoch launch failed");
+        return HC_OK;
+    }
+};
+
 template <typename OffT>
 static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
                          int32_t *dist, int direction, int alpha,
@@ -502,73 +534,36 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     SsspWs *ws = nullptr;
     HC_TRY(get_sssp_ws(g, &ws));
     const PlanHost *ph = nullptr;
-    HC_TRY(get_plan(g, 0, g->n, PlanConfig{}, &ph));
+    HC_TRY(get_plan(g, 0, g->n, tuning().pull, &ph));
     GraphView<OffT> v = view<OffT>(g);
     const long long sweep = sssp_sweep_bytes(g, ph);
-    const int grid = grid_blocks(8);
+    const int grid = ws->f.grid;
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
-        max_epochs, (long long)g->m);
+        max_epochs, (long long)g->m, sweep, grid);
     HC_HIP(hipGetLastError());
-    hipEvent_t ev0, ev1, evb[2];
-    HC_HIP(hipEventCreate(&ev0));
-    HC_HIP(hipEventCreate(&ev1));
-    HC_HIP(hipEventCreateWithFlags(&evb[0], hipEventDisableTiming));
-    HC_HIP(hipEventCreateWithFlags(&evb[1], hipEventDisableTiming));
-    StepCtl *hslot = ws->f.h_ctl;
-    hslot->done = 0;
-    HC_HIP(hipEventRecord(ev0, s));
-    // Enqueue batches of steps; the host checks batch b-1's control block
-    // while batch b is already queued, so the device never idles on a poll.
+    SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
-    const int batch = 8;
-    const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2 * batch + 2;
-    long long launches = 0;
-    int rc = HC_OK;
-    bool done = false;
-    int b = 0;
-    while (!done && launches < cap) {
-        for (int i = 0; i < batch; i++) {
-            sssp_step_kernel<OffT><<<grid, kBlock, 0, s>>>(ph->dev, v, dist, ws->f.fb,
-                                                           ws->f.ctl, sweep);
-            launches++;
-        }
-        if (hipGetLastError() != hipSuccess) {
-            rc = fail(HC_ERR_HIP, "step launch failed");
-            break;
-        }
-        hipMemcpyAsync(&hslot->done, &ws->f.ctl->done, sizeof(int32_t),
-                       hipMemcpyDeviceToHost, s);
-        hipEventRecord(evb[b & 1], s);
-        if (b > 0) {
-            hipEventSynchronize(evb[(b - 1) & 1]);
-            done = __atomic_load_n(&hslot->done, __ATOMIC_ACQUIRE) != 0;
-        }
-        b++;
-    }
-    HC_HIP(hipEventRecord(ev1, s));
-    HC_HIP(hipEventSynchronize(ev1));
+    const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;
     float ms = 0;
-    HC_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-    StepCtl final_ctl;
-    HC_HIP(hipMemcpy(&final_ctl, ws->f.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
-    hipEventDestroy(evb[0]);
-    hipEventDestroy(evb[1]);
-    if (rc != HC_OK) return rc;
-    if (!final_ctl.done && max_epochs <= 0)
-        return fail(HC_ERR_HIP, "SSSP loop did not converge in %lld launches", launches);
+    long long launches = 0;
+    StepCtl fc;
+    HC_TRY(run_epoch_loop(&ws->f, s, cap, &SsspLaunch<OffT>::launch, &L, &ms,
+                          &launches, &fc));
+    if (!fc.done && max_epochs <= 0)
+        return fail(HC_ERR_HIP, "SSSP loop did not converge in %lld epochs", launches);
     if (stats) {
-        stats->ms = ms;
-        stats->epochs = final_ctl.epoch;
-        stats->push_epochs = final_ctl.push_epochs;
-        stats->pull_epochs = final_ctl.pull_epochs;
-        stats->launches = (int32_t)launches;
+        std::memset(stats, 0, sizeof(*stats));
+        // Device wall clock from the init kernel to the controller of the
+        // last epoch (launches queued past it are not part of the loop).
+        stats->ms = double(fc.t_end - fc.t_start) / device_state().clock_khz;
+        stats->epochs = fc.epoch;
+        stats->push_epochs = fc.push_epochs;
+        stats->pull_epochs = fc.pull_epochs;
+        stats->launches = (int32_t)(3 * launches);
         stats->reached = -1;
-        stats->residual = 0;
-        stats->alg_bytes = (double)final_ctl.alg_bytes;
+        stats->alg_bytes = (double)fc.alg_bytes;
     }
     return HC_OK;
 }
@@ -607,7 +602,7 @@ int hc_sssp(hc_graph_t g, int32_t source, int32_t *dist,
     HC_CHECK_ARG(source >= 0 && source < g->n, "source %d outside [0,%d)",
                  source, g->n);
     int direction = opts ? opts->direction : 0;
-    int alpha = opts && opts->alpha > 0 ? opts->alpha : 8;
+    int alpha = opts && opts->alpha > 0 ? opts->alpha : tuning().sssp_alpha;
     int max_epochs = opts ? opts->max_epochs : 0;
     HC_CHECK_ARG(direction >= 0 && direction <= 2, "direction must be 0, 1 or 2");
     HC_TRY(ensure_init());

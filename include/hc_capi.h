@@ -186,6 +186,24 @@ int hc_graph_upload_range(const hc_host_graph_t *g, int32_t start_id,
 int hc_graph_free(hc_graph_t g);
 int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
 
+/* Tuning knobs, process-wide (the autotuner's interface; no reference
+ * counterpart -- the reference fixed these per generated driver,
+ * src/kernels/heterogeneous/sssp.cuh:180-278).  Keys:
+ *   grid_per_cu            workgroups per CU of full-device launches (8)
+ *   sssp.alpha             SSSP pull when frontier edges > m/alpha (8)
+ *   pull.hub_threshold     degree above which a row is split into chunks
+ *   pull.hub_chunk         edges per chunk (>= 256)
+ *   pull.block_threshold   degree above which a workgroup takes a row
+ *   pull.wave_threshold    ... a wavefront
+ *   pull.g16_threshold     ... 16 lanes
+ *   pull.g4_threshold      ... 4 lanes (else one lane)
+ *   bu.wave_threshold / bu.g16_threshold / bu.g4_threshold   BFS bottom-up
+ * Plans are rebuilt for new values; changing knobs while another thread
+ * runs on the same graph is not supported. */
+int hc_set_tuning(const char *key, int64_t value);
+int hc_get_tuning(const char *key, int64_t *value);
+int hc_reset_tuning(void);
+
 /* ------------------------------------------------------------------------ */
 /* Epoch kernels (one bulk-synchronous sweep of [start_id, end_id))         */
 /* ------------------------------------------------------------------------ */
@@ -287,6 +305,20 @@ typedef struct hc_pr_opts {
  * start value); scores (device, num_nodes) in FP32. */
 int hc_pagerank(hc_graph_t g, float *scores, const hc_pr_opts_t *opts,
                 hc_run_stats_t *stats, hc_stream_t stream);
+
+/* Per-epoch record of the last hc_sssp (algorithm 0) or hc_bfs (1) run on g:
+ * the direction each epoch ran (0 push / top-down, 1 pull / bottom-up), the
+ * size and degree sum of the frontier it produced and the edges it examined.
+ * No reference counterpart; it exposes what the reference's commented-out
+ * per-iteration prints showed (src/kernels/cpu/bfs.cuh:82-83, 99-100). */
+typedef struct hc_epoch_trace {
+    int32_t count;
+    int8_t mode[128];
+    uint32_t frontier[128];
+    uint64_t frontier_edges[128];
+    uint64_t scanned[128];
+} hc_epoch_trace_t;
+int hc_get_epoch_trace(hc_graph_t g, int algorithm, hc_epoch_trace_t *out);
 
 #ifdef __cplusplus
 }

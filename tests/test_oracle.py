@@ -1,0 +1,266 @@
+"""Pins of the CPU oracle (oracle/oracle.c) -- CPU only.
+
+Each oracle function is checked against answers it does not compute itself:
+the known-answer fixtures (scipy Dijkstra / unweighted shortest paths and a
+numpy PageRank iteration plus dense solve, tests/golden/make_golden.py),
+brute force on tiny graphs (Floyd-Warshall, exhaustive first-neighbour
+scans), closed forms (paths, stars, cycles, complete and regular graphs) and
+invariants.  A planted mistake (an extra weight unit, a dropped damping
+term, a wrong parent) must fail the same pins."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import golden_files, load_golden
+
+INF = 0x3FFFFFFF
+
+
+def csr_from_edges(n, edges, weighted=True):
+    """Symmetric CSR from (u, v, w) triples (rows sorted, lightest duplicate
+    kept, no self-loops) built with plain Python, independent of the
+    library's builder."""
+    rows = [dict() for _ in range(n)]
+    for u, v, w in edges:
+        if u == v:
+            continue
+        for a, b in ((u, v), (v, u)):
+            rows[a][b] = min(w, rows[a].get(b, w))
+    index = np.zeros(n + 1, np.int64)
+    nbr, wt = [], []
+    for u in range(n):
+        for v in sorted(rows[u]):
+            nbr.append(v)
+            wt.append(rows[u][v])
+        index[u + 1] = len(nbr)
+    return index, np.array(nbr, np.int32), np.array(wt, np.int32)
+
+
+def floyd(index, nbr, wt):
+    n = index.size - 1
+    d = np.full((n, n), np.inf)
+    np.fill_diagonal(d, 0)
+    for u in range(n):
+        for i in range(index[u], index[u + 1]):
+            v = nbr[i]
+            d[v, u] = min(d[v, u], wt[i])  # edge v -> u (pull row u lists v)
+    for k in range(n):
+        d = np.minimum(d, d[:, [k]] + d[[k], :])
+    return np.where(np.isinf(d), INF, d).astype(np.int64)
+
+
+# ---------------------------------------------------------------------------
+# Fixtures
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", golden_files())
+def test_sssp_oracles_match_fixtures(oracle, name):
+    fx = load_golden(name)
+    idx, nb, w = fx["index"], fx["neighbors"], fx["weights"]
+    for i, s in enumerate(fx["sources"]):
+        s = int(s)
+        want = fx["sssp"][i]
+        assert np.array_equal(oracle.sssp_dijkstra(idx, nb, w, s), want)
+        assert np.array_equal(oracle.sssp_pull_serial(idx, nb, w, s), want)
+        init = np.full(idx.size - 1, INF, np.int32)
+        init[s] = 0
+        d, _, epochs = oracle.sssp_pull_omp(idx, nb, w, init, 4)
+        assert np.array_equal(d, want) and epochs >= 1
+
+
+@pytest.mark.parametrize("name", golden_files())
+def test_bfs_oracles_match_fixtures(oracle, name):
+    fx = load_golden(name)
+    idx, nb = fx["index"], fx["neighbors"]
+    for i, s in enumerate(fx["sources"]):
+        s = int(s)
+        depth = oracle.bfs_depth(idx, nb, s)
+        assert np.array_equal(depth, fx["depth"][i])
+        par, _, dirs = oracle.bfs_do(idx, nb, s, 15, 18, 4)
+        assert oracle.verify_parents(depth, par, s) == 0
+        # every reached level is one epoch: epochs = eccentricity + 1
+        assert len(dirs) == depth.max() + 1
+
+
+@pytest.mark.parametrize("name", golden_files())
+def test_pagerank_oracles_match_fixtures(oracle, name):
+    fx = load_golden(name)
+    idx, nb = fx["index"], fx["neighbors"]
+    p64 = oracle.pr_jacobi_f64(idx, nb, 0.85, 20)
+    assert np.allclose(p64, fx["pr20"], rtol=0, atol=1e-15)
+    p32 = oracle.pr_jacobi_f32(idx, nb, 0.85, 20)
+    assert np.abs(p32 - fx["pr20"]).sum() < 1e-6
+    if fx["prfix"].size:
+        pinf = oracle.pr_jacobi_f64(idx, nb, 0.85, 400)
+        assert np.allclose(pinf, fx["prfix"], rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Brute force and closed forms
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(6))
+def test_sssp_vs_floyd_warshall(oracle, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 40))
+    m = int(rng.integers(0, 3 * n))
+    edges = [(int(rng.integers(n)), int(rng.integers(n)), int(rng.integers(0, 256)))
+             for _ in range(m)]
+    idx, nb, w = csr_from_edges(n, edges)
+    fw = floyd(idx, nb, w)
+    for s in range(0, n, max(1, n // 5)):
+        assert np.array_equal(oracle.sssp_dijkstra(idx, nb, w, s), fw[s])
+        assert np.array_equal(oracle.sssp_pull_serial(idx, nb, w, s), fw[s])
+
+
+def test_sssp_closed_forms(oracle):
+    n = 50
+    # path with weight i+1 on edge (i, i+1): prefix sums
+    idx, nb, w = csr_from_edges(n, [(i, i + 1, i + 1) for i in range(n - 1)])
+    d = oracle.sssp_dijkstra(idx, nb, w, 0)
+    assert np.array_equal(d, np.cumsum(np.r_[0, np.arange(1, n)]))
+    # star: center 0, leaf i weight i; from leaf 3 the distance to leaf j is 3+j
+    idx, nb, w = csr_from_edges(10, [(0, i, i) for i in range(1, 10)])
+    d = oracle.sssp_pull_serial(idx, nb, w, 3)
+    assert d[0] == 3 and all(d[j] == 3 + j for j in range(1, 10) if j != 3) and d[3] == 0
+    # disconnected vertex stays at the sentinel; saturation: INF + w == INF
+    idx, nb, w = csr_from_edges(4, [(0, 1, 255), (1, 2, 255)])
+    d = oracle.sssp_dijkstra(idx, nb, w, 0)
+    assert list(d) == [0, 255, 510, INF]
+
+
+def test_bfs_closed_forms(oracle):
+    n = 13
+    idx, nb, _ = csr_from_edges(n, [(i, (i + 1) % n, 1) for i in range(n)])
+    d = oracle.bfs_depth(idx, nb, 0)
+    assert list(d) == [min(k, n - k) for k in range(n)]
+    idx, nb, _ = csr_from_edges(7, [(i, j, 1) for i in range(7) for j in range(i + 1, 7)])
+    assert list(oracle.bfs_depth(idx, nb, 2)) == [1, 1, 0, 1, 1, 1, 1]
+    idx, nb, _ = csr_from_edges(6, [(0, 1, 1), (2, 3, 1)])
+    assert list(oracle.bfs_depth(idx, nb, 0)) == [0, 1, -1, -1, -1, -1]
+
+
+def test_pagerank_closed_forms(oracle):
+    # regular graphs (cycle, complete): 1/N is the fixed point
+    for edges, n in ([(i, (i + 1) % 17, 1) for i in range(17)], 17), \
+                    ([(i, j, 1) for i in range(6) for j in range(i + 1, 6)], 6):
+        idx, nb, _ = csr_from_edges(n, edges)
+        p = oracle.pr_jacobi_f64(idx, nb, 0.85, 30)
+        assert np.allclose(p, 1.0 / n, atol=1e-15)
+    # star with k leaves: c = b + d*k*l, l = b + d*c/k  (dangling mass 0)
+    k = 7
+    n = k + 1
+    idx, nb, _ = csr_from_edges(n, [(0, i, 1) for i in range(1, n)])
+    p = oracle.pr_jacobi_f64(idx, nb, 0.85, 500)
+    b, d = 0.15 / n, 0.85
+    c = (b + d * k * b) / (1 - d * d)
+    leaf = b + d * c / k
+    assert abs(p[0] - c) < 1e-14 and np.allclose(p[1:], leaf, atol=1e-14)
+
+
+def test_pagerank_gauss_seidel_reaches_jacobi_fixed_point(oracle, hc):
+    g = hc.HostGraph.generate("kron", 9, 8, weighted=False)
+    idx, nb = g.index, g.neighbors
+    gs, sweeps = oracle.pr_gauss_seidel_f32(idx, nb, 1000)
+    fix = oracle.pr_jacobi_f64(idx, nb, 0.85, 300)
+    assert sweeps < 1000
+    # stopping rule is a per-sweep change of 5e-8, so the error is larger
+    assert np.abs(gs - fix).max() < 1e-4 * fix.max()
+
+
+def test_bfs_pull_epoch_first_neighbour(oracle, hc):
+    """epoch_bfs_pull_cpu restatement vs an exhaustive scan."""
+    g = hc.HostGraph.generate("kron", 10, 8, weighted=False)
+    idx, nb = g.index, g.neighbors
+    n = g.num_nodes
+    rng = np.random.default_rng(3)
+    parents = np.where(rng.random(n) < 0.3, rng.integers(0, n, n), -1).astype(np.int32)
+    front_set = rng.random(n) < 0.1
+    words = np.zeros((n + 31) // 32, np.uint32)
+    for v in np.flatnonzero(front_set):
+        words[v >> 5] |= np.uint32(1 << (v & 31))
+    par, nxt, cnt = oracle.bfs_pull_epoch(idx, nb, parents, 100, 900, words)
+    exp = parents.copy()
+    exp_cnt = 0
+    for u in range(100, 900):
+        if parents[u] != -1:
+            continue
+        hits = [v for v in nb[idx[u]:idx[u + 1]] if front_set[v]]
+        if hits:
+            exp[u] = hits[0]
+            exp_cnt += 1
+            assert (nxt[u >> 5] >> (u & 31)) & 1
+    assert cnt == exp_cnt and np.array_equal(par, exp)
+
+
+def test_jacobi_epoch_brackets(oracle, hc):
+    """One Jacobi epoch from d0 is >= the fixpoint and <= d0; iterating it
+    reaches the Dijkstra fixpoint."""
+    g = hc.HostGraph.generate("uniform", 9, 6, weighted=True)
+    idx, nb, w = g.index, g.neighbors, g.weights
+    d = np.full(g.num_nodes, INF, np.int32)
+    d[5] = 0
+    fix = oracle.sssp_dijkstra(idx, nb, w, 5)
+    for _ in range(g.num_nodes):
+        nd, c = oracle.sssp_jacobi_epoch(idx, nb, w, 0, g.num_nodes, d)
+        assert np.all(nd <= d) and np.all(nd >= fix)
+        d = nd
+        if c == 0:
+            break
+    assert np.array_equal(d, fix)
+
+
+# ---------------------------------------------------------------------------
+# Planted mistakes fail the pins
+# ---------------------------------------------------------------------------
+
+def test_planted_mistakes_are_caught(oracle):
+    fx = load_golden("kron10.npz")
+    idx, nb, w = fx["index"], fx["neighbors"], fx["weights"]
+    s = int(fx["sources"][0])
+    # an extra unit on every weight (a mis-restated relax) changes distances
+    assert not np.array_equal(oracle.sssp_dijkstra(idx, nb, w + 1, s), fx["sssp"][0])
+    # a dropped damping factor changes PageRank
+    assert np.abs(oracle.pr_jacobi_f64(idx, nb, 1.0, 20) - fx["pr20"]).sum() > 1e-3
+    # a wrong parent / source / unreached flag is reported
+    depth = fx["depth"][0]
+    par, _, _ = oracle.bfs_do(idx, nb, s, 15, 18, 2)
+    bad = par.copy()
+    u = int(np.flatnonzero(depth == 2)[0])
+    bad[u] = s  # depth 0 parent for a depth-2 vertex
+    assert oracle.verify_parents(depth, bad, s) == 1
+    bad = par.copy()
+    bad[s] = int(np.flatnonzero(depth == 1)[0])
+    assert oracle.verify_parents(depth, bad, s) == 1
+    bad = par.copy()
+    bad[int(np.flatnonzero(depth == 1)[0])] = -1
+    assert oracle.verify_parents(depth, bad, s) == 1
+
+
+def test_equal_edge_ranges_restatement(oracle, hc):
+    """Oracle restatement of compute_equal_edge_ranges vs a direct Python
+    transcription of the same loop (src/graph.cuh:156-201)."""
+    g = hc.HostGraph.generate("kron", 12, 8, weighted=False)
+    idx = g.index
+    for S, chunk in [(24, 1), (8, 32), (5, 7), (1, 1)]:
+        got = oracle.equal_edge_ranges(idx, S, chunk)
+        n = idx.size - 1
+        avg = int(idx[-1]) // S
+        exp = [-1] * (S + 1)
+        exp[0] = 0
+        end = seg = sdeg = 0
+        while end != n:
+            sdeg += int(idx[end + 1] - idx[end])
+            if sdeg >= avg:
+                if seg + 1 <= S:
+                    exp[seg + 1] = end
+                sdeg = 0
+                seg += 1
+            end += 1
+        if seg != S and seg + 1 <= S:
+            exp[seg + 1] = end
+        for i in range(1, S):
+            if exp[i] >= 0 and exp[i] % chunk:
+                exp[i] = (exp[i] // chunk + (exp[i] % chunk >= chunk // 2)) * chunk
+        assert list(got) == exp
