@@ -142,7 +142,7 @@ _SIGS = {
     "hc_reset_tuning": ([], C.c_int),
 }
 
-TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "pull.hub_threshold", "pull.hub_chunk",
+TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "pull.stage", "bu.stage", "pull.hub_threshold", "pull.hub_chunk",
                "pull.block_threshold", "pull.wave_threshold", "pull.g16_threshold",
                "pull.g4_threshold", "bu.wave_threshold", "bu.g16_threshold",
                "bu.g4_threshold")

@@ -23,7 +23,9 @@
 
 namespace hc {
 
-__device__ __forceinline__ bool bit_test(const uint32_t *__restrict__ bm, int32_t v) {
+// Frontier bitmap words, the first ones (the hubs' bits) from an LDS copy;
+// the bitmap is read-only during a bottom-up epoch, so the copy is exact.
+__device__ __forceinline__ bool bit_test(const Staged<uint32_t> &bm, int32_t v) {
     return (bm[v >> 5] >> (v & 31)) & 1u;
 }
 
@@ -31,7 +33,7 @@ __device__ __forceinline__ bool bit_test(const uint32_t *__restrict__ bm, int32_
 // within the wavefront); returns the first neighbour in the frontier.
 template <int G, typename OffT>
 __device__ __forceinline__ bool bu_scan(const GraphView<OffT> &g,
-                                        const uint32_t *__restrict__ frontier,
+                                        const Staged<uint32_t> &frontier,
                                         OffT b, OffT e, int lane, int gbase,
                                         int32_t &parent, uint32_t &scanned) {
     if (G == 1) {
@@ -70,7 +72,7 @@ template <int G, typename OffT>
 __device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
                                          const GraphView<OffT> &g,
                                          const int32_t *__restrict__ visited,
-                                         const uint32_t *__restrict__ frontier,
+                                         const Staged<uint32_t> &frontier,
                                          bool &found, int32_t &u, int32_t &par,
                                          uint32_t &scanned) {
     constexpr int PER = kBlock / G;
@@ -99,7 +101,7 @@ __device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
 template <bool LOOP, typename OffT>
 __device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
                          int32_t *__restrict__ visited, int32_t *__restrict__ parents,
-                         int32_t level, const uint32_t *__restrict__ frontier,
+                         int32_t level, const Staged<uint32_t> &frontier,
                          uint32_t *__restrict__ next, uint32_t &cnt,
                          unsigned long long &edges_acc, uint32_t &scanned) {
     const int total = p.unit_start[kNumClasses];
@@ -195,10 +197,13 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
     }
 }
 
+// `nstage` leading frontier-bitmap words are staged in dynamic LDS for
+// bottom-up epochs.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ depth,
-    int32_t *__restrict__ parents, FrontierBufs fb, StepCtl *ctl) {
+    int32_t *__restrict__ parents, FrontierBufs fb, StepCtl *ctl, int32_t nstage) {
+    extern __shared__ uint32_t s_words[];
     if (ctl->done) return;
     const int mode = ctl->mode;
     const int k = ctl->epoch, cur = k & 1, nxt = cur ^ 1;
@@ -212,7 +217,8 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     uint32_t cnt = 0, scanned = 0;
     unsigned long long eacc = 0;
     if (mode == kPull)
-        bu_sweep<true>(p, g, depth, parents, level, frontier, next, cnt, eacc, scanned);
+        bu_sweep<true>(p, g, depth, parents, level, stage(frontier, nstage, s_words), next,
+                       cnt, eacc, scanned);
     else
         td_sweep(g, depth, parents, level, next, fb, ctl, cur, nxt, cnt, eacc, scanned);
     write_partials(fb.part, cnt, eacc, scanned);
@@ -364,10 +370,12 @@ template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_pull_fused_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ parents,
     const uint32_t *__restrict__ frontier, uint32_t *__restrict__ next,
-    int32_t *num_nodes) {
+    int32_t *num_nodes, int32_t nstage) {
+    extern __shared__ uint32_t s_words[];
     uint32_t nnew = 0, scanned = 0;
     unsigned long long eacc = 0;
-    bu_sweep<false>(p, g, parents, nullptr, 0, frontier, next, nnew, eacc, scanned);
+    bu_sweep<false>(p, g, parents, nullptr, 0, stage(frontier, nstage, s_words), next,
+                    nnew, eacc, scanned);
     unsigned long long s = block_sum_u64(nnew);
     if (threadIdx.x == 0 && s) atomicAdd(num_nodes, (int32_t)s);
 }
@@ -385,7 +393,7 @@ __global__ void bfs_pull_thread_kernel(GraphView<OffT> g,
          u += stride) {
         if (parents[u] != -1) continue;
         int32_t par = -1;
-        if (bu_scan<1>(g, frontier, g.rbeg(u), g.rend(u), 0, 0, par, scanned)) {
+        if (bu_scan<1>(g, unstaged(frontier), g.rbeg(u), g.rend(u), 0, 0, par, scanned)) {
             parents[u] = par;
             atomicOr(&next[u >> 5], 1u << (u & 31));
             nnew++;
@@ -408,7 +416,7 @@ __global__ void bfs_pull_wave_kernel(GraphView<OffT> g,
     for (int32_t u = start + wid; u < end; u += waves) {
         if (parents[u] != -1) continue;
         int32_t par = -1;
-        if (bu_scan<64>(g, frontier, g.rbeg(u), g.rend(u), lane_id(), 0, par,
+        if (bu_scan<64>(g, unstaged(frontier), g.rbeg(u), g.rend(u), lane_id(), 0, par,
                         scanned) &&
             lane_id() == 0) {
             parents[u] = par;
@@ -495,10 +503,11 @@ struct BfsLaunch {
     StepCtl *ctl;
     int grid;
     hipStream_t s;
+    int32_t nstage;
     static int launch(void *arg) {
         auto *a = static_cast<BfsLaunch *>(arg);
-        bfs_step_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->plan, a->g, a->depth,
-                                                            a->parents, a->fb, a->ctl);
+        bfs_step_kernel<OffT><<<a->grid, kBlock, a->nstage * sizeof(uint32_t), a->s>>>(
+            a->plan, a->g, a->depth, a->parents, a->fb, a->ctl, a->nstage);
         bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
         compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "BFS epoch launch failed");
@@ -524,7 +533,8 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
         v, depth, parents, ws->f.fb, ws->f.ctl, src, direction, alpha, beta, 0,
         (long long)g->m, plan_rows, grid);
     HC_HIP(hipGetLastError());
-    BfsLaunch<OffT> L{ph->dev, v, depth, parents, ws->f.fb, ws->f.ctl, grid, s};
+    BfsLaunch<OffT> L{ph->dev, v, depth, parents, ws->f.fb, ws->f.ctl, grid, s,
+                      std::min<int32_t>(tuning().stage_bu, ws->f.fb.words)};
     const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
     long long launches = 0;
@@ -609,8 +619,9 @@ int hc_bfs_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
             HC_TRY(get_plan(g, start_id, end_id, tuning().bu, &ph));
             if (ph->total_units == 0) return HC_OK;
             int nb = std::max(1, std::min(ph->total_units, grid_blocks()));
-            bfs_pull_fused_kernel<OffT><<<nb, kBlock, 0, s>>>(
-                ph->dev, v, parents, frontier, next_frontier, num_nodes);
+            const int32_t ns = std::min<int32_t>(tuning().stage_bu, (g->n + 31) / 32);
+            bfs_pull_fused_kernel<OffT><<<nb, kBlock, ns * sizeof(uint32_t), s>>>(
+                ph->dev, v, parents, frontier, next_frontier, num_nodes, ns);
         }
         HC_HIP(hipGetLastError());
         return HC_OK;

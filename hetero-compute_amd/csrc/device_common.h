@@ -96,6 +96,8 @@ struct PlanConfig {
 struct Tuning {
     int grid_per_cu = 8;   // workgroups per CU of the fused / step launches
     int sssp_alpha = 8;    // SSSP goes pull when frontier edges > m / alpha
+    int stage_pull = 8192; // per-vertex entries staged in LDS by pull sweeps
+    int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
     PlanConfig pull;       // SSSP and PageRank pull sweeps
     PlanConfig bu;         // BFS bottom-up sweeps (no chunk / WG classes)
     Tuning() {
@@ -169,6 +171,38 @@ __device__ __forceinline__ uint32_t load_acquire_u32(const uint32_t *p) {
 __device__ __forceinline__ uint64_t load_acquire_u64(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Reader of a per-vertex array whose first k entries are staged in LDS.  In a
+// degree-sorted graph the low ids are the hubs, which receive a large share
+// of all neighbour references (about a third of them for the top 16K
+// vertices of a Kronecker graph), so those gathers turn into LDS reads and
+// the random L1->L2 request stream shrinks accordingly.
+template <typename T>
+struct Staged {
+    const T *g;  // global array
+    const T *s;  // LDS copy of g[0, k)
+    int32_t k;
+    __device__ __forceinline__ T operator[](int32_t v) const { return v < k ? s[v] : g[v]; }
+};
+
+template <typename T>
+__device__ __forceinline__ Staged<T> unstaged(const T *g) {
+    return Staged<T>{g, nullptr, 0};
+}
+
+// Copies g[0, k) into the workgroup's LDS buffer (all threads; ends with a
+// barrier).  The copy is a snapshot: later writes to g[0, k) by other
+// workgroups are not seen through it, which the callers allow for.
+template <typename T>
+__device__ __forceinline__ Staged<T> stage(const T *g, int32_t k, T *lds) {
+    for (int i = threadIdx.x; i < k; i += blockDim.x) lds[i] = g[i];
+    __syncthreads();
+    return Staged<T>{g, lds, k};
+}
+
+// Largest dynamic LDS a staged kernel asks for (leaves room for the static
+// LDS of the reductions within a 64 KiB workgroup allocation).
+constexpr int kMaxStageBytes = 60 * 1024;
 
 // Sets bit u of bm for every lane with pred; lanes that share a word are
 // merged into one atomicOr (the degree-sorted classes hand a wavefront

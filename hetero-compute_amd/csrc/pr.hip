@@ -47,7 +47,7 @@ __device__ __forceinline__ KSum group_ksum(KSum k) {
 // Kahan-compensated sum of contrib over edges [b, e) strided by G.
 template <int G, typename OffT>
 __device__ __forceinline__ KSum row_ksum(const GraphView<OffT> &g,
-                                         const float *__restrict__ contrib,
+                                         const Staged<float> &contrib,
                                          OffT b, OffT e, int lane) {
     float sum = 0.0f, comp = 0.0f;
     OffT i = b + OffT(lane);
@@ -143,6 +143,7 @@ __device__ __forceinline__ double pr_finalize(const PrEpoch &ep, const double *i
 template <int G, typename OffT>
 __device__ __forceinline__ void pr_group(const PlanDev &p, int c, int k,
                                          const GraphView<OffT> &g,
+                                         const Staged<float> &cr,
                                          const PrEpoch &ep, const PrBufs &b,
                                          double &racc) {
     constexpr int PER = kBlock / G;
@@ -150,7 +151,7 @@ __device__ __forceinline__ void pr_group(const PlanDev &p, int c, int k,
     const int idx = k * PER + gid;
     if (idx < p.cnt[c]) {
         int32_t u = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
-        KSum s = row_ksum<G>(g, ep.c_in, g.rbeg(u), g.rend(u), lane);
+        KSum s = row_ksum<G>(g, cr, g.rbeg(u), g.rend(u), lane);
         if (G > 1) s = group_ksum<G>(s);
         if (lane == 0) racc += pr_finalize(ep, b.inv_deg, u, s);
     }
@@ -160,9 +161,13 @@ __device__ __forceinline__ void pr_group(const PlanDev &p, int c, int k,
 // Returns this thread's residual contribution (hub residuals go to
 // b.hub_resid instead, so the total does not depend on which workgroup
 // finishes a hub).
+// contrib is read-only within a sweep, so the LDS snapshot of its hub entries
+// (staged from ep.c_in) is exact.
 template <typename OffT>
 __device__ double pr_sweep(const PlanDev &p, const GraphView<OffT> &g,
-                           const PrEpoch &ep, const PrBufs &b) {
+                           const PrEpoch &ep, const PrBufs &b, int32_t nstage,
+                           float *lds) {
+    const Staged<float> cr = stage(ep.c_in, nstage, lds);
     double racc = 0.0;
     const int gstride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.zero_cnt; i += gstride) {
@@ -176,7 +181,7 @@ __device__ double pr_sweep(const PlanDev &p, const GraphView<OffT> &g,
         const int k = unit - p.unit_start[c];
         if (c == 0) {
             HubChunk h = p.chunks[k];
-            KSum s = block_ksum(row_ksum<kBlock>(g, ep.c_in, OffT(h.b), OffT(h.e),
+            KSum s = block_ksum(row_ksum<kBlock>(g, cr, OffT(h.b), OffT(h.e),
                                                  threadIdx.x));
             // Chunk partials are combined by the controller kernel.
             if (threadIdx.x == 0) {
@@ -185,17 +190,17 @@ __device__ double pr_sweep(const PlanDev &p, const GraphView<OffT> &g,
             }
         } else if (c == 1) {
             int32_t u = p.list[1] ? p.list[1][k] : p.beg[1] + k;
-            KSum s = block_ksum(row_ksum<kBlock>(g, ep.c_in, g.rbeg(u), g.rend(u),
+            KSum s = block_ksum(row_ksum<kBlock>(g, cr, g.rbeg(u), g.rend(u),
                                                  threadIdx.x));
             if (threadIdx.x == 0) racc += pr_finalize(ep, b.inv_deg, u, s);
         } else if (c == 2) {
-            pr_group<64>(p, c, k, g, ep, b, racc);
+            pr_group<64>(p, c, k, g, cr, ep, b, racc);
         } else if (c == 3) {
-            pr_group<16>(p, c, k, g, ep, b, racc);
+            pr_group<16>(p, c, k, g, cr, ep, b, racc);
         } else if (c == 4) {
-            pr_group<4>(p, c, k, g, ep, b, racc);
+            pr_group<4>(p, c, k, g, cr, ep, b, racc);
         } else {
-            pr_group<1>(p, c, k, g, ep, b, racc);
+            pr_group<1>(p, c, k, g, cr, ep, b, racc);
         }
     }
     return racc;
@@ -204,12 +209,13 @@ __device__ double pr_sweep(const PlanDev &p, const GraphView<OffT> &g,
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void pr_step_kernel(
     PlanDev p, GraphView<OffT> g, float *s0, float *s1, float *c0, float *c1,
-    PrBufs b, PrCtl *ctl, float base, float damp) {
+    PrBufs b, PrCtl *ctl, float base, float damp, int32_t nstage) {
+    extern __shared__ float s_stagef[];
     if (ctl->done) return;
     const int it = ctl->iter;
     const bool odd = it & 1;
     PrEpoch ep{odd ? s1 : s0, odd ? c1 : c0, odd ? s0 : s1, odd ? c0 : c1, base, damp};
-    double r = block_sum_f64(pr_sweep(p, g, ep, b));
+    double r = block_sum_f64(pr_sweep(p, g, ep, b, nstage, s_stagef));
     if (threadIdx.x == 0) b.partials[blockIdx.x] = r;
 }
 
@@ -290,8 +296,10 @@ __global__ __launch_bounds__(kBlock) void pr_init_kernel(
 // adds the fixed-order residual to *residual_out.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void pr_epoch_kernel(PlanDev p, GraphView<OffT> g,
-                                                          PrEpoch ep, PrBufs b) {
-    double r = block_sum_f64(pr_sweep(p, g, ep, b));
+                                                          PrEpoch ep, PrBufs b,
+                                                          int32_t nstage) {
+    extern __shared__ float s_stagef[];
+    double r = block_sum_f64(pr_sweep(p, g, ep, b, nstage, s_stagef));
     if (threadIdx.x == 0) b.partials[blockIdx.x] = r;
 }
 
@@ -396,6 +404,7 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     HC_TRY(get_pr_ws(g, ph->dev.cnt[0], ph->dev.num_hubs, &ws));
     GraphView<OffT> v = view<OffT>(g);
     const int grid = grid_blocks();
+    const int32_t ns = std::min<int32_t>(tuning().stage_pull, g->n);
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     const float base = (1.0f - damp) / (float)g->n;
     pr_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
@@ -408,9 +417,9 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     HC_HIP(hipEventCreate(&e1));
     HC_HIP(hipEventRecord(e0, s));
     for (int i = 0; i < max_iters; i++) {
-        pr_step_kernel<OffT><<<grid, kBlock, 0, s>>>(ph->dev, v, ws->s[0], ws->s[1],
-                                                     ws->c[0], ws->c[1], b, ws->ctl,
-                                                     base, damp);
+        pr_step_kernel<OffT><<<grid, kBlock, ns * sizeof(float), s>>>(
+            ph->dev, v, ws->s[0], ws->s[1], ws->c[0], ws->c[1], b, ws->ctl, base,
+            damp, ns);
         pr_ctl_kernel<<<1, kBlock, 0, s>>>(ph->dev, ws->s[0], ws->s[1], ws->c[0],
                                            ws->c[1], b, ws->ctl, base, damp, grid);
     }
@@ -491,10 +500,13 @@ int hc_pr_pull_epoch(hc_graph_t g, int32_t start_id, int32_t end_id,
     PrEpoch ep{score_in, contrib, score_out, contrib_out,
                (1.0f - damping) / (float)g->n, damping};
     const int nb = std::max(1, std::min(std::max(ph->total_units, 1), grid_blocks()));
+    const int32_t ns = std::min<int32_t>(tuning().stage_pull, g->n);
     if (g->off64)
-        pr_epoch_kernel<uint64_t><<<nb, kBlock, 0, s>>>(ph->dev, view<uint64_t>(g), ep, b);
+        pr_epoch_kernel<uint64_t><<<nb, kBlock, ns * sizeof(float), s>>>(
+            ph->dev, view<uint64_t>(g), ep, b, ns);
     else
-        pr_epoch_kernel<uint32_t><<<nb, kBlock, 0, s>>>(ph->dev, view<uint32_t>(g), ep, b);
+        pr_epoch_kernel<uint32_t><<<nb, kBlock, ns * sizeof(float), s>>>(
+            ph->dev, view<uint32_t>(g), ep, b, ns);
     pr_epoch_finish_kernel<<<1, kBlock, 0, s>>>(ph->dev, ep, b, nb, residual_out);
     HC_HIP(hipGetLastError());
     return HC_OK;

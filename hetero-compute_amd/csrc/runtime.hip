@@ -57,7 +57,10 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
     struct {
         const char *k;
         int *p;
-    } i32[] = {{"grid_per_cu", &t.grid_per_cu}, {"sssp.alpha", &t.sssp_alpha}};
+    } i32[] = {{"grid_per_cu", &t.grid_per_cu},
+               {"sssp.alpha", &t.sssp_alpha},
+               {"pull.stage", &t.stage_pull},
+               {"bu.stage", &t.stage_bu}};
     for (auto &x : i32)
         if (!strcmp(key, x.k)) {
             *is_int = true;
@@ -483,7 +486,11 @@ int hc_set_tuning(const char *key, int64_t value) {
     int64_t *p = knob_ref(key, &is_int, &iref);
     if (!p) return fail(HC_ERR_NOT_FOUND, "unknown tuning key '%s'", key);
     if (is_int) {
-        HC_CHECK_ARG(value >= 1 && value <= 1 << 20, "%s out of range", key);
+        const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage");
+        HC_CHECK_ARG(value >= (stage ? 0 : 1) && value <= 1 << 20, "%s out of range", key);
+        if (stage)
+            HC_CHECK_ARG(value * 4 <= kMaxStageBytes, "%s above %d entries", key,
+                         kMaxStageBytes / 4);
         *iref = (int)value;
     } else {
         HC_CHECK_ARG(value >= 0, "%s must be >= 0", key);

@@ -29,7 +29,7 @@ namespace hc {
 // independent gathers per trip keep several HBM requests in flight per lane.
 template <int G, typename OffT>
 __device__ __forceinline__ int32_t row_min(const GraphView<OffT> &g,
-                                           const int32_t *__restrict__ dist,
+                                           const Staged<int32_t> &dist,
                                            OffT b, OffT e, int lane) {
     int32_t m = HC_INF_DIST;
     OffT i = b + OffT(lane);
@@ -69,6 +69,7 @@ __device__ __forceinline__ int32_t block_min(int32_t m) {
 template <int G, typename OffT>
 __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
                                            const GraphView<OffT> &g,
+                                           const Staged<int32_t> &rd,
                                            int32_t *__restrict__ dist,
                                            bool &chg, int32_t &u) {
     constexpr int PER = kBlock / G;
@@ -76,7 +77,7 @@ __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
     const int idx = k * PER + gid;
     if (idx < p.cnt[c]) {
         int32_t uu = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
-        int32_t m = row_min<G>(g, dist, g.rbeg(uu), g.rend(uu), lane);
+        int32_t m = row_min<G>(g, rd, g.rbeg(uu), g.rend(uu), lane);
         if (G > 1) m = group_min<G>(m);
         if (lane == 0) {
             int32_t cur = dist[uu];
@@ -94,8 +95,13 @@ __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
 // vertices are set in the next-frontier bitmap and counted (first setting
 // only: hub chunks may lower the same vertex twice); otherwise changes are
 // counted for the epoch API.
+// Neighbour distances are read through `rd` (hub entries from the LDS
+// snapshot); the row's own distance is read and written in global memory.
+// A stale snapshot only delays an improvement to a later epoch: values only
+// decrease, and an epoch without any change had nothing stale to read.
 template <bool LOOP, typename OffT>
 __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
+                                const Staged<int32_t> &rd,
                                 int32_t *__restrict__ dist, uint32_t *next_bm,
                                 uint32_t &cnt, unsigned long long &edges_acc) {
     const int total = p.unit_start[kNumClasses];
@@ -118,7 +124,7 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
                 b = g.rbeg(uu);
                 e = g.rend(uu);
             }
-            int32_t m = block_min(row_min<kBlock>(g, dist, b, e, threadIdx.x));
+            int32_t m = block_min(row_min<kBlock>(g, rd, b, e, threadIdx.x));
             if (threadIdx.x == 0) {
                 if (c == 0) {
                     if (m < dist[uu]) {
@@ -135,13 +141,13 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
                 u = uu;
             }
         } else if (c == 2) {
-            pull_group<64>(p, c, k, g, dist, chg, u);
+            pull_group<64>(p, c, k, g, rd, dist, chg, u);
         } else if (c == 3) {
-            pull_group<16>(p, c, k, g, dist, chg, u);
+            pull_group<16>(p, c, k, g, rd, dist, chg, u);
         } else if (c == 4) {
-            pull_group<4>(p, c, k, g, dist, chg, u);
+            pull_group<4>(p, c, k, g, rd, dist, chg, u);
         } else {
-            pull_group<1>(p, c, k, g, dist, chg, u);
+            pull_group<1>(p, c, k, g, rd, dist, chg, u);
         }
         if (LOOP) {
             if (wave_set_bit(next_bm, u, chg)) {
@@ -228,10 +234,12 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
 // Kernels of the device-driven loop
 // ---------------------------------------------------------------------------
 
+// `stage` hub distances are staged in dynamic LDS for pull epochs.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, FrontierBufs fb,
-    StepCtl *ctl) {
+    StepCtl *ctl, int32_t nstage) {
+    extern __shared__ int32_t s_stage[];
     if (ctl->done) return;
     const int mode = ctl->mode;
     const int cur = ctl->epoch & 1, nxt = cur ^ 1;
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     uint32_t cnt = 0;
     unsigned long long eacc = 0, scanned = 0;
     if (mode == kPull)
-        sssp_pull_sweep<true>(p, g, dist, fb.bm[nxt], cnt, eacc);
+        sssp_pull_sweep<true>(p, g, stage(dist, nstage, s_stage), dist, fb.bm[nxt], cnt, eacc);
     else
         sssp_push_sweep(g, dist, fb, ctl, cur, nxt, cnt, eacc);
     write_partials(fb.part, cnt, eacc, scanned);
@@ -359,7 +367,7 @@ __global__ void sssp_pull_thread_kernel(GraphView<OffT> g,
     const int32_t stride = gridDim.x * blockDim.x;
     for (int32_t u = start + blockIdx.x * blockDim.x + threadIdx.x; u < end;
          u += stride) {
-        int32_t m = row_min<1>(g, dist, g.rbeg(u), g.rend(u), 0);
+        int32_t m = row_min<1>(g, unstaged(dist), g.rbeg(u), g.rend(u), 0);
         if (m < dist[u]) {
             dist[u] = m;
             upd++;
@@ -378,7 +386,7 @@ __global__ void sssp_pull_wave_kernel(GraphView<OffT> g,
     const int32_t waves = gridDim.x * (blockDim.x / kWave);
     const int32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     for (int32_t u = start + wid; u < end; u += waves) {
-        int32_t m = group_min<64>(row_min<64>(g, dist, g.rbeg(u), g.rend(u), lane_id()));
+        int32_t m = group_min<64>(row_min<64>(g, unstaged(dist), g.rbeg(u), g.rend(u), lane_id()));
         if (lane_id() == 0 && m < dist[u]) {
             dist[u] = m;
             upd++;
@@ -410,10 +418,11 @@ __global__ void sssp_pull_block_kernel(GraphView<OffT> g,
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist,
-    int32_t *updated) {
+    int32_t *updated, int32_t nstage) {
+    extern __shared__ int32_t s_stage[];
     uint32_t upd = 0;
     unsigned long long eacc = 0;
-    sssp_pull_sweep<false>(p, g, dist, nullptr, upd, eacc);
+    sssp_pull_sweep<false>(p, g, stage(dist, nstage, s_stage), dist, nullptr, upd, eacc);
     unsigned long long s = block_sum_u64(upd);
     if (threadIdx.x == 0 && s) atomicAdd(updated, (int32_t)s);
 }
@@ -489,7 +498,9 @@ static int launch_pull_epoch(hc_graph_s *g, int mapping, int32_t start,
             HC_TRY(get_plan(g, start, end, tuning().pull, &ph));
             if (ph->total_units == 0) return HC_OK;
             int nb = std::max(1, std::min(ph->total_units, grid_blocks()));
-            sssp_pull_fused_kernel<OffT><<<nb, kBlock, 0, s>>>(ph->dev, v, dist, updated);
+            const int32_t ns = std::min<int32_t>(tuning().stage_pull, g->n);
+            sssp_pull_fused_kernel<OffT><<<nb, kBlock, ns * sizeof(int32_t), s>>>(
+                ph->dev, v, dist, updated, ns);
         }
     }
     HC_HIP(hipGetLastError());
@@ -517,9 +528,11 @@ struct SsspLaunch {
     StepCtl *ctl;
     int grid;
     hipStream_t s;
+    int32_t nstage;
     static int launch(void *arg) {
         auto *a = static_cast<SsspLaunch *>(arg);
-        sssp_step_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->plan, a->g, a->dist, a->fb, a->ctl);
+        sssp_step_kernel<OffT><<<a->grid, kBlock, a->nstage * sizeof(int32_t), a->s>>>(
+            a->plan, a->g, a->dist, a->fb, a->ctl, a->nstage);
         sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
         compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 0);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "SSSP epoch launch failed");
@@ -543,7 +556,8 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
         max_epochs, (long long)g->m, sweep, grid);
     HC_HIP(hipGetLastError());
-    SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s};
+    SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s,
+                       std::min<int32_t>(tuning().stage_pull, g->n)};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
     const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;
     float ms = 0;
