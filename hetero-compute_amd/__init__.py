@@ -87,6 +87,12 @@ class BfsOpts(C.Structure):
                 ("direction", C.c_int32)]
 
 
+class ScheduleBlock(C.Structure):
+    _fields_ = [("device", C.c_int32), ("device_id", C.c_int32),
+                ("start_segment", C.c_int32), ("end_segment", C.c_int32),
+                ("kernel", C.c_int32), ("blocks", C.c_int32), ("threads", C.c_int32)]
+
+
 class PrOpts(C.Structure):
     _fields_ = [("damping", C.c_float), ("tolerance", C.c_double),
                 ("max_iters", C.c_int32)]
@@ -140,6 +146,30 @@ _SIGS = {
     "hc_set_tuning": ([C.c_char_p, C.c_int64], C.c_int),
     "hc_get_tuning": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
     "hc_reset_tuning": ([], C.c_int),
+    "hc_kernel_from_string": ([C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
+    "hc_kernel_to_string": ([C.c_int32], C.c_char_p),
+    "hc_kernel_device_type": ([C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+    "hc_device_from_string": ([C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
+    "hc_device_to_string": ([C.c_int32], C.c_char_p),
+    "hc_device_type_of": ([C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+    "hc_schedule_parse": ([C.c_char_p, C.POINTER(_P)], C.c_int),
+    "hc_schedule_load": ([C.c_char_p, C.POINTER(_P)], C.c_int),
+    "hc_schedule_create": ([C.POINTER(ScheduleBlock), C.c_int32, C.c_int32,
+                            C.POINTER(_P)], C.c_int),
+    "hc_schedule_info": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "hc_schedule_get_block": ([_P, C.c_int32, C.POINTER(ScheduleBlock)], C.c_int),
+    "hc_schedule_to_text": ([_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
+    "hc_schedule_free": ([_P], C.c_int),
+    "hc_hetero_create": ([C.POINTER(HostGraphStruct), _P, C.POINTER(_P)], C.c_int),
+    "hc_hetero_free": ([_P], C.c_int),
+    "hc_sssp_heterogeneous": ([_P, _P, _P, C.POINTER(RunStats)], C.c_int),
+    "hc_pr_heterogeneous": ([_P, C.POINTER(PrOpts), _P, C.POINTER(RunStats)], C.c_int),
+    "hc_bfs_heterogeneous": ([_P, C.c_int32, C.c_int32, C.c_int32, _P, _P,
+                              C.POINTER(RunStats)], C.c_int),
+    "hc_cpu_sssp_pull_epoch": ([C.POINTER(HostGraphStruct), _P, C.c_int32, C.c_int32,
+                                C.c_int, C.c_int, _P], None),
+    "hc_cpu_pr_pull_epoch": ([C.POINTER(HostGraphStruct), C.c_float, _P, _P, _P, _P, _P,
+                              C.c_int32, C.c_int32, C.c_int, C.c_int], C.c_double),
 }
 
 TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "pull.stage", "bu.stage", "pull.hub_threshold", "pull.hub_chunk",
@@ -523,3 +553,153 @@ def epoch_trace(g: DeviceGraph, algorithm: str) -> list[dict]:
 def gteps(num_edges_directed: int, ms: float) -> float:
     """Reference GTEPS: m / 2 / t (src/benchmarks/gpu_benchmark.cuh:590)."""
     return num_edges_directed / 2.0 / (ms / 1e3) / 1e9
+
+
+# ---------------------------------------------------------------------------
+# Segment schedules and host+GPU co-execution
+# ---------------------------------------------------------------------------
+
+KERNELS = ("SSSP CPU one-to-one", "SSSP GPU one-to-one", "SSSP GPU warp-red",
+           "SSSP GPU block-red", "SSSP GPU bucketed", "PR CPU one-to-one",
+           "PR GPU one-to-one", "PR GPU warp-red", "PR GPU block-red", "PR GPU bucketed",
+           "BFS CPU pull", "BFS GPU one-to-one", "BFS GPU warp", "BFS GPU bucketed")
+DEVICES = ("Intel i7-9700K", "Intel Xeon E5-2686", "NVIDIA Quadro RTX 4000",
+           "NVIDIA Tesla K80", "NVIDIA Tesla M60", "AMD Instinct MI355X", "Host CPU")
+DEV_CPU, DEV_GPU = 0, 1
+
+
+def kernel_id(name: str) -> int:
+    k = C.c_int32()
+    check(lib().hc_kernel_from_string(name.encode(), C.byref(k)))
+    return k.value
+
+
+def device_id(name: str) -> int:
+    d = C.c_int32()
+    check(lib().hc_device_from_string(name.encode(), C.byref(d)))
+    return d.value
+
+
+class Schedule:
+    """A segment schedule (.skd), see hc_schedule_parse."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def parse(cls, text: str) -> "Schedule":
+        h = _P()
+        check(lib().hc_schedule_parse(text.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load(cls, path: str) -> "Schedule":
+        h = _P()
+        check(lib().hc_schedule_load(os.fsencode(path), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create(cls, blocks: list[dict], num_segments: int) -> "Schedule":
+        """blocks: dicts with device (name), start/end segment (inclusive),
+        kernel (name), optional device_id, blocks, threads."""
+        arr = (ScheduleBlock * len(blocks))()
+        for i, b in enumerate(blocks):
+            arr[i] = ScheduleBlock(device_id(b["device"]), b.get("device_id", 0),
+                                   b["start"], b["end"], kernel_id(b["kernel"]),
+                                   b.get("blocks", 0), b.get("threads", 0))
+        h = _P()
+        check(lib().hc_schedule_create(arr, len(blocks), num_segments, C.byref(h)))
+        return cls(h)
+
+    @property
+    def num_segments(self) -> int:
+        nb, ns = C.c_int32(), C.c_int32()
+        check(lib().hc_schedule_info(self.h, C.byref(nb), C.byref(ns)))
+        return ns.value
+
+    @property
+    def blocks(self) -> list[dict]:
+        nb, ns = C.c_int32(), C.c_int32()
+        check(lib().hc_schedule_info(self.h, C.byref(nb), C.byref(ns)))
+        out = []
+        for i in range(nb.value):
+            b = ScheduleBlock()
+            check(lib().hc_schedule_get_block(self.h, i, C.byref(b)))
+            out.append({"device": DEVICES[b.device], "device_id": b.device_id,
+                        "start": b.start_segment, "end": b.end_segment,
+                        "kernel": KERNELS[b.kernel], "blocks": b.blocks,
+                        "threads": b.threads})
+        return out
+
+    def text(self) -> str:
+        need = C.c_size_t()
+        check(lib().hc_schedule_to_text(self.h, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        check(lib().hc_schedule_to_text(self.h, buf, need.value, C.byref(need)))
+        return buf.value.decode()
+
+    def free(self):
+        if self.h:
+            lib().hc_schedule_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Hetero:
+    """A schedule bound to a host graph (hc_hetero_create).  The host graph
+    must outlive this object."""
+
+    def __init__(self, g: HostGraph, schedule: Schedule):
+        self._g = g
+        self.h = _P()
+        check(lib().hc_hetero_create(g.struct_ref(), schedule.h, C.byref(self.h)))
+
+    def sssp(self, source: int | None = None, init: np.ndarray | None = None) -> Result:
+        n = self._g.num_nodes
+        if init is None:
+            init = np.full(n, INF_DIST, np.int32)
+            init[source] = 0
+        init = np.ascontiguousarray(init, np.int32)
+        out = np.empty(n, np.int32)
+        st = RunStats()
+        check(lib().hc_sssp_heterogeneous(self.h, _ptr(init), _ptr(out), C.byref(st)))
+        return Result(out, st.as_dict())
+
+    def pagerank(self, damping: float = 0.85, tolerance: float = 1e-4,
+                 max_iters: int = 20) -> Result:
+        out = np.empty(self._g.num_nodes, np.float32)
+        st = RunStats()
+        o = PrOpts(damping, tolerance, max_iters)
+        check(lib().hc_pr_heterogeneous(self.h, C.byref(o), _ptr(out), C.byref(st)))
+        return Result(out, st.as_dict())
+
+    def bfs(self, source: int, alpha: int = 15, beta: int = 18) -> Result:
+        n = self._g.num_nodes
+        par = np.empty(n, np.int32)
+        dep = np.empty(n, np.int32)
+        st = RunStats()
+        check(lib().hc_bfs_heterogeneous(self.h, source, alpha, beta, _ptr(par),
+                                         _ptr(dep), C.byref(st)))
+        return Result(dep, st.as_dict(), par)
+
+    def free(self):
+        if self.h:
+            lib().hc_hetero_free(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
