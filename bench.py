@@ -82,6 +82,64 @@ def run_graph(hc, kind, scale, degree, steps, warmup, seed):
     return out
 
 
+def kernel_roofline(hc, res, algo, count):
+    """Average launch duration and algorithmic bytes of the step kernel
+    (sssp_step_kernel / bfs_step_kernel) from HIP events around each launch."""
+    dg = res["dg"]
+    n = res["n"]
+    buf = hc.DeviceArray(n, np.int32)
+    hc.set_tuning("profile.kernel_events", 1)
+    try:
+        k_ms = k_bytes = 0.0
+        k_launch = 0
+        for s in map(int, res["sources"][:count]):
+            if algo == "sssp":
+                st = hc.sssp(dg, s, out=buf, fetch=False).stats
+            else:
+                st = hc.bfs(dg, s, depth_out=buf, want_parents=False, fetch=False).stats
+            k_ms += st["kernel_ms"]
+            k_bytes += st["alg_bytes"]
+            k_launch += st["kernel_launches"]
+    finally:
+        hc.set_tuning("profile.kernel_events", 0)
+        buf.free()
+    avg_s = k_ms / 1e3 / max(k_launch, 1)
+    per_launch = k_bytes / max(k_launch, 1)
+    achieved = per_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"{algo}_step_kernel", "avg_launch_us": round(avg_s * 1e6, 2),
+            "alg_bytes_per_launch": round(per_launch, 1), "launches_timed": k_launch,
+            "sources_timed": count}
+
+
+def committed_traffic(kernel, kind, scale):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), or None when no summary for this workload exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{kind}{scale}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_probe(nbytes, reps):
+    """A plain device copy of known size (torch's copy kernel): the reference
+    point that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "reps": reps}), flush=True)
+
+
 def check_exact(hc, oracle, res, count):
     """Bit-exactness of the first `count` sources against the oracle."""
     g, dg = res["g"], res["dg"]
@@ -134,6 +192,10 @@ def main():
     ap.add_argument("--check", type=int, default=1,
                     help="sources checked bit-exact against the oracle")
     ap.add_argument("--pagerank", type=int, default=1)
+    ap.add_argument("--roofline-sources", type=int, default=16,
+                    help="sources of the event-timed pass over the dominant kernel")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -141,6 +203,9 @@ def main():
               "single MI355X (multi-GPU sharding is a design note only).",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe, 3)
+        return
 
     import __graft_entry__ as ge
     hc = ge.package()
@@ -179,26 +244,15 @@ def main():
             s_ok, b_ok = check_exact(hc, oracle, r, args.check)
             exact[kind] = {"sssp": s_ok, "bfs": b_ok, "sources_checked": args.check}
 
-    # Dominant kernel over the timed region: the device-driven epoch kernel of
-    # the algorithm with the larger share of device time.
+    # Dominant kernel: the step kernel of the algorithm with the larger share
+    # of device time.  Its launches are timed with HIP events on the stream
+    # they run on, in a second pass over the same sources (events between
+    # kernels are kept out of the headline timing).
     tot_sssp = sum(r["sssp_ms"].sum() for r in results.values())
     tot_bfs = sum(r["bfs_ms"].sum() for r in results.values())
-    if tot_sssp >= tot_bfs:
-        kname = "sssp_step_kernel"
-        byts = sum(r["sssp_bytes"].sum() for r in results.values())
-        launches = sum(r["sssp_launches"].sum() for r in results.values())
-        ms = tot_sssp
-    else:
-        kname = "bfs_step_kernel"
-        byts = sum(r["bfs_bytes"].sum() for r in results.values())
-        launches = sum(r["bfs_launches"].sum() for r in results.values())
-        ms = tot_bfs
-    achieved = byts / (ms / 1e3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "kernel": kname,
-                "avg_launch_us": round(ms * 1e3 / launches, 2),
-                "alg_bytes_per_launch": round(byts / launches, 1)}
+    algo = "sssp" if tot_sssp >= tot_bfs else "bfs"
+    roofline = kernel_roofline(hc, R, algo, min(args.steps, args.roofline_sources))
+    roofline["traffic"] = committed_traffic(roofline["kernel"], main_kind, args.scale)
 
     cpu = None
     if not args.no_cpu:

@@ -65,7 +65,9 @@ class RunStats(C.Structure):
     _fields_ = [("ms", C.c_double), ("epochs", C.c_int32),
                 ("push_epochs", C.c_int32), ("pull_epochs", C.c_int32),
                 ("launches", C.c_int32), ("reached", C.c_int64),
-                ("residual", C.c_double), ("alg_bytes", C.c_double)]
+                ("residual", C.c_double), ("alg_bytes", C.c_double),
+                ("kernel_ms", C.c_double), ("kernel_launches", C.c_int32),
+                ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -146,6 +148,7 @@ _SIGS = {
     "hc_set_tuning": ([C.c_char_p, C.c_int64], C.c_int),
     "hc_get_tuning": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
     "hc_reset_tuning": ([], C.c_int),
+    "hc_sssp_multi": ([_P, _P, C.c_int32, _P, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
     "hc_kernel_from_string": ([C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
     "hc_kernel_to_string": ([C.c_int32], C.c_char_p),
     "hc_kernel_device_type": ([C.c_int32, C.POINTER(C.c_int32)], C.c_int),
@@ -172,7 +175,9 @@ _SIGS = {
                               C.c_int32, C.c_int32, C.c_int, C.c_int], C.c_double),
 }
 
-TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "pull.stage", "bu.stage", "pull.hub_threshold", "pull.hub_chunk",
+TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "pull.stage", "bu.stage", "profile.kernel_events",
+               "multi.graph",
+               "pull.hub_threshold", "pull.hub_chunk",
                "pull.block_threshold", "pull.wave_threshold", "pull.g16_threshold",
                "pull.g4_threshold", "bu.wave_threshold", "bu.g16_threshold",
                "bu.g4_threshold")
@@ -539,6 +544,24 @@ def pr_pull_epoch(g: DeviceGraph, start: int, end: int, damping: float,
                                  inv_degree.ptr, score_out.ptr, contrib_out.ptr,
                                  r.ptr, None))
     return float(r.get()[0])
+
+
+MULTI_WIDTH = 64
+
+
+def sssp_multi(g: DeviceGraph, sources, out: DeviceArray | None = None,
+               max_epochs: int = 0, fetch: bool = True) -> Result:
+    """SSSP from up to 64 sources in one batch (hc_sssp_multi).  values is
+    the [num_sources, N] distance matrix (row q from sources[q])."""
+    src = np.ascontiguousarray(sources, np.int32)
+    d = out if out is not None else DeviceArray(g.num_nodes * MULTI_WIDTH, np.int32)
+    st = RunStats()
+    check(lib().hc_sssp_multi(g.h, _ptr(src), src.size, d.ptr, max_epochs, C.byref(st),
+                              None))
+    vals = None
+    if fetch:
+        vals = d.get().reshape(g.num_nodes, MULTI_WIDTH)[:, :src.size].T.copy()
+    return Result(vals, st.as_dict())
 
 
 def epoch_trace(g: DeviceGraph, algorithm: str) -> list[dict]:

@@ -504,10 +504,13 @@ struct BfsLaunch {
     int grid;
     hipStream_t s;
     int32_t nstage;
+    KernelTimer *kt;
     static int launch(void *arg) {
         auto *a = static_cast<BfsLaunch *>(arg);
+        a->kt->mark(a->s);
         bfs_step_kernel<OffT><<<a->grid, kBlock, a->nstage * sizeof(uint32_t), a->s>>>(
             a->plan, a->g, a->depth, a->parents, a->fb, a->ctl, a->nstage);
+        a->kt->mark(a->s);
         bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
         compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "BFS epoch launch failed");
@@ -533,8 +536,9 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
         v, depth, parents, ws->f.fb, ws->f.ctl, src, direction, alpha, beta, 0,
         (long long)g->m, plan_rows, grid);
     HC_HIP(hipGetLastError());
+    ws->f.kt.reset();
     BfsLaunch<OffT> L{ph->dev, v, depth, parents, ws->f.fb, ws->f.ctl, grid, s,
-                      std::min<int32_t>(tuning().stage_bu, ws->f.fb.words)};
+                      std::min<int32_t>(tuning().stage_bu, ws->f.fb.words), &ws->f.kt};
     const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
     long long launches = 0;
@@ -553,6 +557,10 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
         stats->launches = (int32_t)(3 * launches);
         stats->reached = (int64_t)fc.reached;
         stats->alg_bytes = (double)fc.alg_bytes;
+        if (ws->f.kt.on) {
+            stats->kernel_launches = std::min(fc.epoch, ws->f.kt.pairs());
+            stats->kernel_ms = ws->f.kt.sum_ms(stats->kernel_launches);
+        }
     }
     return HC_OK;
 }

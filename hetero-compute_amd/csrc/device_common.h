@@ -98,6 +98,8 @@ struct Tuning {
     int sssp_alpha = 8;    // SSSP goes pull when frontier edges > m / alpha
     int stage_pull = 8192; // per-vertex entries staged in LDS by pull sweeps
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
+    int kernel_events = 0; // 1: time every launch of a loop's main kernel
+    int multi_graph = 1;   // batched SSSP: relaunch a captured hipGraph
     PlanConfig pull;       // SSSP and PageRank pull sweeps
     PlanConfig bu;         // BFS bottom-up sweeps (no chunk / WG classes)
     Tuning() {
@@ -109,6 +111,45 @@ struct Tuning {
     }
 };
 Tuning &tuning();
+
+// Per-launch timing of a loop's main kernel with HIP events recorded on the
+// kernel's own stream just before and after it (enabled by the
+// kernel_events knob).  Events are pooled and reused across runs.
+struct KernelTimer {
+    std::vector<hipEvent_t> ev;
+    int used = 0;
+    bool on = false;
+    void reset() {
+        on = tuning().kernel_events != 0;
+        used = 0;
+    }
+    void mark(hipStream_t s) {
+        if (!on) return;
+        if (used == (int)ev.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) {
+                on = false;
+                return;
+            }
+            ev.push_back(e);
+        }
+        hipEventRecord(ev[used++], s);
+    }
+    // Sum over the first n (before, after) pairs; the stream must have
+    // completed them.
+    double sum_ms(int n) const {
+        double t = 0;
+        for (int i = 0; i < n && 2 * i + 1 < used; i++) {
+            float x = 0;
+            if (hipEventElapsedTime(&x, ev[2 * i], ev[2 * i + 1]) == hipSuccess) t += x;
+        }
+        return t;
+    }
+    int pairs() const { return used / 2; }
+    ~KernelTimer() {
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+    }
+};
 
 // Host-side owner of a plan's device arrays.
 struct PlanHost {
@@ -301,6 +342,7 @@ struct hc_graph_s {
     void *ws_sssp = nullptr;
     void *ws_bfs = nullptr;
     void *ws_pr = nullptr;
+    void *ws_multi = nullptr;
 };
 
 namespace hc {
@@ -315,6 +357,7 @@ inline GraphView<OffT> view(const hc_graph_s *g) {
 void free_sssp_ws(hc_graph_s *g);
 void free_bfs_ws(hc_graph_s *g);
 void free_pr_ws(hc_graph_s *g);
+void free_multi_ws(hc_graph_s *g);
 }  // namespace hc
 
 #endif

@@ -99,6 +99,7 @@ struct FrontierWs {
     void *dmem = nullptr;
     int64_t chunk_cap = 0;
     int32_t grid = 0;
+    KernelTimer kt;  // step-kernel launches of the current run
 };
 
 int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws);

@@ -1,0 +1,507 @@
+// Batched SSSP: up to 64 sources of one graph in one device-driven epoch
+// loop, one lane per source.
+//
+// The reference runs one source at a time (sssp_pull_gpu,
+// src/kernels/gpu/sssp.cuh:42-100, called per source by the benchmarks,
+// src/benchmarks/gpu_benchmark.cuh:541-594).  Here the 64 distance arrays are
+// stored vertex-major, dist[v * 64 + q], so one wavefront handles a vertex for
+// all 64 sources and every neighbour row it reads is a 256-byte coalesced
+// segment shared by the sources.
+//
+// Each epoch relaxes a vertex u only against neighbours whose distance
+// changed since u last saw them: chg[v] holds one bit per 8-source group
+// (bit k: some source of lanes 8k..8k+7 improved v), and a lane reads
+// dist[v][q] only when its group's bit is set.  The invariant "for every edge
+// (u,v) and source q, dist[u][q] <= dist[v][q] + w, or v is flagged for q's
+// group" holds at every epoch boundary, so an epoch that changes nothing
+// leaves every source's Bellman-Ford fixpoint: the exact shortest distances,
+// bit-identical to a single-source run.
+//
+// Mapping: degree-bucketed units of one wavefront -- a chunk of a hub's edge
+// list (partial minimum combined by atomicMin), one vertex, 4 vertices of
+// degree <= 16 (16 lanes each) or 16 vertices of degree <= 4 (4 lanes each).
+// The loop is device-driven: step + controller kernels captured in a hipGraph
+// of kEpochsPerGraph epochs that the host relaunches until the controller
+// reports convergence.
+#include <cstring>
+
+#include "../../include/hc_batch.h"
+#include "device_common.h"
+
+namespace hc {
+
+constexpr int kQ = 64;               // sources per batch (= lanes)
+constexpr int kEpochsPerGraph = 4;   // epochs captured per hipGraph launch
+
+struct MultiCtl {
+    int32_t done;
+    int32_t epoch;
+    int32_t max_epochs;
+    int32_t pad;
+    int32_t *dist;                  // vertex-major [n][64]
+    unsigned long long changed[2];  // vertices changed in epoch e (slot e&1)
+    unsigned long long bytes;       // algorithmic bytes of the epochs run
+    long long t_start, t_end;
+};
+
+// Byte of vertex u in a word-packed flag array.
+__device__ __forceinline__ uint32_t flag_of(const uint32_t *f, int32_t u) {
+    return (f[u >> 2] >> ((u & 3) * 8)) & 0xFFu;
+}
+
+// 8-bit group mask of a 64-lane ballot: bit k set when any lane 8k..8k+7 is.
+__device__ __forceinline__ uint32_t group_mask(uint64_t b) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) m |= ((b >> (8 * k)) & 0xFFull) ? (1u << k) : 0u;
+    return m;
+}
+
+template <typename OffT>
+struct MultiArgs {
+    PlanDev p;
+    GraphView<OffT> g;
+    uint32_t *chg[3];
+    MultiCtl *ctl;
+    int32_t n;
+};
+
+template <typename OffT>
+__device__ __forceinline__ OffT wave_max(OffT v) {
+#pragma unroll
+    for (int s = kWave / 2; s > 0; s >>= 1) {
+        OffT o = __shfl_xor(v, s, kWave);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Relaxes V = 64/G vertices against their flagged neighbours.  Lane l works
+// on sub-vertex l / G, whose edges are [my_b, my_e) (empty for padding);
+// best[k] is this lane's (source's) candidate for sub-vertex k.  Returns a
+// bit per k: some neighbour of sub-vertex k was flagged for some group.
+template <int G, typename OffT>
+__device__ __forceinline__ uint32_t relax_group(const MultiArgs<OffT> &a,
+                                                const uint32_t *cur,
+                                                const int32_t *dist, OffT my_b,
+                                                OffT my_e, int32_t *best,
+                                                unsigned long long &bytes) {
+    constexpr int V = kQ / G;
+    const int lane = lane_id();
+    const int li = lane % G;
+    const int grp = lane >> 3;
+    const OffT maxlen = G == kQ ? my_e - my_b : wave_max<OffT>(my_e - my_b);
+    uint32_t touched = 0;
+    for (OffT it = 0; it < maxlen; it += OffT(G)) {
+        const OffT i = my_b + it + OffT(li);
+        int32_t v = 0;
+        uint32_t m = 0;
+        int32_t w = 0;
+        const bool in = i < my_e;
+        if (in) {
+            v = ldnt(&a.g.adj[i]);
+            m = flag_of(cur, v);
+            if (m) w = a.g.wt[i];
+        }
+        uint64_t bal = __ballot(m != 0);
+        bytes += 5ull * (unsigned long long)__popcll(__ballot(in));
+        while (bal) {
+            // Up to four flagged neighbours per round, their rows loaded
+            // before any is used.
+            int j[4];
+            int32_t vj[4], wj[4];
+            uint32_t mj[4];
+            int32_t dv[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (bal) {
+                    j[r] = __ffsll((unsigned long long)bal) - 1;
+                    bal &= bal - 1;
+                    vj[r] = __builtin_amdgcn_readlane(v, j[r]);
+                    wj[r] = __builtin_amdgcn_readlane(w, j[r]);
+                    mj[r] = __builtin_amdgcn_readlane(m, j[r]);
+                } else {
+                    j[r] = -1;
+                    vj[r] = 0;
+                    wj[r] = 0;
+                    mj[r] = 0;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                dv[r] = HC_INF_DIST;
+                if ((mj[r] >> grp) & 1u) dv[r] = dist[(size_t)vj[r] * kQ + lane];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (j[r] < 0) continue;
+                const int sv = j[r] / G;  // wave-uniform
+                const int32_t c = relax(dv[r], wj[r]);
+#pragma unroll
+                for (int k = 0; k < V; k++)
+                    if (k == sv) best[k] = min(best[k], c);
+                touched |= 1u << sv;
+                bytes += 4ull + 32ull * (unsigned long long)__popc(mj[r]);
+            }
+        }
+    }
+    return touched;
+}
+
+// Commits best[k] for the V sub-vertices (vertex of sub-vertex k is lane
+// k*G's my_u), flags improved groups in nxt, and returns the number of
+// vertices changed.  Plain row update: the unit owns these vertices.
+template <int V>
+__device__ __forceinline__ int commit(int32_t *dist, uint32_t *nxt, int32_t my_u,
+                                      const int32_t *best, uint32_t touched,
+                                      unsigned long long &bytes) {
+    constexpr int G = kQ / V;
+    const int lane = lane_id();
+    int changed = 0;
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        if (!((touched >> k) & 1u)) continue;  // wave-uniform
+        const int32_t u = __builtin_amdgcn_readlane(my_u, k * G);
+        int32_t *row = dist + (size_t)u * kQ;
+        const int32_t old = row[lane];
+        const bool imp = best[k] < old;
+        if (imp) row[lane] = best[k];
+        const uint64_t ib = __ballot(imp);
+        bytes += 256ull + 4ull * (unsigned long long)__popcll(ib);
+        if (ib) {
+            const uint32_t gm = group_mask(ib);
+            if (lane == 0) atomicOr(&nxt[u >> 2], gm << ((u & 3) * 8));
+            changed++;
+        }
+    }
+    return changed;
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void multi_step_kernel(MultiArgs<OffT> a) {
+    MultiCtl *ctl = a.ctl;
+    if (ctl->done) return;
+    const int ep = ctl->epoch;
+    int32_t *dist = ctl->dist;
+    const uint32_t *cur = a.chg[ep % 3];
+    uint32_t *nxt = a.chg[(ep + 1) % 3];
+    uint32_t *clr = a.chg[(ep + 2) % 3];
+    const int words = (a.n + 3) >> 2;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < words; i += gridDim.x * kBlock)
+        clr[i] = 0;
+
+    const PlanDev &p = a.p;
+    const int lane = lane_id();
+    // Wave units: chunks, then vertices of classes 1-2, then groups of 4
+    // (class 3), then groups of 16 (classes 4-5).
+    const int n0 = p.cnt[0];
+    const int n12 = p.cnt[1] + p.cnt[2];
+    const int n3 = (p.cnt[3] + 3) / 4;
+    const int n45 = (p.cnt[4] + p.cnt[5] + 15) / 16;
+    const int total = n0 + n12 + n3 + n45;
+    const int wstride = gridDim.x * (kBlock / kWave);
+    unsigned long long bytes = 0;
+    int changed = 0;
+    auto vid = [&](int c, int i) -> int32_t {
+        return p.list[c] ? p.list[c][i] : p.beg[c] + i;
+    };
+    for (int unit = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; unit < total;
+         unit += wstride) {
+        if (unit < n0) {
+            const HubChunk hc = p.chunks[unit];
+            int32_t best[1] = {HC_INF_DIST};
+            uint32_t t = relax_group<kQ>(a, cur, dist, OffT(hc.b), OffT(hc.e), best, bytes);
+            if (t) {
+                const int32_t old = atomicMin(&dist[(size_t)hc.u * kQ + lane], best[0]);
+                const uint64_t ib = __ballot(best[0] < old);
+                bytes += 256ull;
+                if (ib) {
+                    if (lane == 0) atomicOr(&nxt[hc.u >> 2], group_mask(ib) << ((hc.u & 3) * 8));
+                    changed++;
+                }
+            }
+        } else if (unit < n0 + n12) {
+            const int i = unit - n0;
+            const int32_t u = i < p.cnt[1] ? vid(1, i) : vid(2, i - p.cnt[1]);
+            int32_t best[1] = {HC_INF_DIST};
+            uint32_t t = relax_group<kQ>(a, cur, dist, a.g.rbeg(u), a.g.rend(u), best, bytes);
+            changed += commit<1>(dist, nxt, u, best, t, bytes);
+        } else if (unit < n0 + n12 + n3) {
+            const int i = (unit - n0 - n12) * 4 + lane / 16;
+            const bool ok = i < p.cnt[3];
+            const int32_t u = ok ? vid(3, i) : 0;
+            const OffT b = ok ? a.g.rbeg(u) : OffT(0), e = ok ? a.g.rend(u) : OffT(0);
+            int32_t best[4] = {HC_INF_DIST, HC_INF_DIST, HC_INF_DIST, HC_INF_DIST};
+            uint32_t t = relax_group<16>(a, cur, dist, b, e, best, bytes);
+            changed += commit<4>(dist, nxt, u, best, t, bytes);
+        } else {
+            const int c4 = p.cnt[4];
+            const int i = (unit - n0 - n12 - n3) * 16 + lane / 4;
+            const bool ok = i < c4 + p.cnt[5];
+            const int32_t u = ok ? (i < c4 ? vid(4, i) : vid(5, i - c4)) : 0;
+            const OffT b = ok ? a.g.rbeg(u) : OffT(0), e = ok ? a.g.rend(u) : OffT(0);
+            int32_t best[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) best[k] = HC_INF_DIST;
+            uint32_t t = relax_group<4>(a, cur, dist, b, e, best, bytes);
+            changed += commit<16>(dist, nxt, u, best, t, bytes);
+        }
+    }
+    // One pair of atomics per workgroup.
+    __shared__ unsigned long long s_bytes, s_changed;
+    if (threadIdx.x == 0) {
+        s_bytes = 0;
+        s_changed = 0;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        atomicAdd(&s_bytes, bytes);
+        atomicAdd(&s_changed, (unsigned long long)changed);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&ctl->changed[ep & 1], s_changed);
+        atomicAdd(&ctl->bytes, s_bytes);
+    }
+}
+
+__global__ void multi_ctl_kernel(MultiCtl *ctl, unsigned long long sweep_bytes) {
+    if (threadIdx.x != 0 || ctl->done) return;
+    const int ep = ctl->epoch;
+    const unsigned long long c = ctl->changed[ep & 1];
+    ctl->changed[(ep + 1) & 1] = 0;
+    ctl->bytes += sweep_bytes;  // offsets and the flag clear, per epoch
+    ctl->epoch = ep + 1;
+    ctl->t_end = wall_clock64();
+    if (c == 0 || ep + 1 >= ctl->max_epochs) ctl->done = 1;
+}
+
+__global__ void multi_fill_kernel(int32_t *dist, size_t count, uint32_t *f0, uint32_t *f1,
+                                  uint32_t *f2, int words) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += stride)
+        dist[i] = HC_INF_DIST;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)words;
+         i += stride) {
+        f0[i] = 0;
+        f1[i] = 0;
+        f2[i] = 0;
+    }
+}
+
+struct MultiSources {
+    int32_t v[kQ];
+};
+
+__global__ void multi_seed_kernel(MultiCtl *ctl, int32_t *dist, uint32_t *f0,
+                                  MultiSources src, int num, int max_epochs) {
+    const int q = threadIdx.x;
+    if (q < num) {
+        const int32_t s = src.v[q];
+        dist[(size_t)s * kQ + q] = 0;
+        atomicOr(&f0[s >> 2], (1u << (q >> 3)) << ((s & 3) * 8));
+    }
+    if (q == 0) {
+        ctl->done = max_epochs <= 0 ? 1 : 0;
+        ctl->epoch = 0;
+        ctl->max_epochs = max_epochs;
+        ctl->dist = dist;
+        ctl->changed[0] = 0;
+        ctl->changed[1] = 0;
+        ctl->bytes = 0;
+        ctl->t_start = wall_clock64();
+        ctl->t_end = ctl->t_start;
+    }
+}
+
+// Per-graph workspace: flags, controller, and the captured epoch graph.
+struct MultiWs {
+    uint32_t *flags = nullptr;  // 3 arrays of `words` words
+    MultiCtl *ctl = nullptr;
+    MultiCtl *h_ctl = nullptr;  // pinned mirror for polling
+    int words = 0;
+    int grid = 0;
+    const PlanHost *plan = nullptr;
+    hipStream_t gstream = nullptr;  // stream the graph was captured on
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    KernelTimer kt;
+    ~MultiWs() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        if (flags) hipFree(flags);
+        if (ctl) hipFree(ctl);
+        if (h_ctl) hipHostFree(h_ctl);
+    }
+};
+
+void free_multi_ws(hc_graph_s *g) {
+    delete static_cast<MultiWs *>(g->ws_multi);
+    g->ws_multi = nullptr;
+}
+
+// Plan of the batched sweep: hub chunks of 2048 edges above degree 4096,
+// wavefront per vertex above 16, 16 lanes above 4, else 4 lanes.
+static PlanConfig multi_config() {
+    PlanConfig c;
+    c.hub_threshold = 4096;
+    c.hub_chunk = 2048;
+    c.block_threshold = INT64_MAX;
+    c.wave_threshold = 16;
+    c.g16_threshold = 4;
+    c.g4_threshold = 0;
+    return c;
+}
+
+static int get_multi_ws(hc_graph_s *g, MultiWs **out) {
+    auto *ws = static_cast<MultiWs *>(g->ws_multi);
+    if (ws && ws->grid != grid_blocks()) {
+        free_multi_ws(g);
+        ws = nullptr;
+    }
+    if (!ws) {
+        ws = new (std::nothrow) MultiWs();
+        if (!ws) return fail(HC_ERR_OOM, "host allocation failed");
+        ws->words = (g->n + 3) / 4;
+        ws->grid = grid_blocks();
+        hipError_t e1 = hipMalloc(&ws->flags, sizeof(uint32_t) * 3 * (size_t)std::max(ws->words, 1));
+        hipError_t e2 = hipMalloc(&ws->ctl, sizeof(MultiCtl));
+        hipError_t e3 = hipHostMalloc(&ws->h_ctl, sizeof(MultiCtl), hipHostMallocDefault);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+            delete ws;
+            return fail(HC_ERR_OOM, "batched-SSSP workspace allocation failed");
+        }
+        int rc = get_plan(g, 0, g->n, multi_config(), &ws->plan);
+        if (rc != HC_OK) {
+            delete ws;
+            return rc;
+        }
+        g->ws_multi = ws;
+    }
+    *out = ws;
+    return HC_OK;
+}
+
+template <typename OffT>
+static int launch_multi_epochs(const MultiArgs<OffT> &a, int grid, int epochs,
+                               unsigned long long sweep, hipStream_t s, KernelTimer *kt) {
+    for (int i = 0; i < epochs; i++) {
+        if (kt) kt->mark(s);
+        multi_step_kernel<OffT><<<grid, kBlock, 0, s>>>(a);
+        if (kt) kt->mark(s);
+        multi_ctl_kernel<<<1, kWave, 0, s>>>(a.ctl, sweep);
+    }
+    return hipGetLastError() == hipSuccess ? HC_OK
+                                           : fail(HC_ERR_HIP, "batched SSSP launch failed");
+}
+
+template <typename OffT>
+static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *dist,
+                     int max_epochs, hc_run_stats_t *stats, hipStream_t s) {
+    MultiWs *ws = nullptr;
+    HC_TRY(get_multi_ws(g, &ws));
+    uint32_t *f[3] = {ws->flags, ws->flags + ws->words, ws->flags + 2 * (size_t)ws->words};
+    MultiArgs<OffT> a{ws->plan->dev, view<OffT>(g), {f[0], f[1], f[2]}, ws->ctl, g->n};
+    const int grid = ws->grid;
+    const unsigned long long sweep =
+        (unsigned long long)sizeof(OffT) * (g->n + 1ull) + 4ull * ws->words;
+    MultiSources src{};
+    for (int q = 0; q < num; q++) src.v[q] = sources[q];
+    const size_t cells = (size_t)g->n * kQ;
+    const int fill_grid = std::max(1, std::min(grid, (int)((cells + kBlock - 1) / kBlock)));
+    multi_fill_kernel<<<fill_grid, kBlock, 0, s>>>(dist, cells, f[0], f[1], f[2], ws->words);
+    multi_seed_kernel<<<1, kQ, 0, s>>>(ws->ctl, dist, f[0], src, num,
+                                       max_epochs > 0 ? max_epochs : g->n + 1);
+    HC_HIP(hipGetLastError());
+
+    ws->kt.reset();
+    const bool use_graph = tuning().multi_graph != 0 && !ws->kt.on;
+    if (use_graph && (!ws->exec || ws->gstream != s)) {
+        if (ws->exec) hipGraphExecDestroy(ws->exec);
+        if (ws->graph) hipGraphDestroy(ws->graph);
+        ws->exec = nullptr;
+        ws->graph = nullptr;
+        // The kernels read dist and the epoch from the controller, so one
+        // captured graph serves every call on this graph handle.
+        HC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = launch_multi_epochs(a, grid, kEpochsPerGraph, sweep, s, nullptr);
+        hipGraph_t graph = nullptr;
+        hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (rc != HC_OK) return rc;
+        if (ce != hipSuccess) return fail(HC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+        ws->graph = graph;
+        HC_HIP(hipGraphInstantiate(&ws->exec, graph, nullptr, nullptr, 0));
+        ws->gstream = s;
+    }
+    // Relaunch until the controller reports convergence; the done flag of
+    // batch b-1 is read while batch b runs.
+    hipEvent_t evb[2];
+    HC_HIP(hipEventCreateWithFlags(&evb[0], hipEventDisableTiming));
+    HC_HIP(hipEventCreateWithFlags(&evb[1], hipEventDisableTiming));
+    MultiCtl *h = ws->h_ctl;
+    h->done = 0;
+    const long long cap = (long long)(max_epochs > 0 ? max_epochs : g->n + 1) + 2 * kEpochsPerGraph;
+    long long launched = 0;
+    bool done = false;
+    int rc = HC_OK;
+    for (int b = 0; !done && launched < cap; b++) {
+        if (use_graph) {
+            if (hipGraphLaunch(ws->exec, s) != hipSuccess) {
+                rc = fail(HC_ERR_HIP, "graph launch failed");
+                break;
+            }
+        } else {
+            rc = launch_multi_epochs(a, grid, kEpochsPerGraph, sweep, s, &ws->kt);
+            if (rc != HC_OK) break;
+        }
+        launched += kEpochsPerGraph;
+        hipMemcpyAsync(&h->done, &ws->ctl->done, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+        hipEventRecord(evb[b & 1], s);
+        if (b > 0) {
+            hipEventSynchronize(evb[(b - 1) & 1]);
+            done = __atomic_load_n(&h->done, __ATOMIC_ACQUIRE) != 0;
+        }
+    }
+    hipEventDestroy(evb[0]);
+    hipEventDestroy(evb[1]);
+    HC_TRY(rc);
+    MultiCtl c;
+    HC_HIP(hipMemcpyAsync(&c, ws->ctl, sizeof(MultiCtl), hipMemcpyDeviceToHost, s));
+    HC_HIP(hipStreamSynchronize(s));
+    if (!c.done) return fail(HC_ERR_HIP, "batched SSSP did not converge in %lld epochs", launched);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->ms = double(c.t_end - c.t_start) / device_state().clock_khz;
+        stats->epochs = c.epoch;
+        stats->pull_epochs = c.epoch;
+        stats->launches = (int32_t)(2 * launched + 2);
+        stats->reached = num;
+        stats->alg_bytes = (double)c.bytes;
+        if (ws->kt.on) {
+            stats->kernel_launches = std::min(c.epoch, ws->kt.pairs());
+            stats->kernel_ms = ws->kt.sum_ms(stats->kernel_launches);
+        }
+    }
+    return HC_OK;
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" int hc_sssp_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources,
+                             int32_t *dist, int32_t max_epochs, hc_run_stats_t *stats,
+                             hc_stream_t stream) {
+    HC_CHECK_ARG(g && sources && dist, "NULL argument");
+    HC_CHECK_ARG(g->weighted, "SSSP needs a weighted graph");
+    HC_CHECK_ARG(g->row0 == 0 && g->rows == g->n, "batched runs need a whole-graph handle");
+    HC_CHECK_ARG(num_sources >= 1 && num_sources <= kQ, "num_sources must be in [1, %d]", kQ);
+    for (int q = 0; q < num_sources; q++)
+        HC_CHECK_ARG(sources[q] >= 0 && sources[q] < g->n, "source %d = %d outside [0,%d)", q,
+                     sources[q], g->n);
+    HC_TRY(ensure_init());
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    return g->off64 ? run_multi<uint64_t>(g, sources, num_sources, dist, max_epochs, stats, s)
+                    : run_multi<uint32_t>(g, sources, num_sources, dist, max_epochs, stats, s);
+}

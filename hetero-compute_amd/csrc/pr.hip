@@ -315,6 +315,7 @@ __global__ __launch_bounds__(kBlock) void pr_epoch_finish_kernel(PlanDev p, PrEp
 // ---------------------------------------------------------------------------
 
 struct PrWs {
+    KernelTimer kt;  // step-kernel launches of the current run
     float *s[2] = {nullptr, nullptr};
     float *c[2] = {nullptr, nullptr};
     double *inv_deg = nullptr;
@@ -416,10 +417,13 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     HC_HIP(hipEventCreate(&e0));
     HC_HIP(hipEventCreate(&e1));
     HC_HIP(hipEventRecord(e0, s));
+    ws->kt.reset();
     for (int i = 0; i < max_iters; i++) {
+        ws->kt.mark(s);
         pr_step_kernel<OffT><<<grid, kBlock, ns * sizeof(float), s>>>(
             ph->dev, v, ws->s[0], ws->s[1], ws->c[0], ws->c[1], b, ws->ctl, base,
             damp, ns);
+        ws->kt.mark(s);
         pr_ctl_kernel<<<1, kBlock, 0, s>>>(ph->dev, ws->s[0], ws->s[1], ws->c[0],
                                            ws->c[1], b, ws->ctl, base, damp, grid);
     }
@@ -446,6 +450,10 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
         stats->reached = -1;
         stats->residual = c.residual;
         stats->alg_bytes = c.alg_bytes;
+        if (ws->kt.on) {
+            stats->kernel_launches = std::min(c.iter, ws->kt.pairs());
+            stats->kernel_ms = ws->kt.sum_ms(stats->kernel_launches);
+        }
     }
     return HC_OK;
 }

@@ -60,7 +60,9 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
     } i32[] = {{"grid_per_cu", &t.grid_per_cu},
                {"sssp.alpha", &t.sssp_alpha},
                {"pull.stage", &t.stage_pull},
-               {"bu.stage", &t.stage_bu}};
+               {"bu.stage", &t.stage_bu},
+               {"profile.kernel_events", &t.kernel_events},
+               {"multi.graph", &t.multi_graph}};
     for (auto &x : i32)
         if (!strcmp(key, x.k)) {
             *is_int = true;
@@ -471,6 +473,7 @@ int hc_graph_free(hc_graph_t g) {
     free_sssp_ws(g);
     free_bfs_ws(g);
     free_pr_ws(g);
+    free_multi_ws(g);
     for (auto &kv : g->plans) hipFree(kv.second.dmem);
     hipFree(g->d_off);
     hipFree(g->d_adj);
@@ -487,7 +490,9 @@ int hc_set_tuning(const char *key, int64_t value) {
     if (!p) return fail(HC_ERR_NOT_FOUND, "unknown tuning key '%s'", key);
     if (is_int) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage");
-        HC_CHECK_ARG(value >= (stage ? 0 : 1) && value <= 1 << 20, "%s out of range", key);
+        const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph");
+        HC_CHECK_ARG(value >= (stage || flag ? 0 : 1) && value <= (flag ? 1 : 1 << 20),
+                     "%s out of range", key);
         if (stage)
             HC_CHECK_ARG(value * 4 <= kMaxStageBytes, "%s above %d entries", key,
                          kMaxStageBytes / 4);

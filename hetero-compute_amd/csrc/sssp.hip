@@ -529,10 +529,13 @@ struct SsspLaunch {
     int grid;
     hipStream_t s;
     int32_t nstage;
+    KernelTimer *kt;
     static int launch(void *arg) {
         auto *a = static_cast<SsspLaunch *>(arg);
+        a->kt->mark(a->s);
         sssp_step_kernel<OffT><<<a->grid, kBlock, a->nstage * sizeof(int32_t), a->s>>>(
             a->plan, a->g, a->dist, a->fb, a->ctl, a->nstage);
+        a->kt->mark(a->s);
         sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
         compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 0);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "SSSP epoch launch failed");
@@ -556,8 +559,9 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
         max_epochs, (long long)g->m, sweep, grid);
     HC_HIP(hipGetLastError());
+    ws->f.kt.reset();
     SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s,
-                       std::min<int32_t>(tuning().stage_pull, g->n)};
+                       std::min<int32_t>(tuning().stage_pull, g->n), &ws->f.kt};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
     const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;
     float ms = 0;
@@ -578,6 +582,10 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
         stats->launches = (int32_t)(3 * launches);
         stats->reached = -1;
         stats->alg_bytes = (double)fc.alg_bytes;
+        if (ws->f.kt.on) {
+            stats->kernel_launches = std::min(fc.epoch, ws->f.kt.pairs());
+            stats->kernel_ms = ws->f.kt.sum_ms(stats->kernel_launches);
+        }
     }
     return HC_OK;
 }

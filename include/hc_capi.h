@@ -112,6 +112,11 @@ typedef struct hc_run_stats {
     int64_t reached;       /* vertices reached (BFS), -1 if not tracked       */
     double residual;       /* PageRank: final L1 residual (FP64)              */
     double alg_bytes;      /* algorithmic HBM bytes of the epochs run          */
+    double kernel_ms;      /* with profile.kernel_events: summed HIP-event
+                              durations of the main kernel's launches (the
+                              step kernels of the epochs run), else 0          */
+    int32_t kernel_launches; /* launches summed into kernel_ms                */
+    int32_t reserved;
 } hc_run_stats_t;
 
 /* ------------------------------------------------------------------------ */
@@ -198,6 +203,9 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   pull.g16_threshold     ... 16 lanes
  *   pull.g4_threshold      ... 4 lanes (else one lane)
  *   bu.wave_threshold / bu.g16_threshold / bu.g4_threshold   BFS bottom-up
+ *   pull.stage / bu.stage  per-vertex entries / bitmap words staged in LDS
+ *   profile.kernel_events  1: time each launch of the loop's main kernel
+ *                          (hc_run_stats_t.kernel_ms); 0 by default
  * Plans are rebuilt for new values; changing knobs while another thread
  * runs on the same graph is not supported. */
 int hc_set_tuning(const char *key, int64_t value);

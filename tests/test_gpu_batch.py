@@ -1,0 +1,108 @@
+"""GPU parity of batched queries: 64-source SSSP in one device-driven loop
+must give, for every source, exactly the distances of Dijkstra (and of the
+single-source GPU run)."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def kron14(hc):
+    return hc.HostGraph.generate("kron", 14, 16, weighted=True)
+
+
+def _check_columns(oracle, g, sources, vals):
+    for q, s in enumerate(sources):
+        ref = oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, int(s))
+        assert np.array_equal(vals[q], ref), (q, int(s))
+
+
+def test_multi_64_sources_kron14(hc, gpu, oracle, kron14):
+    g = kron14
+    sources = g.pick_sources(64)
+    with hc.DeviceGraph(g) as dg:
+        r = hc.sssp_multi(dg, sources)
+        _check_columns(oracle, g, sources, r.values)
+        assert r.stats["epochs"] >= 2
+        # Same columns as the single-source runs.
+        for q in (0, 17, 63):
+            assert np.array_equal(hc.sssp(dg, int(sources[q])).values, r.values[q])
+
+
+def test_multi_graph_and_plain_launches_agree(hc, gpu, kron14):
+    sources = kron14.pick_sources(64, seed=7)
+    with hc.DeviceGraph(kron14) as dg:
+        a = hc.sssp_multi(dg, sources)
+        hc.set_tuning("multi.graph", 0)
+        try:
+            b = hc.sssp_multi(dg, sources)
+        finally:
+            hc.set_tuning("multi.graph", 1)
+        c = hc.sssp_multi(dg, sources)  # captured graph relaunched
+    assert np.array_equal(a.values, b.values)
+    assert np.array_equal(a.values, c.values)
+    assert a.stats["epochs"] == b.stats["epochs"] == c.stats["epochs"]
+
+
+@pytest.mark.parametrize("count", [1, 5, 33])
+def test_multi_partial_batches_and_duplicates(hc, gpu, oracle, kron14, count):
+    g = kron14
+    sources = g.pick_sources(count, seed=11)
+    if count > 2:
+        sources[1] = sources[0]  # the same source in two columns
+    with hc.DeviceGraph(g) as dg:
+        out = hc.DeviceArray(g.num_nodes * hc.MULTI_WIDTH, np.int32)
+        r = hc.sssp_multi(dg, sources, out=out)
+        full = out.get().reshape(g.num_nodes, hc.MULTI_WIDTH)
+    _check_columns(oracle, g, sources, r.values)
+    assert np.all(full[:, count:] == hc.INF_DIST)
+
+
+@pytest.mark.parametrize("name", ["kron10.npz", "disconnected.npz", "multi.npz",
+                                  "zeroweight.npz", "path10.npz", "star9.npz"])
+def test_multi_golden(hc, gpu, name):
+    fx = load_golden(name)
+    g = hc.HostGraph.from_csr(fx["index"], fx["neighbors"], fx["weights"])
+    with hc.DeviceGraph(g) as dg:
+        r = hc.sssp_multi(dg, fx["sources"])
+    for q in range(len(fx["sources"])):
+        assert np.array_equal(r.values[q], fx["sssp"][q]), (name, q)
+
+
+def test_multi_unsorted_ids_and_wide_offsets(hc, gpu, oracle, kron14):
+    """Vertex ids not in degree order (plan lists instead of ranges) and
+    8-byte offsets."""
+    g = kron14
+    n = g.num_nodes
+    perm = np.random.default_rng(5).permutation(n).astype(np.int32)
+    deg = np.diff(g.index)
+    src = perm[np.repeat(np.arange(n), deg)]
+    dst = perm[g.neighbors]
+    h = hc.HostGraph.from_edges(src, dst, n, weights=g.weights, symmetrize=False,
+                                relabel=False)
+    sources = perm[g.pick_sources(64, seed=3)]
+    os.environ["HC_FORCE_OFF64"] = "1"
+    try:
+        dg = hc.DeviceGraph(h)
+    finally:
+        del os.environ["HC_FORCE_OFF64"]
+    with dg:
+        info = dg.info()
+        assert info["offset_bytes"] == 8 and not info["degree_sorted"]
+        r = hc.sssp_multi(dg, sources)
+    _check_columns(oracle, h, sources, r.values)
+
+
+def test_multi_rejects_bad_input(hc, gpu, kron14):
+    with hc.DeviceGraph(kron14) as dg:
+        with pytest.raises(hc.HcError):
+            hc.sssp_multi(dg, np.arange(65))
+        with pytest.raises(hc.HcError):
+            hc.sssp_multi(dg, [kron14.num_nodes])
