@@ -149,6 +149,12 @@ _SIGS = {
     "hc_get_tuning": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
     "hc_reset_tuning": ([], C.c_int),
     "hc_sssp_multi": ([_P, _P, C.c_int32, _P, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
+    "hc_graph_batch_upload": ([C.POINTER(HostGraphStruct), C.c_int32, C.POINTER(_P)], C.c_int),
+    "hc_graph_batch_free": ([_P], C.c_int),
+    "hc_graph_batch_info": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                             C.POINTER(C.c_int64), C.POINTER(C.c_size_t)], C.c_int),
+    "hc_graph_batch_node_offsets": ([_P, _P], C.c_int),
+    "hc_bfs_many": ([_P, _P, _P, _P, C.c_int32, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
     "hc_kernel_from_string": ([C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
     "hc_kernel_to_string": ([C.c_int32], C.c_char_p),
     "hc_kernel_device_type": ([C.c_int32, C.POINTER(C.c_int32)], C.c_int),
@@ -562,6 +568,60 @@ def sssp_multi(g: DeviceGraph, sources, out: DeviceArray | None = None,
     if fetch:
         vals = d.get().reshape(g.num_nodes, MULTI_WIDTH)[:, :src.size].T.copy()
     return Result(vals, st.as_dict())
+
+
+class GraphBatch:
+    """Many host graphs resident in HBM as one batch (hc_graph_batch_upload)."""
+
+    def __init__(self, graphs: list[HostGraph]):
+        arr = (HostGraphStruct * len(graphs))(*[g._s for g in graphs])
+        self.h = _P()
+        check(lib().hc_graph_batch_upload(arr, len(graphs), C.byref(self.h)))
+        ng, tn, te, nb = C.c_int32(), C.c_int64(), C.c_int64(), C.c_size_t()
+        check(lib().hc_graph_batch_info(self.h, C.byref(ng), C.byref(tn), C.byref(te),
+                                        C.byref(nb)))
+        self.num_graphs, self.total_nodes, self.total_edges = ng.value, tn.value, te.value
+        self.device_bytes = nb.value
+        self.node_offsets = np.zeros(self.num_graphs + 1, np.int64)
+        check(lib().hc_graph_batch_node_offsets(self.h, _ptr(self.node_offsets)))
+
+    def bfs(self, sources, alpha: int = 15, beta: int = 18, want_parents: bool = True,
+            depth: DeviceArray | None = None, parents: DeviceArray | None = None,
+            fetch: bool = True):
+        """One BFS per graph in one launch; returns (depths, parents, stats)
+        with per-graph arrays split by node_offsets."""
+        src = np.ascontiguousarray(sources, np.int32)
+        dd = depth if depth is not None else DeviceArray(self.total_nodes, np.int32)
+        dp = parents if parents is not None else (
+            DeviceArray(self.total_nodes, np.int32) if want_parents else None)
+        st = RunStats()
+        check(lib().hc_bfs_many(self.h, _ptr(src), dd.ptr, dp.ptr if dp else None, alpha,
+                                beta, C.byref(st), None))
+        if not fetch:
+            return None, None, st.as_dict()
+        o = self.node_offsets
+        d = dd.get()
+        p = dp.get() if dp is not None else None
+        ds = [d[o[i]:o[i + 1]] for i in range(self.num_graphs)]
+        ps = [p[o[i]:o[i + 1]] for i in range(self.num_graphs)] if p is not None else None
+        return ds, ps, st.as_dict()
+
+    def free(self):
+        if self.h:
+            lib().hc_graph_batch_free(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def epoch_trace(g: DeviceGraph, algorithm: str) -> list[dict]:

@@ -29,6 +29,31 @@ int hc_sssp_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources,
                   int32_t *dist, int32_t max_epochs, hc_run_stats_t *stats,
                   hc_stream_t stream);
 
+/* Many graphs resident in HBM as one batch: concatenated 4-byte offsets and
+ * neighbour ids (each graph below 2^32 edges), descriptors and a
+ * longest-first work order.  Graph i's vertex v is batch vertex
+ * node_offsets[i] + v. */
+typedef struct hc_graph_batch_s *hc_graph_batch_t;
+int hc_graph_batch_upload(const hc_host_graph_t *graphs, int32_t num_graphs,
+                          hc_graph_batch_t *out);
+int hc_graph_batch_free(hc_graph_batch_t b);
+int hc_graph_batch_info(hc_graph_batch_t b, int32_t *num_graphs,
+                        int64_t *total_nodes, int64_t *total_edges,
+                        size_t *device_bytes);
+/* out: num_graphs + 1 prefix sums of the vertex counts. */
+int hc_graph_batch_node_offsets(hc_graph_batch_t b, int64_t *out);
+
+/* One direction-optimizing BFS per graph (bfs_do_cpu's rule with alpha,
+ * beta; 0 selects 15 and 18), sources[i] (host) in graph i, all graphs in
+ * one persistent launch: each workgroup runs whole BFSs with the bitmaps of
+ * its graph in LDS.  depth (and parents, may be NULL): device arrays of
+ * total_nodes int32 laid out by node_offsets.  Depths are identical to
+ * hc_bfs per graph; parents are valid BFS parents.  stats: push_epochs /
+ * pull_epochs summed over graphs, ms of the launch. */
+int hc_bfs_many(hc_graph_batch_t b, const int32_t *sources, int32_t *depth,
+                int32_t *parents, int32_t alpha, int32_t beta,
+                hc_run_stats_t *stats, hc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

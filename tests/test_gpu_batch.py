@@ -106,3 +106,84 @@ def test_multi_rejects_bad_input(hc, gpu, kron14):
             hc.sssp_multi(dg, np.arange(65))
         with pytest.raises(hc.HcError):
             hc.sssp_multi(dg, [kron14.num_nodes])
+
+
+def _graph_set(hc, count, seed, scales=(6, 12), efs=(2, 32)):
+    rng = np.random.RandomState(seed)
+    graphs = []
+    for i in range(count):
+        kind = "kron" if i % 3 else "uniform"
+        graphs.append(hc.HostGraph.generate(kind, int(rng.randint(scales[0], scales[1] + 1)),
+                                            int(rng.randint(efs[0], efs[1] + 1)),
+                                            weighted=False, seed=1000 + i))
+    return graphs
+
+
+def _check_many(oracle, graphs, sources, depths, parents, stats):
+    pushes = pulls = 0
+    for i, (g, s) in enumerate(zip(graphs, sources)):
+        ref = oracle.bfs_depth(g.index, g.neighbors, int(s))
+        assert np.array_equal(depths[i], ref), i
+        assert oracle.verify_parents(ref, parents[i], int(s)) == 0, i
+        _, _, dirs = oracle.bfs_do(g.index, g.neighbors, int(s))
+        pushes += int((dirs == 0).sum())
+        pulls += int((dirs == 1).sum())
+    assert stats["push_epochs"] == pushes
+    assert stats["pull_epochs"] == pulls
+
+
+def test_bfs_many_mixed_sizes(hc, gpu, oracle):
+    graphs = _graph_set(hc, 48, 2024)
+    sources = [int(g.pick_sources(1, seed=i)[0]) for i, g in enumerate(graphs)]
+    with hc.GraphBatch(graphs) as b:
+        assert b.total_edges == sum(g.num_edges for g in graphs)
+        depths, parents, st = b.bfs(sources)
+        _check_many(oracle, graphs, sources, depths, parents, st)
+        # A second call with other sources reuses the resident batch.
+        sources2 = [int(g.pick_sources(2, seed=i)[1]) for i, g in enumerate(graphs)]
+        depths2, parents2, st2 = b.bfs(sources2)
+        _check_many(oracle, graphs, sources2, depths2, parents2, st2)
+
+
+def test_bfs_many_matches_single_graph_runs(hc, gpu):
+    graphs = _graph_set(hc, 6, 77, scales=(10, 13), efs=(8, 32))
+    sources = [int(g.pick_sources(1)[0]) for g in graphs]
+    with hc.GraphBatch(graphs) as b:
+        depths, _, st = b.bfs(sources, want_parents=False)
+    for g, s, d in zip(graphs, sources, depths):
+        with hc.DeviceGraph(g) as dg:
+            assert np.array_equal(hc.bfs(dg, s).values, d)
+
+
+def test_bfs_many_large_graph_global_bitmaps(hc, gpu, oracle):
+    """A graph above the LDS bitmap limit (2^17 vertices) next to small ones."""
+    big = hc.HostGraph.generate("kron", 18, 4, weighted=False, seed=5)
+    graphs = [big] + _graph_set(hc, 5, 9)
+    sources = [int(g.pick_sources(1)[0]) for g in graphs]
+    with hc.GraphBatch(graphs) as b:
+        depths, parents, st = b.bfs(sources)
+    _check_many(oracle, graphs, sources, depths, parents, st)
+
+
+def test_bfs_many_golden_and_isolated_source(hc, gpu):
+    names = ["path10.npz", "star9.npz", "cycle12.npz", "complete6.npz", "disconnected.npz"]
+    fxs = [load_golden(n) for n in names]
+    graphs = [hc.HostGraph.from_csr(fx["index"], fx["neighbors"]) for fx in fxs]
+    sources = [int(fx["sources"][0]) for fx in fxs]
+    # An isolated vertex as source: only itself at depth 0.
+    iso = hc.HostGraph.from_edges([0, 1], [1, 2], 5)
+    graphs.append(iso)
+    sources.append(4)
+    with hc.GraphBatch(graphs) as b:
+        depths, parents, _ = b.bfs(sources)
+    for i, fx in enumerate(fxs):
+        assert np.array_equal(depths[i], fx["depth"][0]), names[i]
+    assert depths[-1].tolist() == [-1, -1, -1, -1, 0]
+    assert parents[-1].tolist() == [-1, -1, -1, -1, 4]
+
+
+def test_bfs_many_rejects_bad_sources(hc, gpu):
+    graphs = _graph_set(hc, 2, 1)
+    with hc.GraphBatch(graphs) as b:
+        with pytest.raises(hc.HcError):
+            b.bfs([0, graphs[1].num_nodes])
