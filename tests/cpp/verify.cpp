@@ -1,0 +1,255 @@
+// Verification program for the C++ drop-in layer, in the shape of the
+// reference's tests (tests/sssp/sssp_verify.cu, tests/bfs/bfs_verify.cu,
+// tests/pr/pr_verify.cu): every registered kernel is run through the
+// reference-named driver and checked against the oracle.  Unlike the
+// reference's verifiers (which always exit 0), any mismatch exits 1.
+//
+//   verify cpu            host-core drivers only (no GPU needed)
+//   verify gpu            GPU and heterogeneous drivers
+//   verify skd <file>     prints the parsed schedule blocks
+#include <unistd.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/hc/kernels.h"
+#include "../../oracle/oracle.h"
+
+namespace {
+
+int g_failures = 0;
+
+void expect(bool ok, const std::string &what) {
+    std::printf("%s %s\n", ok ? "PASS" : "FAIL", what.c_str());
+    if (!ok) g_failures++;
+}
+
+template <typename NeighborT>
+std::vector<int32_t> ids_of(const GenericCSRGraph<NeighborT> &g) {
+    std::vector<int32_t> v(g.num_edges);
+    for (offset_t i = 0; i < g.num_edges; i++) {
+        if constexpr (std::is_same<NeighborT, wnode_t>::value)
+            v[i] = g.neighbors[i].v;
+        else
+            v[i] = g.neighbors[i];
+    }
+    return v;
+}
+
+std::vector<int32_t> weights_of(const CSRWGraph &g) {
+    std::vector<int32_t> w(g.num_edges);
+    for (offset_t i = 0; i < g.num_edges; i++) w[i] = g.neighbors[i].w;
+    return w;
+}
+
+std::vector<int32_t> dijkstra(const CSRWGraph &g, nid_t src) {
+    auto ids = ids_of(g);
+    auto w = weights_of(g);
+    std::vector<int32_t> d(g.num_nodes);
+    or_sssp_dijkstra(g.num_nodes, g.index, ids.data(), w.data(), src, d.data());
+    return d;
+}
+
+std::vector<int32_t> depth_of(const CSRUWGraph &g, nid_t src) {
+    std::vector<int32_t> d(g.num_nodes);
+    or_bfs_depth(g.num_nodes, g.index, g.neighbors, src, d.data());
+    return d;
+}
+
+bool same(const weight_t *a, const std::vector<int32_t> &b) {
+    return std::memcmp(a, b.data(), sizeof(int32_t) * b.size()) == 0;
+}
+
+double l1_vs_f32(const float *s, const CSRUWGraph &g, int iters) {
+    std::vector<float> ref(g.num_nodes);
+    or_pr_jacobi_f32(g.num_nodes, g.index, g.neighbors, 0.85f, iters, ref.data());
+    double l1 = 0;
+    for (nid_t u = 0; u < g.num_nodes; u++) l1 += std::fabs((double)s[u] - (double)ref[u]);
+    return l1;
+}
+
+// Unweighted copy of a weighted graph (same CSR).
+CSRUWGraph unweighted(const CSRWGraph &g) {
+    CSRUWGraph u;
+    u.num_nodes = g.num_nodes;
+    u.num_edges = g.num_edges;
+    u.index = new offset_t[(size_t)g.num_nodes + 1];
+    std::memcpy(u.index, g.index, sizeof(offset_t) * ((size_t)g.num_nodes + 1));
+    u.neighbors = new nid_t[(size_t)g.num_edges];
+    for (offset_t i = 0; i < g.num_edges; i++) u.neighbors[i] = g.neighbors[i].v;
+    return u;
+}
+
+int iterations_to_converge(const CSRUWGraph &g) {
+    // Iterations the 1e-4 FP64-residual rule takes on this graph (FP64
+    // oracle), to compare the FP32 scores at the same iteration count.
+    std::vector<double> a(g.num_nodes), b(g.num_nodes);
+    for (int it = 1; it <= 20; it++) {
+        or_pr_jacobi_f64(g.num_nodes, g.index, g.neighbors, 0.85, it, a.data());
+        or_pr_jacobi_f64(g.num_nodes, g.index, g.neighbors, 0.85, it - 1, b.data());
+        double r = 0;
+        for (nid_t u = 0; u < g.num_nodes; u++) r += std::fabs(a[u] - b[u]);
+        if (r < 1e-4) return it;
+    }
+    return 20;
+}
+
+void cpu_checks() {
+    CSRWGraph g = generate_graph<CSRWGraph>(false, 11, 16);
+    CSRUWGraph ug = unweighted(g);
+    SourcePicker<CSRWGraph> sp(&g);
+    std::vector<int32_t> picks(4);
+    {
+        hc::HostView<wnode_t> v(g);
+        hc::check_or_exit(hc_pick_sources(&v.c, 37525, 4, picks.data()), "hc_pick_sources");
+    }
+    bool picker_ok = true;
+    for (int i = 0; i < 4; i++) picker_ok &= sp.next_vertex() == picks[i];
+    expect(picker_ok, "SourcePicker sequence equals hc_pick_sources(37525)");
+
+    nid_t *seg = compute_equal_edge_ranges(g, 8);
+    std::vector<int32_t> oseg(9);
+    or_equal_edge_ranges(g.num_nodes, g.index, 8, 1, oseg.data());
+    bool seg_ok = seg[0] == 0 && seg[8] == g.num_nodes;
+    for (int i = 0; i < 8; i++) seg_ok &= seg[i] <= seg[i + 1];
+    expect(seg_ok, "compute_equal_edge_ranges: monotone cover of [0,N)");
+    delete[] seg;
+
+    sp.reset();
+    for (int k = 0; k < 2; k++) {
+        const nid_t src = sp.next_vertex();
+        std::vector<weight_t> init(g.num_nodes, INF_WEIGHT);
+        init[src] = 0;
+        for (SSSPCPU ker : get_kernels(SSSPCPU::undefined)) {
+            weight_t *dist = nullptr;
+            sssp_pull_cpu(g, get_kernel(ker), init.data(), &dist);
+            expect(same(dist, dijkstra(g, src)), to_string(ker) + " source " + std::to_string(src));
+            delete[] dist;
+        }
+        nid_t *parents = nullptr;
+        bfs_do_cpu<15, 18>(ug, src, &parents);
+        auto depth = depth_of(ug, src);
+        expect(or_verify_parents(ug.num_nodes, depth.data(), parents, src) == 0,
+               "bfs_do_cpu<15,18> parents, source " + std::to_string(src));
+        delete[] parents;
+    }
+    float *score = nullptr;
+    pr_pull_cpu(g, PRCPU::one_to_one, nullptr, &score);
+    const double l1 = l1_vs_f32(score, ug, iterations_to_converge(ug));
+    expect(l1 <= 1e-6, "pr_pull_cpu within 1e-6 L1 of the FP32 reference (" + std::to_string(l1) + ")");
+    delete[] score;
+}
+
+const char *kSchedule =
+    "AMD Instinct MI355X\n0 2\n%s 0 0\nHost CPU\n3 3\n%s\nAMD Instinct MI355X\n4 7\n%s 64 256\n";
+
+std::string write_schedule(const char *gpu_kernel, const char *cpu_kernel) {
+    char text[512];
+    std::snprintf(text, sizeof(text), kSchedule, gpu_kernel, cpu_kernel, gpu_kernel);
+    std::string path = std::string(std::getenv("TMPDIR") ? std::getenv("TMPDIR") : "/tmp") +
+                       "/hc_verify_" + std::to_string(::getpid()) + ".skd";
+    std::ofstream(path) << text;
+    return path;
+}
+
+void gpu_checks() {
+    CSRWGraph g = generate_graph<CSRWGraph>(false, 13, 16);
+    CSRUWGraph ug = unweighted(g);
+    SourcePicker<CSRWGraph> sp(&g);
+    for (int k = 0; k < 2; k++) {
+        const nid_t src = sp.next_vertex();
+        const auto ref = dijkstra(g, src);
+        std::vector<weight_t> init(g.num_nodes, INF_WEIGHT);
+        init[src] = 0;
+        for (SSSPGPU ker : get_kernels(SSSPGPU::undefined)) {
+            weight_t *dist = nullptr;
+            sssp_pull_gpu(g, get_kernel(ker), init.data(), &dist, 64, 256);
+            expect(same(dist, ref), to_string(ker) + " source " + std::to_string(src));
+            delete[] dist;
+        }
+        weight_t *dist = nullptr;
+        sssp_gpu(g, src, &dist);
+        expect(same(dist, ref), "sssp_gpu (push/pull) source " + std::to_string(src));
+        delete[] dist;
+
+        const auto depth = depth_of(ug, src);
+        for (BFSGPU ker : get_kernels(BFSGPU::undefined)) {
+            nid_t *parents = nullptr;
+            bfs_gpu(ug, src, &parents, get_kernel(ker), 64, 256);
+            expect(or_verify_parents(ug.num_nodes, depth.data(), parents, src) == 0,
+                   to_string(ker) + " parents, source " + std::to_string(src));
+            delete[] parents;
+        }
+        {
+            nid_t *parents = nullptr;
+            bfs_do_gpu<15, 18>(ug, src, &parents);
+            expect(or_verify_parents(ug.num_nodes, depth.data(), parents, src) == 0,
+                   "bfs_do_gpu<15,18> parents, source " + std::to_string(src));
+            delete[] parents;
+        }
+
+        const std::string skd = write_schedule("SSSP GPU warp-red", "SSSP CPU one-to-one");
+        sssp_pull_heterogeneous(g, init.data(), &dist, skd.c_str());
+        expect(same(dist, ref), "sssp_pull_heterogeneous (.skd) source " + std::to_string(src));
+        delete[] dist;
+        std::remove(skd.c_str());
+
+        const std::string bskd = write_schedule("BFS GPU bucketed", "BFS CPU pull");
+        nid_t *parents = nullptr;
+        bfs_do_heterogeneous(ug, src, &parents, bskd.c_str());
+        expect(or_verify_parents(ug.num_nodes, depth.data(), parents, src) == 0,
+               "bfs_do_heterogeneous (.skd) parents, source " + std::to_string(src));
+        delete[] parents;
+        std::remove(bskd.c_str());
+    }
+    const int iters = iterations_to_converge(ug);
+    for (PRGPU ker : get_kernels(PRGPU::undefined)) {
+        float *score = nullptr;
+        pr_pull_gpu(g, get_kernel(ker), nullptr, &score);
+        const double l1 = l1_vs_f32(score, ug, iters);
+        expect(l1 <= 1e-6, to_string(ker) + " within 1e-6 L1 (" + std::to_string(l1) + ")");
+        delete[] score;
+    }
+    const std::string pskd = write_schedule("PR GPU bucketed", "PR CPU one-to-one");
+    float *score = nullptr;
+    pr_pull_heterogeneous(g, nullptr, &score, pskd.c_str());
+    const double l1 = l1_vs_f32(score, ug, iters);
+    expect(l1 <= 1e-6, "pr_pull_heterogeneous (.skd) within 1e-6 L1 (" + std::to_string(l1) + ")");
+    delete[] score;
+    std::remove(pskd.c_str());
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const std::string mode = argc > 1 ? argv[1] : "cpu";
+    if (mode == "cpu") {
+        cpu_checks();
+    } else if (mode == "gpu") {
+        gpu_checks();
+    } else if (mode == "skd" && argc > 2) {
+        hc_schedule_t s = nullptr;
+        hc::check_or_exit(hc_schedule_load(argv[2], &s), "hc_schedule_load");
+        int32_t nb = 0, ns = 0;
+        hc_schedule_info(s, &nb, &ns);
+        std::printf("segments %d blocks %d\n", ns, nb);
+        for (int i = 0; i < nb; i++) {
+            hc_schedule_block_t b;
+            hc_schedule_get_block(s, i, &b);
+            std::printf("%s#%d [%d,%d] %s %d %d\n", hc_device_to_string(b.device), b.device_id,
+                        b.start_segment, b.end_segment, hc_kernel_to_string(b.kernel), b.blocks,
+                        b.threads);
+        }
+        hc_schedule_free(s);
+    } else {
+        std::fprintf(stderr, "usage: verify cpu|gpu|skd <file>\n");
+        return 2;
+    }
+    std::printf("%d failure(s)\n", g_failures);
+    return g_failures ? 1 : 0;
+}
