@@ -95,6 +95,24 @@ class ScheduleBlock(C.Structure):
                 ("kernel", C.c_int32), ("blocks", C.c_int32), ("threads", C.c_int32)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("memory_budget", C.c_int64)]
+
+
+class QueryInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("kind", C.c_int32), ("graph", C.c_int32),
+                ("source", C.c_int32), ("batch_size", C.c_int32), ("slot", C.c_int32),
+                ("num_nodes", C.c_int64), ("num_edges", C.c_int64), ("queue_ms", C.c_double),
+                ("run_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("batches", C.c_int64), ("batched_queries", C.c_int64),
+                ("uploads", C.c_int64), ("evictions", C.c_int64),
+                ("bytes_in_use", C.c_int64), ("bytes_peak", C.c_int64)]
+
+
 class PrOpts(C.Structure):
     _fields_ = [("damping", C.c_float), ("tolerance", C.c_double),
                 ("max_iters", C.c_int32)]
@@ -149,12 +167,24 @@ _SIGS = {
     "hc_get_tuning": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
     "hc_reset_tuning": ([], C.c_int),
     "hc_sssp_multi": ([_P, _P, C.c_int32, _P, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
+    "hc_bfs_multi": ([_P, _P, C.c_int32, _P, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
     "hc_graph_batch_upload": ([C.POINTER(HostGraphStruct), C.c_int32, C.POINTER(_P)], C.c_int),
     "hc_graph_batch_free": ([_P], C.c_int),
     "hc_graph_batch_info": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                              C.POINTER(C.c_int64), C.POINTER(C.c_size_t)], C.c_int),
     "hc_graph_batch_node_offsets": ([_P, _P], C.c_int),
     "hc_bfs_many": ([_P, _P, _P, _P, C.c_int32, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
+    "hc_graph_share": ([_P, C.POINTER(_P)], C.c_int),
+    "hc_multi_extract": ([_P, C.c_int32, C.c_int32, C.c_int32, _P, _P], C.c_int),
+    "hc_server_create": ([C.POINTER(ServerOpts), C.POINTER(_P)], C.c_int),
+    "hc_server_destroy": ([_P], C.c_int),
+    "hc_server_add_graph": ([_P, C.POINTER(HostGraphStruct), C.c_int32,
+                             C.POINTER(C.c_int32)], C.c_int),
+    "hc_server_submit": ([_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64)], C.c_int),
+    "hc_server_wait": ([_P, C.c_int64, C.POINTER(QueryInfo)], C.c_int),
+    "hc_server_fetch": ([_P, C.c_int64, _P], C.c_int),
+    "hc_server_release": ([_P, C.c_int64], C.c_int),
+    "hc_server_get_stats": ([_P, C.POINTER(ServerStats)], C.c_int),
     "hc_kernel_from_string": ([C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
     "hc_kernel_to_string": ([C.c_int32], C.c_char_p),
     "hc_kernel_device_type": ([C.c_int32, C.POINTER(C.c_int32)], C.c_int),
@@ -624,6 +654,21 @@ class GraphBatch:
             pass
 
 
+def bfs_multi(g: DeviceGraph, sources, out: DeviceArray | None = None,
+              max_epochs: int = 0, fetch: bool = True) -> Result:
+    """BFS depths from up to 64 sources in one batch (hc_bfs_multi).  values
+    is the [num_sources, N] depth matrix with -1 for unreachable vertices."""
+    src = np.ascontiguousarray(sources, np.int32)
+    d = out if out is not None else DeviceArray(g.num_nodes * MULTI_WIDTH, np.int32)
+    st = RunStats()
+    check(lib().hc_bfs_multi(g.h, _ptr(src), src.size, d.ptr, max_epochs, C.byref(st), None))
+    vals = None
+    if fetch:
+        vals = d.get().reshape(g.num_nodes, MULTI_WIDTH)[:, :src.size].T.copy()
+        vals[vals == INF_DIST] = -1
+    return Result(vals, st.as_dict())
+
+
 def epoch_trace(g: DeviceGraph, algorithm: str) -> list[dict]:
     """Per-epoch record of the last sssp/bfs run on g."""
     t = EpochTrace()
@@ -784,5 +829,70 @@ class Hetero:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Serving loop
+# ---------------------------------------------------------------------------
+
+QUERY_BFS, QUERY_SSSP = 0, 1
+
+
+class Server:
+    """Resident/dynamic graphs and continuously batched BFS/SSSP queries
+    (hc_server_*).  Host graphs added here are kept alive by the object."""
+
+    def __init__(self, slots: int = 4, max_batch: int = 64, memory_budget: int = 0):
+        self.h = _P()
+        o = ServerOpts(slots, max_batch, memory_budget)
+        check(lib().hc_server_create(C.byref(o), C.byref(self.h)))
+        self._graphs = []
+
+    def add_graph(self, g: HostGraph, resident: bool = True) -> int:
+        gid = C.c_int32()
+        check(lib().hc_server_add_graph(self.h, g.struct_ref(), int(resident), C.byref(gid)))
+        self._graphs.append(g)
+        return gid.value
+
+    def submit(self, graph: int, kind: int, source: int) -> int:
+        t = C.c_int64()
+        check(lib().hc_server_submit(self.h, graph, kind, int(source), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        i = QueryInfo()
+        check(lib().hc_server_wait(self.h, ticket, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def fetch(self, ticket: int) -> np.ndarray:
+        info = self.wait(ticket)
+        out = np.empty(info["num_nodes"], np.int32)
+        check(lib().hc_server_fetch(self.h, ticket, _ptr(out)))
+        return out
+
+    def release(self, ticket: int):
+        check(lib().hc_server_release(self.h, ticket))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().hc_server_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            lib().hc_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

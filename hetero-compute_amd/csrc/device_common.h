@@ -329,6 +329,7 @@ struct hc_graph_s {
     bool weighted = false;
     bool sorted = false;    // degrees non-increasing over the held rows
     bool off64 = false;
+    bool shared = false;    // device arrays belong to another handle
     int32_t num_nonzero = 0;
     int64_t max_degree = 0;
     void *d_off = nullptr;

@@ -64,6 +64,7 @@ struct MultiArgs {
     uint32_t *chg[3];
     MultiCtl *ctl;
     int32_t n;
+    int32_t unit;  // 1: every edge weighs 1 (breadth-first depths)
 };
 
 template <typename OffT>
@@ -101,7 +102,7 @@ __device__ __forceinline__ uint32_t relax_group(const MultiArgs<OffT> &a,
         if (in) {
             v = ldnt(&a.g.adj[i]);
             m = flag_of(cur, v);
-            if (m) w = a.g.wt[i];
+            if (m) w = a.unit ? 1 : a.g.wt[i];
         }
         uint64_t bal = __ballot(m != 0);
         bytes += 5ull * (unsigned long long)__popcll(__ballot(in));
@@ -322,13 +323,16 @@ struct MultiWs {
     int words = 0;
     int grid = 0;
     const PlanHost *plan = nullptr;
-    hipStream_t gstream = nullptr;  // stream the graph was captured on
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
+    // Captured epoch graphs, [0] weighted, [1] unit weights.
+    hipStream_t gstream[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
     KernelTimer kt;
     ~MultiWs() {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
+        for (int k = 0; k < 2; k++) {
+            if (exec[k]) hipGraphExecDestroy(exec[k]);
+            if (graph[k]) hipGraphDestroy(graph[k]);
+        }
         if (flags) hipFree(flags);
         if (ctl) hipFree(ctl);
         if (h_ctl) hipHostFree(h_ctl);
@@ -397,11 +401,11 @@ static int launch_multi_epochs(const MultiArgs<OffT> &a, int grid, int epochs,
 
 template <typename OffT>
 static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *dist,
-                     int max_epochs, hc_run_stats_t *stats, hipStream_t s) {
+                     int max_epochs, int unit, hc_run_stats_t *stats, hipStream_t s) {
     MultiWs *ws = nullptr;
     HC_TRY(get_multi_ws(g, &ws));
     uint32_t *f[3] = {ws->flags, ws->flags + ws->words, ws->flags + 2 * (size_t)ws->words};
-    MultiArgs<OffT> a{ws->plan->dev, view<OffT>(g), {f[0], f[1], f[2]}, ws->ctl, g->n};
+    MultiArgs<OffT> a{ws->plan->dev, view<OffT>(g), {f[0], f[1], f[2]}, ws->ctl, g->n, unit};
     const int grid = ws->grid;
     const unsigned long long sweep =
         (unsigned long long)sizeof(OffT) * (g->n + 1ull) + 4ull * ws->words;
@@ -416,11 +420,11 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
 
     ws->kt.reset();
     const bool use_graph = tuning().multi_graph != 0 && !ws->kt.on;
-    if (use_graph && (!ws->exec || ws->gstream != s)) {
-        if (ws->exec) hipGraphExecDestroy(ws->exec);
-        if (ws->graph) hipGraphDestroy(ws->graph);
-        ws->exec = nullptr;
-        ws->graph = nullptr;
+    if (use_graph && (!ws->exec[unit] || ws->gstream[unit] != s)) {
+        if (ws->exec[unit]) hipGraphExecDestroy(ws->exec[unit]);
+        if (ws->graph[unit]) hipGraphDestroy(ws->graph[unit]);
+        ws->exec[unit] = nullptr;
+        ws->graph[unit] = nullptr;
         // The kernels read dist and the epoch from the controller, so one
         // captured graph serves every call on this graph handle.
         HC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -429,9 +433,9 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
         hipError_t ce = hipStreamEndCapture(s, &graph);
         if (rc != HC_OK) return rc;
         if (ce != hipSuccess) return fail(HC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
-        ws->graph = graph;
-        HC_HIP(hipGraphInstantiate(&ws->exec, graph, nullptr, nullptr, 0));
-        ws->gstream = s;
+        ws->graph[unit] = graph;
+        HC_HIP(hipGraphInstantiate(&ws->exec[unit], graph, nullptr, nullptr, 0));
+        ws->gstream[unit] = s;
     }
     // Relaunch until the controller reports convergence; the done flag of
     // batch b-1 is read while batch b runs.
@@ -446,7 +450,7 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
     int rc = HC_OK;
     for (int b = 0; !done && launched < cap; b++) {
         if (use_graph) {
-            if (hipGraphLaunch(ws->exec, s) != hipSuccess) {
+            if (hipGraphLaunch(ws->exec[unit], s) != hipSuccess) {
                 rc = fail(HC_ERR_HIP, "graph launch failed");
                 break;
             }
@@ -489,11 +493,20 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
 
 using namespace hc;
 
-extern "C" int hc_sssp_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources,
-                             int32_t *dist, int32_t max_epochs, hc_run_stats_t *stats,
-                             hc_stream_t stream) {
+__global__ void multi_extract_kernel(const int32_t *__restrict__ mat, int32_t n, int col,
+                                     int32_t *__restrict__ out, int minus1) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < (size_t)n; v += stride) {
+        const int32_t x = mat[v * kQ + col];
+        out[v] = (minus1 && x == HC_INF_DIST) ? -1 : x;
+    }
+}
+
+static int multi_entry(hc_graph_t g, const int32_t *sources, int32_t num_sources,
+                       int32_t *dist, int32_t max_epochs, int unit, hc_run_stats_t *stats,
+                       hc_stream_t stream) {
     HC_CHECK_ARG(g && sources && dist, "NULL argument");
-    HC_CHECK_ARG(g->weighted, "SSSP needs a weighted graph");
+    HC_CHECK_ARG(unit || g->weighted, "SSSP needs a weighted graph");
     HC_CHECK_ARG(g->row0 == 0 && g->rows == g->n, "batched runs need a whole-graph handle");
     HC_CHECK_ARG(num_sources >= 1 && num_sources <= kQ, "num_sources must be in [1, %d]", kQ);
     for (int q = 0; q < num_sources; q++)
@@ -502,6 +515,35 @@ extern "C" int hc_sssp_multi(hc_graph_t g, const int32_t *sources, int32_t num_s
     HC_TRY(ensure_init());
     std::lock_guard<std::mutex> lk(g->mu);
     hipStream_t s = pick_stream(stream);
-    return g->off64 ? run_multi<uint64_t>(g, sources, num_sources, dist, max_epochs, stats, s)
-                    : run_multi<uint32_t>(g, sources, num_sources, dist, max_epochs, stats, s);
+    return g->off64
+               ? run_multi<uint64_t>(g, sources, num_sources, dist, max_epochs, unit, stats, s)
+               : run_multi<uint32_t>(g, sources, num_sources, dist, max_epochs, unit, stats, s);
 }
+
+extern "C" {
+
+int hc_sssp_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources, int32_t *dist,
+                  int32_t max_epochs, hc_run_stats_t *stats, hc_stream_t stream) {
+    return multi_entry(g, sources, num_sources, dist, max_epochs, 0, stats, stream);
+}
+
+int hc_bfs_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources, int32_t *depth,
+                 int32_t max_epochs, hc_run_stats_t *stats, hc_stream_t stream) {
+    return multi_entry(g, sources, num_sources, depth, max_epochs, 1, stats, stream);
+}
+
+int hc_multi_extract(const int32_t *matrix, int32_t num_nodes, int32_t column,
+                     int32_t unreached_minus1, int32_t *out, hc_stream_t stream) {
+    HC_CHECK_ARG(matrix && out, "NULL argument");
+    HC_CHECK_ARG(num_nodes >= 0 && column >= 0 && column < kQ, "bad column %d", column);
+    HC_TRY(ensure_init());
+    if (num_nodes == 0) return HC_OK;
+    hipStream_t s = pick_stream(stream);
+    const int grid = std::max(1, std::min(grid_blocks(), (num_nodes + kBlock - 1) / kBlock));
+    multi_extract_kernel<<<grid, kBlock, 0, s>>>(matrix, num_nodes, column, out,
+                                                 unreached_minus1);
+    HC_HIP(hipGetLastError());
+    return HC_OK;
+}
+
+}  // extern "C"

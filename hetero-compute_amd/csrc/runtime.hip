@@ -475,10 +475,36 @@ int hc_graph_free(hc_graph_t g) {
     free_pr_ws(g);
     free_multi_ws(g);
     for (auto &kv : g->plans) hipFree(kv.second.dmem);
-    hipFree(g->d_off);
-    hipFree(g->d_adj);
-    hipFree(g->d_wt);
+    if (!g->shared) {
+        hipFree(g->d_off);
+        hipFree(g->d_adj);
+        hipFree(g->d_wt);
+    }
     delete g;
+    return HC_OK;
+}
+
+int hc_graph_share(hc_graph_t g, hc_graph_t *out) {
+    HC_CHECK_ARG(g && out, "NULL argument");
+    auto *h = new (std::nothrow) hc_graph_s();
+    if (!h) return fail(HC_ERR_OOM, "host allocation failed");
+    std::lock_guard<std::mutex> lk(g->mu);
+    h->n = g->n;
+    h->row0 = g->row0;
+    h->rows = g->rows;
+    h->m = g->m;
+    h->weighted = g->weighted;
+    h->sorted = g->sorted;
+    h->off64 = g->off64;
+    h->num_nonzero = g->num_nonzero;
+    h->max_degree = g->max_degree;
+    h->d_off = g->d_off;
+    h->d_adj = g->d_adj;
+    h->d_wt = g->d_wt;
+    h->bytes = 0;
+    h->h_off = g->h_off;
+    h->shared = true;
+    *out = h;
     return HC_OK;
 }
 

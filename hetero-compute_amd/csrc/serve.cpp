@@ -1,0 +1,524 @@
+// Serving loop: resident and dynamic graphs, BFS / SSSP queries admitted by
+// device-memory footprint and batched as they arrive (continuous batching).
+// Host C++ (g++) over the library's C ABI; one worker thread and HIP stream
+// per slot.  No reference counterpart: the reference runs one query per
+// driver call (src/kernels/gpu/sssp.cuh:42-100, src/kernels/gpu/bfs.cuh:
+// 21-95) and uploads the graph each time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/hc_batch.h"
+#include "../../include/hc_serve.h"
+#include "hc_internal.h"
+
+namespace {
+
+constexpr int kWidth = HC_MULTI_WIDTH;
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Device result storage shared by the queries of one sweep.
+struct Result {
+    int32_t *p = nullptr;
+    size_t bytes = 0;
+    int refs = 0;
+};
+
+struct Query {
+    int32_t graph = 0, kind = 0, source = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    int32_t status = HC_OK;
+    int32_t batch = 0, slot = -1;
+    bool done = false;
+    std::shared_ptr<Result> res;
+    int32_t column = -1;  // column of a batched result, -1 for a single one
+};
+
+struct GraphEntry {
+    hc_host_graph_t host{};
+    bool resident = false;
+    hc_graph_t dev = nullptr;             // owner of the device arrays
+    std::vector<hc_graph_t> per_slot;     // shares, one per slot
+    int64_t bytes = 0;                    // budgeted bytes while uploaded
+    bool uploading = false;
+    int in_use = 0;
+    double last_use = 0;
+};
+
+}  // namespace
+
+struct hc_server_s {
+    hc_server_opts_t opts{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done, cv_mem;
+    std::deque<int64_t> queue;
+    std::map<int64_t, Query> queries;   // stable iterators across inserts
+    std::deque<GraphEntry> graphs;      // stable references across inserts
+    std::vector<std::thread> workers;
+    std::vector<hipStream_t> streams;
+    std::multimap<size_t, int32_t *> pool;  // idle result buffers by size
+    int64_t pooled = 0;                     // bytes held by the pool
+    int64_t in_use = 0;                     // live results + dynamic graphs
+    int64_t next_ticket = 1;
+    int running = 0;
+    int waiting_mem = 0;  // slots holding a batch until memory is released
+    bool stop = false;
+    hc_server_stats_t st{};
+    // Fetch path (client threads, serialised).
+    std::mutex fetch_mu;
+    int32_t *fetch_buf = nullptr;
+    size_t fetch_cap = 0;
+    hipStream_t fetch_stream = nullptr;
+};
+
+namespace {
+
+int64_t node_bytes(const GraphEntry &g, int width) {
+    return (int64_t)g.host.num_nodes * 4 * width;
+}
+
+void note_usage(hc_server_s *s) {
+    s->st.bytes_in_use = s->in_use + s->pooled;
+    s->st.bytes_peak = std::max(s->st.bytes_peak, s->st.bytes_in_use);
+}
+
+// Makes room for `need` more bytes under the budget: trims the buffer pool,
+// then evicts idle dynamic graphs (not `keep`).  Called with mu held; may
+// drop it around device frees.  Returns true when the bytes fit.
+bool make_room(hc_server_s *s, std::unique_lock<std::mutex> &lk, int64_t need, int keep) {
+    const int64_t budget = s->opts.memory_budget;
+    while (s->in_use + s->pooled + need > budget && !s->pool.empty()) {
+        auto it = s->pool.begin();
+        int32_t *p = it->second;
+        s->pooled -= (int64_t)it->first;
+        s->pool.erase(it);
+        lk.unlock();
+        (void)hipFree(p);
+        lk.lock();
+    }
+    while (s->in_use + s->pooled + need > budget) {
+        int victim = -1;
+        for (int i = 0; i < (int)s->graphs.size(); i++) {
+            GraphEntry &g = s->graphs[i];
+            if (i == keep || g.resident || !g.dev || g.in_use || g.uploading) continue;
+            if (victim < 0 || g.last_use < s->graphs[victim].last_use) victim = i;
+        }
+        if (victim < 0) return false;
+        GraphEntry &g = s->graphs[victim];
+        std::vector<hc_graph_t> hs = g.per_slot;
+        hc_graph_t owner = g.dev;
+        g.per_slot.assign(g.per_slot.size(), nullptr);
+        g.dev = nullptr;
+        s->in_use -= g.bytes;
+        s->st.evictions++;
+        lk.unlock();
+        for (hc_graph_t h : hs)
+            if (h) hc_graph_free(h);
+        hc_graph_free(owner);
+        lk.lock();
+    }
+    return true;
+}
+
+int32_t *take_buffer(hc_server_s *s, size_t bytes) {
+    auto it = s->pool.find(bytes);
+    if (it == s->pool.end()) return nullptr;
+    int32_t *p = it->second;
+    s->pool.erase(it);
+    s->pooled -= (int64_t)bytes;
+    return p;
+}
+
+void finish(hc_server_s *s, const std::vector<int64_t> &batch, int status, double t_done) {
+    for (int64_t t : batch) {
+        Query &q = s->queries[t];
+        q.status = status;
+        q.t_done = t_done;
+        q.done = true;
+        s->st.completed++;
+        if (status != HC_OK) s->st.failed++;
+    }
+    s->cv_done.notify_all();
+}
+
+void release_result(hc_server_s *s, std::shared_ptr<Result> &r) {
+    if (!r) return;
+    if (--r->refs == 0) {
+        s->in_use -= (int64_t)r->bytes;
+        if (r->p) {
+            s->pool.emplace(r->bytes, r->p);
+            s->pooled += (int64_t)r->bytes;
+        }
+        r->p = nullptr;
+        s->cv_mem.notify_all();
+    }
+    r.reset();
+}
+
+void worker(hc_server_s *s, int slot) {
+    hipStream_t stream = s->streams[slot];
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        // Admission is in arrival order: while one slot waits for memory no
+        // other slot starts a younger batch, so results of older queries are
+        // the only ones that can hold the budget.
+        s->cv_work.wait(lk, [&] { return s->stop || (!s->queue.empty() && s->waiting_mem == 0); });
+        if (s->queue.empty()) {
+            if (s->stop) break;
+            continue;
+        }
+        // The oldest query and every waiting query of the same graph and
+        // kind, in arrival order, up to max_batch.
+        const int64_t head = s->queue.front();
+        const int gid = s->queries[head].graph, kind = s->queries[head].kind;
+        std::vector<int64_t> batch;
+        for (auto it = s->queue.begin(); it != s->queue.end() &&
+                                         (int)batch.size() < s->opts.max_batch;) {
+            const Query &q = s->queries[*it];
+            if (q.graph == gid && q.kind == kind) {
+                batch.push_back(*it);
+                it = s->queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        GraphEntry &g = s->graphs[gid];
+        g.in_use++;
+        // Admission: result memory (and the graph, if it is not in HBM).
+        const int width = batch.size() > 1 ? kWidth : 1;
+        const int64_t res_bytes = node_bytes(g, width);
+        // Waits for released results when the budget is spent; a client
+        // that holds results while waiting for later queries must release
+        // them first.
+        int status = HC_OK;
+        for (;;) {
+            while (g.uploading) s->cv_mem.wait(lk);
+            const int64_t need = res_bytes + (g.dev ? 0 : g.bytes);
+            if (need > s->opts.memory_budget) {
+                status = hc::fail(HC_ERR_OOM, "query needs %lld bytes, budget %lld",
+                                  (long long)need, (long long)s->opts.memory_budget);
+                break;
+            }
+            const bool fits = make_room(s, lk, need, gid);
+            if (g.uploading) continue;  // another slot began this graph's upload
+            if (fits) break;
+            if (s->stop && s->running == 0) {
+                status = hc::fail(HC_ERR_BUSY, "server stopped with the memory budget spent");
+                break;
+            }
+            s->waiting_mem++;
+            s->cv_mem.wait(lk);
+            s->waiting_mem--;
+        }
+        s->cv_work.notify_all();
+        if (status != HC_OK) {
+            g.in_use--;
+            finish(s, batch, status, now_ms());
+            continue;
+        }
+        s->in_use += res_bytes;
+        bool upload = false;
+        if (!g.dev) {
+            s->in_use += g.bytes;
+            g.uploading = true;
+            upload = true;
+        }
+        note_usage(s);
+        int32_t *buf = take_buffer(s, (size_t)res_bytes);
+        std::vector<int32_t> sources;
+        for (int64_t t : batch) sources.push_back(s->queries[t].source);
+        s->running++;
+        lk.unlock();
+
+        // Device work, outside the lock.
+        if (upload) {
+            hc_graph_t d = nullptr;
+            status = hc_graph_upload(&g.host, &d);
+            lk.lock();
+            g.uploading = false;
+            if (status == HC_OK) {
+                g.dev = d;
+                s->st.uploads++;
+            } else {
+                s->in_use -= g.bytes;
+            }
+            s->cv_mem.notify_all();
+            lk.unlock();
+        }
+        hc_graph_t h = nullptr;
+        if (status == HC_OK) {
+            lk.lock();
+            if (!g.per_slot[slot]) status = hc_graph_share(g.dev, &g.per_slot[slot]);
+            h = g.per_slot[slot];
+            lk.unlock();
+        }
+        if (status == HC_OK && !buf) {
+            void *p = nullptr;
+            status = hipMalloc(&p, (size_t)res_bytes) == hipSuccess
+                         ? HC_OK
+                         : hc::fail(HC_ERR_OOM, "result allocation of %lld bytes failed",
+                                    (long long)res_bytes);
+            buf = static_cast<int32_t *>(p);
+        }
+        const double t_start = now_ms();
+        if (status == HC_OK) {
+            hc_run_stats_t rs{};
+            if (width == 1 && kind == HC_QUERY_BFS) {
+                hc_bfs_opts_t o{15, 18, 0};
+                status = hc_bfs(h, sources[0], buf, nullptr, &o, &rs, stream);
+            } else if (width == 1) {
+                status = hc_sssp(h, sources[0], buf, nullptr, &rs, stream);
+            } else if (kind == HC_QUERY_BFS) {
+                status = hc_bfs_multi(h, sources.data(), (int32_t)sources.size(), buf, 0, &rs,
+                                      stream);
+            } else {
+                status = hc_sssp_multi(h, sources.data(), (int32_t)sources.size(), buf, 0, &rs,
+                                       stream);
+            }
+            if (status == HC_OK && hipStreamSynchronize(stream) != hipSuccess)
+                status = hc::fail(HC_ERR_HIP, "stream synchronisation failed");
+        }
+        const double t_done = now_ms();
+
+        lk.lock();
+        s->running--;
+        g.in_use--;
+        g.last_use = t_done;
+        auto res = std::make_shared<Result>();
+        res->p = buf;
+        res->bytes = (size_t)res_bytes;
+        res->refs = (int)batch.size();
+        if (status != HC_OK) {
+            res->refs = 1;
+            std::shared_ptr<Result> r = res;
+            release_result(s, r);
+        }
+        for (size_t i = 0; i < batch.size(); i++) {
+            Query &q = s->queries[batch[i]];
+            q.t_start = t_start;
+            q.batch = (int32_t)batch.size();
+            q.slot = slot;
+            if (status == HC_OK) {
+                q.res = res;
+                q.column = width == 1 ? -1 : (int32_t)i;
+            }
+        }
+        s->st.batches++;
+        if (batch.size() > 1) s->st.batched_queries += (int64_t)batch.size();
+        finish(s, batch, status, t_done);
+        note_usage(s);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int hc_server_create(const hc_server_opts_t *opts, hc_server_t *out) {
+    HC_CHECK_ARG(out, "NULL argument");
+    HC_TRY(hc_init(0));
+    auto *s = new (std::nothrow) hc_server_s();
+    if (!s) return hc::fail(HC_ERR_OOM, "host allocation failed");
+    s->opts.slots = opts && opts->slots > 0 ? opts->slots : 4;
+    s->opts.max_batch = opts && opts->max_batch > 0 ? opts->max_batch : kWidth;
+    s->opts.memory_budget = opts ? opts->memory_budget : 0;
+    if (s->opts.slots > 16 || s->opts.max_batch > kWidth) {
+        delete s;
+        return hc::fail(HC_ERR_INVALID, "slots must be <= 16 and max_batch <= %d", kWidth);
+    }
+    if (s->opts.memory_budget <= 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            delete s;
+            return hc::fail(HC_ERR_HIP, "hipMemGetInfo failed");
+        }
+        s->opts.memory_budget = (int64_t)(free_b / 2);
+    }
+    s->streams.resize(s->opts.slots);
+    for (int i = 0; i < s->opts.slots; i++)
+        if (hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking) != hipSuccess) {
+            for (int j = 0; j < i; j++) (void)hipStreamDestroy(s->streams[j]);
+            delete s;
+            return hc::fail(HC_ERR_HIP, "stream creation failed");
+        }
+    if (hipStreamCreateWithFlags(&s->fetch_stream, hipStreamNonBlocking) != hipSuccess) {
+        for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+        delete s;
+        return hc::fail(HC_ERR_HIP, "stream creation failed");
+    }
+    for (int i = 0; i < s->opts.slots; i++) s->workers.emplace_back(worker, s, i);
+    *out = s;
+    return HC_OK;
+}
+
+int hc_server_destroy(hc_server_t s) {
+    if (!s) return HC_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_mem.notify_all();
+    for (std::thread &t : s->workers) t.join();
+    for (auto &kv : s->queries)
+        if (kv.second.res && --kv.second.res->refs == 0 && kv.second.res->p)
+            (void)hipFree(kv.second.res->p);
+    for (auto &kv : s->pool) (void)hipFree(kv.second);
+    for (GraphEntry &g : s->graphs) {
+        for (hc_graph_t h : g.per_slot)
+            if (h) hc_graph_free(h);
+        if (g.dev) hc_graph_free(g.dev);
+    }
+    if (s->fetch_buf) (void)hipFree(s->fetch_buf);
+    for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+    (void)hipStreamDestroy(s->fetch_stream);
+    delete s;
+    return HC_OK;
+}
+
+int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
+                        int32_t *graph_id) {
+    HC_CHECK_ARG(s && g && graph_id, "NULL argument");
+    HC_CHECK_ARG(g->num_nodes > 0 && g->index, "empty graph");
+    GraphEntry e;
+    e.host = *g;
+    e.resident = resident != 0;
+    e.per_slot.assign(s->opts.slots, nullptr);
+    // Budgeted footprint of a dynamic graph: CSR plus per-slot workspaces
+    // (queues, bitmaps, flags), estimated at 24 bytes per vertex per slot.
+    const int64_t csr = 8ll * (g->num_nodes + 1) + 4ll * g->num_edges * (g->weights ? 2 : 1);
+    e.bytes = csr + 24ll * g->num_nodes * s->opts.slots;
+    if (e.resident) HC_TRY(hc_graph_upload(g, &e.dev));
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (e.resident) s->st.uploads++;
+    s->graphs.push_back(std::move(e));
+    *graph_id = (int32_t)s->graphs.size() - 1;
+    return HC_OK;
+}
+
+int hc_server_submit(hc_server_t s, int32_t graph_id, int32_t kind, int32_t source,
+                     int64_t *ticket) {
+    HC_CHECK_ARG(s && ticket, "NULL argument");
+    HC_CHECK_ARG(kind == HC_QUERY_BFS || kind == HC_QUERY_SSSP, "unknown query kind %d", kind);
+    std::lock_guard<std::mutex> lk(s->mu);
+    HC_CHECK_ARG(graph_id >= 0 && graph_id < (int)s->graphs.size(), "unknown graph %d", graph_id);
+    const GraphEntry &g = s->graphs[graph_id];
+    HC_CHECK_ARG(source >= 0 && source < g.host.num_nodes, "source %d outside [0,%d)", source,
+                 g.host.num_nodes);
+    HC_CHECK_ARG(kind == HC_QUERY_BFS || g.host.weights, "SSSP needs a weighted graph");
+    HC_CHECK_ARG(!s->stop, "server is shutting down");
+    const int64_t t = s->next_ticket++;
+    Query q;
+    q.graph = graph_id;
+    q.kind = kind;
+    q.source = source;
+    q.t_submit = now_ms();
+    s->queries.emplace(t, std::move(q));
+    s->queue.push_back(t);
+    s->st.submitted++;
+    *ticket = t;
+    s->cv_work.notify_one();
+    return HC_OK;
+}
+
+int hc_server_wait(hc_server_t s, int64_t ticket, hc_query_info_t *info) {
+    HC_CHECK_ARG(s, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->queries.find(ticket);
+    if (it == s->queries.end()) return hc::fail(HC_ERR_NOT_FOUND, "unknown ticket %lld", (long long)ticket);
+    s->cv_done.wait(lk, [&] { return it->second.done; });
+    const Query &q = it->second;
+    if (info) {
+        const GraphEntry &g = s->graphs[q.graph];
+        info->status = q.status;
+        info->kind = q.kind;
+        info->graph = q.graph;
+        info->source = q.source;
+        info->batch_size = q.batch;
+        info->slot = q.slot;
+        info->num_nodes = g.host.num_nodes;
+        info->num_edges = g.host.num_edges;
+        info->queue_ms = q.t_start - q.t_submit;
+        info->run_ms = q.t_done - q.t_start;
+        info->latency_ms = q.t_done - q.t_submit;
+    }
+    return HC_OK;
+}
+
+int hc_server_fetch(hc_server_t s, int64_t ticket, int32_t *host_out) {
+    HC_CHECK_ARG(s && host_out, "NULL argument");
+    std::shared_ptr<Result> res;
+    int32_t column = -1, kind = 0;
+    int64_t n = 0;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->queries.find(ticket);
+        if (it == s->queries.end())
+            return hc::fail(HC_ERR_NOT_FOUND, "unknown ticket %lld", (long long)ticket);
+        s->cv_done.wait(lk, [&] { return it->second.done; });
+        if (it->second.status != HC_OK)
+            return hc::fail(it->second.status, "query %lld failed", (long long)ticket);
+        res = it->second.res;
+        column = it->second.column;
+        kind = it->second.kind;
+        n = s->graphs[it->second.graph].host.num_nodes;
+    }
+    std::lock_guard<std::mutex> fl(s->fetch_mu);
+    const int32_t *src = res->p;
+    if (column >= 0) {
+        const size_t need = sizeof(int32_t) * (size_t)n;
+        if (s->fetch_cap < need) {
+            if (s->fetch_buf) (void)hipFree(s->fetch_buf);
+            s->fetch_buf = nullptr;
+            s->fetch_cap = 0;
+            void *p = nullptr;
+            if (hipMalloc(&p, need) != hipSuccess)
+                return hc::fail(HC_ERR_OOM, "fetch buffer allocation failed");
+            s->fetch_buf = static_cast<int32_t *>(p);
+            s->fetch_cap = need;
+        }
+        HC_TRY(hc_multi_extract(res->p, (int32_t)n, column, kind == HC_QUERY_BFS ? 1 : 0,
+                                s->fetch_buf, s->fetch_stream));
+        src = s->fetch_buf;
+    }
+    if (hipMemcpyAsync(host_out, src, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost,
+                       s->fetch_stream) != hipSuccess ||
+        hipStreamSynchronize(s->fetch_stream) != hipSuccess)
+        return hc::fail(HC_ERR_HIP, "result copy failed");
+    return HC_OK;
+}
+
+int hc_server_release(hc_server_t s, int64_t ticket) {
+    HC_CHECK_ARG(s, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->queries.find(ticket);
+    if (it == s->queries.end()) return hc::fail(HC_ERR_NOT_FOUND, "unknown ticket %lld", (long long)ticket);
+    s->cv_done.wait(lk, [&] { return it->second.done; });
+    release_result(s, it->second.res);
+    s->queries.erase(it);
+    note_usage(s);
+    return HC_OK;
+}
+
+int hc_server_get_stats(hc_server_t s, hc_server_stats_t *out) {
+    HC_CHECK_ARG(s && out, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    return HC_OK;
+}
+
+}  // extern "C"

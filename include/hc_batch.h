@@ -29,6 +29,21 @@ int hc_sssp_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources,
                   int32_t *dist, int32_t max_epochs, hc_run_stats_t *stats,
                   hc_stream_t stream);
 
+/* Breadth-first depths from up to 64 sources at once: the same batched
+ * sweep with every edge weighing 1 (weights, if any, are ignored).
+ * depth[v * 64 + q] is the BFS level of v from sources[q], HC_INF_DIST when
+ * unreachable (hc_bfs reports -1 there); identical to hc_bfs's depths
+ * otherwise. */
+int hc_bfs_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources,
+                 int32_t *depth, int32_t max_epochs, hc_run_stats_t *stats,
+                 hc_stream_t stream);
+
+/* Copies column `column` of a batched result (num_nodes x 64, vertex-major)
+ * into out (device, num_nodes int32), HC_INF_DIST mapped to -1 when
+ * unreached_minus1 is set (BFS depths).  Asynchronous on stream. */
+int hc_multi_extract(const int32_t *matrix, int32_t num_nodes, int32_t column,
+                     int32_t unreached_minus1, int32_t *out, hc_stream_t stream);
+
 /* Many graphs resident in HBM as one batch: concatenated 4-byte offsets and
  * neighbour ids (each graph below 2^32 edges), descriptors and a
  * longest-first work order.  Graph i's vertex v is batch vertex

@@ -189,6 +189,11 @@ int hc_graph_upload(const hc_host_graph_t *g, hc_graph_t *out);
 int hc_graph_upload_range(const hc_host_graph_t *g, int32_t start_id,
                           int32_t end_id, hc_graph_t *out);
 int hc_graph_free(hc_graph_t g);
+/* A second handle over the same device CSR with its own plans and
+ * workspaces, so that runs on different streams can proceed concurrently
+ * (runs on one handle are serialised).  g must outlive the share; freeing
+ * the share leaves g's arrays in place.  No reference counterpart. */
+int hc_graph_share(hc_graph_t g, hc_graph_t *out);
 int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
 
 /* Tuning knobs, process-wide (the autotuner's interface; no reference

@@ -1,0 +1,84 @@
+/*
+ * hc_serve.h -- serving loop: graphs resident in HBM, BFS / SSSP queries
+ * admitted by device-memory footprint and batched as they arrive.
+ *
+ * No reference counterpart (the reference runs one query per driver call and
+ * uploads the graph each time, src/kernels/gpu/sssp.cuh:42-100).  A server
+ * owns `slots` worker threads, each with its own HIP stream and its own
+ * handle on every graph (hc_graph_share), so queries on different slots
+ * overlap on the device.  A free slot takes the oldest waiting query and
+ * every other waiting query of the same graph and kind, up to max_batch:
+ * one query runs the single-source path (hc_bfs / hc_sssp), two or more run
+ * one batched sweep (hc_bfs_multi / hc_sssp_multi).  Before it runs, a
+ * batch reserves its result memory from the budget; when the budget is
+ * spent the slot waits for released results (and evicts idle dynamic
+ * graphs).  Results are bit-identical to running each query alone.
+ */
+#ifndef HC_SERVE_H
+#define HC_SERVE_H
+
+#include "hc_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum hc_query_kind { HC_QUERY_BFS = 0, HC_QUERY_SSSP = 1 };
+
+typedef struct hc_server_opts {
+    int32_t slots;         /* worker threads / streams (default 4)            */
+    int32_t max_batch;     /* queries per batched sweep, 1..64 (default 64)   */
+    int64_t memory_budget; /* HBM bytes for results and dynamic graphs
+                              (default: half of the free HBM)                 */
+} hc_server_opts_t;
+
+typedef struct hc_query_info {
+    int32_t status;       /* HC_OK, or the error of the run                   */
+    int32_t kind;
+    int32_t graph;
+    int32_t source;
+    int32_t batch_size;   /* queries in the sweep that answered this one      */
+    int32_t slot;
+    int64_t num_nodes;
+    int64_t num_edges;    /* directed edges of the graph                       */
+    double queue_ms;      /* submit -> start of its sweep                     */
+    double run_ms;        /* sweep start -> result in HBM (host observed)     */
+    double latency_ms;    /* submit -> result in HBM                          */
+} hc_query_info_t;
+
+typedef struct hc_server_stats {
+    int64_t submitted, completed, failed;
+    int64_t batches, batched_queries;
+    int64_t uploads, evictions;
+    int64_t bytes_in_use, bytes_peak;
+} hc_server_stats_t;
+
+typedef struct hc_server_s *hc_server_t;
+
+int hc_server_create(const hc_server_opts_t *opts, hc_server_t *out);
+/* Waits for every submitted query, then frees everything. */
+int hc_server_destroy(hc_server_t s);
+
+/* Registers a graph.  resident = 1: uploaded now and kept.  resident = 0:
+ * uploaded when a query first needs it, evicted when idle and memory is
+ * needed; the caller keeps g's arrays alive until hc_server_destroy. */
+int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
+                        int32_t *graph_id);
+
+/* Enqueues a query; returns at once with a ticket. */
+int hc_server_submit(hc_server_t s, int32_t graph_id, int32_t kind, int32_t source,
+                     int64_t *ticket);
+/* Blocks until the query has finished; fills info (may be NULL). */
+int hc_server_wait(hc_server_t s, int64_t ticket, hc_query_info_t *info);
+/* Copies the finished query's result to host memory: num_nodes int32 (BFS
+ * depth, -1 unreached; SSSP distance, HC_INF_DIST unreachable). */
+int hc_server_fetch(hc_server_t s, int64_t ticket, int32_t *host_out);
+/* Frees the query's result; the ticket becomes invalid. */
+int hc_server_release(hc_server_t s, int64_t ticket);
+int hc_server_get_stats(hc_server_t s, hc_server_stats_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HC_SERVE_H */

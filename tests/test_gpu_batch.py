@@ -187,3 +187,24 @@ def test_bfs_many_rejects_bad_sources(hc, gpu):
     with hc.GraphBatch(graphs) as b:
         with pytest.raises(hc.HcError):
             b.bfs([0, graphs[1].num_nodes])
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_bfs_multi_depths(hc, gpu, oracle, weighted):
+    g = hc.HostGraph.generate("kron", 14, 16, weighted=weighted)
+    sources = g.pick_sources(64, seed=21)
+    with hc.DeviceGraph(g) as dg:
+        r = hc.bfs_multi(dg, sources)
+        for q in (0, 1, 40, 63):
+            assert np.array_equal(hc.bfs(dg, int(sources[q])).values, r.values[q])
+    for q, s in enumerate(sources):
+        assert np.array_equal(r.values[q], oracle.bfs_depth(g.index, g.neighbors, int(s))), q
+
+
+def test_bfs_multi_disconnected_golden(hc, gpu):
+    fx = load_golden("disconnected.npz")
+    g = hc.HostGraph.from_csr(fx["index"], fx["neighbors"])
+    with hc.DeviceGraph(g) as dg:
+        r = hc.bfs_multi(dg, fx["sources"])
+    for q in range(len(fx["sources"])):
+        assert np.array_equal(r.values[q], fx["depth"][q])

@@ -1,0 +1,118 @@
+"""GPU tests of the serving loop: continuous batching, admission by memory
+footprint and dynamic graphs.  Every result must be bit-identical to the
+same query run alone, and to the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def graphs(hc):
+    return {
+        "kron": hc.HostGraph.generate("kron", 14, 16, weighted=True, seed=1),
+        "unif": hc.HostGraph.generate("uniform", 13, 16, weighted=True, seed=2),
+        "dyn": [hc.HostGraph.generate("kron", s, 8, weighted=True, seed=10 + s)
+                for s in (10, 11, 12)],
+    }
+
+
+def _workload(graphs, ids, count, seed):
+    rng = np.random.RandomState(seed)
+    qs = []
+    for _ in range(count):
+        r = rng.rand()
+        if r < 0.2:
+            k = int(rng.randint(len(graphs["dyn"])))
+            gid, g = ids["dyn"][k], graphs["dyn"][k]
+        elif r < 0.6:
+            gid, g = ids["kron"], graphs["kron"]
+        else:
+            gid, g = ids["unif"], graphs["unif"]
+        kind = 0 if rng.rand() < 0.7 else 1
+        src = int(g.pick_sources(1, seed=int(rng.randint(1 << 30)))[0])
+        qs.append((gid, kind, src, g))
+    return qs
+
+
+def _server(hc, graphs, **kw):
+    srv = hc.Server(**kw)
+    ids = {"kron": srv.add_graph(graphs["kron"]), "unif": srv.add_graph(graphs["unif"]),
+           "dyn": [srv.add_graph(g, resident=False) for g in graphs["dyn"]]}
+    return srv, ids
+
+
+def _answers(srv, tickets):
+    out = []
+    for t in tickets:
+        info = srv.wait(t)
+        assert info["status"] == 0
+        out.append((srv.fetch(t), info))
+        srv.release(t)
+    return out
+
+
+def test_mix_equals_alone(hc, gpu, oracle, graphs):
+    # Alone: one slot, no batching, each query waited for before the next.
+    srv, ids = _server(hc, graphs, slots=1, max_batch=1)
+    qs = _workload(graphs, ids, 80, 7)
+    alone = []
+    for gid, kind, src, _ in qs:
+        t = srv.submit(gid, kind, src)
+        alone.append(_answers(srv, [t])[0][0])
+    srv.close()
+    # Mix: every query submitted at once to a batching multi-slot server.
+    srv, ids = _server(hc, graphs, slots=4, max_batch=64)
+    tickets = [srv.submit(gid, kind, src) for gid, kind, src, _ in qs]
+    mixed = _answers(srv, tickets)
+    st = srv.stats()
+    srv.close()
+    assert max(info["batch_size"] for _, info in mixed) > 1, "no query was batched"
+    assert st["batched_queries"] > 0 and st["completed"] == len(qs)
+    for i, ((gid, kind, src, g), a, (m, info)) in enumerate(zip(qs, alone, mixed)):
+        assert np.array_equal(a, m), (i, kind, info["batch_size"])
+        if i % 8 == 0:  # oracle on a sample
+            ref = (oracle.bfs_depth(g.index, g.neighbors, src) if kind == 0
+                   else oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
+            assert np.array_equal(m, ref), (i, kind)
+
+
+def test_memory_budget_evicts_dynamic_graphs(hc, gpu, oracle, graphs):
+    # Budget: room for one 64-wide result of the largest dynamic graph and
+    # little else, so dynamic graphs are evicted and re-uploaded.
+    n_max = max(g.num_nodes for g in graphs["dyn"])
+    g_max = max(8 * (g.num_nodes + 1) + 8 * g.num_edges + 48 * g.num_nodes
+                for g in graphs["dyn"])
+    budget = n_max * 4 * 64 + g_max + 65536
+    srv = hc.Server(slots=2, max_batch=64, memory_budget=budget)
+    dyn = [srv.add_graph(g, resident=False) for g in graphs["dyn"]]
+    rng = np.random.RandomState(3)
+    qs = []
+    for r in range(3):
+        for k, g in enumerate(graphs["dyn"]):
+            for s in g.pick_sources(4, seed=100 * r + k):
+                qs.append((dyn[k], int(rng.randint(2)), int(s), g))
+    tickets = [srv.submit(gid, kind, src) for gid, kind, src, _ in qs]
+    res = _answers(srv, tickets)
+    st = srv.stats()
+    srv.close()
+    assert st["evictions"] > 0 and st["uploads"] > len(graphs["dyn"])
+    assert st["bytes_peak"] <= budget
+    for (gid, kind, src, g), (m, _) in zip(qs, res):
+        ref = (oracle.bfs_depth(g.index, g.neighbors, src) if kind == 0
+               else oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
+        assert np.array_equal(m, ref)
+
+
+def test_server_rejects_bad_queries(hc, gpu, graphs):
+    srv = hc.Server(slots=1)
+    gid = srv.add_graph(graphs["kron"])
+    with pytest.raises(hc.HcError):
+        srv.submit(gid, 0, graphs["kron"].num_nodes)
+    with pytest.raises(hc.HcError):
+        srv.submit(gid + 5, 0, 0)
+    with pytest.raises(hc.HcError):
+        srv.wait(12345)
+    srv.close()
