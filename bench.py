@@ -140,6 +140,164 @@ def copy_probe(nbytes, reps):
     print(json.dumps({"copy_probe_bytes": n * 4, "reps": reps}), flush=True)
 
 
+def run_batched_sssp(hc, oracle, scale, degree, seed, check):
+    """64 SSSP sources of one Kronecker graph in one batched sweep vs the
+    same 64 sources run one after another (device time of each)."""
+    g = hc.HostGraph.generate("kron", scale, degree, weighted=True, seed=seed)
+    sources = g.pick_sources(64)
+    out = {"graph": f"kron{scale} ef{degree}", "n": g.num_nodes, "m": int(g.num_edges)}
+    with hc.DeviceGraph(g) as dg:
+        buf = hc.DeviceArray(g.num_nodes * hc.MULTI_WIDTH, np.int32)
+        hc.sssp_multi(dg, sources, out=buf, fetch=False)  # warm-up (capture)
+        r = hc.sssp_multi(dg, sources, out=buf, fetch=False)
+        one = hc.DeviceArray(g.num_nodes, np.int32)
+        hc.sssp(dg, int(sources[0]), out=one, fetch=False)
+        seq_ms = 0.0
+        for s_ in sources:
+            seq_ms += hc.sssp(dg, int(s_), out=one, fetch=False).stats["ms"]
+        exact = True
+        if check:
+            mat = buf.get().reshape(g.num_nodes, hc.MULTI_WIDTH)
+            for q in range(64):
+                single = hc.sssp(dg, int(sources[q])).values
+                exact &= bool(np.array_equal(mat[:, q], single))
+            for q in range(min(check, 64)):
+                exact &= bool(np.array_equal(mat[:, q], oracle.sssp_dijkstra(
+                    g.index, g.neighbors, g.weights, int(sources[q]))))
+        buf.free()
+        one.free()
+    b_ms = r.stats["ms"]
+    out.update({
+        "batch_ms": round(b_ms, 3), "sequential_ms": round(seq_ms, 3),
+        "batch_gteps": round(hc.gteps(64 * g.num_edges, b_ms), 3),
+        "sequential_gteps": round(hc.gteps(64 * g.num_edges, seq_ms), 3),
+        "speedup": round(seq_ms / b_ms, 2), "epochs": r.stats["epochs"],
+        "alg_gbs": round(r.stats["alg_bytes"] / (b_ms / 1e3) / 1e9, 1),
+        "bit_exact_vs_single_and_oracle": exact})
+    return out
+
+
+def run_many_graph_bfs(hc, oracle, count, seed, check):
+    """One BFS on each of `count` Kronecker graphs (scale U{10..16}, edge
+    factor U{8..32}) in one persistent launch vs one hc_bfs call per graph."""
+    rng = np.random.RandomState(seed)
+    graphs = []
+    t0 = time.time()
+    for i in range(count):
+        graphs.append(hc.HostGraph.generate("kron", int(rng.randint(10, 17)),
+                                            int(rng.randint(8, 33)), weighted=False,
+                                            seed=seed + 1 + i))
+    gen_s = time.time() - t0
+    sources = [int(g.pick_sources(1)[0]) for g in graphs]
+    total_m = sum(g.num_edges for g in graphs)
+    out = {"graphs": count, "total_edges": int(total_m), "generate_s": round(gen_s, 1)}
+    with hc.GraphBatch(graphs) as b:
+        depth = hc.DeviceArray(b.total_nodes, np.int32)
+        b.bfs(sources, want_parents=False, depth=depth, fetch=False)  # warm-up
+        _, _, st = b.bfs(sources, want_parents=False, depth=depth, fetch=False)
+        all_depth = depth.get()
+        depth.free()
+        offs = b.node_offsets
+    seq_ms = 0.0
+    exact = True
+    one = None
+    for i, g in enumerate(graphs):
+        with hc.DeviceGraph(g) as dg:
+            one = hc.DeviceArray(g.num_nodes, np.int32)
+            hc.bfs(dg, sources[i], depth_out=one, want_parents=False, fetch=False)
+            r = hc.bfs(dg, sources[i], depth_out=one, want_parents=False)
+            seq_ms += r.stats["ms"]
+            exact &= bool(np.array_equal(r.values, all_depth[offs[i]:offs[i + 1]]))
+            one.free()
+    for i in range(min(check, count)):
+        g = graphs[i]
+        exact &= bool(np.array_equal(all_depth[offs[i]:offs[i + 1]],
+                                     oracle.bfs_depth(g.index, g.neighbors, sources[i])))
+    out.update({"batch_ms": round(st["ms"], 3), "sequential_ms": round(seq_ms, 3),
+                "batch_gteps": round(hc.gteps(total_m, st["ms"]), 3),
+                "sequential_gteps": round(hc.gteps(total_m, seq_ms), 3),
+                "speedup": round(seq_ms / st["ms"], 2),
+                "levels": {"push": st["push_epochs"], "pull": st["pull_epochs"]},
+                "bit_exact_vs_single_and_oracle": exact})
+    return out
+
+
+def run_serving(hc, oracle, big, unif, rates, queries, seed, check):
+    """Poisson arrivals of 70% BFS / 30% SSSP queries; 20% target small
+    graphs uploaded on demand.  Latency: arrival -> result in HBM."""
+    import threading
+    rng = np.random.RandomState(seed)
+    dyn = [hc.HostGraph.generate("kron", int(s), 16, weighted=True, seed=seed + int(s))
+           for s in rng.randint(12, 19, size=8)]
+    srv = hc.Server(slots=4, max_batch=64)
+    gids = [srv.add_graph(big), srv.add_graph(unif)] + [srv.add_graph(g, resident=False)
+                                                        for g in dyn]
+    hosts = [big, unif] + dyn
+    results = []
+    for rate in rates:
+        qs = []
+        for _ in range(queries):
+            u = rng.rand()
+            k = (2 + int(rng.randint(len(dyn)))) if u < 0.2 else (0 if u < 0.6 else 1)
+            kind = hc.QUERY_BFS if rng.rand() < 0.7 else hc.QUERY_SSSP
+            src = int(hosts[k].pick_sources(1, seed=int(rng.randint(1 << 30)))[0])
+            qs.append((k, kind, src))
+        gaps = rng.exponential(1.0 / rate, size=queries)
+        arrive = np.cumsum(gaps)
+        tickets = [None] * queries
+        lag = np.zeros(queries)
+        start = time.perf_counter()
+
+        def submitter():
+            for i, (k, kind, src) in enumerate(qs):
+                delay = start + arrive[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                lag[i] = max(0.0, time.perf_counter() - start - arrive[i])
+                tickets[i] = srv.submit(gids[k], kind, src)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, edges, batches, checked, exact, errors = [], 0, [], 0, True, []
+        for i in range(queries):
+            while tickets[i] is None:
+                time.sleep(0.0001)
+            info = srv.wait(tickets[i])
+            if info["status"] != 0:
+                errors.append(info.get("error", str(info["status"])))
+                srv.release(tickets[i])
+                continue
+            lat.append(info["latency_ms"] + lag[i] * 1e3)
+            edges += info["num_edges"]
+            batches.append(info["batch_size"])
+            k, kind, src = qs[i]
+            if checked < check and k >= 2:
+                g = hosts[k]
+                ref = (oracle.bfs_depth(g.index, g.neighbors, src) if kind == hc.QUERY_BFS
+                       else oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
+                exact &= bool(np.array_equal(srv.fetch(tickets[i]), ref))
+                checked += 1
+            srv.release(tickets[i])
+        wall = time.perf_counter() - start
+        th.join()
+        lat = np.array(lat)
+        results.append({"rate_qps": rate, "queries": queries,
+                        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                        "sustained_gteps": round(edges / 2 / wall / 1e9, 3),
+                        "achieved_qps": round(queries / wall, 1),
+                        "mean_batch": round(float(np.mean(batches)), 2),
+                        "checked_vs_oracle": checked, "bit_exact": exact,
+                        "failed": len(errors), "errors": errors[:3]})
+    st = srv.stats()
+    srv.close()
+    return {"graphs": {"kron": big.num_nodes, "uniform": unif.num_nodes,
+                       "dynamic_scales": [int(np.log2(g.num_nodes)) for g in dyn]},
+            "rates": results, "uploads": st["uploads"], "evictions": st["evictions"],
+            "spills": st["spills"],
+            "batched_queries": st["batched_queries"]}
+
+
 def check_exact(hc, oracle, res, count):
     """Bit-exactness of the first `count` sources against the oracle."""
     g, dg = res["g"], res["dg"]
@@ -194,6 +352,13 @@ def main():
     ap.add_argument("--pagerank", type=int, default=1)
     ap.add_argument("--roofline-sources", type=int, default=16,
                     help="sources of the event-timed pass over the dominant kernel")
+    ap.add_argument("--batch", type=int, default=1,
+                    help="run the batched forms (64-source SSSP, many-graph BFS)")
+    ap.add_argument("--batch-scale", type=int, default=22)
+    ap.add_argument("--many-graphs", type=int, default=512)
+    ap.add_argument("--serve", type=int, default=1, help="run the serving loop")
+    ap.add_argument("--serve-scale", type=int, default=24)
+    ap.add_argument("--serve-queries", type=int, default=400)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -254,6 +419,25 @@ def main():
     roofline = kernel_roofline(hc, R, algo, min(args.steps, args.roofline_sources))
     roofline["traffic"] = committed_traffic(roofline["kernel"], main_kind, args.scale)
 
+    batched = None
+    if args.batch:
+        log("[batch] 64-source SSSP")
+        batched = {"sssp64": run_batched_sssp(hc, oracle, args.batch_scale, args.degree,
+                                              args.seed, args.check)}
+        log("[batch] many-graph BFS")
+        batched["bfs_many"] = run_many_graph_bfs(hc, oracle, args.many_graphs, 4242,
+                                                 args.check)
+    serving = None
+    if args.serve:
+        log("[serve] generating graphs")
+        big = hc.HostGraph.generate("kron", args.serve_scale, args.degree, weighted=True,
+                                    seed=args.seed)
+        unif = results["uniform"]["g"] if "uniform" in results else hc.HostGraph.generate(
+            "uniform", args.scale, args.degree, weighted=True, seed=args.seed)
+        serving = run_serving(hc, oracle, big, unif, [1000, 5000, 20000], args.serve_queries,
+                              777, max(args.check, 4))
+        del big
+
     cpu = None
     if not args.no_cpu:
         cb = cpu_baseline(oracle, R, args.cpu_budget, args.cpu_max_sources)
@@ -295,6 +479,8 @@ def main():
                         **{f"sssp_{k}": float(r["sssp_epochs"].mean())
                            for k, r in results.items()}},
         "pagerank": pr,
+        "batched": batched,
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "bit_exact": exact,

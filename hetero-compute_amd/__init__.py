@@ -109,7 +109,7 @@ class QueryInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
                 ("batches", C.c_int64), ("batched_queries", C.c_int64),
-                ("uploads", C.c_int64), ("evictions", C.c_int64),
+                ("uploads", C.c_int64), ("evictions", C.c_int64), ("spills", C.c_int64),
                 ("bytes_in_use", C.c_int64), ("bytes_peak", C.c_int64)]
 
 
@@ -862,9 +862,13 @@ class Server:
         return t.value
 
     def wait(self, ticket: int) -> dict:
+        """Query info; a failed query has status != 0 and "error" set."""
         i = QueryInfo()
         check(lib().hc_server_wait(self.h, ticket, C.byref(i)))
-        return {f: getattr(i, f) for f, _ in i._fields_}
+        d = {f: getattr(i, f) for f, _ in i._fields_}
+        if d["status"] != HC_OK:
+            d["error"] = lib().hc_last_error().decode(errors="replace")
+        return d
 
     def fetch(self, ticket: int) -> np.ndarray:
         info = self.wait(ticket)

@@ -107,7 +107,8 @@ int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
     HC_HIP(hipEventRecord(ev1, s));
     HC_HIP(hipEventSynchronize(ev1));
     HC_HIP(hipEventElapsedTime(ms, ev0, ev1));
-    HC_HIP(hipMemcpy(final_ctl, ws->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
+    HC_HIP(hipMemcpyAsync(final_ctl, ws->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, s));
+    HC_HIP(hipStreamSynchronize(s));
     hipEventDestroy(ev0);
     hipEventDestroy(ev1);
     hipEventDestroy(evb[0]);

@@ -470,7 +470,8 @@ int hc_bfs_many(hc_graph_batch_t b, const int32_t *sources, int32_t *depth,
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     unsigned long long st[3] = {0, 0, 0};
-    HC_HIP(hipMemcpy(st, b->queue + 2, sizeof(st), hipMemcpyDeviceToHost));
+    HC_HIP(hipMemcpyAsync(st, b->queue + 2, sizeof(st), hipMemcpyDeviceToHost, s));
+    HC_HIP(hipStreamSynchronize(s));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->ms = ms;

@@ -20,8 +20,8 @@
 // Mapping: degree-bucketed units of one wavefront -- a chunk of a hub's edge
 // list (partial minimum combined by atomicMin), one vertex, 4 vertices of
 // degree <= 16 (16 lanes each) or 16 vertices of degree <= 4 (4 lanes each).
-// The loop is device-driven: step + controller kernels captured in a hipGraph
-// of kEpochsPerGraph epochs that the host relaunches until the controller
+// The loop is device-driven: step + controller kernels form a hipGraph of
+// kEpochsPerGraph epochs that the host relaunches until the controller
 // reports convergence.
 #include <cstring>
 
@@ -31,7 +31,7 @@
 namespace hc {
 
 constexpr int kQ = 64;               // sources per batch (= lanes)
-constexpr int kEpochsPerGraph = 4;   // epochs captured per hipGraph launch
+constexpr int kEpochsPerGraph = 4;   // epochs per hipGraph launch
 
 struct MultiCtl {
     int32_t done;
@@ -323,8 +323,7 @@ struct MultiWs {
     int words = 0;
     int grid = 0;
     const PlanHost *plan = nullptr;
-    // Captured epoch graphs, [0] weighted, [1] unit weights.
-    hipStream_t gstream[2] = {nullptr, nullptr};
+    // Epoch graphs, [0] weighted, [1] unit weights.
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     KernelTimer kt;
@@ -420,22 +419,41 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
 
     ws->kt.reset();
     const bool use_graph = tuning().multi_graph != 0 && !ws->kt.on;
-    if (use_graph && (!ws->exec[unit] || ws->gstream[unit] != s)) {
-        if (ws->exec[unit]) hipGraphExecDestroy(ws->exec[unit]);
-        if (ws->graph[unit]) hipGraphDestroy(ws->graph[unit]);
-        ws->exec[unit] = nullptr;
-        ws->graph[unit] = nullptr;
+    if (use_graph && !ws->exec[unit]) {
         // The kernels read dist and the epoch from the controller, so one
-        // captured graph serves every call on this graph handle.
-        HC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = launch_multi_epochs(a, grid, kEpochsPerGraph, sweep, s, nullptr);
+        // graph serves every call on this graph handle.  It is built node
+        // by node rather than captured from a stream: capture would forbid
+        // other threads (serving slots) from touching the legacy stream
+        // meanwhile.
         hipGraph_t graph = nullptr;
-        hipError_t ce = hipStreamEndCapture(s, &graph);
-        if (rc != HC_OK) return rc;
-        if (ce != hipSuccess) return fail(HC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+        HC_HIP(hipGraphCreate(&graph, 0));
         ws->graph[unit] = graph;
+        hipGraphNode_t prev = nullptr;
+        for (int i = 0; i < kEpochsPerGraph; i++) {
+            hipKernelNodeParams kp{};
+            void *step_args[] = {(void *)&a};
+            kp.func = reinterpret_cast<void *>(&multi_step_kernel<OffT>);
+            kp.gridDim = dim3(grid);
+            kp.blockDim = dim3(kBlock);
+            kp.sharedMemBytes = 0;
+            kp.kernelParams = step_args;
+            kp.extra = nullptr;
+            hipGraphNode_t step = nullptr, ctl = nullptr;
+            HC_HIP(hipGraphAddKernelNode(&step, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+            MultiCtl *cp = a.ctl;
+            unsigned long long sw = sweep;
+            void *ctl_args[] = {(void *)&cp, (void *)&sw};
+            hipKernelNodeParams cpar{};
+            cpar.func = reinterpret_cast<void *>(&multi_ctl_kernel);
+            cpar.gridDim = dim3(1);
+            cpar.blockDim = dim3(kWave);
+            cpar.sharedMemBytes = 0;
+            cpar.kernelParams = ctl_args;
+            cpar.extra = nullptr;
+            HC_HIP(hipGraphAddKernelNode(&ctl, graph, &step, 1, &cpar));
+            prev = ctl;
+        }
         HC_HIP(hipGraphInstantiate(&ws->exec[unit], graph, nullptr, nullptr, 0));
-        ws->gstream[unit] = s;
     }
     // Relaunch until the controller reports convergence; the done flag of
     // batch b-1 is read while batch b runs.

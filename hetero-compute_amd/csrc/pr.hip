@@ -435,7 +435,8 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     PrCtl c;
-    HC_HIP(hipMemcpy(&c, ws->ctl, sizeof(PrCtl), hipMemcpyDeviceToHost));
+    HC_HIP(hipMemcpyAsync(&c, ws->ctl, sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+    HC_HIP(hipStreamSynchronize(s));
     const float *final_s = (c.iter & 1) ? ws->s[1] : ws->s[0];
     HC_HIP(hipMemcpyAsync(scores, final_s, sizeof(float) * g->n,
                           hipMemcpyDeviceToDevice, s));

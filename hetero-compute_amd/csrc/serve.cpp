@@ -15,6 +15,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -33,9 +34,13 @@ double now_ms() {
 
 // Device result storage shared by the queries of one sweep.
 struct Result {
-    int32_t *p = nullptr;
+    std::mutex mu;                // held while the device copy is read
+    int32_t *p = nullptr;         // device copy (nullptr once spilled)
     size_t bytes = 0;
     int refs = 0;
+    bool budgeted = true;         // bytes still counted in the budget
+    bool spilling = false;
+    std::vector<int32_t> host;    // host copy after a spill
 };
 
 struct Query {
@@ -44,6 +49,7 @@ struct Query {
     int32_t status = HC_OK;
     int32_t batch = 0, slot = -1;
     bool done = false;
+    std::string error;  // message of a failed run
     std::shared_ptr<Result> res;
     int32_t column = -1;  // column of a batched result, -1 for a single one
 };
@@ -144,9 +150,11 @@ int32_t *take_buffer(hc_server_s *s, size_t bytes) {
 }
 
 void finish(hc_server_s *s, const std::vector<int64_t> &batch, int status, double t_done) {
+    const std::string msg = status != HC_OK ? hc_last_error() : "";
     for (int64_t t : batch) {
         Query &q = s->queries[t];
         q.status = status;
+        q.error = msg;
         q.t_done = t_done;
         q.done = true;
         s->st.completed++;
@@ -157,8 +165,9 @@ void finish(hc_server_s *s, const std::vector<int64_t> &batch, int status, doubl
 
 void release_result(hc_server_s *s, std::shared_ptr<Result> &r) {
     if (!r) return;
-    if (--r->refs == 0) {
-        s->in_use -= (int64_t)r->bytes;
+    if (--r->refs == 0 && !r->spilling) {
+        if (r->budgeted) s->in_use -= (int64_t)r->bytes;
+        r->budgeted = false;
         if (r->p) {
             s->pool.emplace(r->bytes, r->p);
             s->pooled += (int64_t)r->bytes;
@@ -167,6 +176,51 @@ void release_result(hc_server_s *s, std::shared_ptr<Result> &r) {
         s->cv_mem.notify_all();
     }
     r.reset();
+}
+
+// Moves the largest finished, unreleased result in HBM to host memory and
+// returns its buffer to the pool, so that admission in arrival order always
+// progresses while clients hold results.  Called with mu held; drops it
+// around the copy.  Returns false when nothing can be spilled.
+bool spill_one(hc_server_s *s, std::unique_lock<std::mutex> &lk) {
+    std::shared_ptr<Result> best;
+    for (auto &kv : s->queries) {
+        const Query &q = kv.second;
+        if (!q.done || !q.res || !q.res->p || q.res->spilling) continue;
+        if (!best || q.res->bytes > best->bytes) best = q.res;
+    }
+    if (!best) return false;
+    best->spilling = true;
+    lk.unlock();
+    bool ok = true;
+    {
+        std::lock_guard<std::mutex> rl(best->mu);
+        try {
+            best->host.resize(best->bytes / sizeof(int32_t));
+        } catch (...) {
+            ok = false;
+        }
+        ok = ok && hipMemcpy(best->host.data(), best->p, best->bytes, hipMemcpyDeviceToHost) ==
+                       hipSuccess;
+    }
+    lk.lock();
+    best->spilling = false;
+    if (ok) {
+        s->pool.emplace(best->bytes, best->p);
+        s->pooled += (int64_t)best->bytes;
+        best->p = nullptr;
+        if (best->budgeted) s->in_use -= (int64_t)best->bytes;
+        best->budgeted = false;
+        s->st.spills++;
+    } else {
+        best->host.clear();
+    }
+    if (best->refs == 0) {  // released while spilling
+        std::shared_ptr<Result> r = best;
+        r->refs = 1;
+        release_result(s, r);
+    }
+    return ok;
 }
 
 void worker(hc_server_s *s, int slot) {
@@ -216,6 +270,7 @@ void worker(hc_server_s *s, int slot) {
             const bool fits = make_room(s, lk, need, gid);
             if (g.uploading) continue;  // another slot began this graph's upload
             if (fits) break;
+            if (spill_one(s, lk)) continue;
             if (s->stop && s->running == 0) {
                 status = hc::fail(HC_ERR_BUSY, "server stopped with the memory budget spent");
                 break;
@@ -244,7 +299,9 @@ void worker(hc_server_s *s, int slot) {
         s->running++;
         lk.unlock();
 
-        // Device work, outside the lock.
+        // Device work, outside the lock.  A stale error of an earlier call
+        // on this thread must not be charged to this batch.
+        (void)hipGetLastError();
         if (upload) {
             hc_graph_t d = nullptr;
             status = hc_graph_upload(&g.host, &d);
@@ -321,6 +378,7 @@ void worker(hc_server_s *s, int slot) {
         if (batch.size() > 1) s->st.batched_queries += (int64_t)batch.size();
         finish(s, batch, status, t_done);
         note_usage(s);
+        s->cv_mem.notify_all();  // its results can now be spilled
     }
 }
 
@@ -442,6 +500,8 @@ int hc_server_wait(hc_server_t s, int64_t ticket, hc_query_info_t *info) {
     if (it == s->queries.end()) return hc::fail(HC_ERR_NOT_FOUND, "unknown ticket %lld", (long long)ticket);
     s->cv_done.wait(lk, [&] { return it->second.done; });
     const Query &q = it->second;
+    // A failed query's message becomes this thread's hc_last_error().
+    if (q.status != HC_OK) hc::fail(q.status, "%s", q.error.c_str());
     if (info) {
         const GraphEntry &g = s->graphs[q.graph];
         info->status = q.status;
@@ -471,11 +531,21 @@ int hc_server_fetch(hc_server_t s, int64_t ticket, int32_t *host_out) {
             return hc::fail(HC_ERR_NOT_FOUND, "unknown ticket %lld", (long long)ticket);
         s->cv_done.wait(lk, [&] { return it->second.done; });
         if (it->second.status != HC_OK)
-            return hc::fail(it->second.status, "query %lld failed", (long long)ticket);
+            return hc::fail(it->second.status, "query %lld failed: %s", (long long)ticket,
+                            it->second.error.c_str());
         res = it->second.res;
         column = it->second.column;
         kind = it->second.kind;
         n = s->graphs[it->second.graph].host.num_nodes;
+    }
+    std::lock_guard<std::mutex> rl(res->mu);
+    if (!res->p) {  // spilled to host
+        const bool minus1 = kind == HC_QUERY_BFS;
+        for (int64_t v = 0; v < n; v++) {
+            const int32_t x = column >= 0 ? res->host[(size_t)v * kWidth + column] : res->host[v];
+            host_out[v] = (column >= 0 && minus1 && x == HC_INF_DIST) ? -1 : x;
+        }
+        return HC_OK;
     }
     std::lock_guard<std::mutex> fl(s->fetch_mu);
     const int32_t *src = res->p;

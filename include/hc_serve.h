@@ -11,8 +11,10 @@
  * one query runs the single-source path (hc_bfs / hc_sssp), two or more run
  * one batched sweep (hc_bfs_multi / hc_sssp_multi).  Before it runs, a
  * batch reserves its result memory from the budget; when the budget is
- * spent the slot waits for released results (and evicts idle dynamic
- * graphs).  Results are bit-identical to running each query alone.
+ * spent the slot trims its buffer pool, evicts idle dynamic graphs, moves
+ * finished but unreleased results to host memory, and otherwise waits for
+ * running sweeps.  Admission is in arrival order.  Results are
+ * bit-identical to running each query alone.
  */
 #ifndef HC_SERVE_H
 #define HC_SERVE_H
@@ -50,6 +52,7 @@ typedef struct hc_server_stats {
     int64_t submitted, completed, failed;
     int64_t batches, batched_queries;
     int64_t uploads, evictions;
+    int64_t spills;             /* finished results moved to host memory     */
     int64_t bytes_in_use, bytes_peak;
 } hc_server_stats_t;
 

@@ -48,7 +48,7 @@ def _answers(srv, tickets):
     out = []
     for t in tickets:
         info = srv.wait(t)
-        assert info["status"] == 0
+        assert info["status"] == 0, info
         out.append((srv.fetch(t), info))
         srv.release(t)
     return out
