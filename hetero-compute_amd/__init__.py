@@ -185,6 +185,8 @@ _SIGS = {
     "hc_server_fetch": ([_P, C.c_int64, _P], C.c_int),
     "hc_server_release": ([_P, C.c_int64], C.c_int),
     "hc_server_get_stats": ([_P, C.POINTER(ServerStats)], C.c_int),
+    "hc_cpu_epoch_ms": ([C.POINTER(HostGraphStruct), C.c_int32, C.c_int32, C.c_int32,
+                         C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "hc_kernel_from_string": ([C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
     "hc_kernel_to_string": ([C.c_int32], C.c_char_p),
     "hc_kernel_device_type": ([C.c_int32, C.POINTER(C.c_int32)], C.c_int),
@@ -212,11 +214,11 @@ _SIGS = {
 }
 
 TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "pull.stage", "bu.stage", "profile.kernel_events",
-               "multi.graph",
-               "pull.hub_threshold", "pull.hub_chunk",
+               "multi.graph", "multi.grid_per_cu", "pull.hub_threshold", "pull.hub_chunk",
                "pull.block_threshold", "pull.wave_threshold", "pull.g16_threshold",
                "pull.g4_threshold", "bu.wave_threshold", "bu.g16_threshold",
-               "bu.g4_threshold")
+               "bu.g4_threshold", "multi.hub_threshold", "multi.hub_chunk",
+               "multi.wave_threshold", "multi.g16_threshold")
 
 
 def set_tuning(key: str, value: int):

@@ -99,15 +99,23 @@ struct Tuning {
     int stage_pull = 8192; // per-vertex entries staged in LDS by pull sweeps
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
     int kernel_events = 0; // 1: time every launch of a loop's main kernel
-    int multi_graph = 1;   // batched SSSP: relaunch a captured hipGraph
+    int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
+    int multi_grid_per_cu = 8;  // workgroups per CU of the batched sweep
     PlanConfig pull;       // SSSP and PageRank pull sweeps
     PlanConfig bu;         // BFS bottom-up sweeps (no chunk / WG classes)
+    PlanConfig multi;      // batched sweeps: chunks, wave, 16-lane, 4-lane
     Tuning() {
         bu.hub_threshold = INT64_MAX;
         bu.block_threshold = INT64_MAX;
         bu.wave_threshold = 256;
         bu.g16_threshold = 32;
         bu.g4_threshold = 8;
+        multi.hub_threshold = 4096;
+        multi.hub_chunk = 2048;
+        multi.block_threshold = INT64_MAX;
+        multi.wave_threshold = 16;
+        multi.g16_threshold = 4;
+        multi.g4_threshold = 0;
     }
 };
 Tuning &tuning();

@@ -245,6 +245,50 @@ double hc_cpu_pr_pull_epoch(const hc_host_graph_t *g, float damping,
     return resid;
 }
 
+int hc_cpu_epoch_ms(const hc_host_graph_t *g, int32_t algorithm, int32_t start_id,
+                    int32_t end_id, int32_t repeats, double *ms) {
+    HC_CHECK_ARG(g && ms && g->index, "NULL argument");
+    HC_CHECK_ARG(algorithm >= 0 && algorithm <= 2, "algorithm must be 0 (SSSP), 1 (BFS) or 2 (PR)");
+    HC_CHECK_ARG(0 <= start_id && start_id <= end_id && end_id <= g->num_nodes,
+                 "range [%d,%d) outside [0,%d)", start_id, end_id, g->num_nodes);
+    HC_CHECK_ARG(algorithm != 0 || g->weights, "SSSP needs a weighted graph");
+    if (repeats <= 0) repeats = 1;
+    const size_t n = (size_t)g->num_nodes;
+    // Work arrays in a steady state: every vertex is at distance 1 (SSSP
+    // sweeps then scan whole rows without improving), every vertex unvisited
+    // with an empty frontier (bottom-up scans whole rows), uniform scores.
+    std::vector<int32_t> state(n, algorithm == 0 ? 1 : -1);
+    std::vector<uint32_t> front((n + 31) / 32, 0), next((n + 31) / 32, 0);
+    std::vector<float> score(n, 1.0f / (float)n), contrib(n, 0.0f), s_out(n), c_out(n);
+    std::vector<double> inv(n, 0.0);
+    for (size_t u = 0; u < n; u++) {
+        const int64_t d = g->index[u + 1] - g->index[u];
+        inv[u] = d > 0 ? 1.0 / (double)d : 0.0;
+        contrib[u] = (float)((double)score[u] * inv[u]);
+    }
+    double total = 0.0;
+    for (int r = 0; r < repeats; r++) {
+        int32_t updated = 0;
+        const double t0 = now_ms();
+#pragma omp parallel
+        {
+            const int tid = omp_get_thread_num(), nt = omp_get_num_threads();
+            if (algorithm == 0) {
+                hc_cpu_sssp_pull_epoch(g, state.data(), start_id, end_id, tid, nt, &updated);
+            } else if (algorithm == 1) {
+                cpu_bfs_pull(*g, state.data(), start_id, end_id, front.data(), next.data(), tid,
+                             nt);
+            } else {
+                hc_cpu_pr_pull_epoch(g, 0.85f, score.data(), contrib.data(), inv.data(),
+                                     s_out.data(), c_out.data(), start_id, end_id, tid, nt);
+            }
+        }
+        total += now_ms() - t0;
+    }
+    *ms = total / repeats;
+    return HC_OK;
+}
+
 int hc_hetero_create(const hc_host_graph_t *g, hc_schedule_t s, hc_hetero_t *out) {
     HC_CHECK_ARG(g && s && out, "NULL argument");
     HC_CHECK_ARG(g->index && g->num_nodes > 0, "empty graph");

@@ -343,22 +343,10 @@ void free_multi_ws(hc_graph_s *g) {
     g->ws_multi = nullptr;
 }
 
-// Plan of the batched sweep: hub chunks of 2048 edges above degree 4096,
-// wavefront per vertex above 16, 16 lanes above 4, else 4 lanes.
-static PlanConfig multi_config() {
-    PlanConfig c;
-    c.hub_threshold = 4096;
-    c.hub_chunk = 2048;
-    c.block_threshold = INT64_MAX;
-    c.wave_threshold = 16;
-    c.g16_threshold = 4;
-    c.g4_threshold = 0;
-    return c;
-}
-
 static int get_multi_ws(hc_graph_s *g, MultiWs **out) {
     auto *ws = static_cast<MultiWs *>(g->ws_multi);
-    if (ws && ws->grid != grid_blocks()) {
+    const int grid = grid_blocks(tuning().multi_grid_per_cu);
+    if (ws && ws->grid != grid) {
         free_multi_ws(g);
         ws = nullptr;
     }
@@ -366,18 +354,13 @@ static int get_multi_ws(hc_graph_s *g, MultiWs **out) {
         ws = new (std::nothrow) MultiWs();
         if (!ws) return fail(HC_ERR_OOM, "host allocation failed");
         ws->words = (g->n + 3) / 4;
-        ws->grid = grid_blocks();
+        ws->grid = grid;
         hipError_t e1 = hipMalloc(&ws->flags, sizeof(uint32_t) * 3 * (size_t)std::max(ws->words, 1));
         hipError_t e2 = hipMalloc(&ws->ctl, sizeof(MultiCtl));
         hipError_t e3 = hipHostMalloc(&ws->h_ctl, sizeof(MultiCtl), hipHostMallocDefault);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
             delete ws;
             return fail(HC_ERR_OOM, "batched-SSSP workspace allocation failed");
-        }
-        int rc = get_plan(g, 0, g->n, multi_config(), &ws->plan);
-        if (rc != HC_OK) {
-            delete ws;
-            return rc;
         }
         g->ws_multi = ws;
     }
@@ -403,6 +386,19 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
                      int max_epochs, int unit, hc_run_stats_t *stats, hipStream_t s) {
     MultiWs *ws = nullptr;
     HC_TRY(get_multi_ws(g, &ws));
+    // Degree classes of the batched sweep (tunable): hub chunks, wavefront,
+    // 16-lane and 4-lane vertices.
+    const PlanHost *ph = nullptr;
+    HC_TRY(get_plan(g, 0, g->n, tuning().multi, &ph));
+    if (ph != ws->plan) {  // epoch graphs bake the plan in
+        for (int k = 0; k < 2; k++) {
+            if (ws->exec[k]) hipGraphExecDestroy(ws->exec[k]);
+            if (ws->graph[k]) hipGraphDestroy(ws->graph[k]);
+            ws->exec[k] = nullptr;
+            ws->graph[k] = nullptr;
+        }
+        ws->plan = ph;
+    }
     uint32_t *f[3] = {ws->flags, ws->flags + ws->words, ws->flags + 2 * (size_t)ws->words};
     MultiArgs<OffT> a{ws->plan->dev, view<OffT>(g), {f[0], f[1], f[2]}, ws->ctl, g->n, unit};
     const int grid = ws->grid;

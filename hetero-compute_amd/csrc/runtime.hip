@@ -51,6 +51,10 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
         {"bu.wave_threshold", &t.bu.wave_threshold},
         {"bu.g16_threshold", &t.bu.g16_threshold},
         {"bu.g4_threshold", &t.bu.g4_threshold},
+        {"multi.hub_threshold", &t.multi.hub_threshold},
+        {"multi.hub_chunk", &t.multi.hub_chunk},
+        {"multi.wave_threshold", &t.multi.wave_threshold},
+        {"multi.g16_threshold", &t.multi.g16_threshold},
     };
     for (auto &x : i64)
         if (!strcmp(key, x.k)) return x.p;
@@ -62,7 +66,8 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"pull.stage", &t.stage_pull},
                {"bu.stage", &t.stage_bu},
                {"profile.kernel_events", &t.kernel_events},
-               {"multi.graph", &t.multi_graph}};
+               {"multi.graph", &t.multi_graph},
+               {"multi.grid_per_cu", &t.multi_grid_per_cu}};
     for (auto &x : i32)
         if (!strcmp(key, x.k)) {
             *is_int = true;
@@ -525,8 +530,8 @@ int hc_set_tuning(const char *key, int64_t value) {
         *iref = (int)value;
     } else {
         HC_CHECK_ARG(value >= 0, "%s must be >= 0", key);
-        if (!strcmp(key, "pull.hub_chunk"))
-            HC_CHECK_ARG(value >= 256, "pull.hub_chunk must be >= 256");
+        if (!strcmp(key, "pull.hub_chunk") || !strcmp(key, "multi.hub_chunk"))
+            HC_CHECK_ARG(value >= 256, "%s must be >= 256", key);
         *p = value;
     }
     return HC_OK;

@@ -278,20 +278,25 @@ int hc_schedule_get_block(hc_schedule_t s, int32_t i, hc_schedule_block_t *out) 
 
 int hc_schedule_to_text(hc_schedule_t s, char *buf, size_t cap, size_t *needed) {
     HC_CHECK_ARG(s, "NULL schedule");
-    std::string text;
-    int prev_dev = -1, prev_id = -1;
+    // One section per device instance, in order of first appearance, as
+    // write_schedule emits one per DeviceSchedule (sched.py:279-286).
+    std::vector<std::pair<int, int>> inst;
     for (const hc_schedule_block_t &b : s->blocks) {
-        if (b.device != prev_dev || b.device_id != prev_id) {
-            text += kDevices[b.device].name;
-            text += '\n';
-            prev_dev = b.device;
-            prev_id = b.device_id;
-        }
-        text += std::to_string(b.start_segment) + ' ' + std::to_string(b.end_segment) + '\n';
-        text += kKernels[b.kernel].name;
-        if (kKernels[b.kernel].device_type == HC_DEV_GPU)
-            text += ' ' + std::to_string(b.blocks) + ' ' + std::to_string(b.threads);
+        std::pair<int, int> k{b.device, b.device_id};
+        if (std::find(inst.begin(), inst.end(), k) == inst.end()) inst.push_back(k);
+    }
+    std::string text;
+    for (const auto &k : inst) {
+        text += kDevices[k.first].name;
         text += '\n';
+        for (const hc_schedule_block_t &b : s->blocks) {
+            if (b.device != k.first || b.device_id != k.second) continue;
+            text += std::to_string(b.start_segment) + ' ' + std::to_string(b.end_segment) + '\n';
+            text += kKernels[b.kernel].name;
+            if (kKernels[b.kernel].device_type == HC_DEV_GPU)
+                text += ' ' + std::to_string(b.blocks) + ' ' + std::to_string(b.threads);
+            text += '\n';
+        }
     }
     if (needed) *needed = text.size() + 1;
     if (!buf) return HC_OK;

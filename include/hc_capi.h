@@ -211,6 +211,10 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   pull.stage / bu.stage  per-vertex entries / bitmap words staged in LDS
  *   profile.kernel_events  1: time each launch of the loop's main kernel
  *                          (hc_run_stats_t.kernel_ms); 0 by default
+ *   multi.graph            1: batched loops relaunch an epoch hipGraph
+ *   multi.grid_per_cu      workgroups per CU of the batched sweep (8)
+ *   multi.hub_threshold / multi.hub_chunk / multi.wave_threshold /
+ *   multi.g16_threshold    degree classes of the batched sweep
  * Plans are rebuilt for new values; changing knobs while another thread
  * runs on the same graph is not supported. */
 int hc_set_tuning(const char *key, int64_t value);

@@ -103,8 +103,9 @@ int hc_schedule_info(hc_schedule_t s, int32_t *num_blocks,
                      int32_t *num_segments);
 int hc_schedule_get_block(hc_schedule_t s, int32_t i, hc_schedule_block_t *out);
 /* Writes .skd text into buf (cap bytes incl. NUL; buf may be NULL to size
- * it); *needed gets the size including the NUL.  A device line is written
- * whenever (device, device_id) differs from the previous block's. */
+ * it); *needed gets the size including the NUL.  One device line per device
+ * instance (in order of first appearance) followed by its blocks, as
+ * write_schedule does; parsing the text back gives the same instances. */
 int hc_schedule_to_text(hc_schedule_t s, char *buf, size_t cap, size_t *needed);
 int hc_schedule_free(hc_schedule_t s);
 
@@ -152,6 +153,15 @@ double hc_cpu_pr_pull_epoch(const hc_host_graph_t *g, float damping,
                             const double *inv_degree, float *score_out,
                             float *contrib_out, int32_t start_id,
                             int32_t end_id, int tid, int nthreads);
+
+/* Host-core epoch time of rows [start_id, end_id) with all threads of the
+ * default OpenMP team, mean over `repeats` (algorithm 0 SSSP pull, 1 BFS
+ * bottom-up, 2 PageRank), on a steady state in which every row is scanned
+ * in full: the per-segment CPU measurement the schedule autotuner uses (the
+ * reference measures segments with TreeBenchmark,
+ * src/benchmarks/benchmark.cuh:87-309). */
+int hc_cpu_epoch_ms(const hc_host_graph_t *g, int32_t algorithm, int32_t start_id,
+                    int32_t end_id, int32_t repeats, double *ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,78 @@
+"""Autotuner logic on the CPU: the host/GPU split of a segment profile, the
+merged schedule it emits, the cache, and the per-segment host timing."""
+from __future__ import annotations
+
+import ctypes as C
+import importlib
+
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def at(hc):
+    return importlib.import_module("hetero-compute_amd.autotune")
+
+
+def _profile(gpu_ms, cpu_ms, n=1 << 20, alg="sssp"):
+    segs = []
+    for i, (g, c) in enumerate(zip(gpu_ms, cpu_ms)):
+        segs.append({"start": i * 10, "end": i * 10 + 10, "edges": 100, "cpu_ms": c,
+                     "gpu": {"SSSP GPU bucketed": (g, 0, 0),
+                             "SSSP GPU warp-red": (g * 1.5, 256, 256)}})
+    return {"algorithm": alg, "num_nodes": n, "segments": segs}
+
+
+def test_all_gpu_when_host_is_slow(hc, at):
+    blocks, makespan, plan = at.schedule_from_profile(_profile([1.0] * 6, [50.0] * 6))
+    assert plan["cpu_segments"] == []
+    assert blocks == [{"device": "AMD Instinct MI355X", "start": 0, "end": 5,
+                       "kernel": "SSSP GPU bucketed", "blocks": 0, "threads": 0}]
+    assert makespan == pytest.approx(6.0)
+
+
+def test_split_lowers_makespan(hc, at):
+    # Segment 2 is slow on the GPU and cheap on the host: it moves; the
+    # exchange cost (8 bytes per vertex over 25 GB/s) is charged once.
+    gpu = [1.0, 1.0, 9.0, 1.0]
+    cpu = [30.0, 30.0, 2.0, 30.0]
+    blocks, makespan, plan = at.schedule_from_profile(_profile(gpu, cpu, n=1000))
+    assert plan["cpu_segments"] == [2]
+    assert makespan == pytest.approx(3.0 + 8000 / 25e9 * 1e3)
+    assert [b["device"] for b in blocks] == ["AMD Instinct MI355X", "Host CPU",
+                                             "AMD Instinct MI355X"]
+    s = hc.Schedule.create(blocks, 4)
+    key = lambda b: b["start"]  # noqa: E731  (sections group blocks per device)
+    assert sorted(hc.Schedule.parse(s.text()).blocks, key=key) == sorted(s.blocks, key=key)
+    # An exchange dearer than the gain keeps everything on the GPU.
+    _, _, plan2 = at.schedule_from_profile(_profile(gpu, cpu, n=1 << 32))
+    assert plan2["cpu_segments"] == []
+
+
+def test_best_gpu_kernel_per_segment(hc, at):
+    prof = _profile([2.0, 2.0], [100.0, 100.0])
+    prof["segments"][1]["gpu"]["SSSP GPU warp-red"] = (0.5, 1024, 256)
+    blocks, _, plan = at.schedule_from_profile(prof)
+    assert plan["gpu_kernels"] == ["SSSP GPU bucketed", "SSSP GPU warp-red"]
+    assert blocks[1] == {"device": "AMD Instinct MI355X", "start": 1, "end": 1,
+                         "kernel": "SSSP GPU warp-red", "blocks": 1024, "threads": 256}
+
+
+def test_cache_roundtrip(hc, at, tmp_path):
+    g = hc.HostGraph.generate("kron", 8, 8)
+    key = f"dev|sssp|{at.signature(g)}"
+    p = str(tmp_path / "c.json")
+    at.save(p, key, {"knobs": {"grid_per_cu": 4}})
+    at.save(p, "other", {"knobs": {}})
+    assert at.load(p, key) == {"knobs": {"grid_per_cu": 4}}
+    assert at.load(p, "missing") is None
+
+
+@pytest.mark.parametrize("alg", [0, 1, 2])
+def test_cpu_epoch_timing(hc, alg):
+    g = hc.HostGraph.generate("kron", 12, 16, weighted=True)
+    ms = C.c_double()
+    hc.check(hc.lib().hc_cpu_epoch_ms(g.struct_ref(), alg, 0, g.num_nodes, 2, C.byref(ms)))
+    assert ms.value > 0
+    with pytest.raises(hc.HcError):
+        hc.check(hc.lib().hc_cpu_epoch_ms(g.struct_ref(), 3, 0, 1, 1, C.byref(ms)))
