@@ -127,6 +127,42 @@ def committed_traffic(kernel, kind, scale):
     return None
 
 
+def class_sweep(hc, kind, scale, degree, seed, reps=3):
+    """Times one SSSP pull sweep per degree class of the bucketed plan (the
+    epoch API on each class's vertex range, at the fixpoint so nothing is
+    written), then the whole graph: where the sweep's time goes."""
+    g = hc.HostGraph.generate(kind, scale, degree, weighted=True, seed=seed)
+    out = []
+    with hc.DeviceGraph(g) as dg:
+        s = int(g.pick_sources(1)[0])
+        dist = hc.DeviceArray.from_host(hc.sssp(dg, s).values)
+        deg = np.diff(g.index)
+        t = hc.tuning()
+        thr = [("hub chunks", t["pull.hub_threshold"]), ("workgroup", t["pull.block_threshold"]),
+               ("wavefront", t["pull.wave_threshold"]), ("16 lanes", t["pull.g16_threshold"]),
+               ("4 lanes", t["pull.g4_threshold"]), ("1 lane", 0)]
+        lo = 0
+        ranges = []
+        for name, T in thr:
+            hi = lo + int(np.count_nonzero(deg[lo:] > T))
+            ranges.append((name, lo, hi))
+            lo = hi
+        ranges.append(("all rows", 0, g.num_nodes))
+        for name, a, b in ranges:
+            ms = []
+            for _ in range(reps):
+                hc.synchronize()
+                t0 = time.perf_counter()
+                hc.sssp_pull_epoch(dg, dist, a, b, mapping=hc.MAP_BUCKETED)
+                hc.synchronize()
+                ms.append((time.perf_counter() - t0) * 1e3)
+            e = int(g.index[b] - g.index[a])
+            out.append({"class": name, "rows": b - a, "edges": e, "ms": round(min(ms), 4),
+                        "gedges_per_s": round(e / (min(ms) / 1e3) / 1e9, 1)})
+        dist.free()
+    print(json.dumps({"class_sweep": f"{kind}{scale} ef{degree}", "classes": out}), flush=True)
+
+
 def copy_probe(nbytes, reps):
     """A plain device copy of known size (torch's copy kernel): the reference
     point that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
@@ -359,6 +395,8 @@ def main():
     ap.add_argument("--serve", type=int, default=1, help="run the serving loop")
     ap.add_argument("--serve-scale", type=int, default=24)
     ap.add_argument("--serve-queries", type=int, default=400)
+    ap.add_argument("--class-sweep", default="", metavar="KIND",
+                    help="only time the SSSP pull sweep per degree class on KIND (kron|uniform)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -375,6 +413,9 @@ def main():
     import __graft_entry__ as ge
     hc = ge.package()
     hc.check(hc.lib().hc_init(0))
+    if args.class_sweep:
+        class_sweep(hc, args.class_sweep, args.scale, args.degree, args.seed)
+        return
     dev = hc.device_info()
     log(f"device {dev['name']} {dev['arch']} CUs={dev['compute_units']}")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

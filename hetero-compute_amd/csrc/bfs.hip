@@ -530,15 +530,17 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     for (int c = 1; c < kNumClasses; c++) plan_rows += ph->dev.cnt[c];
     if (!depth) depth = ws->depth;
     GraphView<OffT> v = view<OffT>(g);
-    const int grid = ws->f.grid;
+    const int32_t nstage = std::min<int32_t>(tuning().stage_bu, ws->f.fb.words);
+    const int grid = resident_grid(reinterpret_cast<const void *>(&bfs_step_kernel<OffT>),
+                                   kBlock, nstage * sizeof(uint32_t), ws->f.grid);
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     bfs_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, depth, parents, ws->f.fb, ws->f.ctl, src, direction, alpha, beta, 0,
         (long long)g->m, plan_rows, grid);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
-    BfsLaunch<OffT> L{ph->dev, v, depth, parents, ws->f.fb, ws->f.ctl, grid, s,
-                      std::min<int32_t>(tuning().stage_bu, ws->f.fb.words), &ws->f.kt};
+    BfsLaunch<OffT> L{ph->dev, v, depth, parents, ws->f.fb, ws->f.ctl, grid, s, nstage,
+                      &ws->f.kt};
     const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
     long long launches = 0;

@@ -324,6 +324,10 @@ DeviceState &device_state();
 int ensure_init();
 // Workgroups for a full-device launch: per_cu per CU (0 = the tuned value).
 int grid_blocks(int per_cu = 0);
+// Workgroups of `kernel` (block threads, lds_bytes dynamic LDS) that fit on
+// the device at once, at most cap: a grid-stride launch of this size has no
+// second wave of late-starting workgroups.
+int resident_grid(const void *kernel, int block, size_t lds_bytes, int cap);
 hipStream_t pick_stream(hc_stream_t s);
 
 }  // namespace hc

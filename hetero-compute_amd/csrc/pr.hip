@@ -404,8 +404,9 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     PrWs *ws = nullptr;
     HC_TRY(get_pr_ws(g, ph->dev.cnt[0], ph->dev.num_hubs, &ws));
     GraphView<OffT> v = view<OffT>(g);
-    const int grid = grid_blocks();
     const int32_t ns = std::min<int32_t>(tuning().stage_pull, g->n);
+    const int grid = resident_grid(reinterpret_cast<const void *>(&pr_step_kernel<OffT>), kBlock,
+                                   ns * sizeof(float), grid_blocks());
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     const float base = (1.0f - damp) / (float)g->n;
     pr_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(

@@ -32,6 +32,25 @@ int grid_blocks(int per_cu) {
     return std::max(1, device_state().cus * per_cu);
 }
 
+int resident_grid(const void *kernel, int block, size_t lds_bytes, int cap) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void *, int, size_t>, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(kernel, block, lds_bytes);
+    auto it = cache.find(key);
+    int per_cu = 0;
+    if (it != cache.end()) {
+        per_cu = it->second;
+    } else {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds_bytes) !=
+                hipSuccess ||
+            per_cu <= 0)
+            per_cu = 1;
+        cache.emplace(key, per_cu);
+    }
+    return std::max(1, std::min(cap, device_state().cus * per_cu));
+}
+
 namespace {
 int64_t g_knob_tmp;
 int64_t *knob_ref(const char *key, bool *is_int, int **iref) {

@@ -553,15 +553,16 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     HC_TRY(get_plan(g, 0, g->n, tuning().pull, &ph));
     GraphView<OffT> v = view<OffT>(g);
     const long long sweep = sssp_sweep_bytes(g, ph);
-    const int grid = ws->f.grid;
+    const int32_t nstage = std::min<int32_t>(tuning().stage_pull, g->n);
+    const int grid = resident_grid(reinterpret_cast<const void *>(&sssp_step_kernel<OffT>),
+                                   kBlock, nstage * sizeof(int32_t), ws->f.grid);
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
         max_epochs, (long long)g->m, sweep, grid);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
-    SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s,
-                       std::min<int32_t>(tuning().stage_pull, g->n), &ws->f.kt};
+    SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s, nstage, &ws->f.kt};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
     const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;
     float ms = 0;
