@@ -286,6 +286,54 @@ def schedule_from_profile(profile: dict, pcie_gbs: float = 25.0):
     return blocks, cur, plan
 
 
+def write_yaml_profiles(profile: dict, g, outdir: str, cpu_name: str = "Host CPU",
+                        gpu_name: str = "AMD Instinct MI355X") -> list[str]:
+    """Writes the segment profile as the reference's per-kernel YAML files
+    (layer results, src/benchmarks/benchmark.cuh:184-207): one file per
+    device and kernel with `config: {device, kernel}` and `results: -
+    segments: [...]`, the input of util/scheduler/sched.py.  Launch shapes
+    are part of the GPU kernel name, as the reference's profiles carry them."""
+    os.makedirs(outdir, exist_ok=True)
+    alg = profile["algorithm"]
+    deg = np.diff(g.index)
+    rows = []
+    for sgm in profile["segments"]:
+        d = deg[sgm["start"]:sgm["end"]]
+        rows.append({"start": sgm["start"], "end": sgm["end"],
+                     "avg_deg": float(d.mean()) if d.size else 0.0,
+                     "min_deg": int(d.min()) if d.size else 0,
+                     "max_deg": int(d.max()) if d.size else 0,
+                     "num_edges": sgm["edges"]})
+
+    def emit(path, device, kernel, millis):
+        lines = ["config:", f"  device: {device}", f"  kernel: {kernel}", "results:",
+                 "  - segments:"]
+        for r, ms in zip(rows, millis):
+            gteps = r["num_edges"] / 2 / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+            lines += [f"      - start: {r['start']}", f"        end: {r['end']}",
+                      f"        avg_deg: {r['avg_deg']:.6g}", f"        min_deg: {r['min_deg']}",
+                      f"        max_deg: {r['max_deg']}", f"        num_edges: {r['num_edges']}",
+                      f"        millis: {ms:.6g}", f"        gteps: {gteps:.6g}"]
+        with open(path, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        return path
+
+    out = [emit(os.path.join(outdir, f"{alg}_cpu_{CPU_KERNEL[alg].split()[-1]}.yaml"), cpu_name,
+                CPU_KERNEL[alg], [sg["cpu_ms"] for sg in profile["segments"]])]
+    for name in GPU_KERNELS[alg]:
+        if not all(name in sg["gpu"] for sg in profile["segments"]):
+            continue
+        # The reference profiles one launch shape per file; use the shape
+        # that is best over most segments.
+        shapes = [sg["gpu"][name][1:] for sg in profile["segments"]]
+        b, t = max(set(shapes), key=shapes.count)
+        kname = f"{name} {b} {t}"
+        tag = name.split()[-1]
+        out.append(emit(os.path.join(outdir, f"{alg}_gpu_{tag}.yaml"), gpu_name, kname,
+                        [sg["gpu"][name][0] for sg in profile["segments"]]))
+    return out
+
+
 def tune_schedule(g, algorithm: str, num_segments: int = 24, pcie_gbs: float = 25.0):
     prof = profile_segments(g, algorithm, num_segments)
     blocks, makespan, plan = schedule_from_profile(prof, pcie_gbs)

@@ -76,3 +76,28 @@ def test_cpu_epoch_timing(hc, alg):
     assert ms.value > 0
     with pytest.raises(hc.HcError):
         hc.check(hc.lib().hc_cpu_epoch_ms(g.struct_ref(), 3, 0, 1, 1, C.byref(ms)))
+
+
+def test_yaml_profiles_in_reference_layout(hc, at, tmp_path):
+    import yaml
+    g = hc.HostGraph.generate("kron", 8, 8)
+    bounds = g.equal_edge_ranges(3)
+    segs = []
+    for i in range(3):
+        a, b = int(bounds[i]), int(bounds[i + 1])
+        segs.append({"start": a, "end": b, "edges": int(g.index[b] - g.index[a]),
+                     "cpu_ms": 1.0 + i, "gpu": {"SSSP GPU bucketed": (0.5, 0, 0),
+                                                  "SSSP GPU warp-red": (0.7, 256, 256)}})
+    files = at.write_yaml_profiles({"algorithm": "sssp", "num_nodes": g.num_nodes,
+                                    "segments": segs}, g, str(tmp_path))
+    assert len(files) == 3
+    docs = [yaml.safe_load(open(f)) for f in files]
+    cpu = [d for d in docs if d["config"]["device"] == "Host CPU"][0]
+    assert cpu["config"]["kernel"] == "SSSP CPU one-to-one"
+    assert [s_["millis"] for s_ in cpu["results"][0]["segments"]] == [1.0, 2.0, 3.0]
+    warp = [d for d in docs if d["config"]["kernel"].startswith("SSSP GPU warp-red")][0]
+    assert warp["config"]["kernel"] == "SSSP GPU warp-red 256 256"
+    # The kernel line of a profile is a valid schedule kernel line.
+    assert hc.kernel_id(warp["config"]["kernel"]) == hc.KERNELS.index("SSSP GPU warp-red")
+    seg0 = cpu["results"][0]["segments"][0]
+    assert seg0["start"] == 0 and seg0["num_edges"] == segs[0]["edges"]
