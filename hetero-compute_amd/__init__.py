@@ -76,7 +76,7 @@ class RunStats(C.Structure):
 class EpochTrace(C.Structure):
     _fields_ = [("count", C.c_int32), ("mode", C.c_int8 * 128),
                 ("frontier", C.c_uint32 * 128), ("frontier_edges", C.c_uint64 * 128),
-                ("scanned", C.c_uint64 * 128)]
+                ("scanned", C.c_uint64 * 128), ("us", C.c_double * 128)]
 
 
 class SsspOpts(C.Structure):
@@ -213,7 +213,8 @@ _SIGS = {
                               C.c_int32, C.c_int32, C.c_int, C.c_int], C.c_double),
 }
 
-TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "pull.stage", "bu.stage", "profile.kernel_events",
+TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "sssp.filter", "sssp.stage",
+               "sssp.block_threshold", "pull.stage", "bu.stage", "profile.kernel_events",
                "multi.graph", "multi.grid_per_cu", "pull.hub_threshold", "pull.hub_chunk",
                "pull.block_threshold", "pull.wave_threshold", "pull.g16_threshold",
                "pull.g4_threshold", "bu.wave_threshold", "bu.g16_threshold",
@@ -676,7 +677,8 @@ def epoch_trace(g: DeviceGraph, algorithm: str) -> list[dict]:
     t = EpochTrace()
     check(lib().hc_get_epoch_trace(g.h, 0 if algorithm == "sssp" else 1, C.byref(t)))
     return [{"mode": "pull" if t.mode[i] else "push", "frontier": t.frontier[i],
-             "frontier_edges": t.frontier_edges[i], "scanned": t.scanned[i]}
+             "frontier_edges": t.frontier_edges[i], "scanned": t.scanned[i],
+             "us": round(t.us[i], 2)}
             for i in range(t.count)]
 
 

@@ -45,21 +45,22 @@ else:  # run as a script
 # Candidate values per knob, searched one knob at a time in this order.
 KNOB_SPACE = {
     "sssp": {
-        "grid_per_cu": [2, 4, 8, 16],
-        "sssp.alpha": [4, 8, 16, 32],
-        "pull.stage": [0, 2048, 8192, 15360],
-        "pull.wave_threshold": [32, 64, 96, 192],
-        "pull.g16_threshold": [8, 16, 24, 48],
+        "sssp.alpha": [8, 16, 32],
+        "sssp.filter": [0, 1],
+        "sssp.stage": [0, 2048, 8192],
+        "sssp.block_threshold": [1024, 4096, 16384],
+        "pull.wave_threshold": [48, 96, 192],
+        "pull.g16_threshold": [12, 24, 48],
         "pull.g4_threshold": [2, 5, 8],
-        "pull.block_threshold": [512, 1024, 4096],
-        "pull.hub_threshold": [4096, 16384, 65536],
+        "pull.hub_threshold": [16384, 65536],
+        "grid_per_cu": [4, 8, 16],
     },
     "bfs": {
         "grid_per_cu": [2, 4, 8, 16],
         "bu.stage": [0, 2048, 8192, 15360],
-        "bu.wave_threshold": [128, 256, 1024],
-        "bu.g16_threshold": [16, 32, 128],
-        "bu.g4_threshold": [4, 8, 32],
+        "bu.wave_threshold": [1024, 4096],
+        "bu.g16_threshold": [128, 256, 512],
+        "bu.g4_threshold": [16, 32, 64],
     },
     "pr": {
         "grid_per_cu": [2, 4, 8, 16],

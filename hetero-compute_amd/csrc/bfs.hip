@@ -30,42 +30,71 @@ __device__ __forceinline__ bool bit_test(const Staged<uint32_t> &bm, int32_t v) 
 }
 
 // Scans row [b, e) of u with G lanes from lane `lane` (group base lane gbase
-// within the wavefront); returns the first neighbour in the frontier.
+// within the wavefront); returns the first neighbour in the frontier (CSR
+// order).  The first round reads one position per lane: neighbours are in
+// id order, ids in degree order, and the low-id hubs are the likeliest
+// parents, so most scans end there.  Later rounds read four positions per
+// lane (predicated) and test their bits together; the first hit is the
+// earliest position of the earliest round with one.  `scanned` counts the
+// positions examined.
 template <int G, typename OffT>
 __device__ __forceinline__ bool bu_scan(const GraphView<OffT> &g,
                                         const Staged<uint32_t> &frontier,
                                         OffT b, OffT e, int lane, int gbase,
                                         int32_t &parent, uint32_t &scanned) {
-    if (G == 1) {
-        for (OffT i = b; i < e; i++) {
-            int32_t v = ldnt(&g.adj[i]);
+    const int L = G == 1 ? 0 : lane;
+    {
+        const OffT k = b + OffT(L);
+        const int32_t v = k < e ? ldnt(&g.adj[k]) : -1;
+        const bool h = v >= 0 && bit_test(frontier, v);
+        if (G == 1) {
+            if (v < 0) return false;
             scanned++;
-            if (bit_test(frontier, v)) {
+            if (h) {
                 parent = v;
                 return true;
             }
-        }
-        return false;
-    } else {
-        for (OffT base = b; base < e; base += OffT(G)) {
-            OffT i = base + OffT(lane);
-            bool hit = false;
-            int32_t v = 0;
-            if (i < e) {
-                v = ldnt(&g.adj[i]);
-                scanned++;
-                hit = bit_test(frontier, v);
-            }
-            uint64_t m = __ballot(hit);
-            uint64_t gm = G == 64 ? m : ((m >> gbase) & ((uint64_t(1) << G) - 1));
+        } else {
+            scanned += v >= 0 ? 1u : 0u;
+            const uint64_t m = __ballot(h);
+            const uint64_t gm = G == 64 ? m : ((m >> gbase) & ((uint64_t(1) << G) - 1));
             if (gm) {
-                int first = __ffsll((unsigned long long)gm) - 1;
-                parent = __shfl(v, gbase + first);
+                parent = __shfl(v, gbase + __ffsll((unsigned long long)gm) - 1);
                 return true;
             }
         }
-        return false;
     }
+    for (OffT base = b + OffT(G); base < e; base += OffT(4 * G)) {
+        int32_t v[4];
+        bool h[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const OffT k = base + OffT(r * G + L);
+            v[r] = k < e ? ldnt(&g.adj[k]) : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) h[r] = v[r] >= 0 && bit_test(frontier, v[r]);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (G == 1) {
+                if (v[r] < 0) return false;
+                scanned++;
+                if (h[r]) {
+                    parent = v[r];
+                    return true;
+                }
+            } else {
+                scanned += v[r] >= 0 ? 1u : 0u;
+                const uint64_t m = __ballot(h[r]);
+                const uint64_t gm = G == 64 ? m : ((m >> gbase) & ((uint64_t(1) << G) - 1));
+                if (gm) {
+                    parent = __shfl(v[r], gbase + __ffsll((unsigned long long)gm) - 1);
+                    return true;
+                }
+            }
+        }
+    }
+    return false;
 }
 
 template <int G, typename OffT>

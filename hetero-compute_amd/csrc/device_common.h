@@ -95,7 +95,10 @@ struct PlanConfig {
 // caller; read when plans and workspaces are built).
 struct Tuning {
     int grid_per_cu = 8;   // workgroups per CU of the fused / step launches
-    int sssp_alpha = 8;    // SSSP goes pull when frontier edges > m / alpha
+    int sssp_alpha = 16;   // SSSP goes pull when frontier edges > m / alpha
+    int sssp_filter = 0;   // SSSP pulls gather only neighbours changed last epoch
+    int sssp_stage = 0;    // per-vertex entries staged in LDS by SSSP pull sweeps
+    int64_t sssp_block_threshold = 4096;  // SSSP: degree above which a WG takes a row
     int stage_pull = 8192; // per-vertex entries staged in LDS by pull sweeps
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
     int kernel_events = 0; // 1: time every launch of a loop's main kernel
@@ -107,9 +110,9 @@ struct Tuning {
     Tuning() {
         bu.hub_threshold = INT64_MAX;
         bu.block_threshold = INT64_MAX;
-        bu.wave_threshold = 256;
-        bu.g16_threshold = 32;
-        bu.g4_threshold = 8;
+        bu.wave_threshold = 1024;
+        bu.g16_threshold = 256;
+        bu.g4_threshold = 32;
         multi.hub_threshold = 4096;
         multi.hub_chunk = 2048;
         multi.block_threshold = INT64_MAX;
@@ -119,6 +122,14 @@ struct Tuning {
     }
 };
 Tuning &tuning();
+// Plan of the SSSP pull sweeps: the shared pull classes with SSSP's own
+// workgroup threshold.
+inline PlanConfig sssp_plan_config() {
+    PlanConfig c = tuning().pull;
+    c.block_threshold = std::max<int64_t>(tuning().sssp_block_threshold, c.wave_threshold);
+    if (c.hub_threshold < c.block_threshold) c.hub_threshold = c.block_threshold;
+    return c;
+}
 
 // Per-launch timing of a loop's main kernel with HIP events recorded on the
 // kernel's own stream just before and after it (enabled by the

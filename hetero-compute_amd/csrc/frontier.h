@@ -64,7 +64,9 @@ struct StepCtl {
     unsigned long long reached;
     unsigned long long alg_bytes;  // algorithmic bytes moved by the epochs
     int32_t plan_rows;
-    int32_t pad;
+    int32_t filter;                // SSSP: next pull gathers changed neighbours only
+    int32_t filter_knob;           // SSSP: filtering allowed (tuning knob)
+    int32_t pad2;
     // Device wall clock (constant rate): start of the loop, end of the last
     // epoch, and end of each traced epoch.
     long long t_start;

@@ -134,11 +134,14 @@ extern "C" int hc_get_epoch_trace(hc_graph_t g, int algorithm,
     HC_HIP(hipMemcpy(&c, static_cast<FrontierWs *>(ws)->ctl, sizeof(StepCtl),
                      hipMemcpyDeviceToHost));
     out->count = std::min(c.epoch, kTraceLen);
+    const double khz = device_state().clock_khz;
     for (int i = 0; i < out->count; i++) {
         out->mode[i] = c.tr_mode[i];
         out->frontier[i] = c.tr_n[i];
         out->frontier_edges[i] = c.tr_e[i];
         out->scanned[i] = c.tr_scan[i];
+        const long long t0 = i == 0 ? c.t_start : c.tr_t[i - 1];
+        out->us[i] = khz > 0 ? double(c.tr_t[i] - t0) * 1e3 / khz : 0.0;
     }
     return HC_OK;
 }

@@ -44,30 +44,48 @@ __device__ __forceinline__ KSum group_ksum(KSum k) {
     return k;
 }
 
-// Kahan-compensated sum of contrib over edges [b, e) strided by G.
+// Kahan-compensated sum of contrib over edges [b, e) strided by G, in edge
+// order per lane.  Trips of four edges with predicated loads, and the next
+// trip's ids loaded before this trip's gathers (one memory round trip per
+// trip); padding lanes skip the update, so the sum is the same sequence of
+// Kahan steps as a plain loop over the lane's edges.
 template <int G, typename OffT>
 __device__ __forceinline__ KSum row_ksum(const GraphView<OffT> &g,
                                          const Staged<float> &contrib,
                                          OffT b, OffT e, int lane) {
     float sum = 0.0f, comp = 0.0f;
     OffT i = b + OffT(lane);
-    for (; i + OffT(3 * G) < e; i += OffT(4 * G)) {
-        float x0 = contrib[ldnt(&g.adj[i])], x1 = contrib[ldnt(&g.adj[i + G])],
-              x2 = contrib[ldnt(&g.adj[i + 2 * G])], x3 = contrib[ldnt(&g.adj[i + 3 * G])];
-        float xs[4] = {x0, x1, x2, x3};
+    if (i >= e) return KSum{0.0f, 0.0f};
+    int32_t v[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            float y = xs[j] - comp;
+    for (int r = 0; r < 4; r++) {
+        const OffT k = i + OffT(r * G);
+        v[r] = k < e ? ldnt(&g.adj[k]) : -1;
+    }
+    for (;;) {
+        const OffT ni = i + OffT(4 * G);
+        int32_t nv[4] = {-1, -1, -1, -1};
+        if (ni < e) {  // short rows skip the prefetch instructions entirely
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const OffT k = ni + OffT(r * G);
+                nv[r] = k < e ? ldnt(&g.adj[k]) : -1;
+            }
+        }
+        float x[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) x[r] = v[r] >= 0 ? contrib[v[r]] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (v[r] < 0) continue;
+            float y = x[r] - comp;
             float t = sum + y;
             comp = (t - sum) - y;
             sum = t;
         }
-    }
-    for (; i < e; i += OffT(G)) {
-        float y = contrib[ldnt(&g.adj[i])] - comp;
-        float t = sum + y;
-        comp = (t - sum) - y;
-        sum = t;
+        if (ni >= e) break;
+        for (int r = 0; r < 4; r++) v[r] = nv[r];
+        i = ni;
     }
     return KSum{sum, -comp};
 }

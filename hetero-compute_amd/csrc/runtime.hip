@@ -61,6 +61,7 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
         const char *k;
         int64_t *p;
     } i64[] = {
+        {"sssp.block_threshold", &t.sssp_block_threshold},
         {"pull.hub_threshold", &t.pull.hub_threshold},
         {"pull.hub_chunk", &t.pull.hub_chunk},
         {"pull.block_threshold", &t.pull.block_threshold},
@@ -82,6 +83,8 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
         int *p;
     } i32[] = {{"grid_per_cu", &t.grid_per_cu},
                {"sssp.alpha", &t.sssp_alpha},
+               {"sssp.filter", &t.sssp_filter},
+               {"sssp.stage", &t.sssp_stage},
                {"pull.stage", &t.stage_pull},
                {"bu.stage", &t.stage_bu},
                {"profile.kernel_events", &t.kernel_events},
@@ -539,8 +542,10 @@ int hc_set_tuning(const char *key, int64_t value) {
     int64_t *p = knob_ref(key, &is_int, &iref);
     if (!p) return fail(HC_ERR_NOT_FOUND, "unknown tuning key '%s'", key);
     if (is_int) {
-        const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage");
-        const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph");
+        const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
+                           !strcmp(key, "sssp.stage");
+        const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph") ||
+                          !strcmp(key, "sssp.filter");
         HC_CHECK_ARG(value >= (stage || flag ? 0 : 1) && value <= (flag ? 1 : 1 << 20),
                      "%s out of range", key);
         if (stage)

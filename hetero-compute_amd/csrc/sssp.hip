@@ -25,24 +25,62 @@ namespace hc {
 // Row reductions
 // ---------------------------------------------------------------------------
 
-// Minimum candidate over edges [b, e) taken G apart starting at b+lane; four
-// independent gathers per trip keep several HBM requests in flight per lane.
-template <int G, typename OffT>
+__device__ __forceinline__ bool changed_bit(const Staged<uint32_t> &bm, int32_t v) {
+    return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+// Minimum candidate over edges [b, e) taken G apart starting at b+lane.
+// Trips of four edges per lane with predicated loads (a short row is one
+// trip of parallel loads, not a serial tail), and the next trip's ids and
+// weights are loaded before this trip's distance gathers, so each trip costs
+// about one memory round trip instead of two.
+// FILTER: only neighbours set in `changed` (the vertices whose distance
+// changed in the previous epoch) are gathered; the others cannot lower the
+// row below its current value.  `gathered` counts the gathers.
+template <int G, bool FILTER, typename OffT>
 __device__ __forceinline__ int32_t row_min(const GraphView<OffT> &g,
                                            const Staged<int32_t> &dist,
-                                           OffT b, OffT e, int lane) {
+                                           const Staged<uint32_t> &changed,
+                                           OffT b, OffT e, int lane, uint32_t &gathered) {
     int32_t m = HC_INF_DIST;
     OffT i = b + OffT(lane);
-    for (; i + OffT(3 * G) < e; i += OffT(4 * G)) {
-        int32_t v0 = ldnt(&g.adj[i]), v1 = ldnt(&g.adj[i + G]),
-                v2 = ldnt(&g.adj[i + 2 * G]), v3 = ldnt(&g.adj[i + 3 * G]);
-        int32_t w0 = ldnt(&g.wt[i]), w1 = ldnt(&g.wt[i + G]),
-                w2 = ldnt(&g.wt[i + 2 * G]), w3 = ldnt(&g.wt[i + 3 * G]);
-        int32_t d0 = dist[v0], d1 = dist[v1], d2 = dist[v2], d3 = dist[v3];
-        m = min(m, min(min(relax(d0, w0), relax(d1, w1)),
-                       min(relax(d2, w2), relax(d3, w3))));
+    if (i >= e) return m;
+    int32_t v[4], w[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const OffT k = i + OffT(r * G);
+        v[r] = k < e ? ldnt(&g.adj[k]) : -1;
+        w[r] = k < e ? ldnt(&g.wt[k]) : 0;
     }
-    for (; i < e; i += OffT(G)) m = min(m, relax(dist[ldnt(&g.adj[i])], ldnt(&g.wt[i])));
+    for (;;) {
+        const OffT ni = i + OffT(4 * G);
+        int32_t nv[4] = {-1, -1, -1, -1}, nw[4] = {0, 0, 0, 0};
+        if (ni < e) {  // short rows skip the prefetch instructions entirely
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const OffT k = ni + OffT(r * G);
+                nv[r] = k < e ? ldnt(&g.adj[k]) : -1;
+                nw[r] = k < e ? ldnt(&g.wt[k]) : 0;
+            }
+        }
+        bool use[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) use[r] = v[r] >= 0 && (!FILTER || changed_bit(changed, v[r]));
+        int32_t d[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) d[r] = use[r] ? dist[v[r]] : HC_INF_DIST;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            m = min(m, relax(d[r], w[r]));
+            gathered += use[r] ? 1u : 0u;
+        }
+        if (ni >= e) break;
+        for (int r = 0; r < 4; r++) {
+            v[r] = nv[r];
+            w[r] = nw[r];
+        }
+        i = ni;
+    }
     return m;
 }
 
@@ -66,18 +104,19 @@ __device__ __forceinline__ int32_t block_min(int32_t m) {
 // Fused degree-bucketed pull sweep
 // ---------------------------------------------------------------------------
 
-template <int G, typename OffT>
+template <int G, bool FILTER, typename OffT>
 __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
                                            const GraphView<OffT> &g,
                                            const Staged<int32_t> &rd,
+                                           const Staged<uint32_t> &changed,
                                            int32_t *__restrict__ dist,
-                                           bool &chg, int32_t &u) {
+                                           bool &chg, int32_t &u, uint32_t &gathered) {
     constexpr int PER = kBlock / G;
     const int gid = threadIdx.x / G, lane = threadIdx.x % G;
     const int idx = k * PER + gid;
     if (idx < p.cnt[c]) {
         int32_t uu = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
-        int32_t m = row_min<G>(g, rd, g.rbeg(uu), g.rend(uu), lane);
+        int32_t m = row_min<G, FILTER>(g, rd, changed, g.rbeg(uu), g.rend(uu), lane, gathered);
         if (G > 1) m = group_min<G>(m);
         if (lane == 0) {
             int32_t cur = dist[uu];
@@ -99,11 +138,14 @@ __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
 // snapshot); the row's own distance is read and written in global memory.
 // A stale snapshot only delays an improvement to a later epoch: values only
 // decrease, and an epoch without any change had nothing stale to read.
-template <bool LOOP, typename OffT>
+// FILTER: gather only neighbours in `changed` (see row_min); `gathered`
+// counts the gathers.
+template <bool LOOP, bool FILTER, typename OffT>
 __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
-                                const Staged<int32_t> &rd,
+                                const Staged<int32_t> &rd, const Staged<uint32_t> &changed,
                                 int32_t *__restrict__ dist, uint32_t *next_bm,
-                                uint32_t &cnt, unsigned long long &edges_acc) {
+                                uint32_t &cnt, unsigned long long &edges_acc,
+                                uint32_t &gathered) {
     const int total = p.unit_start[kNumClasses];
     for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
         int c = 0;
@@ -124,7 +166,8 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
                 b = g.rbeg(uu);
                 e = g.rend(uu);
             }
-            int32_t m = block_min(row_min<kBlock>(g, rd, b, e, threadIdx.x));
+            int32_t m = block_min(row_min<kBlock, FILTER>(g, rd, changed, b, e, threadIdx.x,
+                                                          gathered));
             if (threadIdx.x == 0) {
                 if (c == 0) {
                     if (m < dist[uu]) {
@@ -141,13 +184,13 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
                 u = uu;
             }
         } else if (c == 2) {
-            pull_group<64>(p, c, k, g, rd, dist, chg, u);
+            pull_group<64, FILTER>(p, c, k, g, rd, changed, dist, chg, u, gathered);
         } else if (c == 3) {
-            pull_group<16>(p, c, k, g, rd, dist, chg, u);
+            pull_group<16, FILTER>(p, c, k, g, rd, changed, dist, chg, u, gathered);
         } else if (c == 4) {
-            pull_group<4>(p, c, k, g, rd, dist, chg, u);
+            pull_group<4, FILTER>(p, c, k, g, rd, changed, dist, chg, u, gathered);
         } else {
-            pull_group<1>(p, c, k, g, rd, dist, chg, u);
+            pull_group<1, FILTER>(p, c, k, g, rd, changed, dist, chg, u, gathered);
         }
         if (LOOP) {
             if (wave_set_bit(next_bm, u, chg)) {
@@ -168,8 +211,8 @@ template <int G, typename OffT>
 __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
                                            int32_t *__restrict__ dist,
                                            const FrontierBufs &fb, StepCtl *ctl,
-                                           int nxt, int32_t u, OffT b, OffT e,
-                                           int lane, uint32_t &cnt,
+                                           int nxt, uint32_t *wbm, int32_t u, OffT b,
+                                           OffT e, int lane, uint32_t &cnt,
                                            unsigned long long &edges_acc) {
     const int32_t du = dist[u];
     for (OffT i = b + OffT(lane); i < e; i += OffT(G)) {
@@ -180,7 +223,7 @@ __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
             int32_t old = atomicMin(&dist[v], nd);
             if (nd < old) {
                 uint32_t bit = 1u << (v & 31);
-                app = !(atomicOr(&fb.bm[nxt][v >> 5], bit) & bit);
+                app = !(atomicOr(&wbm[v >> 5], bit) & bit);
             }
         }
         if (app) {
@@ -195,7 +238,7 @@ template <typename OffT>
 __device__ void sssp_push_sweep(const GraphView<OffT> &g,
                                 int32_t *__restrict__ dist,
                                 const FrontierBufs &fb, StepCtl *ctl, int cur,
-                                int nxt, uint32_t &cnt,
+                                int nxt, uint32_t *wbm, uint32_t &cnt,
                                 unsigned long long &edges_acc) {
     const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
                    n1 = ctl->cnt[cur][1], n2 = ctl->cnt[cur][2];
@@ -205,24 +248,24 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
     for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
-            push_range<kBlock>(g, dist, fb, ctl, nxt, h.u, OffT(h.b), OffT(h.e),
+            push_range<kBlock>(g, dist, fb, ctl, nxt, wbm, h.u, OffT(h.b), OffT(h.e),
                                threadIdx.x, cnt, edges_acc);
         } else if (unit < u1) {
             int32_t u = fb.q[cur][0][unit - nch];
-            push_range<kBlock>(g, dist, fb, ctl, nxt, u, g.rbeg(u), g.rend(u),
+            push_range<kBlock>(g, dist, fb, ctl, nxt, wbm, u, g.rbeg(u), g.rend(u),
                                threadIdx.x, cnt, edges_acc);
         } else if (unit < u2) {
             uint32_t idx = (unit - u1) * kWavesPerBlock + threadIdx.x / kWave;
             if (idx < n1) {
                 int32_t u = fb.q[cur][1][idx];
-                push_range<kWave>(g, dist, fb, ctl, nxt, u, g.rbeg(u), g.rend(u),
+                push_range<kWave>(g, dist, fb, ctl, nxt, wbm, u, g.rbeg(u), g.rend(u),
                                   lane_id(), cnt, edges_acc);
             }
         } else {
             uint32_t idx = (unit - u2) * kSmallPerUnit + threadIdx.x / kSmallLanes;
             if (idx < n2) {
                 int32_t u = fb.q[cur][2][idx];
-                push_range<kSmallLanes>(g, dist, fb, ctl, nxt, u, g.rbeg(u),
+                push_range<kSmallLanes>(g, dist, fb, ctl, nxt, wbm, u, g.rbeg(u),
                                         g.rend(u), threadIdx.x % kSmallLanes, cnt,
                                         edges_acc);
             }
@@ -242,27 +285,35 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     extern __shared__ int32_t s_stage[];
     if (ctl->done) return;
     const int mode = ctl->mode;
-    const int cur = ctl->epoch & 1, nxt = cur ^ 1;
-    // The bitmap of parity cur deduplicated the frontier this epoch reads;
-    // it is not used again until it becomes the output of epoch k+1.
-    for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words;
-         w += gridDim.x * kBlock)
-        fb.bm[cur][w] = 0;
-    uint32_t cnt = 0;
-    unsigned long long eacc = 0, scanned = 0;
-    if (mode == kPull)
-        sssp_pull_sweep<true>(p, g, stage(dist, nstage, s_stage), dist, fb.bm[nxt], cnt, eacc);
-    else
-        sssp_push_sweep(g, dist, fb, ctl, cur, nxt, cnt, eacc);
-    write_partials(fb.part, cnt, eacc, scanned);
+    const int k = ctl->epoch;
+    const int cur = k & 1, nxt = cur ^ 1;  // queue parities
+    // Bitmaps: changed in epoch k-1 (read), changed in epoch k (written),
+    // and the one to clear for epoch k+1.
+    const uint32_t *rbm = fb.bm[k % 3];
+    uint32_t *wbm = fb.bm[(k + 1) % 3];
+    uint32_t *cbm = fb.bm[(k + 2) % 3];
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words; w += gridDim.x * kBlock)
+        cbm[w] = 0;
+    uint32_t cnt = 0, gathered = 0;
+    unsigned long long eacc = 0;
+    if (mode == kPull) {
+        const Staged<int32_t> rd = stage(dist, nstage, s_stage);
+        if (ctl->filter)
+            sssp_pull_sweep<true, true>(p, g, rd, unstaged(rbm), dist, wbm, cnt, eacc, gathered);
+        else
+            sssp_pull_sweep<true, false>(p, g, rd, unstaged(rbm), dist, wbm, cnt, eacc, gathered);
+    } else {
+        sssp_push_sweep(g, dist, fb, ctl, cur, nxt, wbm, cnt, eacc);
+    }
+    write_partials(fb.part, cnt, eacc, gathered);
 }
 
 // Epoch controller: one workgroup.
 __global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb,
                                                           StepCtl *c) {
     if (c->done) return;
-    unsigned long long out_n, out_e, unused;
-    sum_partials(fb.part, c->grid, out_n, out_e, unused);
+    unsigned long long out_n, out_e, gathered;
+    sum_partials(fb.part, c->grid, out_n, out_e, gathered);
     if (threadIdx.x != 0) return;
     const int k = c->epoch, cur = k & 1;
     const int ran = c->mode;
@@ -273,7 +324,11 @@ __global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb,
         c->push_epochs++;
         trace_epoch(c, k, ran, out_n, out_e, c->in_e);
     } else {
-        c->alg_bytes += (unsigned long long)c->sweep_bytes + 8ull * out_n;
+        // Offsets, ids and own distances always; weight + distance per
+        // gathered edge; one bit per tested neighbour when filtering.
+        c->alg_bytes += (unsigned long long)c->sweep_bytes - 8ull * (unsigned long long)c->m +
+                        8ull * gathered + (c->filter ? (unsigned long long)c->m / 8 : 0ull) +
+                        8ull * out_n;
         c->pull_epochs++;
         trace_epoch(c, k, ran, out_n, out_e, (unsigned long long)c->m);
     }
@@ -281,6 +336,7 @@ __global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb,
     c->epoch = k + 1;
     c->in_n = (long long)out_n;
     c->in_e = (long long)out_e;
+    c->filter = c->filter_knob;  // the changed set of this epoch is complete
     int next = ran;
     if (out_n == 0 || (c->max_epochs > 0 && k + 1 >= c->max_epochs)) {
         c->done = 1;
@@ -303,15 +359,21 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
     GraphView<OffT> g, int32_t *__restrict__ dist,
     const int32_t *__restrict__ init_dist, FrontierBufs fb, StepCtl *ctl,
     int32_t src, int direction, int alpha, int max_epochs, long long m,
-    long long sweep_bytes, int grid) {
+    long long sweep_bytes, int grid, int filter) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride)
         dist[i] = init_dist ? init_dist[i] : (i == src ? 0 : HC_INF_DIST);
     for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words; w += stride) {
-        fb.bm[0][w] = 0;
+        // Epoch 0 reads bitmap 0 as the set changed before it: the source.
+        fb.bm[0][w] = (!init_dist && w == (src >> 5)) ? (1u << (src & 31)) : 0u;
         fb.bm[1][w] = 0;
+        fb.bm[2][w] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // With initial distances the changed set before epoch 0 is unknown:
+        // that epoch gathers every neighbour.
+        ctl->filter_knob = filter;
+        ctl->filter = init_dist ? 0 : filter;
         for (int p = 0; p < 2; p++)
             for (int j = 0; j < 5; j++) ctl->cnt[p][j] = 0;
         ctl->done = 0;
@@ -367,7 +429,9 @@ __global__ void sssp_pull_thread_kernel(GraphView<OffT> g,
     const int32_t stride = gridDim.x * blockDim.x;
     for (int32_t u = start + blockIdx.x * blockDim.x + threadIdx.x; u < end;
          u += stride) {
-        int32_t m = row_min<1>(g, unstaged(dist), g.rbeg(u), g.rend(u), 0);
+        uint32_t gathered = 0;
+        int32_t m = row_min<1, false>(g, unstaged(dist), unstaged<uint32_t>(nullptr), g.rbeg(u),
+                                      g.rend(u), 0, gathered);
         if (m < dist[u]) {
             dist[u] = m;
             upd++;
@@ -386,7 +450,9 @@ __global__ void sssp_pull_wave_kernel(GraphView<OffT> g,
     const int32_t waves = gridDim.x * (blockDim.x / kWave);
     const int32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     for (int32_t u = start + wid; u < end; u += waves) {
-        int32_t m = group_min<64>(row_min<64>(g, unstaged(dist), g.rbeg(u), g.rend(u), lane_id()));
+        uint32_t gathered = 0;
+        int32_t m = group_min<64>(row_min<64, false>(g, unstaged(dist), unstaged<uint32_t>(nullptr),
+                                                     g.rbeg(u), g.rend(u), lane_id(), gathered));
         if (lane_id() == 0 && m < dist[u]) {
             dist[u] = m;
             upd++;
@@ -422,7 +488,9 @@ __global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
     extern __shared__ int32_t s_stage[];
     uint32_t upd = 0;
     unsigned long long eacc = 0;
-    sssp_pull_sweep<false>(p, g, stage(dist, nstage, s_stage), dist, nullptr, upd, eacc);
+    uint32_t gathered = 0;
+    sssp_pull_sweep<false, false>(p, g, stage(dist, nstage, s_stage), unstaged<uint32_t>(nullptr),
+                                  dist, nullptr, upd, eacc, gathered);
     unsigned long long s = block_sum_u64(upd);
     if (threadIdx.x == 0 && s) atomicAdd(updated, (int32_t)s);
 }
@@ -495,10 +563,10 @@ static int launch_pull_epoch(hc_graph_s *g, int mapping, int32_t start,
             break;
         default: {
             const PlanHost *ph = nullptr;
-            HC_TRY(get_plan(g, start, end, tuning().pull, &ph));
+            HC_TRY(get_plan(g, start, end, sssp_plan_config(), &ph));
             if (ph->total_units == 0) return HC_OK;
             int nb = std::max(1, std::min(ph->total_units, grid_blocks()));
-            const int32_t ns = std::min<int32_t>(tuning().stage_pull, g->n);
+            const int32_t ns = std::min<int32_t>(tuning().sssp_stage, g->n);
             sssp_pull_fused_kernel<OffT><<<nb, kBlock, ns * sizeof(int32_t), s>>>(
                 ph->dev, v, dist, updated, ns);
         }
@@ -537,7 +605,7 @@ struct SsspLaunch {
             a->plan, a->g, a->dist, a->fb, a->ctl, a->nstage);
         a->kt->mark(a->s);
         sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
-        compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 0);
+        compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "SSSP epoch launch failed");
         return HC_OK;
     }
@@ -550,16 +618,16 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     SsspWs *ws = nullptr;
     HC_TRY(get_sssp_ws(g, &ws));
     const PlanHost *ph = nullptr;
-    HC_TRY(get_plan(g, 0, g->n, tuning().pull, &ph));
+    HC_TRY(get_plan(g, 0, g->n, sssp_plan_config(), &ph));
     GraphView<OffT> v = view<OffT>(g);
     const long long sweep = sssp_sweep_bytes(g, ph);
-    const int32_t nstage = std::min<int32_t>(tuning().stage_pull, g->n);
+    const int32_t nstage = std::min<int32_t>(tuning().sssp_stage, g->n);
     const int grid = resident_grid(reinterpret_cast<const void *>(&sssp_step_kernel<OffT>),
                                    kBlock, nstage * sizeof(int32_t), ws->f.grid);
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
-        max_epochs, (long long)g->m, sweep, grid);
+        max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
     SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s, nstage, &ws->f.kt};

@@ -200,7 +200,12 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  * counterpart -- the reference fixed these per generated driver,
  * src/kernels/heterogeneous/sssp.cuh:180-278).  Keys:
  *   grid_per_cu            workgroups per CU of full-device launches (8)
- *   sssp.alpha             SSSP pull when frontier edges > m/alpha (8)
+ *   sssp.alpha             SSSP pull when frontier edges > m/alpha (16)
+ *   sssp.filter            1: SSSP pull epochs gather only neighbours whose
+ *                          distance changed in the previous epoch (0)
+ *   sssp.stage             distances staged in LDS by SSSP pull sweeps (0)
+ *   sssp.block_threshold   SSSP: degree above which a workgroup takes a row
+ *                          (4096; the other pull.* classes are shared)
  *   pull.hub_threshold     degree above which a row is split into chunks
  *   pull.hub_chunk         edges per chunk (>= 256)
  *   pull.block_threshold   degree above which a workgroup takes a row
@@ -334,6 +339,8 @@ typedef struct hc_epoch_trace {
     uint32_t frontier[128];
     uint64_t frontier_edges[128];
     uint64_t scanned[128];
+    double us[128];        /* device time of the epoch (end of the previous
+                              epoch's controller to end of this one's)       */
 } hc_epoch_trace_t;
 int hc_get_epoch_trace(hc_graph_t g, int algorithm, hc_epoch_trace_t *out);
 
