@@ -213,7 +213,7 @@ _SIGS = {
                               C.c_int32, C.c_int32, C.c_int, C.c_int], C.c_double),
 }
 
-TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "sssp.filter", "sssp.stage",
+TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "sssp.filter", "sssp.stage", "sssp.grid_per_cu",
                "sssp.block_threshold", "pull.stage", "bu.stage", "profile.kernel_events",
                "multi.graph", "multi.grid_per_cu", "pull.hub_threshold", "pull.hub_chunk",
                "pull.block_threshold", "pull.wave_threshold", "pull.g16_threshold",

@@ -53,7 +53,7 @@ KNOB_SPACE = {
         "pull.g16_threshold": [12, 24, 48],
         "pull.g4_threshold": [2, 5, 8],
         "pull.hub_threshold": [16384, 65536],
-        "grid_per_cu": [4, 8, 16],
+        "sssp.grid_per_cu": [2, 4, 8, 16],
     },
     "bfs": {
         "grid_per_cu": [2, 4, 8, 16],

@@ -80,7 +80,7 @@ struct PlanConfig {
     int64_t hub_threshold = 16384;  // deg > this -> chunks
     int64_t hub_chunk = 4096;       // edges per chunk
     int64_t block_threshold = 1024; // deg > this -> workgroup per vertex
-    int64_t wave_threshold = 96;    // deg > this -> wavefront per vertex
+    int64_t wave_threshold = 192;   // deg > this -> wavefront per vertex
     int64_t g16_threshold = 24;     // deg > this -> 16 lanes
     int64_t g4_threshold = 5;       // deg > this -> 4 lanes, else 1
     bool operator<(const PlanConfig &o) const {
@@ -95,7 +95,8 @@ struct PlanConfig {
 // caller; read when plans and workspaces are built).
 struct Tuning {
     int grid_per_cu = 8;   // workgroups per CU of the fused / step launches
-    int sssp_alpha = 16;   // SSSP goes pull when frontier edges > m / alpha
+    int sssp_alpha = 8;    // SSSP goes pull when frontier edges > m / alpha
+    int sssp_grid_per_cu = 4;  // workgroups per CU of the SSSP loop (at most)
     int sssp_filter = 0;   // SSSP pulls gather only neighbours changed last epoch
     int sssp_stage = 0;    // per-vertex entries staged in LDS by SSSP pull sweeps
     int64_t sssp_block_threshold = 4096;  // SSSP: degree above which a WG takes a row
@@ -103,20 +104,20 @@ struct Tuning {
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
     int kernel_events = 0; // 1: time every launch of a loop's main kernel
     int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
-    int multi_grid_per_cu = 8;  // workgroups per CU of the batched sweep
+    int multi_grid_per_cu = 16;  // workgroups per CU of the batched sweep
     PlanConfig pull;       // SSSP and PageRank pull sweeps
     PlanConfig bu;         // BFS bottom-up sweeps (no chunk / WG classes)
     PlanConfig multi;      // batched sweeps: chunks, wave, 16-lane, 4-lane
     Tuning() {
         bu.hub_threshold = INT64_MAX;
         bu.block_threshold = INT64_MAX;
-        bu.wave_threshold = 1024;
-        bu.g16_threshold = 256;
-        bu.g4_threshold = 32;
-        multi.hub_threshold = 4096;
-        multi.hub_chunk = 2048;
+        bu.wave_threshold = 4096;
+        bu.g16_threshold = 512;
+        bu.g4_threshold = 64;
+        multi.hub_threshold = 2048;
+        multi.hub_chunk = 1024;
         multi.block_threshold = INT64_MAX;
-        multi.wave_threshold = 16;
+        multi.wave_threshold = 32;
         multi.g16_threshold = 4;
         multi.g4_threshold = 0;
     }

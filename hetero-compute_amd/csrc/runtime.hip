@@ -85,6 +85,7 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"sssp.alpha", &t.sssp_alpha},
                {"sssp.filter", &t.sssp_filter},
                {"sssp.stage", &t.sssp_stage},
+               {"sssp.grid_per_cu", &t.sssp_grid_per_cu},
                {"pull.stage", &t.stage_pull},
                {"bu.stage", &t.stage_bu},
                {"profile.kernel_events", &t.kernel_events},

@@ -623,7 +623,8 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const long long sweep = sssp_sweep_bytes(g, ph);
     const int32_t nstage = std::min<int32_t>(tuning().sssp_stage, g->n);
     const int grid = resident_grid(reinterpret_cast<const void *>(&sssp_step_kernel<OffT>),
-                                   kBlock, nstage * sizeof(int32_t), ws->f.grid);
+                                   kBlock, nstage * sizeof(int32_t),
+                                   std::min(ws->f.grid, grid_blocks(tuning().sssp_grid_per_cu)));
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
