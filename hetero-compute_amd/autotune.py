@@ -61,6 +61,7 @@ KNOB_SPACE = {
         "bu.wave_threshold": [1024, 4096],
         "bu.g16_threshold": [128, 256, 512],
         "bu.g4_threshold": [16, 32, 64],
+        "bu.remaining": [0, 1, 4],
     },
     "pr": {
         "grid_per_cu": [2, 4, 8, 16],

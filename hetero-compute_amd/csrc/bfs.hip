@@ -11,13 +11,18 @@
 //    neighbours with compare-and-swap; winners are appended to the next
 //    frontier (wavefront-aggregated) and their degrees summed (scout count).
 //  * Direction-optimizing loop, switching rule of bfs_do_cpu<alpha,beta>
-//    (src/kernels/cpu/bfs.cuh:48-114), evaluated on the device by the last
+//    (src/kernels/cpu/bfs.cuh:48-114), evaluated on the device by a
 //    controller kernel between epochs.  Bottom-up epochs emit the next
 //    frontier as a bitmap only; top-down epochs emit bitmap and queues; the
 //    compaction kernel builds queues from the bitmap when the direction turns
 //    from bottom-up to top-down.
+//  * A bottom-up epoch lists the vertices it leaves unvisited (per-workgroup
+//    list segments, no global counter); the next bottom-up epoch sweeps only
+//    those.  One-lane rows read their first neighbour from a per-vertex
+//    array, so the common first-probe hit costs coalesced loads only.
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "frontier.h"
 
@@ -36,16 +41,19 @@ __device__ __forceinline__ bool bit_test(const Staged<uint32_t> &bm, int32_t v) 
 // parents, so most scans end there.  Later rounds read four positions per
 // lane (predicated) and test their bits together; the first hit is the
 // earliest position of the earliest round with one.  `scanned` counts the
-// positions examined.
-template <int G, typename OffT>
+// positions examined.  FIRST (one lane): the first neighbour is `first_v`,
+// already read from the first-neighbour array (-1: empty row).
+template <int G, bool FIRST = false, typename OffT>
 __device__ __forceinline__ bool bu_scan(const GraphView<OffT> &g,
                                         const Staged<uint32_t> &frontier,
                                         OffT b, OffT e, int lane, int gbase,
-                                        int32_t &parent, uint32_t &scanned) {
+                                        int32_t &parent, uint32_t &scanned,
+                                        int32_t first_v = -1) {
+    static_assert(!FIRST || G == 1, "first-neighbour probes are one-lane");
     const int L = G == 1 ? 0 : lane;
     {
         const OffT k = b + OffT(L);
-        const int32_t v = k < e ? ldnt(&g.adj[k]) : -1;
+        const int32_t v = FIRST ? first_v : (k < e ? ldnt(&g.adj[k]) : -1);
         const bool h = v >= 0 && bit_test(frontier, v);
         if (G == 1) {
             if (v < 0) return false;
@@ -97,12 +105,50 @@ __device__ __forceinline__ bool bu_scan(const GraphView<OffT> &g,
     return false;
 }
 
+// Bottom-up step of one vertex uu by a group of G lanes.  found: a frontier
+// neighbour was found (parent par); open: uu is unvisited and stays so (it
+// has no frontier neighbour).  Results are valid in the group's lane 0.
+// With `first` (one lane): the visited flag, the first neighbour and the row
+// bounds are read together, so most rows cost one round of coalesced loads
+// and a bitmap probe instead of a dependent gather into the edge array.
+template <int G, bool CHECK, typename OffT>
+__device__ __forceinline__ void bu_vertex(const GraphView<OffT> &g,
+                                          const int32_t *__restrict__ visited,
+                                          const int32_t *__restrict__ first,
+                                          const Staged<uint32_t> &frontier, int32_t uu,
+                                          int lane, int gbase, bool &found, bool &open,
+                                          int32_t &par, uint32_t &scanned) {
+    if (G == 1 && first) {
+        const int32_t vis = CHECK ? visited[uu] : -1;
+        const int32_t f = first[uu];
+        const OffT b = g.rbeg(uu), e = g.rend(uu);
+        if (vis < 0) {
+            int32_t pr = -1;
+            const bool fnd = bu_scan<1, true>(g, frontier, b, e, 0, 0, pr, scanned, f);
+            found = fnd;
+            open = !fnd;
+            par = pr;
+        }
+        return;
+    }
+    if (!CHECK || visited[uu] < 0) {
+        int32_t pr = -1;
+        bool f = bu_scan<G>(g, frontier, g.rbeg(uu), g.rend(uu), lane, gbase, pr, scanned);
+        if (lane == 0) {
+            found = f;
+            open = !f;
+            par = pr;
+        }
+    }
+}
+
 template <int G, typename OffT>
 __device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
                                          const GraphView<OffT> &g,
                                          const int32_t *__restrict__ visited,
+                                         const int32_t *__restrict__ first,
                                          const Staged<uint32_t> &frontier,
-                                         bool &found, int32_t &u, int32_t &par,
+                                         bool &found, bool &open, int32_t &u, int32_t &par,
                                          uint32_t &scanned) {
     constexpr int PER = kBlock / G;
     const int gid = threadIdx.x / G, lane = threadIdx.x % G;
@@ -110,54 +156,109 @@ __device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
     const int idx = k * PER + gid;
     if (idx < p.cnt[c]) {
         int32_t uu = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
-        if (visited[uu] < 0) {
-            int32_t pr = -1;
-            bool f = bu_scan<G>(g, frontier, g.rbeg(uu), g.rend(uu), lane, gbase,
-                                pr, scanned);
-            if (lane == 0 && f) {
-                found = true;
-                u = uu;
-                par = pr;
-            }
+        u = uu;
+        bu_vertex<G, true>(g, visited, first, frontier, uu, lane, gbase, found, open, par,
+                           scanned);
+    }
+}
+
+// Records the outcome of one bottom-up vertex: LOOP: depth/parent, a bit in
+// the next-frontier bitmap (counted with its degree) and, with `rem`, the
+// vertex on the next unvisited list when it stays open.  Epoch API: visited
+// is parents.  Called by every lane of the wavefront (wave-aggregated).
+template <bool LOOP, typename OffT>
+__device__ __forceinline__ void bu_commit(const GraphView<OffT> &g,
+                                          int32_t *__restrict__ visited,
+                                          int32_t *__restrict__ parents, int32_t level,
+                                          uint32_t *__restrict__ next, int32_t *rem,
+                                          uint32_t *rem_cnt, bool found, bool open,
+                                          int32_t u, int32_t par, uint32_t &cnt,
+                                          unsigned long long &edges_acc) {
+    if (LOOP) {
+        if (found) {
+            visited[u] = level;
+            if (parents) parents[u] = par;
         }
+        if (wave_set_bit(next, u, found)) {
+            cnt++;
+            edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
+        }
+        if (rem) wave_append(rem, rem_cnt, open, u);
+    } else if (found) {
+        visited[u] = par;
+        atomicOr(&next[u >> 5], 1u << (u & 31));
+        cnt++;
     }
 }
 
 // One bottom-up sweep over the plan (classes 2..5; the BFS plan keeps classes
 // 0 and 1 empty).  LOOP: visited is depth; found vertices get depth/parent
-// and a bit in the next-frontier bitmap, counted with their degree.
+// and a bit in the next-frontier bitmap, counted with their degree; with
+// `rem`, vertices that stay unvisited are listed for the next sweep.
 // Epoch API: visited is parents.
 template <bool LOOP, typename OffT>
 __device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
                          int32_t *__restrict__ visited, int32_t *__restrict__ parents,
+                         const int32_t *__restrict__ first,
                          int32_t level, const Staged<uint32_t> &frontier,
-                         uint32_t *__restrict__ next, uint32_t &cnt,
-                         unsigned long long &edges_acc, uint32_t &scanned) {
+                         uint32_t *__restrict__ next, int32_t *rem, uint32_t *rem_cnt,
+                         uint32_t &cnt, unsigned long long &edges_acc, uint32_t &scanned) {
     const int total = p.unit_start[kNumClasses];
     for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
         int c = 0;
         while (unit >= p.unit_start[c + 1]) c++;
         const int k = unit - p.unit_start[c];
-        bool found = false;
+        bool found = false, open = false;
         int32_t u = 0, par = -1;
-        if (c == 2) bu_group<64>(p, c, k, g, visited, frontier, found, u, par, scanned);
-        else if (c == 3) bu_group<16>(p, c, k, g, visited, frontier, found, u, par, scanned);
-        else if (c == 4) bu_group<4>(p, c, k, g, visited, frontier, found, u, par, scanned);
-        else if (c == 5) bu_group<1>(p, c, k, g, visited, frontier, found, u, par, scanned);
-        if (LOOP) {
-            if (found) {
-                visited[u] = level;
-                if (parents) parents[u] = par;
-            }
-            if (wave_set_bit(next, u, found)) {
-                cnt++;
-                edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
-            }
-        } else if (found) {
-            visited[u] = par;
-            atomicOr(&next[u >> 5], 1u << (u & 31));
-            cnt++;
+        if (c == 2)
+            bu_group<64>(p, c, k, g, visited, nullptr, frontier, found, open, u, par, scanned);
+        else if (c == 3)
+            bu_group<16>(p, c, k, g, visited, nullptr, frontier, found, open, u, par, scanned);
+        else if (c == 4)
+            bu_group<4>(p, c, k, g, visited, nullptr, frontier, found, open, u, par, scanned);
+        else if (c == 5)
+            bu_group<1>(p, c, k, g, visited, first, frontier, found, open, u, par, scanned);
+        bu_commit<LOOP>(g, visited, parents, level, next, rem, rem_cnt, found, open, u, par, cnt,
+                        edges_acc);
+    }
+}
+
+// Unvisited lists of the bottom-up loop.  A pull epoch lists, per
+// workgroup, the plan vertices it left unvisited; workgroup b owns the slots
+// [off[b], off[b+1]) of each parity's list (as many as plan vertices it
+// sweeps, so the lists need no global counter) and records its list length
+// in len[parity * grid + b].  A following pull epoch sweeps only the lists.
+struct RemLists {
+    const int32_t *off;
+    uint32_t *len;
+};
+
+// Bottom-up sweep of this workgroup's unvisited list, G lanes per vertex;
+// vertices that stay open go on the next list.
+template <int G, typename OffT>
+__device__ void bu_list_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
+                              int32_t *__restrict__ parents,
+                              const int32_t *__restrict__ first, int32_t level,
+                              const Staged<uint32_t> &frontier, uint32_t *__restrict__ next,
+                              const int32_t *__restrict__ list, uint32_t len, int32_t *rem,
+                              uint32_t *rem_cnt, uint32_t &cnt, unsigned long long &edges_acc,
+                              uint32_t &scanned) {
+    constexpr int PER = kBlock / G;
+    const int gid = threadIdx.x / G, lane = threadIdx.x % G;
+    const int gbase = lane_id() - lane;
+    for (uint32_t base = 0; base < len; base += PER) {
+        const uint32_t idx = base + gid;
+        bool found = false, open = false;
+        int32_t u = 0, par = -1;
+        if (idx < len) {
+            u = list[idx];
+            // Listed vertices are unvisited: only pull epochs ran since
+            // they were listed, and each visits only the vertices it scans.
+            bu_vertex<G, false>(g, depth, G == 1 ? first : nullptr, frontier, u, lane, gbase,
+                                found, open, par, scanned);
         }
+        bu_commit<true>(g, depth, parents, level, next, rem, rem_cnt, found, open, u, par, cnt,
+                        edges_acc);
     }
 }
 
@@ -231,8 +332,10 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ depth,
-    int32_t *__restrict__ parents, FrontierBufs fb, StepCtl *ctl, int32_t nstage) {
+    int32_t *__restrict__ parents, const int32_t *__restrict__ first, FrontierBufs fb,
+    StepCtl *ctl, RemLists rl, int32_t nstage) {
     extern __shared__ uint32_t s_words[];
+    __shared__ uint32_t s_rem;
     if (ctl->done) return;
     const int mode = ctl->mode;
     const int k = ctl->epoch, cur = k & 1, nxt = cur ^ 1;
@@ -245,23 +348,56 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     const int32_t level = k + 1;
     uint32_t cnt = 0, scanned = 0;
     unsigned long long eacc = 0;
-    if (mode == kPull)
-        bu_sweep<true>(p, g, depth, parents, level, stage(frontier, nstage, s_words), next,
-                       cnt, eacc, scanned);
-    else
+    if (mode == kPull) {
+        // The unvisited lists live in the small-degree push queues, which
+        // pull epochs do not use (a switch to push invalidates them).
+        const bool listing = ctl->rem_knob != 0;
+        if (threadIdx.x == 0) s_rem = 0;
+        const Staged<uint32_t> fr = stage(frontier, nstage, s_words);  // syncs s_rem
+        const int32_t off = listing ? rl.off[blockIdx.x] : 0;
+        int32_t *rem = listing ? fb.q[nxt][2] + off : nullptr;
+        if (ctl->rem_valid) {
+            const int32_t *list = fb.q[cur][2] + off;
+            const uint32_t len = rl.len[cur * gridDim.x + blockIdx.x];
+            if (ctl->rem_knob == 16)
+                bu_list_sweep<16>(g, depth, parents, first, level, fr, next, list, len, rem, &s_rem,
+                                  cnt, eacc, scanned);
+            else if (ctl->rem_knob == 4)
+                bu_list_sweep<4>(g, depth, parents, first, level, fr, next, list, len, rem, &s_rem,
+                                 cnt, eacc, scanned);
+            else
+                bu_list_sweep<1>(g, depth, parents, first, level, fr, next, list, len, rem, &s_rem,
+                                 cnt, eacc, scanned);
+        } else
+            bu_sweep<true>(p, g, depth, parents, first, level, fr, next, rem, &s_rem, cnt, eacc,
+                           scanned);
+        if (listing) {
+            __syncthreads();
+            if (threadIdx.x == 0) rl.len[nxt * gridDim.x + blockIdx.x] = s_rem;
+        }
+    } else
         td_sweep(g, depth, parents, level, next, fb, ctl, cur, nxt, cnt, eacc, scanned);
     write_partials(fb.part, cnt, eacc, scanned);
 }
 
 // Direction-optimizing controller (one workgroup): the decisions of
 // bfs_do_cpu (src/kernels/cpu/bfs.cuh:68-103) applied between epochs.
-__global__ __launch_bounds__(kBlock) void bfs_ctl_kernel(FrontierBufs fb,
-                                                         StepCtl *c, int offb) {
+__device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, const RemLists &rl, int offb) {
     if (c->done) return;
     unsigned long long q, sc, scanned;
     sum_partials(fb.part, c->grid, q, sc, scanned);
-    if (threadIdx.x != 0) return;
+    // Unvisited-list lengths read and written by a pull epoch.
+    unsigned long long lin = 0, lout = 0;
     const int k = c->epoch, cur = k & 1;
+    if (c->mode == kPull && c->rem_knob) {
+        for (int i = threadIdx.x; i < c->grid; i += kBlock) {
+            lin += rl.len[cur * c->grid + i];
+            lout += rl.len[(cur ^ 1) * c->grid + i];
+        }
+        lin = block_sum_u64(lin);
+        lout = block_sum_u64(lout);
+    }
+    if (threadIdx.x != 0) return;
     const int ran = c->mode;
     c->reached += q;
     trace_epoch(c, k, ran, q, sc, scanned);
@@ -273,8 +409,10 @@ __global__ __launch_bounds__(kBlock) void bfs_ctl_kernel(FrontierBufs fb,
     } else {
         // visited flag per swept row, id + bitmap bit per examined edge,
         // writes per new vertex, bitmap clear.
-        c->alg_bytes += 4ull * c->plan_rows + 4ull * scanned + scanned / 8 +
-                        12ull * q + 4ull * ((c->n + 31) / 32);
+        // List mode: a list entry per swept row; list writes.
+        const unsigned long long rows = c->rem_valid ? lin : (unsigned long long)c->plan_rows;
+        c->alg_bytes += 4ull * rows + 4ull * scanned + scanned / 8 + 12ull * q +
+                        4ull * ((c->n + 31) / 32) + 4ull * lout;
         c->pull_epochs++;
     }
     for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
@@ -319,14 +457,27 @@ __global__ __launch_bounds__(kBlock) void bfs_ctl_kernel(FrontierBufs fb,
     }
     if (c->max_epochs > 0 && k + 1 >= c->max_epochs) c->done = 1;
     c->need_compact = (!c->done && ran == kPull && next == kPush) ? 1 : 0;
+    // A pull epoch listed the vertices it left open; the next pull reads that
+    // list unless the list mode is switched off.
+    c->rem_valid = (c->rem_knob && ran == kPull && next == kPull) ? 1 : 0;
     c->mode = next;
+    if (c->done) publish_done(fb);
+}
+
+// The controller launch (one workgroup).  A controller run by the step's
+// last workgroup was measured slower: every workgroup then needs a
+// device-scope release fence, which writes back its XCD's L2.
+__global__ __launch_bounds__(kBlock) void bfs_ctl_kernel(FrontierBufs fb, StepCtl *c,
+                                                         RemLists rl, int offb) {
+    bfs_control(fb, c, rl, offb);
 }
 
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
     GraphView<OffT> g, int32_t *__restrict__ depth, int32_t *__restrict__ parents,
     FrontierBufs fb, StepCtl *ctl, int32_t src, int direction, int alpha,
-    int beta, int max_epochs, long long m, int32_t plan_rows, int grid) {
+    int beta, int max_epochs, long long m, int32_t plan_rows, int grid, int rem_knob,
+    int chunk) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride) {
         depth[i] = i == src ? 0 : -1;
@@ -347,6 +498,9 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
         ctl->alpha = alpha;
         ctl->beta = beta;
         ctl->need_compact = 0;
+        ctl->rem_valid = 0;
+        ctl->rem_knob = rem_knob;
+        ctl->chunk = chunk;
         ctl->n = g.n;
         ctl->grid = grid;
         ctl->m = m;
@@ -359,13 +513,8 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
         OffT b = g.rbeg(src), e = g.rend(src);
         int64_t d = int64_t(e - b);
         int cls = degree_class(d);
-        if (cls == 3) {
-            uint32_t nch = 0;
-            for (int64_t x = b; x < int64_t(e); x += kChunkEdges, nch++) {
-                int64_t ce = x + kChunkEdges < int64_t(e) ? x + kChunkEdges : int64_t(e);
-                fb.ch[0][nch] = HubChunk{src, 0, x, ce};
-            }
-            ctl->cnt[0][3] = nch;
+        if (cls == 3) {  // chunk descriptors: written by workgroup 0 below
+            ctl->cnt[0][3] = (uint32_t)((d + chunk - 1) / chunk);
             ctl->cnt[0][4] = 1;
         } else {
             fb.q[0][cls][0] = src;
@@ -391,6 +540,7 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
             ctl->scout = 0;
         }
     }
+    if (blockIdx.x == 0) write_source_chunks(g, fb, src, chunk);
 }
 
 // ---- epoch-API kernels (reference semantics: parents marks visited) ----
@@ -403,8 +553,8 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_fused_kernel(
     extern __shared__ uint32_t s_words[];
     uint32_t nnew = 0, scanned = 0;
     unsigned long long eacc = 0;
-    bu_sweep<false>(p, g, parents, nullptr, 0, stage(frontier, nstage, s_words), next,
-                    nnew, eacc, scanned);
+    bu_sweep<false>(p, g, parents, nullptr, nullptr, 0, stage(frontier, nstage, s_words), next,
+                    nullptr, nullptr, nnew, eacc, scanned);
     unsigned long long s = block_sum_u64(nnew);
     if (threadIdx.x == 0 && s) atomicAdd(num_nodes, (int32_t)s);
 }
@@ -486,21 +636,61 @@ __global__ void bfs_push_epoch_kernel(GraphView<OffT> g,
 // Host side
 // ---------------------------------------------------------------------------
 
+// first[u] = the first neighbour of u in CSR order, or -1 for an empty row.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void first_neighbour_kernel(GraphView<OffT> g,
+                                                                 int32_t *__restrict__ first) {
+    for (int u = blockIdx.x * kBlock + threadIdx.x; u < g.n; u += gridDim.x * kBlock) {
+        const OffT b = g.rbeg(u), e = g.rend(u);
+        first[u] = b < e ? g.adj[b] : -1;
+    }
+}
+
 struct BfsWs {
     FrontierWs f;
     int32_t *depth = nullptr;  // used when the caller passes no depth array
+    int32_t *first = nullptr;  // first neighbour of each vertex (-1: no edges)
+    // Unvisited-list segments: off[grid + 1] then len[2][grid] (sized for
+    // f.grid workgroups), laid out for rem_plan at rem_grid workgroups.
+    int32_t *rem = nullptr;
+    const PlanHost *rem_plan = nullptr;
+    int rem_grid = 0;
 };
+
+// Workgroup b of a grid-strided bottom-up sweep takes units b, b + grid, ...;
+// its list segment holds as many slots as those units have vertices.
+static int bfs_rem_layout(BfsWs *ws, const PlanHost *ph, int grid, hipStream_t s) {
+    if (ws->rem_plan == ph && ws->rem_grid == grid) return HC_OK;
+    std::vector<int32_t> off(grid + 1, 0);
+    const PlanDev &p = ph->dev;
+    for (int c = 2; c < kNumClasses; c++) {
+        const int per = kClassPerUnit[c];
+        for (int u = p.unit_start[c]; u < p.unit_start[c + 1]; u++) {
+            const int first = (u - p.unit_start[c]) * per;
+            off[u % grid + 1] += std::min(per, p.cnt[c] - first);
+        }
+    }
+    for (int b = 0; b < grid; b++) off[b + 1] += off[b];
+    HC_HIP(hipMemcpyAsync(ws->rem, off.data(), sizeof(int32_t) * (grid + 1),
+                          hipMemcpyHostToDevice, s));
+    HC_HIP(hipStreamSynchronize(s));
+    ws->rem_plan = ph;
+    ws->rem_grid = grid;
+    return HC_OK;
+}
 
 void free_bfs_ws(hc_graph_s *g) {
     if (!g->ws_bfs) return;
     auto *ws = static_cast<BfsWs *>(g->ws_bfs);
     frontier_ws_free(&ws->f);
     if (ws->depth) hipFree(ws->depth);
+    if (ws->rem) hipFree(ws->rem);
+    if (ws->first) hipFree(ws->first);
     delete ws;
     g->ws_bfs = nullptr;
 }
 
-static int get_bfs_ws(hc_graph_s *g, BfsWs **out) {
+static int get_bfs_ws(hc_graph_s *g, BfsWs **out, hipStream_t s) {
     // The partial slots are sized for the launch grid; re-allocate when the
     // tuned grid changes.
     if (g->ws_bfs && static_cast<BfsWs *>(g->ws_bfs)->f.grid != grid_blocks()) free_bfs_ws(g);
@@ -511,10 +701,26 @@ static int get_bfs_ws(hc_graph_s *g, BfsWs **out) {
             delete ws;
             return rc;
         }
-        if (hipMalloc(&ws->depth, sizeof(int32_t) * std::max(g->n, 1)) != hipSuccess) {
+        if (hipMalloc(&ws->depth, sizeof(int32_t) * std::max(g->n, 1)) != hipSuccess ||
+            hipMalloc(&ws->rem, sizeof(int32_t) * (3 * size_t(ws->f.grid) + 1)) != hipSuccess ||
+            hipMalloc(&ws->first, sizeof(int32_t) * std::max(g->n, 1)) != hipSuccess) {
             frontier_ws_free(&ws->f);
+            if (ws->depth) hipFree(ws->depth);
+            if (ws->rem) hipFree(ws->rem);
             delete ws;
-            return fail(HC_ERR_OOM, "BFS depth workspace allocation failed");
+            return fail(HC_ERR_OOM, "BFS workspace allocation failed");
+        }
+        if (g->n > 0) {
+            const int nb = std::min((g->n + kBlock - 1) / kBlock, 65536);
+            if (g->off64)
+                first_neighbour_kernel<<<nb, kBlock, 0, s>>>(view<uint64_t>(g), ws->first);
+            else
+                first_neighbour_kernel<<<nb, kBlock, 0, s>>>(view<uint32_t>(g), ws->first);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+                g->ws_bfs = ws;
+                free_bfs_ws(g);
+                return fail(HC_ERR_HIP, "first-neighbour array build failed");
+            }
         }
         g->ws_bfs = ws;
     }
@@ -528,8 +734,10 @@ struct BfsLaunch {
     GraphView<OffT> g;
     int32_t *depth;
     int32_t *parents;
+    const int32_t *first;
     FrontierBufs fb;
     StepCtl *ctl;
+    RemLists rl;
     int grid;
     hipStream_t s;
     int32_t nstage;
@@ -538,9 +746,9 @@ struct BfsLaunch {
         auto *a = static_cast<BfsLaunch *>(arg);
         a->kt->mark(a->s);
         bfs_step_kernel<OffT><<<a->grid, kBlock, a->nstage * sizeof(uint32_t), a->s>>>(
-            a->plan, a->g, a->depth, a->parents, a->fb, a->ctl, a->nstage);
+            a->plan, a->g, a->depth, a->parents, a->first, a->fb, a->ctl, a->rl, a->nstage);
         a->kt->mark(a->s);
-        bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
+        bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, a->rl, (int)sizeof(OffT));
         compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "BFS epoch launch failed");
         return HC_OK;
@@ -552,7 +760,7 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
                         int32_t *parents, int direction, int alpha, int beta,
                         hc_run_stats_t *stats, hipStream_t s) {
     BfsWs *ws = nullptr;
-    HC_TRY(get_bfs_ws(g, &ws));
+    HC_TRY(get_bfs_ws(g, &ws, s));
     const PlanHost *ph = nullptr;
     HC_TRY(get_plan(g, 0, g->n, tuning().bu, &ph));
     int32_t plan_rows = 0;
@@ -562,14 +770,17 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     const int32_t nstage = std::min<int32_t>(tuning().stage_bu, ws->f.fb.words);
     const int grid = resident_grid(reinterpret_cast<const void *>(&bfs_step_kernel<OffT>),
                                    kBlock, nstage * sizeof(uint32_t), ws->f.grid);
+    if (grid > ws->f.grid) return fail(HC_ERR_HIP, "BFS grid %d above its workspace", grid);
+    HC_TRY(bfs_rem_layout(ws, ph, grid, s));
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     bfs_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, depth, parents, ws->f.fb, ws->f.ctl, src, direction, alpha, beta, 0,
-        (long long)g->m, plan_rows, grid);
+        (long long)g->m, plan_rows, grid, tuning().bu_remaining, tuning().push_chunk);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
-    BfsLaunch<OffT> L{ph->dev, v, depth, parents, ws->f.fb, ws->f.ctl, grid, s, nstage,
-                      &ws->f.kt};
+    const RemLists rl{ws->rem, reinterpret_cast<uint32_t *>(ws->rem + grid + 1)};
+    BfsLaunch<OffT> L{ph->dev, v, depth, parents, tuning().bu_first ? ws->first : nullptr,
+                      ws->f.fb, ws->f.ctl, rl, grid, s, nstage, &ws->f.kt};
     const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
     long long launches = 0;

@@ -15,7 +15,7 @@
 //
 // Push frontiers are routed by degree into classes that map to different
 // lane counts when expanded:
-//   chunk: deg > kHubDeg      -> one workgroup per kChunkEdges-edge chunk
+//   chunk: deg > kHubDeg      -> one workgroup per chunk of StepCtl::chunk edges
 //   q[0] : deg > kBigDeg      -> one workgroup per vertex
 //   q[1] : deg > kMidDeg      -> one wavefront per vertex
 //   q[2] : otherwise          -> kSmallLanes lanes per vertex
@@ -30,7 +30,9 @@
 namespace hc {
 
 constexpr int64_t kHubDeg = 1024;
-constexpr int64_t kChunkEdges = 512;
+constexpr int64_t kMinChunkEdges = 128;
+constexpr int kTicketGroups = 64;   // first-level completion counters
+constexpr int kTicketStride = 32;   // words between counters (own cache lines)  // smallest push chunk (sizes the chunk lists)
 constexpr int64_t kBigDeg = 256;
 constexpr int64_t kMidDeg = 16;
 constexpr int kSmallLanes = 8;
@@ -66,7 +68,9 @@ struct StepCtl {
     int32_t plan_rows;
     int32_t filter;                // SSSP: next pull gathers changed neighbours only
     int32_t filter_knob;           // SSSP: filtering allowed (tuning knob)
-    int32_t pad2;
+    int32_t rem_valid;             // BFS: the next pull reads the unvisited lists
+    int32_t rem_knob;              // BFS: lanes per listed vertex, 0: no lists
+    int32_t chunk;                 // edges per hub chunk of push epochs
     // Device wall clock (constant rate): start of the loop, end of the last
     // epoch, and end of each traced epoch.
     long long t_start;
@@ -91,7 +95,14 @@ struct FrontierBufs {
     uint32_t *bm[3];
     int32_t words;             // bitmap words
     Partials part;
+    int32_t *host_done;        // pinned host word the controller sets at the end
 };
+
+// Tells the polling host that the loop has finished (system-scope store
+// into pinned host memory; no copy on the stream).
+__device__ __forceinline__ void publish_done(const FrontierBufs &fb) {
+    if (fb.host_done) __hip_atomic_store(fb.host_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Workspace layout for one graph (SSSP and BFS share the shape).
 struct FrontierWs {
@@ -129,16 +140,52 @@ __device__ __forceinline__ void frontier_append(const GraphView<OffT> &g,
     wave_append(fb.q[p][0], &ctl->cnt[p][0], cls == 0, v);
     wave_append(fb.q[p][1], &ctl->cnt[p][1], cls == 1, v);
     wave_append(fb.q[p][2], &ctl->cnt[p][2], cls == 2, v);
-    if (cls == 3) {
-        int64_t d = int64_t(e - b);
-        uint32_t nch = (uint32_t)((d + kChunkEdges - 1) / kChunkEdges);
-        uint32_t base = atomicAdd(&ctl->cnt[p][3], nch);
-        atomicAdd(&ctl->cnt[p][4], 1u);
-        for (uint32_t j = 0; j < nch; j++) {
-            int64_t cb = int64_t(b) + int64_t(j) * kChunkEdges;
-            int64_t ce = cb + kChunkEdges < int64_t(e) ? cb + kChunkEdges : int64_t(e);
-            fb.ch[p][base + j] = HubChunk{v, 0, cb, ce};
+    // Hub vertices: their chunk slots are reserved with one atomic per
+    // wavefront, and the active lanes write each hub's chunk descriptors
+    // together (a hub can have hundreds).
+    const uint64_t m3 = __ballot(cls == 3);
+    if (m3) {
+        const int64_t ck = ctl->chunk;
+        const uint32_t nch = cls == 3 ? (uint32_t)((int64_t(e - b) + ck - 1) / ck) : 0u;
+        uint32_t total = 0;
+        for (uint64_t t = m3; t; t &= t - 1) total += __shfl(nch, __ffsll((unsigned long long)t) - 1);
+        const int leader = __ffsll((unsigned long long)m3) - 1;
+        uint32_t base = 0;
+        if (lane_id() == leader) {
+            base = atomicAdd(&ctl->cnt[p][3], total);
+            atomicAdd(&ctl->cnt[p][4], (uint32_t)__popcll(m3));
         }
+        base = __shfl(base, leader);
+        const uint64_t act = __ballot(true);
+        const uint32_t rank = (uint32_t)__popcll(act & lanemask_lt());
+        const uint32_t nact = (uint32_t)__popcll(act);
+        for (uint64_t t = m3; t; t &= t - 1) {
+            const int l = __ffsll((unsigned long long)t) - 1;
+            const uint32_t n_l = __shfl(nch, l);
+            const int32_t v_l = __shfl(v, l);
+            const long long b_l = __shfl((long long)b, l);
+            const long long e_l = __shfl((long long)e, l);
+            for (uint32_t j = rank; j < n_l; j += nact) {
+                const long long cb = b_l + (long long)j * ck;
+                const long long ce = cb + ck < e_l ? cb + ck : e_l;
+                fb.ch[p][base + j] = HubChunk{v_l, 0, cb, ce};
+            }
+            base += n_l;
+        }
+    }
+}
+
+// Chunk descriptors of a hub source (parity 0), written by one workgroup.
+template <typename OffT>
+__device__ __forceinline__ void write_source_chunks(const GraphView<OffT> &g,
+                                                    const FrontierBufs &fb, int32_t src,
+                                                    int64_t chunk) {
+    const int64_t b = int64_t(g.rbeg(src)), e = int64_t(g.rend(src));
+    if (degree_class(e - b) != 3) return;
+    const int64_t nch = (e - b + chunk - 1) / chunk;
+    for (int64_t j = threadIdx.x; j < nch; j += blockDim.x) {
+        const int64_t cb = b + j * chunk;
+        fb.ch[0][j] = HubChunk{src, 0, cb, cb + chunk < e ? cb + chunk : e};
     }
 }
 
@@ -176,35 +223,38 @@ __device__ __forceinline__ void write_partials(const Partials &part, uint32_t n,
     }
 }
 
-// Fixed-order sum of the step's partials, run by one workgroup of kBlock
-// threads; results valid in every thread.
+// Sum of the step's partials (integers: exact in any order), run by one
+// workgroup of kBlock threads; results valid in every thread.
 __device__ __forceinline__ void sum_partials(const Partials &part, int grid,
                                              unsigned long long &n,
                                              unsigned long long &e,
                                              unsigned long long &s) {
-    __shared__ unsigned long long r_n[kBlock], r_e[kBlock], r_s[kBlock];
+    __shared__ unsigned long long r[3][kWavesPerBlock];
     unsigned long long a = 0, b = 0, c = 0;
     for (int i = threadIdx.x; i < grid; i += blockDim.x) {
         a += part.n[i];
         b += part.e[i];
         c += part.s[i];
     }
-    r_n[threadIdx.x] = a;
-    r_e[threadIdx.x] = b;
-    r_s[threadIdx.x] = c;
-    __syncthreads();
-    for (int sh = kBlock / 2; sh > 0; sh >>= 1) {
-        if ((int)threadIdx.x < sh) {
-            r_n[threadIdx.x] += r_n[threadIdx.x + sh];
-            r_e[threadIdx.x] += r_e[threadIdx.x + sh];
-            r_s[threadIdx.x] += r_s[threadIdx.x + sh];
-        }
-        __syncthreads();
+#pragma unroll
+    for (int sh = 32; sh > 0; sh >>= 1) {
+        a += __shfl_xor(a, sh, 64);
+        b += __shfl_xor(b, sh, 64);
+        c += __shfl_xor(c, sh, 64);
     }
-    n = r_n[0];
-    e = r_e[0];
-    s = r_s[0];
     __syncthreads();
+    if (lane_id() == 0) {
+        r[0][threadIdx.x / kWave] = a;
+        r[1][threadIdx.x / kWave] = b;
+        r[2][threadIdx.x / kWave] = c;
+    }
+    __syncthreads();
+    n = e = s = 0;
+    for (int w = 0; w < kWavesPerBlock; w++) {
+        n += r[0][w];
+        e += r[1][w];
+        s += r[2][w];
+    }
 }
 
 __device__ __forceinline__ void trace_epoch(StepCtl *c, int k, int mode,
@@ -235,6 +285,7 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
     const int p = k & 1;
     const uint32_t *bm = three_bitmaps ? fb.bm[k % 3] : fb.bm[k & 1];
     const int words = fb.words;
+    const int64_t ck = ctl->chunk;
     const int per_block = (words + gridDim.x - 1) / gridDim.x;
     const int w0 = blockIdx.x * per_block;
     const int w1 = min(words, w0 + per_block);
@@ -253,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
             if (c < 3) {
                 my[c]++;
             } else {
-                my[3] += (uint32_t)((d + kChunkEdges - 1) / kChunkEdges);
+                my[3] += (uint32_t)((d + ck - 1) / ck);
                 my[4]++;
             }
         }
@@ -274,8 +325,8 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
             if (c < 3) {
                 fb.q[p][c][s_base[c] + off[c]++] = v;
             } else {
-                for (int64_t cb = int64_t(b); cb < int64_t(e); cb += kChunkEdges) {
-                    int64_t ce = cb + kChunkEdges < int64_t(e) ? cb + kChunkEdges : int64_t(e);
+                for (int64_t cb = int64_t(b); cb < int64_t(e); cb += ck) {
+                    int64_t ce = cb + ck < int64_t(e) ? cb + ck : int64_t(e);
                     fb.ch[p][s_base[3] + off[3]++] = HubChunk{v, 0, cb, ce};
                 }
             }

@@ -17,7 +17,7 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     int64_t cap = 1;
     for (int32_t i = 0; i < g->rows; i++) {
         int64_t d = g->h_off[i + 1] - g->h_off[i];
-        if (d > kHubDeg) cap += (d + kChunkEdges - 1) / kChunkEdges;
+        if (d > kHubDeg) cap += (d + kMinChunkEdges - 1) / kMinChunkEdges;
     }
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     size_t qbytes = al(sizeof(int32_t) * std::max<int64_t>(n, 1));
@@ -48,6 +48,7 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     o += al(sizeof(unsigned long long) * grid);
     ws->fb.part.s = reinterpret_cast<unsigned long long *>(base + o);
     o += al(sizeof(unsigned long long) * grid);
+
     ws->ctl = reinterpret_cast<StepCtl *>(base + o);
     ws->fb.words = words;
     ws->dmem = base;
@@ -58,6 +59,11 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
         hipFree(base);
         ws->dmem = nullptr;
         return fail(HC_ERR_OOM, "pinned control block allocation failed");
+    }
+    if (hipHostGetDevicePointer(reinterpret_cast<void **>(&ws->fb.host_done), &ws->h_ctl->done,
+                                0) != hipSuccess) {
+        frontier_ws_free(ws);
+        return fail(HC_ERR_HIP, "pinned control block is not device-visible");
     }
     return HC_OK;
 }
@@ -80,9 +86,9 @@ int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
     StepCtl *h = ws->h_ctl;
     h->done = 0;
     HC_HIP(hipEventRecord(ev0, s));
-    // Batches of two epochs: the host reads batch b-1's done flag while batch
-    // b is queued, so at most ~4 epochs are enqueued past the last one (they
-    // return at once) and the device never waits on a poll.
+    // Batches of two epochs: the host reads the done flag after batch b-1 has
+    // finished while batch b is queued, so at most ~4 epochs are enqueued past
+    // the last one (they return at once) and the device never waits on a poll.
     long long epochs = 0;
     int rc = HC_OK;
     bool done = false;
@@ -95,8 +101,7 @@ int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
             epochs++;
         }
         if (rc != HC_OK) break;
-        hipMemcpyAsync(&h->done, &ws->ctl->done, sizeof(int32_t),
-                       hipMemcpyDeviceToHost, s);
+        // The controller sets h->done itself when the loop ends.
         hipEventRecord(evb[b & 1], s);
         if (b > 0) {
             hipEventSynchronize(evb[(b - 1) & 1]);

@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "device_common.h"
+#include "frontier.h"
 
 namespace hc {
 
@@ -88,6 +89,9 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"sssp.grid_per_cu", &t.sssp_grid_per_cu},
                {"pull.stage", &t.stage_pull},
                {"bu.stage", &t.stage_bu},
+               {"bu.remaining", &t.bu_remaining},
+               {"push.chunk", &t.push_chunk},
+               {"bu.first", &t.bu_first},
                {"profile.kernel_events", &t.kernel_events},
                {"multi.graph", &t.multi_graph},
                {"multi.grid_per_cu", &t.multi_grid_per_cu}};
@@ -546,8 +550,14 @@ int hc_set_tuning(const char *key, int64_t value) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
                            !strcmp(key, "sssp.stage");
         const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph") ||
-                          !strcmp(key, "sssp.filter");
-        HC_CHECK_ARG(value >= (stage || flag ? 0 : 1) && value <= (flag ? 1 : 1 << 20),
+                          !strcmp(key, "sssp.filter") || !strcmp(key, "bu.first");
+        if (!strcmp(key, "push.chunk"))
+            HC_CHECK_ARG(value >= kMinChunkEdges, "push.chunk must be >= %d", (int)kMinChunkEdges);
+        if (!strcmp(key, "bu.remaining"))
+            HC_CHECK_ARG(value == 0 || value == 1 || value == 4 || value == 16,
+                         "bu.remaining must be 0, 1, 4 or 16");
+        const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining");
+        HC_CHECK_ARG(value >= (zero_ok ? 0 : 1) && value <= (flag ? 1 : 1 << 20),
                      "%s out of range", key);
         if (stage)
             HC_CHECK_ARG(value * 4 <= kMaxStageBytes, "%s above %d entries", key,

@@ -309,8 +309,7 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
 }
 
 // Epoch controller: one workgroup.
-__global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb,
-                                                          StepCtl *c) {
+__device__ void sssp_control(const FrontierBufs &fb, StepCtl *c) {
     if (c->done) return;
     unsigned long long out_n, out_e, gathered;
     sum_partials(fb.part, c->grid, out_n, out_e, gathered);
@@ -349,6 +348,12 @@ __global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb,
     }
     c->need_compact = (!c->done && ran == kPull && next == kPush) ? 1 : 0;
     c->mode = next;
+    if (c->done) publish_done(fb);
+}
+
+// The controller launch (one workgroup).
+__global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb, StepCtl *c) {
+    sssp_control(fb, c);
 }
 
 // Initialises distances, bitmaps and the control block for one run.
@@ -359,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
     GraphView<OffT> g, int32_t *__restrict__ dist,
     const int32_t *__restrict__ init_dist, FrontierBufs fb, StepCtl *ctl,
     int32_t src, int direction, int alpha, int max_epochs, long long m,
-    long long sweep_bytes, int grid, int filter) {
+    long long sweep_bytes, int grid, int filter, int chunk) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride)
         dist[i] = init_dist ? init_dist[i] : (i == src ? 0 : HC_INF_DIST);
@@ -382,6 +387,7 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
         ctl->direction = direction;
         ctl->alpha = alpha;
         ctl->need_compact = 0;
+        ctl->chunk = chunk;
         ctl->n = g.n;
         ctl->grid = grid;
         ctl->m = m;
@@ -399,13 +405,8 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
             OffT b = g.rbeg(src), e = g.rend(src);
             int64_t d = int64_t(e - b);
             int cls = degree_class(d);
-            if (cls == 3) {
-                uint32_t nch = 0;
-                for (int64_t x = b; x < int64_t(e); x += kChunkEdges, nch++) {
-                    int64_t ce = x + kChunkEdges < int64_t(e) ? x + kChunkEdges : int64_t(e);
-                    fb.ch[0][nch] = HubChunk{src, 0, x, ce};
-                }
-                ctl->cnt[0][3] = nch;
+            if (cls == 3) {  // chunk descriptors: written by workgroup 0 below
+                ctl->cnt[0][3] = (uint32_t)((d + chunk - 1) / chunk);
                 ctl->cnt[0][4] = 1;
             } else {
                 fb.q[0][cls][0] = src;
@@ -418,6 +419,7 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
                         : (d > m / alpha ? kPull : kPush);
         }
     }
+    if (blockIdx.x == 0 && !init_dist) write_source_chunks(g, fb, src, chunk);
 }
 
 template <typename OffT>
@@ -628,7 +630,7 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
-        max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter);
+        max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter, tuning().push_chunk);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
     SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s, nstage, &ws->f.kt};

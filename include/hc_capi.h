@@ -216,6 +216,14 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   pull.g4_threshold      ... 4 lanes (else one lane)
  *   bu.wave_threshold / bu.g16_threshold / bu.g4_threshold   BFS bottom-up
  *   pull.stage / bu.stage  per-vertex entries / bitmap words staged in LDS
+ *   push.chunk             edges per workgroup chunk of a hub vertex in push
+ *                          epochs (>= 128; 512)
+ *   bu.first               1: one-lane bottom-up rows read their first
+ *                          neighbour from a per-vertex array (n int32 per
+ *                          graph) instead of the edge array (1)
+ *   bu.remaining           a bottom-up epoch after another sweeps only the
+ *                          vertices the previous one left unvisited, with
+ *                          this many lanes per vertex (1, 4 or 16; 0: off)
  *   profile.kernel_events  1: time each launch of the loop's main kernel
  *                          (hc_run_stats_t.kernel_ms); 0 by default
  *   multi.graph            1: batched loops relaunch an epoch hipGraph

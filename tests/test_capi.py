@@ -53,6 +53,20 @@ def test_error_codes_without_gpu(hc):
     hc.reset_tuning()
 
 
+def test_every_tuning_key_round_trips(hc):
+    # Each advertised knob accepts its current value and one in its range.
+    try:
+        for key in hc.TUNING_KEYS:
+            v = hc.get_tuning(key)
+            hc.set_tuning(key, v)
+            assert hc.get_tuning(key) == v
+            alt = 1 - v if v in (0, 1) else v // 2
+            hc.set_tuning(key, alt)
+            assert hc.get_tuning(key) == alt, key
+    finally:
+        hc.reset_tuning()
+
+
 def test_device_calls_fail_cleanly_without_gpu(hc):
     import subprocess
     import sys
