@@ -60,8 +60,12 @@ KNOB_SPACE = {
         "bu.stage": [0, 2048, 8192, 15360],
         "bu.wave_threshold": [1024, 4096],
         "bu.g16_threshold": [128, 256, 512],
-        "bu.g4_threshold": [16, 32, 64],
+        "bu.g4_threshold": [8, 16, 32, 64],
         "bu.remaining": [0, 1, 4],
+        "bu.stage_min": [0, 256, 1024, 4096, 1 << 20],
+        "bu.first": [0, 1],
+        "push.chunk": [256, 512, 1024],
+        "compact.grid": [128, 512, 2048],
     },
     "pr": {
         "grid_per_cu": [2, 4, 8, 16],

@@ -333,7 +333,7 @@ template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ depth,
     int32_t *__restrict__ parents, const int32_t *__restrict__ first, FrontierBufs fb,
-    StepCtl *ctl, RemLists rl, int32_t nstage) {
+    StepCtl *ctl, RemLists rl, int32_t nstage, int32_t stage_min) {
     extern __shared__ uint32_t s_words[];
     __shared__ uint32_t s_rem;
     if (ctl->done) return;
@@ -347,18 +347,20 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
         stale[w] = 0;
     const int32_t level = k + 1;
     uint32_t cnt = 0, scanned = 0;
-    unsigned long long eacc = 0;
+    unsigned long long eacc = 0, list_in = 0, list_out = 0;
     if (mode == kPull) {
         // The unvisited lists live in the small-degree push queues, which
         // pull epochs do not use (a switch to push invalidates them).
         const bool listing = ctl->rem_knob != 0;
         if (threadIdx.x == 0) s_rem = 0;
-        const Staged<uint32_t> fr = stage(frontier, nstage, s_words);  // syncs s_rem
         const int32_t off = listing ? rl.off[blockIdx.x] : 0;
+        const uint32_t len = ctl->rem_valid ? rl.len[cur * gridDim.x + blockIdx.x] : 0u;
+        // A short list does not repay copying the bitmap head into LDS.
+        const int32_t ns = (ctl->rem_valid && len < (uint32_t)stage_min) ? 0 : nstage;
+        const Staged<uint32_t> fr = stage(frontier, ns, s_words);  // syncs s_rem
         int32_t *rem = listing ? fb.q[nxt][2] + off : nullptr;
         if (ctl->rem_valid) {
             const int32_t *list = fb.q[cur][2] + off;
-            const uint32_t len = rl.len[cur * gridDim.x + blockIdx.x];
             if (ctl->rem_knob == 16)
                 bu_list_sweep<16>(g, depth, parents, first, level, fr, next, list, len, rem, &s_rem,
                                   cnt, eacc, scanned);
@@ -374,29 +376,25 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
         if (listing) {
             __syncthreads();
             if (threadIdx.x == 0) rl.len[nxt * gridDim.x + blockIdx.x] = s_rem;
+            list_in = len;
+            list_out = s_rem;
         }
     } else
         td_sweep(g, depth, parents, level, next, fb, ctl, cur, nxt, cnt, eacc, scanned);
-    write_partials(fb.part, cnt, eacc, scanned);
+    write_partials(fb.part, cnt, eacc, scanned, list_in, list_out);
 }
 
 // Direction-optimizing controller (one workgroup): the decisions of
 // bfs_do_cpu (src/kernels/cpu/bfs.cuh:68-103) applied between epochs.
-__device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, const RemLists &rl, int offb) {
+__device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, int offb) {
     if (c->done) return;
-    unsigned long long q, sc, scanned;
-    sum_partials(fb.part, c->grid, q, sc, scanned);
-    // Unvisited-list lengths read and written by a pull epoch.
-    unsigned long long lin = 0, lout = 0;
+    unsigned long long sums[kSumSlots];
+    sum_partials(fb.part, sums);
+    // Frontier size and degree sum, edges examined, unvisited-list lengths
+    // read and written by a pull epoch.
+    const unsigned long long q = sums[0], sc = sums[1], scanned = sums[2];
+    const unsigned long long lin = sums[3], lout = sums[4];
     const int k = c->epoch, cur = k & 1;
-    if (c->mode == kPull && c->rem_knob) {
-        for (int i = threadIdx.x; i < c->grid; i += kBlock) {
-            lin += rl.len[cur * c->grid + i];
-            lout += rl.len[(cur ^ 1) * c->grid + i];
-        }
-        lin = block_sum_u64(lin);
-        lout = block_sum_u64(lout);
-    }
     if (threadIdx.x != 0) return;
     const int ran = c->mode;
     c->reached += q;
@@ -468,8 +466,8 @@ __device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, const RemLists &
 // last workgroup was measured slower: every workgroup then needs a
 // device-scope release fence, which writes back its XCD's L2.
 __global__ __launch_bounds__(kBlock) void bfs_ctl_kernel(FrontierBufs fb, StepCtl *c,
-                                                         RemLists rl, int offb) {
-    bfs_control(fb, c, rl, offb);
+                                                         int offb) {
+    bfs_control(fb, c, offb);
 }
 
 template <typename OffT>
@@ -488,6 +486,7 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
         fb.bm[1][w] = 0;
         fb.bm[2][w] = 0;
     }
+    if (blockIdx.x == 0) clear_partials(fb.part);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int p = 0; p < 2; p++)
             for (int j = 0; j < 5; j++) ctl->cnt[p][j] = 0;
@@ -741,15 +740,17 @@ struct BfsLaunch {
     int grid;
     hipStream_t s;
     int32_t nstage;
+    int32_t stage_min;
     KernelTimer *kt;
     static int launch(void *arg) {
         auto *a = static_cast<BfsLaunch *>(arg);
         a->kt->mark(a->s);
         bfs_step_kernel<OffT><<<a->grid, kBlock, a->nstage * sizeof(uint32_t), a->s>>>(
-            a->plan, a->g, a->depth, a->parents, a->first, a->fb, a->ctl, a->rl, a->nstage);
+            a->plan, a->g, a->depth, a->parents, a->first, a->fb, a->ctl, a->rl, a->nstage,
+            a->stage_min);
         a->kt->mark(a->s);
-        bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, a->rl, (int)sizeof(OffT));
-        compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
+        bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
+        compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "BFS epoch launch failed");
         return HC_OK;
     }
@@ -780,7 +781,8 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     ws->f.kt.reset();
     const RemLists rl{ws->rem, reinterpret_cast<uint32_t *>(ws->rem + grid + 1)};
     BfsLaunch<OffT> L{ph->dev, v, depth, parents, tuning().bu_first ? ws->first : nullptr,
-                      ws->f.fb, ws->f.ctl, rl, grid, s, nstage, &ws->f.kt};
+                      ws->f.fb, ws->f.ctl, rl, grid, s, nstage, tuning().bu_stage_min,
+                      &ws->f.kt};
     const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
     long long launches = 0;

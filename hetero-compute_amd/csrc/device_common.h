@@ -102,9 +102,11 @@ struct Tuning {
     int64_t sssp_block_threshold = 4096;  // SSSP: degree above which a WG takes a row
     int stage_pull = 8192; // per-vertex entries staged in LDS by pull sweeps
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
-    int push_chunk = 512;  // edges per hub chunk of push epochs (>= 128)
-    int bu_remaining = 1;
-    int bu_first = 1;      // BFS: one-lane rows probe the first-neighbour array  // BFS: lanes per vertex of unvisited-list sweeps (0: off)
+    int push_chunk = 256;  // edges per hub chunk of push epochs (>= 128)
+    int bu_remaining = 1;  // BFS: lanes per vertex of unvisited-list sweeps (0: off)
+    int bu_first = 1;      // BFS: one-lane rows probe the first-neighbour array
+    int bu_stage_min = 4096;  // BFS: unvisited lists shorter than this skip LDS staging
+    int compact_grid = 4096;  // workgroups of the frontier compaction launch (capped)
     int kernel_events = 0; // 1: time every launch of a loop's main kernel
     int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
     int multi_grid_per_cu = 16;  // workgroups per CU of the batched sweep

@@ -83,10 +83,18 @@ struct StepCtl {
     unsigned long long tr_scan[kTraceLen];
 };
 
+// Step sums.  Every workgroup adds its totals into one of kSumGroups counter
+// groups (kSumStride words apart, each in its own cache line), so no counter
+// takes more than grid / kSumGroups same-address atomics; the controller
+// reads and clears them.  Slots: frontier vertices produced, their degree
+// sum, edges examined, and two algorithm-specific counts.
+constexpr int kSumGroups = 64;
+constexpr int kSumStride = 16;
+constexpr int kSumSlots = 5;
+static_assert(kSumGroups <= kWave, "one wavefront reads the counter groups");
+
 struct Partials {
-    uint32_t *n;               // frontier vertices produced, per workgroup
-    unsigned long long *e;     // their degree sum
-    unsigned long long *s;     // edges examined
+    unsigned long long *acc;   // [kSumGroups][kSumStride]
 };
 
 struct FrontierBufs {
@@ -189,11 +197,13 @@ __device__ __forceinline__ void write_source_chunks(const GraphView<OffT> &g,
     }
 }
 
-// Workgroup sums of three per-thread counters, written to this workgroup's
-// partial slot (thread 0).
+// Workgroup sums of three per-thread counters plus two per-workgroup values
+// (x, y: taken from thread 0), added into this workgroup's counter group.
 __device__ __forceinline__ void write_partials(const Partials &part, uint32_t n,
                                                unsigned long long e,
-                                               unsigned long long s) {
+                                               unsigned long long s,
+                                               unsigned long long x = 0,
+                                               unsigned long long y = 0) {
     __shared__ uint32_t s_n[kWavesPerBlock];
     __shared__ unsigned long long s_e[kWavesPerBlock], s_s[kWavesPerBlock];
 #pragma unroll
@@ -210,51 +220,44 @@ __device__ __forceinline__ void write_partials(const Partials &part, uint32_t n,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t tn = 0;
-        unsigned long long te = 0, ts = 0;
+        unsigned long long v[kSumSlots] = {0, 0, 0, x, y};
         for (int i = 0; i < kWavesPerBlock; i++) {
-            tn += s_n[i];
-            te += s_e[i];
-            ts += s_s[i];
+            v[0] += s_n[i];
+            v[1] += s_e[i];
+            v[2] += s_s[i];
         }
-        part.n[blockIdx.x] = tn;
-        part.e[blockIdx.x] = te;
-        part.s[blockIdx.x] = ts;
+        unsigned long long *g = part.acc + (blockIdx.x % kSumGroups) * kSumStride;
+        for (int k = 0; k < kSumSlots; k++)
+            if (v[k]) atomicAdd(&g[k], v[k]);
     }
 }
 
-// Sum of the step's partials (integers: exact in any order), run by one
-// workgroup of kBlock threads; results valid in every thread.
-__device__ __forceinline__ void sum_partials(const Partials &part, int grid,
-                                             unsigned long long &n,
-                                             unsigned long long &e,
-                                             unsigned long long &s) {
-    __shared__ unsigned long long r[3][kWavesPerBlock];
-    unsigned long long a = 0, b = 0, c = 0;
-    for (int i = threadIdx.x; i < grid; i += blockDim.x) {
-        a += part.n[i];
-        b += part.e[i];
-        c += part.s[i];
-    }
+// Reads and clears the step sums; run by one workgroup of kBlock threads
+// (integer sums: exact in any order); results valid in every thread.
+__device__ __forceinline__ void sum_partials(const Partials &part,
+                                             unsigned long long (&out)[kSumSlots]) {
+    __shared__ unsigned long long r[kSumSlots];
+    if (threadIdx.x < kWave) {
+        unsigned long long v[kSumSlots];
+        unsigned long long *g = part.acc + threadIdx.x * kSumStride;
+        for (int k = 0; k < kSumSlots; k++) {
+            v[k] = threadIdx.x < kSumGroups ? g[k] : 0ull;
+            if (threadIdx.x < kSumGroups) g[k] = 0;
+        }
+        for (int k = 0; k < kSumSlots; k++) {
 #pragma unroll
-    for (int sh = 32; sh > 0; sh >>= 1) {
-        a += __shfl_xor(a, sh, 64);
-        b += __shfl_xor(b, sh, 64);
-        c += __shfl_xor(c, sh, 64);
+            for (int sh = 32; sh > 0; sh >>= 1) v[k] += __shfl_xor(v[k], sh, 64);
+        }
+        if (threadIdx.x == 0)
+            for (int k = 0; k < kSumSlots; k++) r[k] = v[k];
     }
     __syncthreads();
-    if (lane_id() == 0) {
-        r[0][threadIdx.x / kWave] = a;
-        r[1][threadIdx.x / kWave] = b;
-        r[2][threadIdx.x / kWave] = c;
-    }
-    __syncthreads();
-    n = e = s = 0;
-    for (int w = 0; w < kWavesPerBlock; w++) {
-        n += r[0][w];
-        e += r[1][w];
-        s += r[2][w];
-    }
+    for (int k = 0; k < kSumSlots; k++) out[k] = r[k];
+}
+
+// Zeroes the step sums (one workgroup, at the start of a run).
+__device__ __forceinline__ void clear_partials(const Partials &part) {
+    for (int i = threadIdx.x; i < kSumGroups * kSumStride; i += blockDim.x) part.acc[i] = 0;
 }
 
 __device__ __forceinline__ void trace_epoch(StepCtl *c, int k, int mode,
@@ -275,6 +278,11 @@ __device__ __forceinline__ void trace_epoch(StepCtl *c, int k, int mode,
 // Bitmap -> degree-classed queues of parity (epoch & 1).  Each workgroup
 // takes a contiguous run of bitmap words, counts its vertices per class in
 // LDS and reserves its output with one atomic per class.
+// The compaction launch is queued after every epoch but has work only when
+// the direction turns to top-down; a smaller grid keeps its idle launches
+// cheap (tuning knob compact.grid: workgroups, at most the step grid).
+inline int compact_grid(int step_grid) { return std::max(1, std::min(step_grid, tuning().compact_grid)); }
+
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
                                                          FrontierBufs fb,

@@ -23,7 +23,7 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     size_t qbytes = al(sizeof(int32_t) * std::max<int64_t>(n, 1));
     size_t cbytes = al(sizeof(HubChunk) * cap);
     size_t bbytes = al(sizeof(uint32_t) * std::max(words, 1));
-    size_t pbytes = al(sizeof(uint32_t) * grid) + 2 * al(sizeof(unsigned long long) * grid);
+    size_t pbytes = al(sizeof(unsigned long long) * kSumGroups * kSumStride);
     size_t total = 6 * qbytes + 2 * cbytes + 3 * bbytes + pbytes + al(sizeof(StepCtl));
     char *base = nullptr;
     if (hipMalloc(reinterpret_cast<void **>(&base), total) != hipSuccess)
@@ -42,12 +42,8 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
         ws->fb.bm[p] = reinterpret_cast<uint32_t *>(base + o);
         o += bbytes;
     }
-    ws->fb.part.n = reinterpret_cast<uint32_t *>(base + o);
-    o += al(sizeof(uint32_t) * grid);
-    ws->fb.part.e = reinterpret_cast<unsigned long long *>(base + o);
-    o += al(sizeof(unsigned long long) * grid);
-    ws->fb.part.s = reinterpret_cast<unsigned long long *>(base + o);
-    o += al(sizeof(unsigned long long) * grid);
+    ws->fb.part.acc = reinterpret_cast<unsigned long long *>(base + o);
+    o += pbytes;
 
     ws->ctl = reinterpret_cast<StepCtl *>(base + o);
     ws->fb.words = words;

@@ -92,6 +92,8 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"bu.remaining", &t.bu_remaining},
                {"push.chunk", &t.push_chunk},
                {"bu.first", &t.bu_first},
+               {"compact.grid", &t.compact_grid},
+               {"bu.stage_min", &t.bu_stage_min},
                {"profile.kernel_events", &t.kernel_events},
                {"multi.graph", &t.multi_graph},
                {"multi.grid_per_cu", &t.multi_grid_per_cu}};
@@ -556,7 +558,8 @@ int hc_set_tuning(const char *key, int64_t value) {
         if (!strcmp(key, "bu.remaining"))
             HC_CHECK_ARG(value == 0 || value == 1 || value == 4 || value == 16,
                          "bu.remaining must be 0, 1, 4 or 16");
-        const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining");
+        const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining") ||
+                             !strcmp(key, "bu.stage_min");
         HC_CHECK_ARG(value >= (zero_ok ? 0 : 1) && value <= (flag ? 1 : 1 << 20),
                      "%s out of range", key);
         if (stage)

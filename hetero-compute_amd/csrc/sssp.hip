@@ -311,8 +311,9 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
 // Epoch controller: one workgroup.
 __device__ void sssp_control(const FrontierBufs &fb, StepCtl *c) {
     if (c->done) return;
-    unsigned long long out_n, out_e, gathered;
-    sum_partials(fb.part, c->grid, out_n, out_e, gathered);
+    unsigned long long sums[kSumSlots];
+    sum_partials(fb.part, sums);
+    const unsigned long long out_n = sums[0], out_e = sums[1], gathered = sums[2];
     if (threadIdx.x != 0) return;
     const int k = c->epoch, cur = k & 1;
     const int ran = c->mode;
@@ -374,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
         fb.bm[1][w] = 0;
         fb.bm[2][w] = 0;
     }
+    if (blockIdx.x == 0) clear_partials(fb.part);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // With initial distances the changed set before epoch 0 is unknown:
         // that epoch gathers every neighbour.
@@ -607,7 +609,7 @@ struct SsspLaunch {
             a->plan, a->g, a->dist, a->fb, a->ctl, a->nstage);
         a->kt->mark(a->s);
         sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
-        compact_kernel<OffT><<<a->grid, kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
+        compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "SSSP epoch launch failed");
         return HC_OK;
     }

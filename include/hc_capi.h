@@ -217,7 +217,12 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   bu.wave_threshold / bu.g16_threshold / bu.g4_threshold   BFS bottom-up
  *   pull.stage / bu.stage  per-vertex entries / bitmap words staged in LDS
  *   push.chunk             edges per workgroup chunk of a hub vertex in push
- *                          epochs (>= 128; 512)
+ *                          epochs (>= 128; 256)
+ *   bu.stage_min           a workgroup whose unvisited list is shorter than
+ *                          this reads the frontier bitmap without LDS
+ *                          staging (4096)
+ *   compact.grid           workgroups of the frontier compaction launch, at
+ *                          most the step grid (4096)
  *   bu.first               1: one-lane bottom-up rows read their first
  *                          neighbour from a per-vertex array (n int32 per
  *                          graph) instead of the edge array (1)
