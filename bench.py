@@ -166,6 +166,26 @@ def class_sweep(hc, kind, scale, degree, seed, reps=3):
     print(json.dumps({"class_sweep": f"{kind}{scale} ef{degree}", "classes": out}), flush=True)
 
 
+def pull_sweeps(hc, kind, scale, degree, seed, reps):
+    """`reps` full SSSP pull sweeps (epoch API, bucketed plan) at the fixpoint
+    of one source: a fixed, repeatable target for hardware-counter passes."""
+    g = hc.HostGraph.generate(kind, scale, degree, weighted=True, seed=seed)
+    with hc.DeviceGraph(g) as dg:
+        s = int(g.pick_sources(1)[0])
+        dist = hc.DeviceArray.from_host(hc.sssp(dg, s).values)
+        ms = []
+        for _ in range(reps):
+            hc.synchronize()
+            t0 = time.perf_counter()
+            hc.sssp_pull_epoch(dg, dist, 0, g.num_nodes, mapping=hc.MAP_BUCKETED)
+            hc.synchronize()
+            ms.append((time.perf_counter() - t0) * 1e3)
+        dist.free()
+    print(json.dumps({"pull_sweeps": f"{kind}{scale} ef{degree}", "m": g.num_edges,
+                      "ms": [round(x, 4) for x in ms],
+                      "gedges_per_s": round(g.num_edges / min(ms) / 1e6, 1)}), flush=True)
+
+
 def copy_probe(nbytes, reps):
     """A plain device copy of known size (torch's copy kernel): the reference
     point that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
@@ -400,6 +420,8 @@ def main():
     ap.add_argument("--serve-queries", type=int, default=400)
     ap.add_argument("--class-sweep", default="", metavar="KIND",
                     help="only time the SSSP pull sweep per degree class on KIND (kron|uniform)")
+    ap.add_argument("--pull-sweeps", default="", metavar="KIND",
+                    help="only run --steps full SSSP pull sweeps on KIND at the fixpoint")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -418,6 +440,9 @@ def main():
     hc.check(hc.lib().hc_init(0))
     if args.class_sweep:
         class_sweep(hc, args.class_sweep, args.scale, args.degree, args.seed)
+        return
+    if args.pull_sweeps:
+        pull_sweeps(hc, args.pull_sweeps, args.scale, args.degree, args.seed, args.steps)
         return
     dev = hc.device_info()
     log(f"device {dev['name']} {dev['arch']} CUs={dev['compute_units']}")

@@ -269,6 +269,7 @@ __device__ __forceinline__ void td_range(const GraphView<OffT> &g,
                                          int32_t *__restrict__ parents,
                                          int32_t level, uint32_t *__restrict__ next,
                                          const FrontierBufs &fb, StepCtl *ctl,
+                                         const PushOut &out,
                                          int nxt, int32_t u, OffT b, OffT e,
                                          int lane, uint32_t &cnt,
                                          unsigned long long &edges_acc,
@@ -284,7 +285,7 @@ __device__ __forceinline__ void td_range(const GraphView<OffT> &g,
             cnt++;
             edges_acc += (unsigned long long)(g.rend(v) - g.rbeg(v));
         }
-        frontier_append(g, fb, ctl, nxt, app, v);
+        push_out_append(out, g, fb, ctl, nxt, app, v);
     }
 }
 
@@ -292,7 +293,7 @@ template <typename OffT>
 __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
                          int32_t *__restrict__ parents, int32_t level,
                          uint32_t *__restrict__ next, const FrontierBufs &fb,
-                         StepCtl *ctl, int cur, int nxt, uint32_t &cnt,
+                         StepCtl *ctl, const PushOut &out, int cur, int nxt, uint32_t &cnt,
                          unsigned long long &edges_acc, uint32_t &scanned) {
     const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
                    n1 = ctl->cnt[cur][1], n2 = ctl->cnt[cur][2];
@@ -302,24 +303,24 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
     for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
-            td_range<kBlock>(g, depth, parents, level, next, fb, ctl, nxt, h.u,
+            td_range<kBlock>(g, depth, parents, level, next, fb, ctl, out, nxt, h.u,
                              OffT(h.b), OffT(h.e), threadIdx.x, cnt, edges_acc, scanned);
         } else if (unit < u1) {
             int32_t u = fb.q[cur][0][unit - nch];
-            td_range<kBlock>(g, depth, parents, level, next, fb, ctl, nxt, u,
+            td_range<kBlock>(g, depth, parents, level, next, fb, ctl, out, nxt, u,
                              g.rbeg(u), g.rend(u), threadIdx.x, cnt, edges_acc, scanned);
         } else if (unit < u2) {
             uint32_t idx = (unit - u1) * kWavesPerBlock + threadIdx.x / kWave;
             if (idx < n1) {
                 int32_t u = fb.q[cur][1][idx];
-                td_range<kWave>(g, depth, parents, level, next, fb, ctl, nxt, u,
+                td_range<kWave>(g, depth, parents, level, next, fb, ctl, out, nxt, u,
                                 g.rbeg(u), g.rend(u), lane_id(), cnt, edges_acc, scanned);
             }
         } else {
             uint32_t idx = (unit - u2) * kSmallPerUnit + threadIdx.x / kSmallLanes;
             if (idx < n2) {
                 int32_t u = fb.q[cur][2][idx];
-                td_range<kSmallLanes>(g, depth, parents, level, next, fb, ctl, nxt,
+                td_range<kSmallLanes>(g, depth, parents, level, next, fb, ctl, out, nxt,
                                       u, g.rbeg(u), g.rend(u),
                                       threadIdx.x % kSmallLanes, cnt, edges_acc, scanned);
             }
@@ -328,12 +329,13 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
 }
 
 // `nstage` leading frontier-bitmap words are staged in dynamic LDS for
-// bottom-up epochs.
+// bottom-up epochs; top-down epochs stage their output there (`lds_words`
+// words of dynamic LDS in all).
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ depth,
     int32_t *__restrict__ parents, const int32_t *__restrict__ first, FrontierBufs fb,
-    StepCtl *ctl, RemLists rl, int32_t nstage, int32_t stage_min) {
+    StepCtl *ctl, RemLists rl, int32_t nstage, int32_t stage_min, int32_t lds_words) {
     extern __shared__ uint32_t s_words[];
     __shared__ uint32_t s_rem;
     if (ctl->done) return;
@@ -379,8 +381,15 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
             list_in = len;
             list_out = s_rem;
         }
-    } else
-        td_sweep(g, depth, parents, level, next, fb, ctl, cur, nxt, cnt, eacc, scanned);
+    } else {
+        // Top-down: output staged in the dynamic LDS (unused by this mode).
+        __shared__ uint32_t s_out;
+        if (threadIdx.x == 0) s_out = 0;
+        __syncthreads();
+        const PushOut out{reinterpret_cast<int32_t *>(s_words), &s_out, (uint32_t)lds_words};
+        td_sweep(g, depth, parents, level, next, fb, ctl, out, cur, nxt, cnt, eacc, scanned);
+        push_out_flush(out, g, fb, ctl, nxt);
+    }
     write_partials(fb.part, cnt, eacc, scanned, list_in, list_out);
 }
 
@@ -741,13 +750,14 @@ struct BfsLaunch {
     hipStream_t s;
     int32_t nstage;
     int32_t stage_min;
+    int32_t lds_words;
     KernelTimer *kt;
     static int launch(void *arg) {
         auto *a = static_cast<BfsLaunch *>(arg);
         a->kt->mark(a->s);
-        bfs_step_kernel<OffT><<<a->grid, kBlock, a->nstage * sizeof(uint32_t), a->s>>>(
+        bfs_step_kernel<OffT><<<a->grid, kBlock, a->lds_words * sizeof(uint32_t), a->s>>>(
             a->plan, a->g, a->depth, a->parents, a->first, a->fb, a->ctl, a->rl, a->nstage,
-            a->stage_min);
+            a->stage_min, a->lds_words);
         a->kt->mark(a->s);
         bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
         compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
@@ -769,8 +779,9 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     if (!depth) depth = ws->depth;
     GraphView<OffT> v = view<OffT>(g);
     const int32_t nstage = std::min<int32_t>(tuning().stage_bu, ws->f.fb.words);
+    const int32_t lds_words = std::max<int32_t>(nstage, (int32_t)kPushStage);
     const int grid = resident_grid(reinterpret_cast<const void *>(&bfs_step_kernel<OffT>),
-                                   kBlock, nstage * sizeof(uint32_t), ws->f.grid);
+                                   kBlock, lds_words * sizeof(uint32_t), ws->f.grid);
     if (grid > ws->f.grid) return fail(HC_ERR_HIP, "BFS grid %d above its workspace", grid);
     HC_TRY(bfs_rem_layout(ws, ph, grid, s));
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
@@ -782,7 +793,7 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     const RemLists rl{ws->rem, reinterpret_cast<uint32_t *>(ws->rem + grid + 1)};
     BfsLaunch<OffT> L{ph->dev, v, depth, parents, tuning().bu_first ? ws->first : nullptr,
                       ws->f.fb, ws->f.ctl, rl, grid, s, nstage, tuning().bu_stage_min,
-                      &ws->f.kt};
+                      lds_words, &ws->f.kt};
     const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
     long long launches = 0;

@@ -183,6 +183,94 @@ __device__ __forceinline__ void frontier_append(const GraphView<OffT> &g,
     }
 }
 
+// Push-epoch output staged per workgroup in LDS.  Appending straight to the
+// degree-classed queues costs one atomic per wavefront and class on the same
+// few counters, and same-address atomics serialize at about 12 ns each on
+// this GPU (a big top-down epoch issued ~10^5 of them); staged output is
+// classified at the end of the sweep and reserved with one atomic per class
+// and workgroup.  Entries past `cap` go to the queues directly.
+constexpr uint32_t kPushStage = 4096;  // least LDS entries of a push step's buffer
+
+struct PushOut {
+    int32_t *buf;      // LDS, cap entries
+    uint32_t *count;   // LDS: entries appended (past cap: sent to the queues)
+    uint32_t cap;
+};
+
+template <typename OffT>
+__device__ __forceinline__ void push_out_append(const PushOut &o, const GraphView<OffT> &g,
+                                                const FrontierBufs &fb, StepCtl *ctl, int p,
+                                                bool pred, int32_t v) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(o.count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    const uint32_t pos = base + (uint32_t)__popcll(m & lanemask_lt());
+    const bool fits = pos < o.cap;
+    if (pred && fits) o.buf[pos] = v;
+    frontier_append(g, fb, ctl, p, pred && !fits, v);
+}
+
+// Moves the staged entries into the queues of parity p (all threads of the
+// workgroup; called once, after the sweep).
+template <typename OffT>
+__device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
+                               const FrontierBufs &fb, StepCtl *ctl, int p) {
+    __shared__ uint32_t s_n[5], s_base[5];
+    __syncthreads();
+    const uint32_t n = min(*o.count, o.cap);
+    if (threadIdx.x < 5) s_n[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t ck = ctl->chunk;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const int32_t v = o.buf[i];
+        const int64_t d = int64_t(g.rend(v) - g.rbeg(v));
+        const int c = degree_class(d);
+        if (c < 3) {
+            atomicAdd(&s_n[c], 1u);
+        } else {
+            atomicAdd(&s_n[3], (uint32_t)((d + ck - 1) / ck));
+            atomicAdd(&s_n[4], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        const uint32_t k = s_n[threadIdx.x];
+        s_base[threadIdx.x] = k ? atomicAdd(&ctl->cnt[p][threadIdx.x], k) : 0u;
+        s_n[threadIdx.x] = 0;  // now the write cursor of the class
+    }
+    __syncthreads();
+    // Every lane takes part in every round (a hub's chunk descriptors, up to
+    // hundreds, are written by the whole wavefront).
+    for (uint32_t i0 = 0; i0 < n; i0 += blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        int32_t v = 0;
+        int c = -1;
+        OffT b = 0, e = 0;
+        if (i < n) {
+            v = o.buf[i];
+            b = g.rbeg(v);
+            e = g.rend(v);
+            c = degree_class(int64_t(e - b));
+        }
+        if (c >= 0 && c < 3) fb.q[p][c][s_base[c] + atomicAdd(&s_n[c], 1u)] = v;
+        const uint32_t nch = c == 3 ? (uint32_t)((int64_t(e - b) + ck - 1) / ck) : 0u;
+        const uint32_t at = c == 3 ? s_base[3] + atomicAdd(&s_n[3], nch) : 0u;
+        for (uint64_t t = __ballot(c == 3); t; t &= t - 1) {
+            const int l = __ffsll((unsigned long long)t) - 1;
+            const uint32_t n_l = __shfl(nch, l), at_l = __shfl(at, l);
+            const int32_t v_l = __shfl(v, l);
+            const long long b_l = __shfl((long long)b, l), e_l = __shfl((long long)e, l);
+            for (uint32_t j = lane_id(); j < n_l; j += kWave) {
+                const long long cb = b_l + (long long)j * ck;
+                fb.ch[p][at_l + j] = HubChunk{v_l, 0, cb, cb + ck < e_l ? cb + ck : e_l};
+            }
+        }
+    }
+}
+
 // Chunk descriptors of a hub source (parity 0), written by one workgroup.
 template <typename OffT>
 __device__ __forceinline__ void write_source_chunks(const GraphView<OffT> &g,

@@ -211,6 +211,7 @@ template <int G, typename OffT>
 __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
                                            int32_t *__restrict__ dist,
                                            const FrontierBufs &fb, StepCtl *ctl,
+                                           const PushOut &out,
                                            int nxt, uint32_t *wbm, int32_t u, OffT b,
                                            OffT e, int lane, uint32_t &cnt,
                                            unsigned long long &edges_acc) {
@@ -230,14 +231,15 @@ __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
             cnt++;
             edges_acc += (unsigned long long)(g.rend(v) - g.rbeg(v));
         }
-        frontier_append(g, fb, ctl, nxt, app, v);
+        push_out_append(out, g, fb, ctl, nxt, app, v);
     }
 }
 
 template <typename OffT>
 __device__ void sssp_push_sweep(const GraphView<OffT> &g,
                                 int32_t *__restrict__ dist,
-                                const FrontierBufs &fb, StepCtl *ctl, int cur,
+                                const FrontierBufs &fb, StepCtl *ctl,
+                                const PushOut &out, int cur,
                                 int nxt, uint32_t *wbm, uint32_t &cnt,
                                 unsigned long long &edges_acc) {
     const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
@@ -248,24 +250,24 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
     for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
-            push_range<kBlock>(g, dist, fb, ctl, nxt, wbm, h.u, OffT(h.b), OffT(h.e),
+            push_range<kBlock>(g, dist, fb, ctl, out, nxt, wbm, h.u, OffT(h.b), OffT(h.e),
                                threadIdx.x, cnt, edges_acc);
         } else if (unit < u1) {
             int32_t u = fb.q[cur][0][unit - nch];
-            push_range<kBlock>(g, dist, fb, ctl, nxt, wbm, u, g.rbeg(u), g.rend(u),
+            push_range<kBlock>(g, dist, fb, ctl, out, nxt, wbm, u, g.rbeg(u), g.rend(u),
                                threadIdx.x, cnt, edges_acc);
         } else if (unit < u2) {
             uint32_t idx = (unit - u1) * kWavesPerBlock + threadIdx.x / kWave;
             if (idx < n1) {
                 int32_t u = fb.q[cur][1][idx];
-                push_range<kWave>(g, dist, fb, ctl, nxt, wbm, u, g.rbeg(u), g.rend(u),
+                push_range<kWave>(g, dist, fb, ctl, out, nxt, wbm, u, g.rbeg(u), g.rend(u),
                                   lane_id(), cnt, edges_acc);
             }
         } else {
             uint32_t idx = (unit - u2) * kSmallPerUnit + threadIdx.x / kSmallLanes;
             if (idx < n2) {
                 int32_t u = fb.q[cur][2][idx];
-                push_range<kSmallLanes>(g, dist, fb, ctl, nxt, wbm, u, g.rbeg(u),
+                push_range<kSmallLanes>(g, dist, fb, ctl, out, nxt, wbm, u, g.rbeg(u),
                                         g.rend(u), threadIdx.x % kSmallLanes, cnt,
                                         edges_acc);
             }
@@ -277,11 +279,12 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
 // Kernels of the device-driven loop
 // ---------------------------------------------------------------------------
 
-// `stage` hub distances are staged in dynamic LDS for pull epochs.
+// `nstage` hub distances are staged in dynamic LDS for pull epochs; push
+// epochs stage their output there (`lds_entries` entries in all).
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, FrontierBufs fb,
-    StepCtl *ctl, int32_t nstage) {
+    StepCtl *ctl, int32_t nstage, int32_t lds_entries) {
     extern __shared__ int32_t s_stage[];
     if (ctl->done) return;
     const int mode = ctl->mode;
@@ -303,7 +306,13 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
         else
             sssp_pull_sweep<true, false>(p, g, rd, unstaged(rbm), dist, wbm, cnt, eacc, gathered);
     } else {
-        sssp_push_sweep(g, dist, fb, ctl, cur, nxt, wbm, cnt, eacc);
+        // Output staged in the dynamic LDS (unused by push epochs).
+        __shared__ uint32_t s_out;
+        if (threadIdx.x == 0) s_out = 0;
+        __syncthreads();
+        const PushOut out{s_stage, &s_out, (uint32_t)lds_entries};
+        sssp_push_sweep(g, dist, fb, ctl, out, cur, nxt, wbm, cnt, eacc);
+        push_out_flush(out, g, fb, ctl, nxt);
     }
     write_partials(fb.part, cnt, eacc, gathered);
 }
@@ -601,12 +610,13 @@ struct SsspLaunch {
     int grid;
     hipStream_t s;
     int32_t nstage;
+    int32_t lds_entries;
     KernelTimer *kt;
     static int launch(void *arg) {
         auto *a = static_cast<SsspLaunch *>(arg);
         a->kt->mark(a->s);
-        sssp_step_kernel<OffT><<<a->grid, kBlock, a->nstage * sizeof(int32_t), a->s>>>(
-            a->plan, a->g, a->dist, a->fb, a->ctl, a->nstage);
+        sssp_step_kernel<OffT><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t), a->s>>>(
+            a->plan, a->g, a->dist, a->fb, a->ctl, a->nstage, a->lds_entries);
         a->kt->mark(a->s);
         sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
         compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
@@ -626,8 +636,9 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     GraphView<OffT> v = view<OffT>(g);
     const long long sweep = sssp_sweep_bytes(g, ph);
     const int32_t nstage = std::min<int32_t>(tuning().sssp_stage, g->n);
+    const int32_t lds_entries = std::max<int32_t>(nstage, (int32_t)kPushStage);
     const int grid = resident_grid(reinterpret_cast<const void *>(&sssp_step_kernel<OffT>),
-                                   kBlock, nstage * sizeof(int32_t),
+                                   kBlock, lds_entries * sizeof(int32_t),
                                    std::min(ws->f.grid, grid_blocks(tuning().sssp_grid_per_cu)));
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
@@ -635,7 +646,8 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
         max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter, tuning().push_chunk);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
-    SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s, nstage, &ws->f.kt};
+    SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s, nstage, lds_entries,
+                       &ws->f.kt};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
     const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;
     float ms = 0;
