@@ -45,7 +45,7 @@ else:  # run as a script
 # Candidate values per knob, searched one knob at a time in this order.
 KNOB_SPACE = {
     "sssp": {
-        "sssp.alpha": [8, 16, 32],
+        "sssp.alpha": [2, 4, 8, 16],
         "sssp.filter": [0, 1],
         "sssp.stage": [0, 2048, 8192],
         "sssp.block_threshold": [1024, 4096, 16384],

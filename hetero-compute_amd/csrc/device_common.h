@@ -95,7 +95,7 @@ struct PlanConfig {
 // caller; read when plans and workspaces are built).
 struct Tuning {
     int grid_per_cu = 8;   // workgroups per CU of the fused / step launches
-    int sssp_alpha = 8;    // SSSP goes pull when frontier edges > m / alpha
+    int sssp_alpha = 4;    // SSSP goes pull when frontier edges > m / alpha
     int sssp_grid_per_cu = 4;  // workgroups per CU of the SSSP loop (at most)
     int sssp_filter = 0;   // SSSP pulls gather only neighbours changed last epoch
     int sssp_stage = 0;    // per-vertex entries staged in LDS by SSSP pull sweeps

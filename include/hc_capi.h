@@ -201,7 +201,7 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  * src/kernels/heterogeneous/sssp.cuh:180-278).  Keys:
  *   grid_per_cu            workgroups per CU of full-device launches (8); the
  *                          loops launch at most as many as fit at once
- *   sssp.alpha             SSSP pull when frontier edges > m/alpha (8)
+ *   sssp.alpha             SSSP pull when frontier edges > m/alpha (4)
  *   sssp.grid_per_cu       cap on workgroups per CU of the SSSP loop (4)
  *   sssp.filter            1: SSSP pull epochs gather only neighbours whose
  *                          distance changed in the previous epoch (0)

@@ -84,6 +84,31 @@ def test_sssp_bfs_scale16(hc, gpu, oracle, request, which):
             assert b.stats["reached"] == int((depth >= 0).sum())
 
 
+@pytest.mark.parametrize("filt,alpha", [(1, 8), (1, 2), (0, 2), (0, 16)])
+def test_sssp_filter_and_direction_exact(hc, gpu, oracle, kron16, uniform16, filt, alpha):
+    """Change-filtered pull epochs (neighbours changed in the previous epoch
+    only) and other push/pull switch points give the same distances as
+    Dijkstra, on both scale-16 graphs and every golden graph."""
+    try:
+        hc.set_tuning("sssp.filter", filt)
+        hc.set_tuning("sssp.alpha", alpha)
+        for g in (kron16, uniform16):
+            with hc.DeviceGraph(g) as dg:
+                for s in map(int, g.pick_sources(3)):
+                    ref = oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, s)
+                    for direction in (0, 1):
+                        r = hc.sssp(dg, s, direction=direction)
+                        assert np.array_equal(r.values, ref), (s, direction)
+        for name in golden_files():
+            fx = load_golden(name)
+            with hc.DeviceGraph(_graph(hc, fx)) as dg:
+                for i, s in enumerate(fx["sources"]):
+                    r = hc.sssp(dg, int(s))
+                    assert np.array_equal(r.values, fx["sssp"][i]), (name, int(s))
+    finally:
+        hc.reset_tuning()
+
+
 @pytest.mark.parametrize("mapping", [0, 1, 2, 3])
 def test_sssp_pull_driver_mappings(hc, gpu, oracle, kron16, mapping):
     """sssp_pull_gpu semantics (pull epochs from init_dist until no change)
