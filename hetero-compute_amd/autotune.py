@@ -241,6 +241,16 @@ def profile_segments(g, algorithm: str, num_segments: int = 24, shapes=None,
     return {"algorithm": algorithm, "num_nodes": n, "segments": segs}
 
 
+def profile_tree(g, algorithm: str, depth: int, shapes=None, repeats: int = 3) -> dict:
+    """Tree microbenchmark (src/benchmarks/benchmark.cuh:258-291): layer i
+    splits the vertices into 2^i equal-edge segments and profiles each one on
+    the host cores and every GPU kernel, for i = 0..depth."""
+    if depth < 0:
+        raise ValueError("depth must be >= 0")
+    layers = [profile_segments(g, algorithm, 1 << i, shapes, repeats) for i in range(depth + 1)]
+    return {"algorithm": algorithm, "num_nodes": g.num_nodes, "layers": layers}
+
+
 def schedule_from_profile(profile: dict, pcie_gbs: float = 25.0):
     """Greedy host/GPU split of the profiled segments.  The GPU side costs
     the sum of its segments' best kernel times plus, once any segment is on
@@ -292,6 +302,73 @@ def schedule_from_profile(profile: dict, pcie_gbs: float = 25.0):
     return blocks, cur, plan
 
 
+def _segment_rows(g, segments):
+    deg = np.diff(g.index)
+    rows = []
+    for sgm in segments:
+        d = deg[sgm["start"]:sgm["end"]]
+        rows.append({"start": sgm["start"], "end": sgm["end"],
+                     "avg_deg": float(d.mean()) if d.size else 0.0,
+                     "min_deg": int(d.min()) if d.size else 0,
+                     "max_deg": int(d.max()) if d.size else 0,
+                     "num_edges": sgm["edges"]})
+    return rows
+
+
+def _segment_yaml(rows, millis, indent):
+    pad = " " * indent
+    out = []
+    for r, ms in zip(rows, millis):
+        gteps = r["num_edges"] / 2 / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        out += [f"{pad}- start: {r['start']}", f"{pad}  end: {r['end']}",
+                f"{pad}  avg_deg: {r['avg_deg']:.6g}", f"{pad}  min_deg: {r['min_deg']}",
+                f"{pad}  max_deg: {r['max_deg']}", f"{pad}  num_edges: {r['num_edges']}",
+                f"{pad}  millis: {ms:.6g}", f"{pad}  gteps: {gteps:.6g}"]
+    return out
+
+
+def _kernel_series(segments, alg, cpu_name, gpu_name):
+    """(file tag, device, kernel line, per-segment ms) for the host kernel and
+    each GPU kernel profiled on every segment (its most common best shape)."""
+    series = [(f"cpu_{CPU_KERNEL[alg].split()[-1]}", cpu_name, CPU_KERNEL[alg],
+               [sg["cpu_ms"] for sg in segments])]
+    for name in GPU_KERNELS[alg]:
+        if not all(name in sg["gpu"] for sg in segments):
+            continue
+        shapes = [sg["gpu"][name][1:] for sg in segments]
+        b, t = max(set(shapes), key=shapes.count)
+        series.append((f"gpu_{name.split()[-1]}", gpu_name, f"{name} {b} {t}",
+                       [sg["gpu"][name][0] for sg in segments]))
+    return series
+
+
+def write_yaml_tree(tree: dict, g, outdir: str, cpu_name: str = "Host CPU",
+                    gpu_name: str = "AMD Instinct MI355X") -> list[str]:
+    """Writes a tree profile in the layout of the reference's tree_res_t
+    writer (src/benchmarks/benchmark.cuh:213-240): per device and kernel,
+    `config: {device, kernel}` and `results: - depth: i, segments: [...]`."""
+    os.makedirs(outdir, exist_ok=True)
+    alg = tree["algorithm"]
+    per_file = {}
+    for depth, layer in enumerate(tree["layers"]):
+        rows = _segment_rows(g, layer["segments"])
+        for tag, dev, kname, millis in _kernel_series(layer["segments"], alg, cpu_name, gpu_name):
+            entry = per_file.setdefault(tag, {"device": dev, "kernel": kname, "layers": []})
+            entry["layers"].append((depth, rows, millis))
+    out = []
+    for tag, e in per_file.items():
+        if len(e["layers"]) != len(tree["layers"]):
+            continue  # a kernel missing from some layer has no complete tree
+        lines = ["config:", f"  device: {e['device']}", f"  kernel: {e['kernel']}", "results:"]
+        for depth, rows, millis in e["layers"]:
+            lines += [f"  - depth: {depth}", "    segments:"] + _segment_yaml(rows, millis, 6)
+        path = os.path.join(outdir, f"{alg}_{tag}_tree.yaml")
+        with open(path, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        out.append(path)
+    return out
+
+
 def write_yaml_profiles(profile: dict, g, outdir: str, cpu_name: str = "Host CPU",
                         gpu_name: str = "AMD Instinct MI355X") -> list[str]:
     """Writes the segment profile as the reference's per-kernel YAML files
@@ -301,44 +378,16 @@ def write_yaml_profiles(profile: dict, g, outdir: str, cpu_name: str = "Host CPU
     are part of the GPU kernel name, as the reference's profiles carry them."""
     os.makedirs(outdir, exist_ok=True)
     alg = profile["algorithm"]
-    deg = np.diff(g.index)
-    rows = []
-    for sgm in profile["segments"]:
-        d = deg[sgm["start"]:sgm["end"]]
-        rows.append({"start": sgm["start"], "end": sgm["end"],
-                     "avg_deg": float(d.mean()) if d.size else 0.0,
-                     "min_deg": int(d.min()) if d.size else 0,
-                     "max_deg": int(d.max()) if d.size else 0,
-                     "num_edges": sgm["edges"]})
-
-    def emit(path, device, kernel, millis):
-        lines = ["config:", f"  device: {device}", f"  kernel: {kernel}", "results:",
-                 "  - segments:"]
-        for r, ms in zip(rows, millis):
-            gteps = r["num_edges"] / 2 / (ms / 1e3) / 1e9 if ms > 0 else 0.0
-            lines += [f"      - start: {r['start']}", f"        end: {r['end']}",
-                      f"        avg_deg: {r['avg_deg']:.6g}", f"        min_deg: {r['min_deg']}",
-                      f"        max_deg: {r['max_deg']}", f"        num_edges: {r['num_edges']}",
-                      f"        millis: {ms:.6g}", f"        gteps: {gteps:.6g}"]
+    rows = _segment_rows(g, profile["segments"])
+    out = []
+    for tag, dev, kname, millis in _kernel_series(profile["segments"], alg, cpu_name, gpu_name):
+        lines = ["config:", f"  device: {dev}", f"  kernel: {kname}", "results:",
+                 "  - segments:"] + _segment_yaml(rows, millis, 6)
+        path = os.path.join(outdir, f"{alg}_{tag}.yaml")
         with open(path, "w") as f:
             f.write("\n".join(lines) + "\n")
-        return path
-
-    out = [emit(os.path.join(outdir, f"{alg}_cpu_{CPU_KERNEL[alg].split()[-1]}.yaml"), cpu_name,
-                CPU_KERNEL[alg], [sg["cpu_ms"] for sg in profile["segments"]])]
-    for name in GPU_KERNELS[alg]:
-        if not all(name in sg["gpu"] for sg in profile["segments"]):
-            continue
-        # The reference profiles one launch shape per file; use the shape
-        # that is best over most segments.
-        shapes = [sg["gpu"][name][1:] for sg in profile["segments"]]
-        b, t = max(set(shapes), key=shapes.count)
-        kname = f"{name} {b} {t}"
-        tag = name.split()[-1]
-        out.append(emit(os.path.join(outdir, f"{alg}_gpu_{tag}.yaml"), gpu_name, kname,
-                        [sg["gpu"][name][0] for sg in profile["segments"]]))
+        out.append(path)
     return out
-
 
 def tune_schedule(g, algorithm: str, num_segments: int = 24, pcie_gbs: float = 25.0):
     prof = profile_segments(g, algorithm, num_segments)
@@ -389,6 +438,10 @@ def main(argv=None):
     ap.add_argument("--segments", type=int, default=0,
                     help="also build a host/GPU schedule over this many segments")
     ap.add_argument("--cache", default="autotune_cache.json")
+    ap.add_argument("--tree-depth", type=int, default=-1,
+                    help="also run the tree microbenchmark (2^0..2^D segments)")
+    ap.add_argument("--yaml-dir", default="",
+                    help="write the segment/tree profiles here as the reference's YAML")
     args = ap.parse_args(argv)
     hc.check(hc.lib().hc_init(0))
     g = hc.HostGraph.generate(args.kind, args.scale, args.degree,
@@ -403,6 +456,13 @@ def main(argv=None):
         sched, info = tune_schedule(g, args.algorithm, args.segments)
         print(sched.text())
         print(json.dumps({k: v for k, v in info.items() if k != "profile"}))
+        if args.yaml_dir:
+            for path in write_yaml_profiles(info["profile"], g, args.yaml_dir):
+                print(path)
+    if args.tree_depth >= 0 and args.algorithm != "multi":
+        tree = profile_tree(g, args.algorithm, args.tree_depth)
+        for path in write_yaml_tree(tree, g, args.yaml_dir or "."):
+            print(path)
 
 
 if __name__ == "__main__":

@@ -309,23 +309,6 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
     return t;
 }
 
-// Last-workgroup detection: returns true in every thread of the workgroup
-// that finished last.  Every thread releases its own global writes before the
-// ticket is taken, and the last workgroup acquires, so all writes (stores and
-// atomics) of every workgroup are visible to it afterwards.
-__device__ __forceinline__ bool last_block_done(uint32_t *blocks_done) {
-    __shared__ int s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = atomicAdd(blocks_done, 1u);
-        s_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return s_last != 0;
-}
-
 // ------------------------------------------------------------------------
 // Runtime state (host)
 // ------------------------------------------------------------------------

@@ -101,3 +101,33 @@ def test_yaml_profiles_in_reference_layout(hc, at, tmp_path):
     assert hc.kernel_id(warp["config"]["kernel"]) == hc.KERNELS.index("SSSP GPU warp-red")
     seg0 = cpu["results"][0]["segments"][0]
     assert seg0["start"] == 0 and seg0["num_edges"] == segs[0]["edges"]
+
+
+def test_yaml_tree_in_reference_layout(hc, at, tmp_path):
+    """Tree profiles follow the reference's tree_res_t writer: one file per
+    device and kernel, `results` a list of `{depth, segments}` with 2^depth
+    equal-edge segments per layer."""
+    import yaml
+    g = hc.HostGraph.generate("kron", 8, 8)
+    layers = []
+    for depth in range(3):
+        bounds = g.equal_edge_ranges(1 << depth)
+        segs = []
+        for i in range(1 << depth):
+            a, b = int(bounds[i]), int(bounds[i + 1])
+            segs.append({"start": a, "end": b, "edges": int(g.index[b] - g.index[a]),
+                         "cpu_ms": 1.0 + i, "gpu": {"BFS GPU bucketed": (0.25, 0, 0)}})
+        layers.append({"algorithm": "bfs", "num_nodes": g.num_nodes, "segments": segs})
+    files = at.write_yaml_tree({"algorithm": "bfs", "num_nodes": g.num_nodes,
+                                "layers": layers}, g, str(tmp_path))
+    assert len(files) == 2
+    docs = {yaml.safe_load(open(f))["config"]["device"]: yaml.safe_load(open(f)) for f in files}
+    cpu = docs["Host CPU"]
+    assert [r["depth"] for r in cpu["results"]] == [0, 1, 2]
+    for r in cpu["results"]:
+        segs = r["segments"]
+        assert len(segs) == 1 << r["depth"]
+        assert segs[0]["start"] == 0 and segs[-1]["end"] == g.num_nodes
+        assert sum(x["num_edges"] for x in segs) == g.num_edges
+    gpu = docs["AMD Instinct MI355X"]
+    assert hc.kernel_id(gpu["config"]["kernel"]) == hc.KERNELS.index("BFS GPU bucketed")

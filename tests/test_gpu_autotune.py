@@ -62,3 +62,19 @@ def test_profiled_schedule_runs_exact(hc, gpu, oracle, at, kron14, algorithm):
     with hc.Hetero(g, hc.Schedule.create(blocks, 6)) as h:
         r2 = h.sssp(src) if algorithm == "sssp" else h.bfs(src)
     assert np.array_equal(r.values, r2.values)
+
+
+def test_tree_microbenchmark(hc, gpu, at, tmp_path):
+    """The tree microbenchmark profiles 2^0..2^d segments per layer on the
+    host and the GPU kernels and writes complete tree YAML files."""
+    import yaml
+    g = hc.HostGraph.generate("kron", 12, 16, weighted=True)
+    tree = at.profile_tree(g, "sssp", 2, shapes=[(0, 0), (64, 256)], repeats=1)
+    assert [len(l["segments"]) for l in tree["layers"]] == [1, 2, 4]
+    for layer in tree["layers"]:
+        assert all(s["cpu_ms"] > 0 and s["gpu"] for s in layer["segments"])
+    files = at.write_yaml_tree(tree, g, str(tmp_path))
+    assert len(files) >= 2
+    for f in files:
+        doc = yaml.safe_load(open(f))
+        assert [r["depth"] for r in doc["results"]] == [0, 1, 2]
