@@ -63,7 +63,7 @@ KNOB_SPACE = {
         "bu.g4_threshold": [8, 16, 32, 64],
         "bu.remaining": [0, 1, 4],
         "bu.stage_min": [0, 256, 1024, 4096, 1 << 20],
-        "bu.first": [0, 1],
+        "bu.first": [0, 1, 2],
         "push.chunk": [256, 512, 1024],
         "compact.grid": [128, 512, 2048],
     },

@@ -131,6 +131,28 @@ __device__ __forceinline__ void bu_vertex(const GraphView<OffT> &g,
         }
         return;
     }
+    if (G > 1 && first) {
+        // The group tests the first neighbour from the array (one broadcast
+        // load) and scans the row from its second edge only if that misses.
+        const int32_t vis = CHECK ? visited[uu] : -1;
+        const int32_t f = first[uu];
+        if (vis < 0) {
+            int32_t pr = f;
+            bool fnd = f >= 0 && bit_test(frontier, f);
+            if (!fnd) {
+                pr = -1;
+                fnd = bu_scan<G>(g, frontier, g.rbeg(uu) + OffT(1), g.rend(uu), lane, gbase, pr,
+                                 scanned);
+            }
+            if (lane == 0) {
+                scanned++;
+                found = fnd;
+                open = !fnd;
+                par = pr;
+            }
+        }
+        return;
+    }
     if (!CHECK || visited[uu] < 0) {
         int32_t pr = -1;
         bool f = bu_scan<G>(g, frontier, g.rbeg(uu), g.rend(uu), lane, gbase, pr, scanned);
@@ -199,7 +221,7 @@ __device__ __forceinline__ void bu_commit(const GraphView<OffT> &g,
 template <bool LOOP, typename OffT>
 __device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
                          int32_t *__restrict__ visited, int32_t *__restrict__ parents,
-                         const int32_t *__restrict__ first,
+                         const int32_t *__restrict__ first, const int32_t *__restrict__ first_g,
                          int32_t level, const Staged<uint32_t> &frontier,
                          uint32_t *__restrict__ next, int32_t *rem, uint32_t *rem_cnt,
                          uint32_t &cnt, unsigned long long &edges_acc, uint32_t &scanned) {
@@ -211,11 +233,11 @@ __device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
         bool found = false, open = false;
         int32_t u = 0, par = -1;
         if (c == 2)
-            bu_group<64>(p, c, k, g, visited, nullptr, frontier, found, open, u, par, scanned);
+            bu_group<64>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned);
         else if (c == 3)
-            bu_group<16>(p, c, k, g, visited, nullptr, frontier, found, open, u, par, scanned);
+            bu_group<16>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned);
         else if (c == 4)
-            bu_group<4>(p, c, k, g, visited, nullptr, frontier, found, open, u, par, scanned);
+            bu_group<4>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned);
         else if (c == 5)
             bu_group<1>(p, c, k, g, visited, first, frontier, found, open, u, par, scanned);
         bu_commit<LOOP>(g, visited, parents, level, next, rem, rem_cnt, found, open, u, par, cnt,
@@ -335,7 +357,8 @@ template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ depth,
     int32_t *__restrict__ parents, const int32_t *__restrict__ first, FrontierBufs fb,
-    StepCtl *ctl, RemLists rl, int32_t nstage, int32_t stage_min, int32_t lds_words) {
+    StepCtl *ctl, RemLists rl, int32_t nstage, int32_t stage_min, int32_t lds_words,
+    int first_all) {
     extern __shared__ uint32_t s_words[];
     __shared__ uint32_t s_rem;
     if (ctl->done) return;
@@ -373,8 +396,8 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
                 bu_list_sweep<1>(g, depth, parents, first, level, fr, next, list, len, rem, &s_rem,
                                  cnt, eacc, scanned);
         } else
-            bu_sweep<true>(p, g, depth, parents, first, level, fr, next, rem, &s_rem, cnt, eacc,
-                           scanned);
+            bu_sweep<true>(p, g, depth, parents, first, first_all ? first : nullptr, level, fr,
+                           next, rem, &s_rem, cnt, eacc, scanned);
         if (listing) {
             __syncthreads();
             if (threadIdx.x == 0) rl.len[nxt * gridDim.x + blockIdx.x] = s_rem;
@@ -561,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_fused_kernel(
     extern __shared__ uint32_t s_words[];
     uint32_t nnew = 0, scanned = 0;
     unsigned long long eacc = 0;
-    bu_sweep<false>(p, g, parents, nullptr, nullptr, 0, stage(frontier, nstage, s_words), next,
+    bu_sweep<false>(p, g, parents, nullptr, nullptr, nullptr, 0, stage(frontier, nstage, s_words), next,
                     nullptr, nullptr, nnew, eacc, scanned);
     unsigned long long s = block_sum_u64(nnew);
     if (threadIdx.x == 0 && s) atomicAdd(num_nodes, (int32_t)s);
@@ -751,13 +774,14 @@ struct BfsLaunch {
     int32_t nstage;
     int32_t stage_min;
     int32_t lds_words;
+    int first_all;
     KernelTimer *kt;
     static int launch(void *arg) {
         auto *a = static_cast<BfsLaunch *>(arg);
         a->kt->mark(a->s);
         bfs_step_kernel<OffT><<<a->grid, kBlock, a->lds_words * sizeof(uint32_t), a->s>>>(
             a->plan, a->g, a->depth, a->parents, a->first, a->fb, a->ctl, a->rl, a->nstage,
-            a->stage_min, a->lds_words);
+            a->stage_min, a->lds_words, a->first_all);
         a->kt->mark(a->s);
         bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
         compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
@@ -793,7 +817,7 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     const RemLists rl{ws->rem, reinterpret_cast<uint32_t *>(ws->rem + grid + 1)};
     BfsLaunch<OffT> L{ph->dev, v, depth, parents, tuning().bu_first ? ws->first : nullptr,
                       ws->f.fb, ws->f.ctl, rl, grid, s, nstage, tuning().bu_stage_min,
-                      lds_words, &ws->f.kt};
+                      lds_words, tuning().bu_first == 2, &ws->f.kt};
     const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
     long long launches = 0;

@@ -104,7 +104,8 @@ struct Tuning {
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
     int push_chunk = 256;  // edges per hub chunk of push epochs (>= 128)
     int bu_remaining = 1;  // BFS: lanes per vertex of unvisited-list sweeps (0: off)
-    int bu_first = 1;      // BFS: one-lane rows probe the first-neighbour array
+    int bu_first = 2;      // BFS: rows probe the first-neighbour array (1: one-lane
+                           // rows, 2: all rows)
     int bu_stage_min = 4096;  // BFS: unvisited lists shorter than this skip LDS staging
     int compact_grid = 4096;  // workgroups of the frontier compaction launch (capped)
     int kernel_events = 0; // 1: time every launch of a loop's main kernel

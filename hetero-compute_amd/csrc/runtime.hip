@@ -552,13 +552,16 @@ int hc_set_tuning(const char *key, int64_t value) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
                            !strcmp(key, "sssp.stage");
         const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph") ||
-                          !strcmp(key, "sssp.filter") || !strcmp(key, "bu.first");
+                          !strcmp(key, "sssp.filter");
+        if (!strcmp(key, "bu.first"))
+            HC_CHECK_ARG(value >= 0 && value <= 2, "bu.first must be 0, 1 or 2");
         if (!strcmp(key, "push.chunk"))
             HC_CHECK_ARG(value >= kMinChunkEdges, "push.chunk must be >= %d", (int)kMinChunkEdges);
         if (!strcmp(key, "bu.remaining"))
             HC_CHECK_ARG(value == 0 || value == 1 || value == 4 || value == 16,
                          "bu.remaining must be 0, 1, 4 or 16");
         const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining") ||
+                             !strcmp(key, "bu.first") ||
                              !strcmp(key, "bu.stage_min");
         HC_CHECK_ARG(value >= (zero_ok ? 0 : 1) && value <= (flag ? 1 : 1 << 20),
                      "%s out of range", key);

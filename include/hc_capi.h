@@ -223,9 +223,9 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *                          staging (4096)
  *   compact.grid           workgroups of the frontier compaction launch, at
  *                          most the step grid (4096)
- *   bu.first               1: one-lane bottom-up rows read their first
- *                          neighbour from a per-vertex array (n int32 per
- *                          graph) instead of the edge array (1)
+ *   bu.first               bottom-up rows test their first neighbour from a
+ *                          per-vertex array (n int32 per graph) before the
+ *                          edge array: 1 one-lane rows, 2 all rows, 0 off (2)
  *   bu.remaining           a bottom-up epoch after another sweeps only the
  *                          vertices the previous one left unvisited, with
  *                          this many lanes per vertex (1, 4 or 16; 0: off)

@@ -109,6 +109,34 @@ def test_sssp_filter_and_direction_exact(hc, gpu, oracle, kron16, uniform16, fil
         hc.reset_tuning()
 
 
+@pytest.mark.parametrize("first,remaining,stage_min", [(0, 0, 0), (2, 1, 4096), (2, 4, 0),
+                                                     (1, 16, 1 << 20)])
+def test_bfs_bottom_up_variants_exact(hc, gpu, oracle, kron16, uniform16, first, remaining,
+                                      stage_min):
+    """Bottom-up epochs with and without the first-neighbour probes (one-lane
+    rows or all rows), unvisited-list sweeps of 1/4/16 lanes and list-length
+    staging thresholds: depths bit-exact, parents a valid BFS tree, and the
+    direction-optimizing epoch sequence of bfs_do_cpu."""
+    try:
+        hc.set_tuning("bu.first", first)
+        hc.set_tuning("bu.remaining", remaining)
+        hc.set_tuning("bu.stage_min", stage_min)
+        for g in (kron16, uniform16):
+            with hc.DeviceGraph(g) as dg:
+                for s in map(int, g.pick_sources(3)):
+                    depth = oracle.bfs_depth(g.index, g.neighbors, s)
+                    b = hc.bfs(dg, s)
+                    assert np.array_equal(b.values, depth), s
+                    assert oracle.verify_parents(depth, b.parents, s) == 0
+                    _, _, dirs = oracle.bfs_do(g.index, g.neighbors, s)
+                    assert b.stats["epochs"] == len(dirs)
+                    assert b.stats["pull_epochs"] == int(dirs.sum())
+                    b1 = hc.bfs(dg, s, direction=1)
+                    assert np.array_equal(b1.values, depth), s
+    finally:
+        hc.reset_tuning()
+
+
 @pytest.mark.parametrize("mapping", [0, 1, 2, 3])
 def test_sssp_pull_driver_mappings(hc, gpu, oracle, kron16, mapping):
     """sssp_pull_gpu semantics (pull epochs from init_dist until no change)
