@@ -382,15 +382,21 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
     const uint32_t *bm = three_bitmaps ? fb.bm[k % 3] : fb.bm[k & 1];
     const int words = fb.words;
     const int64_t ck = ctl->chunk;
-    const int per_block = (words + gridDim.x - 1) / gridDim.x;
-    const int w0 = blockIdx.x * per_block;
-    const int w1 = min(words, w0 + per_block);
-    __shared__ uint32_t s_cnt[5], s_base[5];
-    for (int base_w = w0; base_w < w1; base_w += kBlock) {
+    // Word w belongs to workgroup w % grid, so the hub words at the start
+    // of a degree-sorted bitmap spread over many workgroups.
+    const int stride = gridDim.x;
+    const int rounds = (words - (int)blockIdx.x + stride * kBlock - 1) / (stride * kBlock);
+    constexpr uint32_t kHubList = 256;
+    __shared__ uint32_t s_cnt[5], s_base[5], s_nhub;
+    __shared__ int32_t s_hub_v[kHubList];
+    __shared__ uint32_t s_hub_at[kHubList], s_hub_pre[kHubList + 1];
+    __shared__ long long s_hub_b[kHubList], s_hub_e[kHubList];
+    for (int r = 0; r < rounds; r++) {
         if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0;
+        if (threadIdx.x == 0) s_nhub = 0;
         __syncthreads();
-        const int w = base_w + threadIdx.x;
-        uint32_t bits = (w < w1) ? bm[w] : 0u;
+        const int w = (int)blockIdx.x + (r * kBlock + (int)threadIdx.x) * stride;
+        uint32_t bits = (w < words) ? bm[w] : 0u;
         // pass 1: count per class (class 3 counts chunks; slot 4 hubs)
         uint32_t my[5] = {0, 0, 0, 0, 0};
         for (uint32_t x = bits; x; x &= x - 1) {
@@ -412,7 +418,9 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
                                       ? atomicAdd(&ctl->cnt[p][threadIdx.x], s_cnt[threadIdx.x])
                                       : 0;
         __syncthreads();
-        // pass 2: write queue entries and chunk descriptors
+        // pass 2: queue entries; hubs are listed with their chunk range and
+        // their descriptors written by the whole workgroup below (serially
+        // by the owning thread only past the list's capacity).
         for (uint32_t x = bits; x; x &= x - 1) {
             int32_t v = w * 32 + __ffs(x) - 1;
             OffT b = g.rbeg(v), e = g.rend(v);
@@ -421,11 +429,44 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
             if (c < 3) {
                 fb.q[p][c][s_base[c] + off[c]++] = v;
             } else {
-                for (int64_t cb = int64_t(b); cb < int64_t(e); cb += ck) {
-                    int64_t ce = cb + ck < int64_t(e) ? cb + ck : int64_t(e);
-                    fb.ch[p][s_base[3] + off[3]++] = HubChunk{v, 0, cb, ce};
+                const uint32_t at = s_base[3] + off[3];
+                off[3] += (uint32_t)((d + ck - 1) / ck);
+                const uint32_t h = atomicAdd(&s_nhub, 1u);
+                if (h < kHubList) {
+                    s_hub_v[h] = v;
+                    s_hub_at[h] = at;
+                    s_hub_b[h] = (long long)b;
+                    s_hub_e[h] = (long long)e;
+                } else {
+                    for (int64_t cb = int64_t(b); cb < int64_t(e); cb += ck) {
+                        int64_t ce = cb + ck < int64_t(e) ? cb + ck : int64_t(e);
+                        fb.ch[p][at + (uint32_t)((cb - int64_t(b)) / ck)] = HubChunk{v, 0, cb, ce};
+                    }
                 }
             }
+        }
+        __syncthreads();
+        const uint32_t nh = min(s_nhub, kHubList);
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (uint32_t h = 0; h < nh; h++) {
+                s_hub_pre[h] = t;
+                t += (uint32_t)((s_hub_e[h] - s_hub_b[h] + ck - 1) / ck);
+            }
+            s_hub_pre[nh] = t;
+        }
+        __syncthreads();
+        // Chunk j of the listed hubs: hub h with pre[h] <= j < pre[h+1].
+        for (uint32_t j = threadIdx.x; j < s_hub_pre[nh]; j += blockDim.x) {
+            uint32_t lo = 0, hi = nh;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (s_hub_pre[mid] <= j) lo = mid; else hi = mid;
+            }
+            const uint32_t q = j - s_hub_pre[lo];
+            const long long cb = s_hub_b[lo] + (long long)q * ck;
+            const long long ce = cb + ck < s_hub_e[lo] ? cb + ck : s_hub_e[lo];
+            fb.ch[p][s_hub_at[lo] + q] = HubChunk{s_hub_v[lo], 0, cb, ce};
         }
         __syncthreads();
     }
