@@ -97,6 +97,7 @@ struct Tuning {
     int grid_per_cu = 8;   // workgroups per CU of the fused / step launches
     int sssp_alpha = 4;    // SSSP goes pull when frontier edges > m / alpha
     int sssp_grid_per_cu = 4;  // workgroups per CU of the SSSP loop (at most)
+    int sssp_dist16 = 0;   // SSSP pulls gather a 16-bit mirror of the distances
     int sssp_filter = 0;   // SSSP pulls gather only neighbours changed last epoch
     int sssp_stage = 0;    // per-vertex entries staged in LDS by SSSP pull sweeps
     int64_t sssp_block_threshold = 4096;  // SSSP: degree above which a WG takes a row

@@ -371,12 +371,23 @@ __device__ __forceinline__ void trace_epoch(StepCtl *c, int k, int mode,
 // cheap (tuning knob compact.grid: workgroups, at most the step grid).
 inline int compact_grid(int step_grid) { return std::max(1, std::min(step_grid, tuning().compact_grid)); }
 
+// SSSP with a 16-bit distance mirror (`d16`): before every pull epoch the
+// same launch rebuilds the mirror from the int32 distances.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
                                                          FrontierBufs fb,
                                                          StepCtl *ctl,
-                                                         int three_bitmaps) {
-    if (ctl->done || !ctl->need_compact) return;
+                                                         int three_bitmaps,
+                                                         const int32_t *dist = nullptr,
+                                                         uint16_t *d16 = nullptr) {
+    if (ctl->done) return;
+    if (d16 && ctl->mode == kPull) {
+        for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += gridDim.x * kBlock) {
+            const int32_t d = dist[i];
+            d16[i] = d >= HC_INF_DIST ? uint16_t(0xFFFF) : (d >= 0xFFFE ? uint16_t(0xFFFE) : uint16_t(d));
+        }
+    }
+    if (!ctl->need_compact) return;
     const int k = ctl->epoch;
     const int p = k & 1;
     const uint32_t *bm = three_bitmaps ? fb.bm[k % 3] : fb.bm[k & 1];

@@ -85,6 +85,7 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
     } i32[] = {{"grid_per_cu", &t.grid_per_cu},
                {"sssp.alpha", &t.sssp_alpha},
                {"sssp.filter", &t.sssp_filter},
+               {"sssp.dist16", &t.sssp_dist16},
                {"sssp.stage", &t.sssp_stage},
                {"sssp.grid_per_cu", &t.sssp_grid_per_cu},
                {"pull.stage", &t.stage_pull},
@@ -552,7 +553,7 @@ int hc_set_tuning(const char *key, int64_t value) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
                            !strcmp(key, "sssp.stage");
         const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph") ||
-                          !strcmp(key, "sssp.filter");
+                          !strcmp(key, "sssp.filter") || !strcmp(key, "sssp.dist16");
         if (!strcmp(key, "bu.first"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "bu.first must be 0, 1 or 2");
         if (!strcmp(key, "push.chunk"))

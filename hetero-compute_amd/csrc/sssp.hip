@@ -37,9 +37,30 @@ __device__ __forceinline__ bool changed_bit(const Staged<uint32_t> &bm, int32_t 
 // FILTER: only neighbours set in `changed` (the vertices whose distance
 // changed in the previous epoch) are gathered; the others cannot lower the
 // row below its current value.  `gathered` counts the gathers.
-template <int G, bool FILTER, typename OffT>
+// Neighbour distances through a 16-bit mirror: half the bytes per gathered
+// entry, so twice as many vertices fit each cache.  0xFFFF: unreached;
+// 0xFFFE: too large for 16 bits, read the int32 array.  The mirror is
+// rebuilt from the int32 distances before every pull epoch and written
+// during the sweep by whoever lowers an entry; a row lowered concurrently by
+// several hub chunks may keep a stale (larger) mirror entry until the next
+// rebuild, which costs nothing in exactness: the row is marked changed, so
+// another epoch runs, and it starts from a fresh mirror.
+struct Dist16Reader {
+    const uint16_t *h;
+    const int32_t *full;
+    __device__ __forceinline__ int32_t operator[](int32_t v) const {
+        const uint32_t x = h[v];
+        return x < 0xFFFEu ? int32_t(x) : (x == 0xFFFFu ? HC_INF_DIST : full[v]);
+    }
+};
+
+__device__ __forceinline__ uint16_t dist16(int32_t d) {
+    return d >= HC_INF_DIST ? uint16_t(0xFFFF) : (d >= 0xFFFE ? uint16_t(0xFFFE) : uint16_t(d));
+}
+
+template <int G, bool FILTER, typename DR, typename OffT>
 __device__ __forceinline__ int32_t row_min(const GraphView<OffT> &g,
-                                           const Staged<int32_t> &dist,
+                                           const DR &dist,
                                            const Staged<uint32_t> &changed,
                                            OffT b, OffT e, int lane, uint32_t &gathered) {
     int32_t m = HC_INF_DIST;
@@ -104,12 +125,12 @@ __device__ __forceinline__ int32_t block_min(int32_t m) {
 // Fused degree-bucketed pull sweep
 // ---------------------------------------------------------------------------
 
-template <int G, bool FILTER, typename OffT>
+template <int G, bool FILTER, typename DR, typename OffT>
 __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
                                            const GraphView<OffT> &g,
-                                           const Staged<int32_t> &rd,
+                                           const DR &rd,
                                            const Staged<uint32_t> &changed,
-                                           int32_t *__restrict__ dist,
+                                           int32_t *__restrict__ dist, uint16_t *d16,
                                            bool &chg, int32_t &u, uint32_t &gathered) {
     constexpr int PER = kBlock / G;
     const int gid = threadIdx.x / G, lane = threadIdx.x % G;
@@ -122,6 +143,7 @@ __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
             int32_t cur = dist[uu];
             if (m < cur) {
                 dist[uu] = m;
+                if (d16) d16[uu] = dist16(m);
                 chg = true;
             }
             u = uu;
@@ -140,10 +162,10 @@ __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
 // decrease, and an epoch without any change had nothing stale to read.
 // FILTER: gather only neighbours in `changed` (see row_min); `gathered`
 // counts the gathers.
-template <bool LOOP, bool FILTER, typename OffT>
+template <bool LOOP, bool FILTER, typename DR, typename OffT>
 __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
-                                const Staged<int32_t> &rd, const Staged<uint32_t> &changed,
-                                int32_t *__restrict__ dist, uint32_t *next_bm,
+                                const DR &rd, const Staged<uint32_t> &changed,
+                                int32_t *__restrict__ dist, uint16_t *d16, uint32_t *next_bm,
                                 uint32_t &cnt, unsigned long long &edges_acc,
                                 uint32_t &gathered) {
     const int total = p.unit_start[kNumClasses];
@@ -173,24 +195,30 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
                     if (m < dist[uu]) {
                         int32_t old = atomicMin(&dist[uu], m);
                         chg = m < old;
+                        // Two chunks lowering the same hub may store their
+                        // mirror values out of order; a stale (larger) entry
+                        // only weakens this epoch, and the hub is marked
+                        // changed, so the next epoch starts from a rebuild.
+                        if (chg && d16) d16[uu] = dist16(m);
                     }
                 } else {
                     int32_t cur = dist[uu];
                     if (m < cur) {
                         dist[uu] = m;
+                        if (d16) d16[uu] = dist16(m);
                         chg = true;
                     }
                 }
                 u = uu;
             }
         } else if (c == 2) {
-            pull_group<64, FILTER>(p, c, k, g, rd, changed, dist, chg, u, gathered);
+            pull_group<64, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
         } else if (c == 3) {
-            pull_group<16, FILTER>(p, c, k, g, rd, changed, dist, chg, u, gathered);
+            pull_group<16, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
         } else if (c == 4) {
-            pull_group<4, FILTER>(p, c, k, g, rd, changed, dist, chg, u, gathered);
+            pull_group<4, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
         } else {
-            pull_group<1, FILTER>(p, c, k, g, rd, changed, dist, chg, u, gathered);
+            pull_group<1, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
         }
         if (LOOP) {
             if (wave_set_bit(next_bm, u, chg)) {
@@ -283,7 +311,7 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
 // epochs stage their output there (`lds_entries` entries in all).
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
-    PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, FrontierBufs fb,
+    PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, uint16_t *d16, FrontierBufs fb,
     StepCtl *ctl, int32_t nstage, int32_t lds_entries) {
     extern __shared__ int32_t s_stage[];
     if (ctl->done) return;
@@ -300,11 +328,23 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     uint32_t cnt = 0, gathered = 0;
     unsigned long long eacc = 0;
     if (mode == kPull) {
-        const Staged<int32_t> rd = stage(dist, nstage, s_stage);
-        if (ctl->filter)
-            sssp_pull_sweep<true, true>(p, g, rd, unstaged(rbm), dist, wbm, cnt, eacc, gathered);
-        else
-            sssp_pull_sweep<true, false>(p, g, rd, unstaged(rbm), dist, wbm, cnt, eacc, gathered);
+        if (d16) {
+            const Dist16Reader rd{d16, dist};
+            if (ctl->filter)
+                sssp_pull_sweep<true, true>(p, g, rd, unstaged(rbm), dist, d16, wbm, cnt, eacc,
+                                            gathered);
+            else
+                sssp_pull_sweep<true, false>(p, g, rd, unstaged(rbm), dist, d16, wbm, cnt, eacc,
+                                             gathered);
+        } else {
+            const Staged<int32_t> rd = stage(dist, nstage, s_stage);
+            if (ctl->filter)
+                sssp_pull_sweep<true, true>(p, g, rd, unstaged(rbm), dist, nullptr, wbm, cnt,
+                                            eacc, gathered);
+            else
+                sssp_pull_sweep<true, false>(p, g, rd, unstaged(rbm), dist, nullptr, wbm, cnt,
+                                             eacc, gathered);
+        }
     } else {
         // Output staged in the dynamic LDS (unused by push epochs).
         __shared__ uint32_t s_out;
@@ -374,10 +414,13 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
     GraphView<OffT> g, int32_t *__restrict__ dist,
     const int32_t *__restrict__ init_dist, FrontierBufs fb, StepCtl *ctl,
     int32_t src, int direction, int alpha, int max_epochs, long long m,
-    long long sweep_bytes, int grid, int filter, int chunk) {
+    long long sweep_bytes, int grid, int filter, int chunk, uint16_t *d16) {
     const int stride = gridDim.x * kBlock;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride)
-        dist[i] = init_dist ? init_dist[i] : (i == src ? 0 : HC_INF_DIST);
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride) {
+        const int32_t d = init_dist ? init_dist[i] : (i == src ? 0 : HC_INF_DIST);
+        dist[i] = d;
+        if (d16) d16[i] = dist16(d);
+    }
     for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words; w += stride) {
         // Epoch 0 reads bitmap 0 as the set changed before it: the source.
         fb.bm[0][w] = (!init_dist && w == (src >> 5)) ? (1u << (src & 31)) : 0u;
@@ -503,7 +546,7 @@ __global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
     unsigned long long eacc = 0;
     uint32_t gathered = 0;
     sssp_pull_sweep<false, false>(p, g, stage(dist, nstage, s_stage), unstaged<uint32_t>(nullptr),
-                                  dist, nullptr, upd, eacc, gathered);
+                                  dist, nullptr, nullptr, upd, eacc, gathered);
     unsigned long long s = block_sum_u64(upd);
     if (threadIdx.x == 0 && s) atomicAdd(updated, (int32_t)s);
 }
@@ -515,12 +558,14 @@ __global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
 
 struct SsspWs {
     FrontierWs f;
+    uint16_t *d16 = nullptr;  // 16-bit distance mirror, allocated on first use
 };
 
 void free_sssp_ws(hc_graph_s *g) {
     if (!g->ws_sssp) return;
     auto *ws = static_cast<SsspWs *>(g->ws_sssp);
     frontier_ws_free(&ws->f);
+    if (ws->d16) hipFree(ws->d16);
     delete ws;
     g->ws_sssp = nullptr;
 }
@@ -605,6 +650,7 @@ struct SsspLaunch {
     PlanDev plan;
     GraphView<OffT> g;
     int32_t *dist;
+    uint16_t *d16;  // 16-bit distance mirror (sssp.dist16), or nullptr
     FrontierBufs fb;
     StepCtl *ctl;
     int grid;
@@ -616,10 +662,11 @@ struct SsspLaunch {
         auto *a = static_cast<SsspLaunch *>(arg);
         a->kt->mark(a->s);
         sssp_step_kernel<OffT><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t), a->s>>>(
-            a->plan, a->g, a->dist, a->fb, a->ctl, a->nstage, a->lds_entries);
+            a->plan, a->g, a->dist, a->d16, a->fb, a->ctl, a->nstage, a->lds_entries);
         a->kt->mark(a->s);
         sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
-        compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
+        compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1,
+                                                                          a->dist, a->d16);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "SSSP epoch launch failed");
         return HC_OK;
     }
@@ -634,6 +681,12 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const PlanHost *ph = nullptr;
     HC_TRY(get_plan(g, 0, g->n, sssp_plan_config(), &ph));
     GraphView<OffT> v = view<OffT>(g);
+    uint16_t *d16 = nullptr;
+    if (tuning().sssp_dist16) {
+        if (!ws->d16 && hipMalloc(&ws->d16, sizeof(uint16_t) * std::max(g->n, 1)) != hipSuccess)
+            return fail(HC_ERR_OOM, "16-bit distance mirror allocation failed");
+        d16 = ws->d16;
+    }
     const long long sweep = sssp_sweep_bytes(g, ph);
     const int32_t nstage = std::min<int32_t>(tuning().sssp_stage, g->n);
     const int32_t lds_entries = std::max<int32_t>(nstage, (int32_t)kPushStage);
@@ -643,10 +696,10 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
-        max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter, tuning().push_chunk);
+        max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter, tuning().push_chunk, d16);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
-    SsspLaunch<OffT> L{ph->dev, v, dist, ws->f.fb, ws->f.ctl, grid, s, nstage, lds_entries,
+    SsspLaunch<OffT> L{ph->dev, v, dist, d16, ws->f.fb, ws->f.ctl, grid, s, nstage, lds_entries,
                        &ws->f.kt};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
     const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;

@@ -205,6 +205,8 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   sssp.grid_per_cu       cap on workgroups per CU of the SSSP loop (4)
  *   sssp.filter            1: SSSP pull epochs gather only neighbours whose
  *                          distance changed in the previous epoch (0)
+ *   sssp.dist16            1: SSSP pull epochs gather a 16-bit mirror of the
+ *                          distances (rebuilt before each pull; exact) (0)
  *   sssp.stage             distances staged in LDS by SSSP pull sweeps (0)
  *   sssp.block_threshold   SSSP: degree above which a workgroup takes a row
  *                          (4096; the other pull.* classes are shared)

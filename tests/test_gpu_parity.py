@@ -137,6 +137,42 @@ def test_bfs_bottom_up_variants_exact(hc, gpu, oracle, kron16, uniform16, first,
         hc.reset_tuning()
 
 
+def test_sssp_dist16_mirror_exact(hc, gpu, oracle, kron16, uniform16):
+    """Pull gathers through the 16-bit distance mirror: bit-exact on both
+    scale-16 graphs in every direction, on the golden graphs, and on a
+    weighted path whose distances pass 65534 (the int32 fallback)."""
+    n = 1000
+    idx = np.zeros(n + 1, np.int64)
+    nb, wt = [], []
+    for u in range(n):
+        for v in (u - 1, u + 1):
+            if 0 <= v < n:
+                nb.append(v)
+                wt.append(255 if (u + v) % 3 else 7)
+        idx[u + 1] = len(nb)
+    path = hc.HostGraph.from_csr(idx, np.array(nb, np.int32), np.array(wt, np.int32))
+    try:
+        hc.set_tuning("sssp.dist16", 1)
+        for g in (kron16, uniform16, path):
+            srcs = [0, n // 2] if g is path else list(map(int, g.pick_sources(3)))
+            with hc.DeviceGraph(g) as dg:
+                for s in srcs:
+                    ref = oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, s)
+                    if g is path:
+                        assert ref.max() > 0xFFFE
+                    for direction in (0, 1, 2):
+                        r = hc.sssp(dg, s, direction=direction)
+                        assert np.array_equal(r.values, ref), (s, direction)
+        for name in golden_files():
+            fx = load_golden(name)
+            with hc.DeviceGraph(_graph(hc, fx)) as dg:
+                for i, s in enumerate(fx["sources"]):
+                    r = hc.sssp(dg, int(s), direction=1)
+                    assert np.array_equal(r.values, fx["sssp"][i]), (name, int(s))
+    finally:
+        hc.reset_tuning()
+
+
 @pytest.mark.parametrize("mapping", [0, 1, 2, 3])
 def test_sssp_pull_driver_mappings(hc, gpu, oracle, kron16, mapping):
     """sssp_pull_gpu semantics (pull epochs from init_dist until no change)
