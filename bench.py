@@ -141,9 +141,13 @@ def class_sweep(hc, kind, scale, degree, seed, reps=3):
         dist = hc.DeviceArray.from_host(hc.sssp(dg, s).values)
         deg = np.diff(g.index)
         t = hc.tuning()
-        thr = [("hub chunks", t["pull.hub_threshold"]), ("workgroup", t["pull.block_threshold"]),
-               ("wavefront", t["pull.wave_threshold"]), ("16 lanes", t["pull.g16_threshold"]),
-               ("4 lanes", t["pull.g4_threshold"]), ("1 lane", 0)]
+        # The SSSP plan's class thresholds (sssp_plan_config).
+        wave = t["pull.wave_threshold"]
+        block = max(t["sssp.block_threshold"], wave)
+        g16 = min(t["sssp.g16_threshold"], wave)
+        thr = [("hub chunks", max(t["pull.hub_threshold"], block)), ("workgroup", block),
+               ("wavefront", wave), ("16 lanes", g16),
+               ("4 lanes", min(t["sssp.g4_threshold"], g16)), ("1 lane", 0)]
         lo = 0
         ranges = []
         for name, T in thr:

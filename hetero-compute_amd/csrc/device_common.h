@@ -95,12 +95,16 @@ struct PlanConfig {
 // caller; read when plans and workspaces are built).
 struct Tuning {
     int grid_per_cu = 8;   // workgroups per CU of the fused / step launches
-    int sssp_alpha = 4;    // SSSP goes pull when frontier edges > m / alpha
+    int sssp_alpha = 0;    // SSSP goes pull when frontier edges > m / alpha (0: by
+                           // graph shape, 4 skewed / 2 flat)
     int sssp_grid_per_cu = 4;  // workgroups per CU of the SSSP loop (at most)
-    int sssp_dist16 = 0;   // SSSP pulls gather a 16-bit mirror of the distances
+    int sssp_dist16 = 2;   // SSSP pulls gather a 16-bit mirror of the distances
+                           // (0 off, 1 on, 2: on for flat graphs)
     int sssp_filter = 0;   // SSSP pulls gather only neighbours changed last epoch
     int sssp_stage = 0;    // per-vertex entries staged in LDS by SSSP pull sweeps
     int64_t sssp_block_threshold = 4096;  // SSSP: degree above which a WG takes a row
+    int64_t sssp_g16_threshold = 48;      // SSSP: ... 16 lanes take a row
+    int64_t sssp_g4_threshold = 8;        // SSSP: ... 4 lanes take a row (else 1)
     int stage_pull = 8192; // per-vertex entries staged in LDS by pull sweeps
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
     int push_chunk = 256;  // edges per hub chunk of push epochs (>= 128)
@@ -134,6 +138,8 @@ Tuning &tuning();
 // workgroup threshold.
 inline PlanConfig sssp_plan_config() {
     PlanConfig c = tuning().pull;
+    c.g16_threshold = std::min<int64_t>(tuning().sssp_g16_threshold, c.wave_threshold);
+    c.g4_threshold = std::min<int64_t>(tuning().sssp_g4_threshold, c.g16_threshold);
     c.block_threshold = std::max<int64_t>(tuning().sssp_block_threshold, c.wave_threshold);
     if (c.hub_threshold < c.block_threshold) c.hub_threshold = c.block_threshold;
     return c;
@@ -351,6 +357,7 @@ struct hc_graph_s {
     int32_t *d_wt = nullptr;
     size_t bytes = 0;
     std::vector<int64_t> h_off;  // local offsets on the host (planning)
+    double hub_share = -1.0;     // edge share of the top 1% rows by degree (lazy)
     std::map<std::tuple<int32_t, int32_t, hc::PlanConfig>, hc::PlanHost> plans;
     std::mutex mu;
     // Per-algorithm workspaces (allocated on first use, freed with the graph).

@@ -63,6 +63,8 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
         int64_t *p;
     } i64[] = {
         {"sssp.block_threshold", &t.sssp_block_threshold},
+        {"sssp.g16_threshold", &t.sssp_g16_threshold},
+        {"sssp.g4_threshold", &t.sssp_g4_threshold},
         {"pull.hub_threshold", &t.pull.hub_threshold},
         {"pull.hub_chunk", &t.pull.hub_chunk},
         {"pull.block_threshold", &t.pull.block_threshold},
@@ -553,7 +555,9 @@ int hc_set_tuning(const char *key, int64_t value) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
                            !strcmp(key, "sssp.stage");
         const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph") ||
-                          !strcmp(key, "sssp.filter") || !strcmp(key, "sssp.dist16");
+                          !strcmp(key, "sssp.filter");
+        if (!strcmp(key, "sssp.dist16"))
+            HC_CHECK_ARG(value >= 0 && value <= 2, "sssp.dist16 must be 0, 1 or 2");
         if (!strcmp(key, "bu.first"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "bu.first must be 0, 1 or 2");
         if (!strcmp(key, "push.chunk"))
@@ -562,7 +566,8 @@ int hc_set_tuning(const char *key, int64_t value) {
             HC_CHECK_ARG(value == 0 || value == 1 || value == 4 || value == 16,
                          "bu.remaining must be 0, 1, 4 or 16");
         const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining") ||
-                             !strcmp(key, "bu.first") ||
+                             !strcmp(key, "bu.first") || !strcmp(key, "sssp.alpha") ||
+                             !strcmp(key, "sssp.dist16") ||
                              !strcmp(key, "bu.stage_min");
         HC_CHECK_ARG(value >= (zero_ok ? 0 : 1) && value <= (flag ? 1 : 1 << 20),
                      "%s out of range", key);

@@ -15,6 +15,8 @@
 //    frontier's degree sum exceeds m/alpha) and stops the loop when nothing
 //    changed, so the host only enqueues launches.
 #include <algorithm>
+#include <functional>
+#include <vector>
 #include <cstring>
 
 #include "frontier.h"
@@ -672,6 +674,30 @@ struct SsspLaunch {
     }
 };
 
+// Share of all edges held by the 1% highest-degree rows (cached; the caller
+// holds g->mu).  Skewed graphs (Kronecker: about half) and flat ones
+// (uniform: about 1%) want different push/pull switch points: a push epoch
+// on a flat graph relaxes edges whose targets are spread over the whole
+// distance array, so its atomics cost far more per edge.
+static double hub_share(hc_graph_s *g) {
+    if (g->hub_share >= 0.0) return g->hub_share;
+    const int32_t rows = g->rows;
+    if (rows <= 0 || g->m <= 0) return g->hub_share = 0.0;
+    const int32_t k = std::max<int32_t>(1, rows / 100);
+    int64_t top = 0;
+    if (g->sorted) {
+        top = g->h_off[k] - g->h_off[0];
+    } else {
+        std::vector<int64_t> deg(rows);
+        for (int32_t i = 0; i < rows; i++) deg[i] = g->h_off[i + 1] - g->h_off[i];
+        std::nth_element(deg.begin(), deg.begin() + (k - 1), deg.end(), std::greater<int64_t>());
+        for (int32_t i = 0; i < k; i++) top += deg[i];
+    }
+    return g->hub_share = double(top) / double(g->m);
+}
+
+static bool flat_graph(hc_graph_s *g) { return hub_share(g) < 0.05; }
+
 template <typename OffT>
 static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
                          int32_t *dist, int direction, int alpha,
@@ -681,8 +707,9 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const PlanHost *ph = nullptr;
     HC_TRY(get_plan(g, 0, g->n, sssp_plan_config(), &ph));
     GraphView<OffT> v = view<OffT>(g);
+    if (alpha <= 0) alpha = tuning().sssp_alpha > 0 ? tuning().sssp_alpha : (flat_graph(g) ? 2 : 4);
     uint16_t *d16 = nullptr;
-    if (tuning().sssp_dist16) {
+    if (tuning().sssp_dist16 == 1 || (tuning().sssp_dist16 == 2 && flat_graph(g))) {
         if (!ws->d16 && hipMalloc(&ws->d16, sizeof(uint16_t) * std::max(g->n, 1)) != hipSuccess)
             return fail(HC_ERR_OOM, "16-bit distance mirror allocation failed");
         d16 = ws->d16;
@@ -763,7 +790,7 @@ int hc_sssp(hc_graph_t g, int32_t source, int32_t *dist,
     HC_CHECK_ARG(source >= 0 && source < g->n, "source %d outside [0,%d)",
                  source, g->n);
     int direction = opts ? opts->direction : 0;
-    int alpha = opts && opts->alpha > 0 ? opts->alpha : tuning().sssp_alpha;
+    int alpha = opts && opts->alpha > 0 ? opts->alpha : 0;  // 0: the loop's default
     int max_epochs = opts ? opts->max_epochs : 0;
     HC_CHECK_ARG(direction >= 0 && direction <= 2, "direction must be 0, 1 or 2");
     HC_TRY(ensure_init());

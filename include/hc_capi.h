@@ -201,15 +201,20 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  * src/kernels/heterogeneous/sssp.cuh:180-278).  Keys:
  *   grid_per_cu            workgroups per CU of full-device launches (8); the
  *                          loops launch at most as many as fit at once
- *   sssp.alpha             SSSP pull when frontier edges > m/alpha (4)
+ *   sssp.alpha             SSSP pull when frontier edges > m/alpha; 0: by graph
+ *                          shape, 4 when the top 1% rows hold >= 5% of the
+ *                          edges (skewed), else 2 (0)
  *   sssp.grid_per_cu       cap on workgroups per CU of the SSSP loop (4)
  *   sssp.filter            1: SSSP pull epochs gather only neighbours whose
  *                          distance changed in the previous epoch (0)
  *   sssp.dist16            1: SSSP pull epochs gather a 16-bit mirror of the
- *                          distances (rebuilt before each pull; exact) (0)
+ *                          distances (rebuilt before each pull; exact); 2: on
+ *                          for flat graphs only (2)
  *   sssp.stage             distances staged in LDS by SSSP pull sweeps (0)
  *   sssp.block_threshold   SSSP: degree above which a workgroup takes a row
- *                          (4096; the other pull.* classes are shared)
+ *                          (4096; hub and wavefront classes are pull.*'s)
+ *   sssp.g16_threshold / sssp.g4_threshold   SSSP: degree above which 16 / 4
+ *                          lanes take a row (48 / 8)
  *   pull.hub_threshold     degree above which a row is split into chunks
  *   pull.hub_chunk         edges per chunk (>= 256)
  *   pull.block_threshold   degree above which a workgroup takes a row
