@@ -310,8 +310,9 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
 // ---------------------------------------------------------------------------
 
 // `nstage` hub distances are staged in dynamic LDS for pull epochs; push
-// epochs stage their output there (`lds_entries` entries in all).
-template <typename OffT>
+// epochs stage their output there (`lds_entries` entries in all).  D16: pull
+// gathers read the 16-bit mirror `d16` instead.
+template <typename OffT, bool D16>
 __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, uint16_t *d16, FrontierBufs fb,
     StepCtl *ctl, int32_t nstage, int32_t lds_entries) {
@@ -330,7 +331,9 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     uint32_t cnt = 0, gathered = 0;
     unsigned long long eacc = 0;
     if (mode == kPull) {
-        if (d16) {
+        // One gather path per instantiation: both in one kernel would make
+        // it hold the registers of both (and spill).
+        if (D16) {
             const Dist16Reader rd{d16, dist};
             if (ctl->filter)
                 sssp_pull_sweep<true, true>(p, g, rd, unstaged(rbm), dist, d16, wbm, cnt, eacc,
@@ -663,8 +666,13 @@ struct SsspLaunch {
     static int launch(void *arg) {
         auto *a = static_cast<SsspLaunch *>(arg);
         a->kt->mark(a->s);
-        sssp_step_kernel<OffT><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t), a->s>>>(
-            a->plan, a->g, a->dist, a->d16, a->fb, a->ctl, a->nstage, a->lds_entries);
+        if (a->d16)
+            sssp_step_kernel<OffT, true><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t), a->s>>>(
+                a->plan, a->g, a->dist, a->d16, a->fb, a->ctl, a->nstage, a->lds_entries);
+        else
+            sssp_step_kernel<OffT, false><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t),
+                                            a->s>>>(a->plan, a->g, a->dist, nullptr, a->fb,
+                                                    a->ctl, a->nstage, a->lds_entries);
         a->kt->mark(a->s);
         sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
         compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1,
@@ -717,7 +725,9 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const long long sweep = sssp_sweep_bytes(g, ph);
     const int32_t nstage = std::min<int32_t>(tuning().sssp_stage, g->n);
     const int32_t lds_entries = std::max<int32_t>(nstage, (int32_t)kPushStage);
-    const int grid = resident_grid(reinterpret_cast<const void *>(&sssp_step_kernel<OffT>),
+    const void *step_fn = d16 ? reinterpret_cast<const void *>(&sssp_step_kernel<OffT, true>)
+                              : reinterpret_cast<const void *>(&sssp_step_kernel<OffT, false>);
+    const int grid = resident_grid(step_fn,
                                    kBlock, lds_entries * sizeof(int32_t),
                                    std::min(ws->f.grid, grid_blocks(tuning().sssp_grid_per_cu)));
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
