@@ -561,6 +561,30 @@ __global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
 // ---------------------------------------------------------------------------
 
 
+// Share of all edges held by the 1% highest-degree rows (cached; the caller
+// holds g->mu).  Skewed graphs (Kronecker: about half) and flat ones
+// (uniform: about 1%) want different push/pull switch points: a push epoch
+// on a flat graph relaxes edges whose targets are spread over the whole
+// distance array, so its atomics cost far more per edge.
+static double hub_share(hc_graph_s *g) {
+    if (g->hub_share >= 0.0) return g->hub_share;
+    const int32_t rows = g->rows;
+    if (rows <= 0 || g->m <= 0) return g->hub_share = 0.0;
+    const int32_t k = std::max<int32_t>(1, rows / 100);
+    int64_t top = 0;
+    if (g->sorted) {
+        top = g->h_off[k] - g->h_off[0];
+    } else {
+        std::vector<int64_t> deg(rows);
+        for (int32_t i = 0; i < rows; i++) deg[i] = g->h_off[i + 1] - g->h_off[i];
+        std::nth_element(deg.begin(), deg.begin() + (k - 1), deg.end(), std::greater<int64_t>());
+        for (int32_t i = 0; i < k; i++) top += deg[i];
+    }
+    return g->hub_share = double(top) / double(g->m);
+}
+
+static bool flat_graph(hc_graph_s *g) { return hub_share(g) < 0.05; }
+
 struct SsspWs {
     FrontierWs f;
     uint16_t *d16 = nullptr;  // 16-bit distance mirror, allocated on first use
@@ -626,7 +650,7 @@ static int launch_pull_epoch(hc_graph_s *g, int mapping, int32_t start,
             break;
         default: {
             const PlanHost *ph = nullptr;
-            HC_TRY(get_plan(g, start, end, sssp_plan_config(), &ph));
+            HC_TRY(get_plan(g, start, end, sssp_plan_config(flat_graph(g)), &ph));
             if (ph->total_units == 0) return HC_OK;
             int nb = std::max(1, std::min(ph->total_units, grid_blocks()));
             const int32_t ns = std::min<int32_t>(tuning().sssp_stage, g->n);
@@ -682,29 +706,6 @@ struct SsspLaunch {
     }
 };
 
-// Share of all edges held by the 1% highest-degree rows (cached; the caller
-// holds g->mu).  Skewed graphs (Kronecker: about half) and flat ones
-// (uniform: about 1%) want different push/pull switch points: a push epoch
-// on a flat graph relaxes edges whose targets are spread over the whole
-// distance array, so its atomics cost far more per edge.
-static double hub_share(hc_graph_s *g) {
-    if (g->hub_share >= 0.0) return g->hub_share;
-    const int32_t rows = g->rows;
-    if (rows <= 0 || g->m <= 0) return g->hub_share = 0.0;
-    const int32_t k = std::max<int32_t>(1, rows / 100);
-    int64_t top = 0;
-    if (g->sorted) {
-        top = g->h_off[k] - g->h_off[0];
-    } else {
-        std::vector<int64_t> deg(rows);
-        for (int32_t i = 0; i < rows; i++) deg[i] = g->h_off[i + 1] - g->h_off[i];
-        std::nth_element(deg.begin(), deg.begin() + (k - 1), deg.end(), std::greater<int64_t>());
-        for (int32_t i = 0; i < k; i++) top += deg[i];
-    }
-    return g->hub_share = double(top) / double(g->m);
-}
-
-static bool flat_graph(hc_graph_s *g) { return hub_share(g) < 0.05; }
 
 template <typename OffT>
 static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
@@ -713,7 +714,7 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     SsspWs *ws = nullptr;
     HC_TRY(get_sssp_ws(g, &ws));
     const PlanHost *ph = nullptr;
-    HC_TRY(get_plan(g, 0, g->n, sssp_plan_config(), &ph));
+    HC_TRY(get_plan(g, 0, g->n, sssp_plan_config(flat_graph(g)), &ph));
     GraphView<OffT> v = view<OffT>(g);
     if (alpha <= 0) alpha = tuning().sssp_alpha > 0 ? tuning().sssp_alpha : (flat_graph(g) ? 2 : 4);
     uint16_t *d16 = nullptr;

@@ -214,7 +214,8 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   sssp.block_threshold   SSSP: degree above which a workgroup takes a row
  *                          (4096; hub and wavefront classes are pull.*'s)
  *   sssp.g16_threshold / sssp.g4_threshold   SSSP: degree above which 16 / 4
- *                          lanes take a row (48 / 8)
+ *                          lanes take a row (0 / 8; 0: 48 for skewed graphs,
+ *                          24 for flat ones)
  *   pull.hub_threshold     degree above which a row is split into chunks
  *   pull.hub_chunk         edges per chunk (>= 256)
  *   pull.block_threshold   degree above which a workgroup takes a row
