@@ -322,7 +322,11 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
     const uint32_t u1 = nch + n0;
     const uint32_t u2 = u1 + (n1 + kWavesPerBlock - 1) / kWavesPerBlock;
     const uint32_t total = u2 + (n2 + kSmallPerUnit - 1) / kSmallPerUnit;
+    // Edges per unit: a hub chunk, a workgroup row (<= kHubDeg), wavefront
+    // rows, or 8-lane rows.
+    const uint32_t unit_max = (uint32_t)max<int64_t>(ctl->chunk, kHubDeg);
     for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
+        if (unit != blockIdx.x) push_out_make_room(out, g, fb, ctl, nxt, unit_max);
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
             td_range<kBlock>(g, depth, parents, level, next, fb, ctl, out, nxt, h.u,

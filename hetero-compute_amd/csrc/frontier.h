@@ -271,6 +271,23 @@ __device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
     }
 }
 
+// Called by the whole workgroup between units of a push sweep: flushes the
+// staged output early when the next unit could overfill the buffer (a unit
+// appends at most `unit_max` entries: its edges), so the per-lane queue
+// atomics of the overflow path stay a last resort.
+template <typename OffT>
+__device__ __forceinline__ void push_out_make_room(const PushOut &o, const GraphView<OffT> &g,
+                                                   const FrontierBufs &fb, StepCtl *ctl, int p,
+                                                   uint32_t unit_max) {
+    __syncthreads();
+    const uint32_t n = *o.count;
+    if (n == 0 || n + unit_max <= o.cap) return;  // uniform: all threads read n after the barrier
+    push_out_flush(o, g, fb, ctl, p);
+    __syncthreads();
+    if (threadIdx.x == 0) *o.count = 0;
+    __syncthreads();
+}
+
 // Chunk descriptors of a hub source (parity 0), written by one workgroup.
 template <typename OffT>
 __device__ __forceinline__ void write_source_chunks(const GraphView<OffT> &g,
