@@ -355,13 +355,13 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
 }
 
 // `nstage` leading frontier-bitmap words are staged in dynamic LDS for
-// bottom-up epochs; top-down epochs stage their output there (`lds_words`
-// words of dynamic LDS in all).
+// bottom-up epochs; top-down epochs stage up to `push_cap` output entries
+// there.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ depth,
     int32_t *__restrict__ parents, const int32_t *__restrict__ first, FrontierBufs fb,
-    StepCtl *ctl, RemLists rl, int32_t nstage, int32_t stage_min, int32_t lds_words,
+    StepCtl *ctl, RemLists rl, int32_t nstage, int32_t stage_min, int32_t push_cap,
     int first_all) {
     extern __shared__ uint32_t s_words[];
     __shared__ uint32_t s_rem;
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
         __shared__ uint32_t s_out;
         if (threadIdx.x == 0) s_out = 0;
         __syncthreads();
-        const PushOut out{reinterpret_cast<int32_t *>(s_words), &s_out, (uint32_t)lds_words};
+        const PushOut out{reinterpret_cast<int32_t *>(s_words), &s_out, (uint32_t)push_cap};
         td_sweep(g, depth, parents, level, next, fb, ctl, out, cur, nxt, cnt, eacc, scanned);
         push_out_flush(out, g, fb, ctl, nxt);
     }
@@ -778,6 +778,7 @@ struct BfsLaunch {
     int32_t nstage;
     int32_t stage_min;
     int32_t lds_words;
+    int32_t push_cap;
     int first_all;
     KernelTimer *kt;
     static int launch(void *arg) {
@@ -785,7 +786,7 @@ struct BfsLaunch {
         a->kt->mark(a->s);
         bfs_step_kernel<OffT><<<a->grid, kBlock, a->lds_words * sizeof(uint32_t), a->s>>>(
             a->plan, a->g, a->depth, a->parents, a->first, a->fb, a->ctl, a->rl, a->nstage,
-            a->stage_min, a->lds_words, a->first_all);
+            a->stage_min, a->push_cap, a->first_all);
         a->kt->mark(a->s);
         bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
         compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
@@ -807,7 +808,10 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     if (!depth) depth = ws->depth;
     GraphView<OffT> v = view<OffT>(g);
     const int32_t nstage = std::min<int32_t>(tuning().stage_bu, ws->f.fb.words);
-    const int32_t lds_words = std::max<int32_t>(nstage, (int32_t)kPushStage);
+    // Dynamic LDS: the bitmap head of bottom-up epochs or the staged output
+    // of top-down epochs, whichever is larger.
+    const int32_t push_cap = tuning().push_stage;
+    const int32_t lds_words = std::max<int32_t>(nstage, push_cap);
     const int grid = resident_grid(reinterpret_cast<const void *>(&bfs_step_kernel<OffT>),
                                    kBlock, lds_words * sizeof(uint32_t), ws->f.grid);
     if (grid > ws->f.grid) return fail(HC_ERR_HIP, "BFS grid %d above its workspace", grid);
@@ -821,7 +825,7 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
     const RemLists rl{ws->rem, reinterpret_cast<uint32_t *>(ws->rem + grid + 1)};
     BfsLaunch<OffT> L{ph->dev, v, depth, parents, tuning().bu_first ? ws->first : nullptr,
                       ws->f.fb, ws->f.ctl, rl, grid, s, nstage, tuning().bu_stage_min,
-                      lds_words, tuning().bu_first == 2, &ws->f.kt};
+                      lds_words, push_cap, tuning().bu_first == 2, &ws->f.kt};
     const long long cap = (long long)g->n + 2;  // levels <= n
     float ms = 0;
     long long launches = 0;

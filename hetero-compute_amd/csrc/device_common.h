@@ -107,6 +107,7 @@ struct Tuning {
     int64_t sssp_g4_threshold = 8;        // SSSP: ... 4 lanes take a row (else 1)
     int stage_pull = 8192; // per-vertex entries staged in LDS by pull sweeps
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
+    int push_stage = 8192; // push epochs: output entries staged in LDS per workgroup
     int push_chunk = 256;  // edges per hub chunk of push epochs (>= 128)
     int bu_remaining = 1;  // BFS: lanes per vertex of unvisited-list sweeps (0: off)
     int bu_first = 2;      // BFS: rows probe the first-neighbour array (1: one-lane

@@ -189,7 +189,6 @@ __device__ __forceinline__ void frontier_append(const GraphView<OffT> &g,
 // this GPU (a big top-down epoch issued ~10^5 of them); staged output is
 // classified at the end of the sweep and reserved with one atomic per class
 // and workgroup.  Entries past `cap` go to the queues directly.
-constexpr uint32_t kPushStage = 8192;  // least LDS entries of a push step's buffer
 
 struct PushOut {
     int32_t *buf;      // LDS, cap entries

@@ -94,6 +94,7 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"bu.stage", &t.stage_bu},
                {"bu.remaining", &t.bu_remaining},
                {"push.chunk", &t.push_chunk},
+               {"push.stage", &t.push_stage},
                {"bu.first", &t.bu_first},
                {"compact.grid", &t.compact_grid},
                {"bu.stage_min", &t.bu_stage_min},
@@ -553,7 +554,7 @@ int hc_set_tuning(const char *key, int64_t value) {
     if (!p) return fail(HC_ERR_NOT_FOUND, "unknown tuning key '%s'", key);
     if (is_int) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
-                           !strcmp(key, "sssp.stage");
+                           !strcmp(key, "sssp.stage") || !strcmp(key, "push.stage");
         const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph") ||
                           !strcmp(key, "sssp.filter");
         if (!strcmp(key, "sssp.dist16"))

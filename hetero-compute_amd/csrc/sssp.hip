@@ -312,12 +312,12 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
 // ---------------------------------------------------------------------------
 
 // `nstage` hub distances are staged in dynamic LDS for pull epochs; push
-// epochs stage their output there (`lds_entries` entries in all).  D16: pull
+// epochs stage up to `push_cap` output entries there.  D16: pull
 // gathers read the 16-bit mirror `d16` instead.
 template <typename OffT, bool D16>
 __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, uint16_t *d16, FrontierBufs fb,
-    StepCtl *ctl, int32_t nstage, int32_t lds_entries) {
+    StepCtl *ctl, int32_t nstage, int32_t push_cap) {
     extern __shared__ int32_t s_stage[];
     if (ctl->done) return;
     const int mode = ctl->mode;
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
         __shared__ uint32_t s_out;
         if (threadIdx.x == 0) s_out = 0;
         __syncthreads();
-        const PushOut out{s_stage, &s_out, (uint32_t)lds_entries};
+        const PushOut out{s_stage, &s_out, (uint32_t)push_cap};
         sssp_push_sweep(g, dist, fb, ctl, out, cur, nxt, wbm, cnt, eacc);
         push_out_flush(out, g, fb, ctl, nxt);
     }
@@ -688,17 +688,18 @@ struct SsspLaunch {
     hipStream_t s;
     int32_t nstage;
     int32_t lds_entries;
+    int32_t push_cap;
     KernelTimer *kt;
     static int launch(void *arg) {
         auto *a = static_cast<SsspLaunch *>(arg);
         a->kt->mark(a->s);
         if (a->d16)
             sssp_step_kernel<OffT, true><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t), a->s>>>(
-                a->plan, a->g, a->dist, a->d16, a->fb, a->ctl, a->nstage, a->lds_entries);
+                a->plan, a->g, a->dist, a->d16, a->fb, a->ctl, a->nstage, a->push_cap);
         else
             sssp_step_kernel<OffT, false><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t),
                                             a->s>>>(a->plan, a->g, a->dist, nullptr, a->fb,
-                                                    a->ctl, a->nstage, a->lds_entries);
+                                                    a->ctl, a->nstage, a->push_cap);
         a->kt->mark(a->s);
         sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
         compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1,
@@ -727,7 +728,8 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     }
     const long long sweep = sssp_sweep_bytes(g, ph);
     const int32_t nstage = std::min<int32_t>(tuning().sssp_stage, g->n);
-    const int32_t lds_entries = std::max<int32_t>(nstage, (int32_t)kPushStage);
+    const int32_t push_cap = tuning().push_stage;
+    const int32_t lds_entries = std::max<int32_t>(nstage, push_cap);
     const void *step_fn = d16 ? reinterpret_cast<const void *>(&sssp_step_kernel<OffT, true>)
                               : reinterpret_cast<const void *>(&sssp_step_kernel<OffT, false>);
     const int grid = resident_grid(step_fn,
@@ -740,6 +742,7 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
     SsspLaunch<OffT> L{ph->dev, v, dist, d16, ws->f.fb, ws->f.ctl, grid, s, nstage, lds_entries,
+                       push_cap,
                        &ws->f.kt};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
     const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;

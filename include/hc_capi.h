@@ -224,6 +224,9 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   pull.g4_threshold      ... 4 lanes (else one lane)
  *   bu.wave_threshold / bu.g16_threshold / bu.g4_threshold   BFS bottom-up
  *   pull.stage / bu.stage  per-vertex entries / bitmap words staged in LDS
+ *   push.stage             output entries a push-epoch workgroup stages in LDS
+ *                          before reserving queue space (<= 15360; 8192);
+ *                          past it, entries go to the queues one atomic each
  *   push.chunk             edges per workgroup chunk of a hub vertex in push
  *                          epochs (>= 128; 256)
  *   bu.stage_min           a workgroup whose unvisited list is shorter than

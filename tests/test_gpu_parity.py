@@ -173,6 +173,31 @@ def test_sssp_dist16_mirror_exact(hc, gpu, oracle, kron16, uniform16):
         hc.reset_tuning()
 
 
+@pytest.mark.parametrize("push_stage", [0, 64])
+def test_push_output_staging_limits_exact(hc, gpu, oracle, kron16, uniform16, push_stage):
+    """Push epochs with no LDS staging (every entry through the queue
+    atomics) and with a tiny staging buffer (flushed between units): BFS
+    depths, the direction-optimizing epoch sequence and SSSP distances stay
+    exact."""
+    try:
+        hc.set_tuning("push.stage", push_stage)
+        for g in (kron16, uniform16):
+            with hc.DeviceGraph(g) as dg:
+                for s in map(int, g.pick_sources(3)):
+                    depth = oracle.bfs_depth(g.index, g.neighbors, s)
+                    for direction in (0, 2):
+                        b = hc.bfs(dg, s, direction=direction)
+                        assert np.array_equal(b.values, depth), (s, direction)
+                        assert oracle.verify_parents(depth, b.parents, s) == 0
+                    _, _, dirs = oracle.bfs_do(g.index, g.neighbors, s)
+                    assert hc.bfs(dg, s).stats["epochs"] == len(dirs)
+                    ref = oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, s)
+                    for direction in (0, 2):
+                        assert np.array_equal(hc.sssp(dg, s, direction=direction).values, ref)
+    finally:
+        hc.reset_tuning()
+
+
 @pytest.mark.parametrize("mapping", [0, 1, 2, 3])
 def test_sssp_pull_driver_mappings(hc, gpu, oracle, kron16, mapping):
     """sssp_pull_gpu semantics (pull epochs from init_dist until no change)
