@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
 
 // Direction-optimizing controller (one workgroup): the decisions of
 // bfs_do_cpu (src/kernels/cpu/bfs.cuh:68-103) applied between epochs.
-__device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, int offb) {
+__device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, StepCtl *trace, int offb) {
     if (c->done) return;
     unsigned long long sums[kSumSlots];
     sum_partials(fb.part, sums);
@@ -434,7 +434,7 @@ __device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, int offb) {
     if (threadIdx.x != 0) return;
     const int ran = c->mode;
     c->reached += q;
-    trace_epoch(c, k, ran, q, sc, scanned);
+    trace_epoch(c, trace, k, ran, q, sc, scanned);
     if (ran == kPush) {
         // queue entry + offsets per frontier vertex; id + depth probe per
         // edge; depth, parent, bitmap and queue writes per new vertex.
@@ -503,7 +503,13 @@ __device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, int offb) {
 // device-scope release fence, which writes back its XCD's L2.
 __global__ __launch_bounds__(kBlock) void bfs_ctl_kernel(FrontierBufs fb, StepCtl *c,
                                                          int offb) {
-    bfs_control(fb, c, offb);
+    __shared__ StepCtl s_c;  // header fields only
+    ctl_head_copy(&s_c, c);
+    __syncthreads();
+    if (s_c.done) return;
+    bfs_control(fb, &s_c, c, offb);
+    __syncthreads();
+    ctl_head_copy(c, &s_c);
 }
 
 template <typename OffT>

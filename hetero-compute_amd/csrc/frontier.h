@@ -364,19 +364,36 @@ __device__ __forceinline__ void clear_partials(const Partials &part) {
     for (int i = threadIdx.x; i < kSumGroups * kSumStride; i += blockDim.x) part.acc[i] = 0;
 }
 
-__device__ __forceinline__ void trace_epoch(StepCtl *c, int k, int mode,
+// Records epoch k: the end time in the controller's copy c, the trace entry
+// in the control block itself (the trace arrays are not copied).
+__device__ __forceinline__ void trace_epoch(StepCtl *c, StepCtl *trace, int k, int mode,
                                             unsigned long long out_n,
                                             unsigned long long out_e,
                                             unsigned long long scanned) {
     const long long t = wall_clock64();
     c->t_end = t;
     if (k < kTraceLen) {
-        c->tr_t[k] = t;
-        c->tr_mode[k] = (int8_t)mode;
-        c->tr_n[k] = (uint32_t)out_n;
-        c->tr_e[k] = out_e;
-        c->tr_scan[k] = scanned;
+        trace->tr_t[k] = t;
+        trace->tr_mode[k] = (int8_t)mode;
+        trace->tr_n[k] = (uint32_t)out_n;
+        trace->tr_e[k] = out_e;
+        trace->tr_scan[k] = scanned;
     }
+}
+
+// The controllers work on an LDS copy of the control block's header (every
+// field before the trace arrays): one coalesced load and one coalesced store
+// instead of a chain of dependent global accesses by one thread.
+constexpr int kCtlHeadWords =
+    (int)(__builtin_offsetof(StepCtl, tr_t) / sizeof(unsigned long long));
+static_assert(__builtin_offsetof(StepCtl, tr_t) % sizeof(unsigned long long) == 0,
+              "the control block header is copied in 8-byte words");
+static_assert(kCtlHeadWords <= kBlock, "one word per thread");
+
+__device__ __forceinline__ void ctl_head_copy(StepCtl *dst, const StepCtl *src) {
+    if (threadIdx.x < kCtlHeadWords)
+        reinterpret_cast<unsigned long long *>(dst)[threadIdx.x] =
+            reinterpret_cast<const unsigned long long *>(src)[threadIdx.x];
 }
 
 // Bitmap -> degree-classed queues of parity (epoch & 1).  Each workgroup

@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
 }
 
 // Epoch controller: one workgroup.
-__device__ void sssp_control(const FrontierBufs &fb, StepCtl *c) {
+__device__ void sssp_control(const FrontierBufs &fb, StepCtl *c, StepCtl *trace) {
     if (c->done) return;
     unsigned long long sums[kSumSlots];
     sum_partials(fb.part, sums);
@@ -378,7 +378,7 @@ __device__ void sssp_control(const FrontierBufs &fb, StepCtl *c) {
         // edge; dist write + bitmap/queue entry per improved vertex.
         c->alg_bytes += 12ull * c->in_n + 12ull * c->in_e + 8ull * out_n;
         c->push_epochs++;
-        trace_epoch(c, k, ran, out_n, out_e, c->in_e);
+        trace_epoch(c, trace, k, ran, out_n, out_e, c->in_e);
     } else {
         // Offsets, ids and own distances always; weight + distance per
         // gathered edge; one bit per tested neighbour when filtering.
@@ -386,7 +386,7 @@ __device__ void sssp_control(const FrontierBufs &fb, StepCtl *c) {
                         8ull * gathered + (c->filter ? (unsigned long long)c->m / 8 : 0ull) +
                         8ull * out_n;
         c->pull_epochs++;
-        trace_epoch(c, k, ran, out_n, out_e, (unsigned long long)c->m);
+        trace_epoch(c, trace, k, ran, out_n, out_e, (unsigned long long)c->m);
     }
     for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
     c->epoch = k + 1;
@@ -410,7 +410,13 @@ __device__ void sssp_control(const FrontierBufs &fb, StepCtl *c) {
 
 // The controller launch (one workgroup).
 __global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb, StepCtl *c) {
-    sssp_control(fb, c);
+    __shared__ StepCtl s_c;  // header fields only
+    ctl_head_copy(&s_c, c);
+    __syncthreads();
+    if (s_c.done) return;
+    sssp_control(fb, &s_c, c);
+    __syncthreads();
+    ctl_head_copy(c, &s_c);
 }
 
 // Initialises distances, bitmaps and the control block for one run.

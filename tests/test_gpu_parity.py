@@ -109,14 +109,31 @@ def test_sssp_filter_and_direction_exact(hc, gpu, oracle, kron16, uniform16, fil
         hc.reset_tuning()
 
 
+def _pull_only_parents(oracle, g, s):
+    """Parents of a BFS made of bottom-up epochs only: each epoch of the
+    oracle's epoch_bfs_pull_cpu restatement gives every unvisited vertex its
+    first frontier neighbour in CSR order."""
+    n = g.index.size - 1
+    parents = np.full(n, -1, np.int32)
+    parents[s] = s
+    front = np.zeros((n + 31) // 32, np.uint32)
+    front[s >> 5] |= np.uint32(1 << (s & 31))
+    while True:
+        parents, nxt, c = oracle.bfs_pull_epoch(g.index, g.neighbors, parents, 0, n, front)
+        if c == 0:
+            return parents
+        front = nxt
+
+
 @pytest.mark.parametrize("first,remaining,stage_min", [(0, 0, 0), (2, 1, 4096), (2, 4, 0),
-                                                     (1, 16, 1 << 20)])
+                                                     (1, 16, 1 << 20), (1, 1, 0)])
 def test_bfs_bottom_up_variants_exact(hc, gpu, oracle, kron16, uniform16, first, remaining,
                                       stage_min):
     """Bottom-up epochs with and without the first-neighbour probes (one-lane
     rows or all rows), unvisited-list sweeps of 1/4/16 lanes and list-length
-    staging thresholds: depths bit-exact, parents a valid BFS tree, and the
-    direction-optimizing epoch sequence of bfs_do_cpu."""
+    staging thresholds: depths bit-exact, parents a valid BFS tree, the
+    direction-optimizing epoch sequence of bfs_do_cpu, and bottom-up-only
+    runs with the parents of the sequential first-neighbour scan."""
     try:
         hc.set_tuning("bu.first", first)
         hc.set_tuning("bu.remaining", remaining)
@@ -133,6 +150,7 @@ def test_bfs_bottom_up_variants_exact(hc, gpu, oracle, kron16, uniform16, first,
                     assert b.stats["pull_epochs"] == int(dirs.sum())
                     b1 = hc.bfs(dg, s, direction=1)
                     assert np.array_equal(b1.values, depth), s
+                    assert np.array_equal(b1.parents, _pull_only_parents(oracle, g, s)), s
     finally:
         hc.reset_tuning()
 
