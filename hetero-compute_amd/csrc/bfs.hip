@@ -317,8 +317,8 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
                          uint32_t *__restrict__ next, const FrontierBufs &fb,
                          StepCtl *ctl, const PushOut &out, int cur, int nxt, uint32_t &cnt,
                          unsigned long long &edges_acc, uint32_t &scanned) {
-    const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
-                   n1 = ctl->cnt[cur][1], n2 = ctl->cnt[cur][2];
+    const uint32_t nch = fb.cnt[cur][3], n0 = fb.cnt[cur][0],
+                   n1 = fb.cnt[cur][1], n2 = fb.cnt[cur][2];
     const uint32_t u1 = nch + n0;
     const uint32_t u2 = u1 + (n1 + kWavesPerBlock - 1) / kWavesPerBlock;
     const uint32_t total = u2 + (n2 + kSmallPerUnit - 1) / kSmallPerUnit;
@@ -420,18 +420,16 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
     write_partials(fb.part, cnt, eacc, scanned, list_in, list_out);
 }
 
-// Direction-optimizing controller (one workgroup): the decisions of
-// bfs_do_cpu (src/kernels/cpu/bfs.cuh:68-103) applied between epochs.
-__device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, StepCtl *trace, int offb) {
-    if (c->done) return;
-    unsigned long long sums[kSumSlots];
-    sum_partials(fb.part, sums);
+// Direction-optimizing controller (one thread, on the header copy c): the
+// decisions of bfs_do_cpu (src/kernels/cpu/bfs.cuh:68-103) applied between
+// epochs, from the step sums of the epoch that ran.
+__device__ void bfs_decide(StepCtl *c, StepCtl *trace, const unsigned long long (&sums)[kSumSlots],
+                           int offb) {
     // Frontier size and degree sum, edges examined, unvisited-list lengths
     // read and written by a pull epoch.
     const unsigned long long q = sums[0], sc = sums[1], scanned = sums[2];
     const unsigned long long lin = sums[3], lout = sums[4];
-    const int k = c->epoch, cur = k & 1;
-    if (threadIdx.x != 0) return;
+    const int k = c->epoch;
     const int ran = c->mode;
     c->reached += q;
     trace_epoch(c, trace, k, ran, q, sc, scanned);
@@ -449,7 +447,6 @@ __device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, StepCtl *trace, 
                         4ull * ((c->n + 31) / 32) + 4ull * lout;
         c->pull_epochs++;
     }
-    for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
     c->epoch = k + 1;
     c->in_n = (long long)q;
     c->in_e = (long long)sc;
@@ -495,21 +492,25 @@ __device__ void bfs_control(const FrontierBufs &fb, StepCtl *c, StepCtl *trace, 
     // list unless the list mode is switched off.
     c->rem_valid = (c->rem_knob && ran == kPull && next == kPull) ? 1 : 0;
     c->mode = next;
-    if (c->done) publish_done(fb);
 }
 
-// The controller launch (one workgroup).  A controller run by the step's
-// last workgroup was measured slower: every workgroup then needs a
-// device-scope release fence, which writes back its XCD's L2.
-__global__ __launch_bounds__(kBlock) void bfs_ctl_kernel(FrontierBufs fb, StepCtl *c,
-                                                         int offb) {
+// Between epochs k and k + 1: the controller (every workgroup decides on its
+// own copy; workgroup 0 records the epoch and stores header k + 1), then
+// the frontier compaction when the loop turns to top-down.  A controller
+// run by the step's last workgroup was measured slower: every workgroup
+// then needs a device-scope release fence, which writes back its XCD's L2.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void bfs_between_kernel(GraphView<OffT> g, FrontierBufs fb,
+                                                             StepCtl *cur, StepCtl *nxt,
+                                                             StepCtl *trace, Partials next_set,
+                                                             int offb) {
     __shared__ StepCtl s_c;  // header fields only
-    ctl_head_copy(&s_c, c);
+    unsigned long long sums[kSumSlots];
+    if (!ctl_begin(&s_c, cur, fb.part, sums)) return;
+    if (threadIdx.x == 0) bfs_decide(&s_c, blockIdx.x == 0 ? trace : nullptr, sums, offb);
     __syncthreads();
-    if (s_c.done) return;
-    bfs_control(fb, &s_c, c, offb);
-    __syncthreads();
-    ctl_head_copy(c, &s_c);
+    ctl_end(&s_c, cur, nxt, fb, next_set);
+    compact_body(g, fb, &s_c, 1);
 }
 
 template <typename OffT>
@@ -528,10 +529,11 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
         fb.bm[1][w] = 0;
         fb.bm[2][w] = 0;
     }
-    if (blockIdx.x == 0) clear_partials(fb.part);
+    if (blockIdx.x == 0)
+        for (int j = 0; j < kSumSets; j++) clear_partials(sum_set(fb.part, j));
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int p = 0; p < 2; p++)
-            for (int j = 0; j < 5; j++) ctl->cnt[p][j] = 0;
+            for (int j = 0; j < 5; j++) fb.cnt[p][j] = 0;
         ctl->done = 0;
         ctl->epoch = 0;
         ctl->max_epochs = max_epochs;
@@ -555,11 +557,11 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(
         int64_t d = int64_t(e - b);
         int cls = degree_class(d);
         if (cls == 3) {  // chunk descriptors: written by workgroup 0 below
-            ctl->cnt[0][3] = (uint32_t)((d + chunk - 1) / chunk);
-            ctl->cnt[0][4] = 1;
+            fb.cnt[0][3] = (uint32_t)((d + chunk - 1) / chunk);
+            fb.cnt[0][4] = 1;
         } else {
             fb.q[0][cls][0] = src;
-            ctl->cnt[0][cls] = 1;
+            fb.cnt[0][cls] = 1;
         }
         ctl->in_n = 1;
         ctl->in_e = d;
@@ -787,15 +789,19 @@ struct BfsLaunch {
     int32_t push_cap;
     int first_all;
     KernelTimer *kt;
-    static int launch(void *arg) {
+    // Epoch k: header slot k & 1, counter set k % 3.
+    static int launch(void *arg, long long k) {
         auto *a = static_cast<BfsLaunch *>(arg);
+        FrontierBufs fb = a->fb;
+        fb.part = sum_set(a->fb.part, k);
+        StepCtl *cur = a->ctl + (k & 1), *nxt = a->ctl + ((k + 1) & 1);
         a->kt->mark(a->s);
         bfs_step_kernel<OffT><<<a->grid, kBlock, a->lds_words * sizeof(uint32_t), a->s>>>(
-            a->plan, a->g, a->depth, a->parents, a->first, a->fb, a->ctl, a->rl, a->nstage,
+            a->plan, a->g, a->depth, a->parents, a->first, fb, cur, a->rl, a->nstage,
             a->stage_min, a->push_cap, a->first_all);
         a->kt->mark(a->s);
-        bfs_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl, (int)sizeof(OffT));
-        compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1);
+        bfs_between_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(
+            a->g, fb, cur, nxt, a->ctl, sum_set(a->fb.part, k + 1), (int)sizeof(OffT));
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "BFS epoch launch failed");
         return HC_OK;
     }
@@ -847,7 +853,7 @@ static int run_bfs_loop(hc_graph_s *g, int32_t src, int32_t *depth,
         stats->epochs = fc.epoch;
         stats->push_epochs = fc.push_epochs;
         stats->pull_epochs = fc.pull_epochs;
-        stats->launches = (int32_t)(3 * launches);
+        stats->launches = (int32_t)(2 * launches);
         stats->reached = (int64_t)fc.reached;
         stats->alg_bytes = (double)fc.alg_bytes;
         if (ws->f.kt.on) {

@@ -114,6 +114,7 @@ struct Tuning {
                            // rows, 2: all rows)
     int bu_stage_min = 4096;  // BFS: unvisited lists shorter than this skip LDS staging
     int compact_grid = 4096;  // workgroups of the frontier compaction launch (capped)
+    int loop_batch = 2;    // epochs the host enqueues per poll of the done flag
     int kernel_events = 0; // 1: time every launch of a loop's main kernel
     int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
     int multi_grid_per_cu = 16;  // workgroups per CU of the batched sweep

@@ -1,15 +1,23 @@
 // Device-resident frontier and epoch-control state shared by the
 // device-driven epoch loops (SSSP push/pull and BFS top-down/bottom-up).
 //
-// Every epoch is three launches on one stream:
-//   step      (full grid)  runs the epoch; each workgroup writes its partial
-//                          counts (frontier size, degree sum, edges examined)
-//                          to its own slot -- no same-address atomics;
-//   control   (1 workgroup) sums the partials in a fixed order, records the
-//                          epoch, picks the next direction or stops;
-//   compact   (full grid)  only after a pull epoch that is followed by a push
-//                          epoch: turns the frontier bitmap into the
-//                          degree-classed queues the push expansion reads.
+// Every epoch is two launches on one stream:
+//   step      (full grid)  runs the epoch; each workgroup adds its counts
+//                          (frontier size, degree sum, edges examined) into
+//                          one of kSumGroups counter groups;
+//   between   (compaction grid) every workgroup sums the counters and
+//                          applies the controller's decision to its own LDS
+//                          copy of the control header (the same inputs give
+//                          the same result everywhere); workgroup 0 records
+//                          the epoch and stores the next header; after a
+//                          pull epoch followed by a push epoch, the grid
+//                          turns the frontier bitmap into the degree-classed
+//                          queues the push expansion reads.
+// The control header is double-buffered (epoch k reads slot k & 1, its
+// controller writes slot (k + 1) & 1) and the counter groups are
+// triple-buffered (the step of epoch k adds into set k % 3, the controller
+// of epoch k clears set (k + 1) % 3), so no launch writes what another
+// workgroup of the same launch reads.  The host passes k with each launch.
 // All cross-workgroup data flows through kernel boundaries, so no device-
 // scope fences are needed inside a kernel, and the host only enqueues.
 //
@@ -42,7 +50,6 @@ constexpr int kTraceLen = 128;
 enum StepMode : int32_t { kPush = 0, kPull = 1 };
 
 struct StepCtl {
-    uint32_t cnt[2][5];            // queue fill [parity][q0,q1,q2,chunks,hubs]
     int32_t done;
     int32_t mode;
     int32_t epoch;
@@ -93,16 +100,25 @@ constexpr int kSumStride = 16;
 constexpr int kSumSlots = 5;
 static_assert(kSumGroups <= kWave, "one wavefront reads the counter groups");
 
+constexpr int kSumSets = 3;
+constexpr int kSumWords = kSumGroups * kSumStride;
+
 struct Partials {
     unsigned long long *acc;   // [kSumGroups][kSumStride]
 };
+
+// Counter set j of the kSumSets consecutive sets at base.
+__host__ __device__ inline Partials sum_set(const Partials &base, long long j) {
+    return Partials{base.acc + (j % kSumSets) * kSumWords};
+}
 
 struct FrontierBufs {
     int32_t *q[2][3];
     HubChunk *ch[2];
     uint32_t *bm[3];
+    uint32_t (*cnt)[5];        // queue fill [parity][q0, q1, q2, chunks, hubs]
     int32_t words;             // bitmap words
-    Partials part;
+    Partials part;             // counter set of the current epoch
     int32_t *host_done;        // pinned host word the controller sets at the end
 };
 
@@ -145,9 +161,9 @@ __device__ __forceinline__ void frontier_append(const GraphView<OffT> &g,
         e = g.rend(v);
         cls = degree_class(int64_t(e - b));
     }
-    wave_append(fb.q[p][0], &ctl->cnt[p][0], cls == 0, v);
-    wave_append(fb.q[p][1], &ctl->cnt[p][1], cls == 1, v);
-    wave_append(fb.q[p][2], &ctl->cnt[p][2], cls == 2, v);
+    wave_append(fb.q[p][0], &fb.cnt[p][0], cls == 0, v);
+    wave_append(fb.q[p][1], &fb.cnt[p][1], cls == 1, v);
+    wave_append(fb.q[p][2], &fb.cnt[p][2], cls == 2, v);
     // Hub vertices: their chunk slots are reserved with one atomic per
     // wavefront, and the active lanes write each hub's chunk descriptors
     // together (a hub can have hundreds).
@@ -160,8 +176,8 @@ __device__ __forceinline__ void frontier_append(const GraphView<OffT> &g,
         const int leader = __ffsll((unsigned long long)m3) - 1;
         uint32_t base = 0;
         if (lane_id() == leader) {
-            base = atomicAdd(&ctl->cnt[p][3], total);
-            atomicAdd(&ctl->cnt[p][4], (uint32_t)__popcll(m3));
+            base = atomicAdd(&fb.cnt[p][3], total);
+            atomicAdd(&fb.cnt[p][4], (uint32_t)__popcll(m3));
         }
         base = __shfl(base, leader);
         const uint64_t act = __ballot(true);
@@ -237,7 +253,7 @@ __device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
     __syncthreads();
     if (threadIdx.x < 5) {
         const uint32_t k = s_n[threadIdx.x];
-        s_base[threadIdx.x] = k ? atomicAdd(&ctl->cnt[p][threadIdx.x], k) : 0u;
+        s_base[threadIdx.x] = k ? atomicAdd(&fb.cnt[p][threadIdx.x], k) : 0u;
         s_n[threadIdx.x] = 0;  // now the write cursor of the class
     }
     __syncthreads();
@@ -336,18 +352,16 @@ __device__ __forceinline__ void write_partials(const Partials &part, uint32_t n,
     }
 }
 
-// Reads and clears the step sums; run by one workgroup of kBlock threads
-// (integer sums: exact in any order); results valid in every thread.
+// Reads the step sums of one counter set; run by every workgroup of a
+// launch that reads them (integer sums: exact in any order); results valid
+// in every thread.
 __device__ __forceinline__ void sum_partials(const Partials &part,
                                              unsigned long long (&out)[kSumSlots]) {
     __shared__ unsigned long long r[kSumSlots];
     if (threadIdx.x < kWave) {
         unsigned long long v[kSumSlots];
-        unsigned long long *g = part.acc + threadIdx.x * kSumStride;
-        for (int k = 0; k < kSumSlots; k++) {
-            v[k] = threadIdx.x < kSumGroups ? g[k] : 0ull;
-            if (threadIdx.x < kSumGroups) g[k] = 0;
-        }
+        const unsigned long long *g = part.acc + threadIdx.x * kSumStride;
+        for (int k = 0; k < kSumSlots; k++) v[k] = threadIdx.x < kSumGroups ? g[k] : 0ull;
         for (int k = 0; k < kSumSlots; k++) {
 #pragma unroll
             for (int sh = 32; sh > 0; sh >>= 1) v[k] += __shfl_xor(v[k], sh, 64);
@@ -359,20 +373,20 @@ __device__ __forceinline__ void sum_partials(const Partials &part,
     for (int k = 0; k < kSumSlots; k++) out[k] = r[k];
 }
 
-// Zeroes the step sums (one workgroup, at the start of a run).
+// Zeroes one counter set (one workgroup).
 __device__ __forceinline__ void clear_partials(const Partials &part) {
-    for (int i = threadIdx.x; i < kSumGroups * kSumStride; i += blockDim.x) part.acc[i] = 0;
+    for (int i = threadIdx.x; i < kSumWords; i += blockDim.x) part.acc[i] = 0;
 }
 
 // Records epoch k: the end time in the controller's copy c, the trace entry
-// in the control block itself (the trace arrays are not copied).
+// in `trace` (slot 0 of the control block; null in all workgroups but one).
 __device__ __forceinline__ void trace_epoch(StepCtl *c, StepCtl *trace, int k, int mode,
                                             unsigned long long out_n,
                                             unsigned long long out_e,
                                             unsigned long long scanned) {
     const long long t = wall_clock64();
     c->t_end = t;
-    if (k < kTraceLen) {
+    if (trace && k < kTraceLen) {
         trace->tr_t[k] = t;
         trace->tr_mode[k] = (int8_t)mode;
         trace->tr_n[k] = (uint32_t)out_n;
@@ -396,10 +410,43 @@ __device__ __forceinline__ void ctl_head_copy(StepCtl *dst, const StepCtl *src) 
             reinterpret_cast<const unsigned long long *>(src)[threadIdx.x];
 }
 
-// Bitmap -> degree-classed queues of parity (epoch & 1).  Each workgroup
+// Prologue of a between-epochs launch (every workgroup): an LDS copy of
+// the header epoch k's step read (`cur`) and that step's sums.  False when
+// the loop has ended.
+__device__ __forceinline__ bool ctl_begin(StepCtl *s_c, const StepCtl *cur, const Partials &part,
+                                          unsigned long long (&sums)[kSumSlots]) {
+    ctl_head_copy(s_c, cur);
+    __syncthreads();
+    if (s_c->done) return false;
+    sum_partials(part, sums);  // ends with a barrier
+    return true;
+}
+
+// Epilogue of the controller (after a barrier that follows the decision on
+// s_c, which advanced its epoch to k + 1; workgroup 0 only): the header of
+// epoch k + 1 goes to its slot `nxt`, the queue counters of parity k & 1
+// (read by epoch k, appended to by epoch k + 1) and the counter set of
+// epoch k + 1 are cleared, and a finished loop is marked in both header
+// slots (launches queued past the end read either) and published to the
+// host.
+__device__ __forceinline__ void ctl_end(const StepCtl *s_c, StepCtl *cur, StepCtl *nxt,
+                                        const FrontierBufs &fb, const Partials &next_set) {
+    if (blockIdx.x != 0) return;
+    ctl_head_copy(nxt, s_c);
+    clear_partials(next_set);
+    const int used = (s_c->epoch - 1) & 1;
+    if (threadIdx.x < 5) fb.cnt[used][threadIdx.x] = 0;
+    if (threadIdx.x == 0 && s_c->done) {
+        cur->done = 1;
+        publish_done(fb);
+    }
+}
+
+// Bitmap -> degree-classed queues of parity (epoch & 1), run by the
+// between-epochs launch with the decided header c.  Each workgroup
 // takes a contiguous run of bitmap words, counts its vertices per class in
 // LDS and reserves its output with one atomic per class.
-// The compaction launch is queued after every epoch but has work only when
+// The launch is queued after every epoch but has compaction work only when
 // the direction turns to top-down; a smaller grid keeps its idle launches
 // cheap (tuning knob compact.grid: workgroups, at most the step grid).
 inline int compact_grid(int step_grid) { return std::max(1, std::min(step_grid, tuning().compact_grid)); }
@@ -407,12 +454,9 @@ inline int compact_grid(int step_grid) { return std::max(1, std::min(step_grid, 
 // SSSP with a 16-bit distance mirror (`d16`): before every pull epoch the
 // same launch rebuilds the mirror from the int32 distances.
 template <typename OffT>
-__global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
-                                                         FrontierBufs fb,
-                                                         StepCtl *ctl,
-                                                         int three_bitmaps,
-                                                         const int32_t *dist = nullptr,
-                                                         uint16_t *d16 = nullptr) {
+__device__ void compact_body(const GraphView<OffT> &g, const FrontierBufs &fb,
+                             const StepCtl *ctl, int three_bitmaps,
+                             const int32_t *dist = nullptr, uint16_t *d16 = nullptr) {
     if (ctl->done) return;
     if (d16 && ctl->mode == kPull) {
         for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += gridDim.x * kBlock) {
@@ -459,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
         __syncthreads();
         if (threadIdx.x < 5)
             s_base[threadIdx.x] = s_cnt[threadIdx.x]
-                                      ? atomicAdd(&ctl->cnt[p][threadIdx.x], s_cnt[threadIdx.x])
+                                      ? atomicAdd(&fb.cnt[p][threadIdx.x], s_cnt[threadIdx.x])
                                       : 0;
         __syncthreads();
         // pass 2: queue entries; hubs are listed with their chunk range and
@@ -516,11 +560,17 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(GraphView<OffT> g,
     }
 }
 
-// Host loop shared by SSSP and BFS: enqueue `launch_epoch` batches and poll
-// the done flag of the previous batch while the next one is queued.
+// Host loop shared by SSSP and BFS: enqueue `launch_epoch(arg, k)` for
+// epochs k = 0, 1, ... in batches and poll the done flag of the previous
+// batch while the next one is queued.  final_ctl: the last header with the
+// trace of slot 0.
 int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
-                   int (*launch_epoch)(void *), void *arg, float *ms,
+                   int (*launch_epoch)(void *, long long), void *arg, float *ms,
                    long long *launches, StepCtl *final_ctl);
+
+// The newer header of the two slots at d (epoch counts differ by one) with
+// the trace of slot 0.
+int read_final_ctl(const StepCtl *d, StepCtl *out, hipStream_t s);
 
 }  // namespace hc
 

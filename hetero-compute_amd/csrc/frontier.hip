@@ -23,8 +23,10 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     size_t qbytes = al(sizeof(int32_t) * std::max<int64_t>(n, 1));
     size_t cbytes = al(sizeof(HubChunk) * cap);
     size_t bbytes = al(sizeof(uint32_t) * std::max(words, 1));
-    size_t pbytes = al(sizeof(unsigned long long) * kSumGroups * kSumStride);
-    size_t total = 6 * qbytes + 2 * cbytes + 3 * bbytes + pbytes + al(sizeof(StepCtl));
+    size_t pbytes = al(sizeof(unsigned long long) * kSumWords * kSumSets);
+    size_t nbytes = al(sizeof(uint32_t) * 2 * 5);
+    size_t total =
+        6 * qbytes + 2 * cbytes + 3 * bbytes + pbytes + nbytes + al(2 * sizeof(StepCtl));
     char *base = nullptr;
     if (hipMalloc(reinterpret_cast<void **>(&base), total) != hipSuccess)
         return fail(HC_ERR_OOM, "frontier workspace of %zu bytes failed", total);
@@ -44,6 +46,8 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     }
     ws->fb.part.acc = reinterpret_cast<unsigned long long *>(base + o);
     o += pbytes;
+    ws->fb.cnt = reinterpret_cast<uint32_t (*)[5]>(base + o);
+    o += nbytes;
 
     ws->ctl = reinterpret_cast<StepCtl *>(base + o);
     ws->fb.words = words;
@@ -71,14 +75,27 @@ void frontier_ws_free(FrontierWs *ws) {
     ws->h_ctl = nullptr;
 }
 
+int read_final_ctl(const StepCtl *d, StepCtl *out, hipStream_t s) {
+    StepCtl h[2];
+    HC_HIP(hipMemcpyAsync(h, d, 2 * sizeof(StepCtl), hipMemcpyDeviceToHost, s));
+    HC_HIP(hipStreamSynchronize(s));
+    *out = h[0];
+    if (h[1].epoch > h[0].epoch)
+        std::memcpy(out, &h[1], sizeof(unsigned long long) * kCtlHeadWords);
+    return HC_OK;
+}
+
 int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
-                   int (*launch_epoch)(void *), void *arg, float *ms,
+                   int (*launch_epoch)(void *, long long), void *arg, float *ms,
                    long long *launches, StepCtl *final_ctl) {
     hipEvent_t ev0, ev1, evb[2];
     HC_HIP(hipEventCreate(&ev0));
     HC_HIP(hipEventCreate(&ev1));
-    HC_HIP(hipEventCreateWithFlags(&evb[0], hipEventDisableTiming));
-    HC_HIP(hipEventCreateWithFlags(&evb[1], hipEventDisableTiming));
+    // The batch events only pace the host: no system-scope fence (an L2
+    // writeback per batch); the done word is a system-scope store of its own.
+    const unsigned pace = hipEventDisableTiming | hipEventDisableSystemFence;
+    HC_HIP(hipEventCreateWithFlags(&evb[0], pace));
+    HC_HIP(hipEventCreateWithFlags(&evb[1], pace));
     StepCtl *h = ws->h_ctl;
     h->done = 0;
     HC_HIP(hipEventRecord(ev0, s));
@@ -90,9 +107,9 @@ int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
     bool done = false;
     int b = 0;
     while (!done && epochs < cap) {
-        const int batch = 2;
+        const int batch = tuning().loop_batch;
         for (int i = 0; i < batch; i++) {
-            rc = launch_epoch(arg);
+            rc = launch_epoch(arg, epochs);
             if (rc != HC_OK) break;
             epochs++;
         }
@@ -108,8 +125,7 @@ int run_epoch_loop(FrontierWs *ws, hipStream_t s, long long cap,
     HC_HIP(hipEventRecord(ev1, s));
     HC_HIP(hipEventSynchronize(ev1));
     HC_HIP(hipEventElapsedTime(ms, ev0, ev1));
-    HC_HIP(hipMemcpyAsync(final_ctl, ws->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, s));
-    HC_HIP(hipStreamSynchronize(s));
+    rc = rc != HC_OK ? rc : read_final_ctl(ws->ctl, final_ctl, s);
     hipEventDestroy(ev0);
     hipEventDestroy(ev1);
     hipEventDestroy(evb[0]);
@@ -132,8 +148,8 @@ extern "C" int hc_get_epoch_trace(hc_graph_t g, int algorithm,
     std::memset(out, 0, sizeof(*out));
     if (!ws) return HC_OK;
     StepCtl c;
-    HC_HIP(hipMemcpy(&c, static_cast<FrontierWs *>(ws)->ctl, sizeof(StepCtl),
-                     hipMemcpyDeviceToHost));
+    HC_HIP(hipDeviceSynchronize());
+    HC_TRY(read_final_ctl(static_cast<FrontierWs *>(ws)->ctl, &c, nullptr));
     out->count = std::min(c.epoch, kTraceLen);
     const double khz = device_state().clock_khz;
     for (int i = 0; i < out->count; i++) {

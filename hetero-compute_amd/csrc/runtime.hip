@@ -98,6 +98,7 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"bu.first", &t.bu_first},
                {"compact.grid", &t.compact_grid},
                {"bu.stage_min", &t.bu_stage_min},
+               {"loop.batch", &t.loop_batch},
                {"profile.kernel_events", &t.kernel_events},
                {"multi.graph", &t.multi_graph},
                {"multi.grid_per_cu", &t.multi_grid_per_cu}};
@@ -561,6 +562,8 @@ int hc_set_tuning(const char *key, int64_t value) {
             HC_CHECK_ARG(value >= 0 && value <= 2, "sssp.dist16 must be 0, 1 or 2");
         if (!strcmp(key, "bu.first"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "bu.first must be 0, 1 or 2");
+        if (!strcmp(key, "loop.batch"))
+            HC_CHECK_ARG(value >= 1 && value <= 16, "loop.batch must be in [1, 16]");
         if (!strcmp(key, "push.chunk"))
             HC_CHECK_ARG(value >= kMinChunkEdges, "push.chunk must be >= %d", (int)kMinChunkEdges);
         if (!strcmp(key, "bu.remaining"))

@@ -272,8 +272,8 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
                                 const PushOut &out, int cur,
                                 int nxt, uint32_t *wbm, uint32_t &cnt,
                                 unsigned long long &edges_acc) {
-    const uint32_t nch = ctl->cnt[cur][3], n0 = ctl->cnt[cur][0],
-                   n1 = ctl->cnt[cur][1], n2 = ctl->cnt[cur][2];
+    const uint32_t nch = fb.cnt[cur][3], n0 = fb.cnt[cur][0],
+                   n1 = fb.cnt[cur][1], n2 = fb.cnt[cur][2];
     const uint32_t u1 = nch + n0;
     const uint32_t u2 = u1 + (n1 + kWavesPerBlock - 1) / kWavesPerBlock;
     const uint32_t total = u2 + (n2 + kSmallPerUnit - 1) / kSmallPerUnit;
@@ -364,14 +364,12 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     write_partials(fb.part, cnt, eacc, gathered);
 }
 
-// Epoch controller: one workgroup.
-__device__ void sssp_control(const FrontierBufs &fb, StepCtl *c, StepCtl *trace) {
-    if (c->done) return;
-    unsigned long long sums[kSumSlots];
-    sum_partials(fb.part, sums);
+// Epoch controller (one thread, on the header copy c), from the step sums
+// of the epoch that ran.
+__device__ void sssp_decide(StepCtl *c, StepCtl *trace,
+                            const unsigned long long (&sums)[kSumSlots]) {
     const unsigned long long out_n = sums[0], out_e = sums[1], gathered = sums[2];
-    if (threadIdx.x != 0) return;
-    const int k = c->epoch, cur = k & 1;
+    const int k = c->epoch;
     const int ran = c->mode;
     if (ran == kPush) {
         // offsets + dist[u] per frontier vertex; id + weight + dist[v] per
@@ -388,7 +386,6 @@ __device__ void sssp_control(const FrontierBufs &fb, StepCtl *c, StepCtl *trace)
         c->pull_epochs++;
         trace_epoch(c, trace, k, ran, out_n, out_e, (unsigned long long)c->m);
     }
-    for (int j = 0; j < 5; j++) c->cnt[cur][j] = 0;
     c->epoch = k + 1;
     c->in_n = (long long)out_n;
     c->in_e = (long long)out_e;
@@ -405,18 +402,24 @@ __device__ void sssp_control(const FrontierBufs &fb, StepCtl *c, StepCtl *trace)
     }
     c->need_compact = (!c->done && ran == kPull && next == kPush) ? 1 : 0;
     c->mode = next;
-    if (c->done) publish_done(fb);
 }
 
-// The controller launch (one workgroup).
-__global__ __launch_bounds__(kBlock) void sssp_ctl_kernel(FrontierBufs fb, StepCtl *c) {
+// Between epochs k and k + 1: the controller (every workgroup decides on its
+// own copy; workgroup 0 records the epoch and stores header k + 1), then
+// the frontier compaction when the loop turns to push and, with the 16-bit
+// mirror, its rebuild before a pull epoch.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void sssp_between_kernel(GraphView<OffT> g, FrontierBufs fb,
+                                                              StepCtl *cur, StepCtl *nxt,
+                                                              StepCtl *trace, Partials next_set,
+                                                              const int32_t *dist, uint16_t *d16) {
     __shared__ StepCtl s_c;  // header fields only
-    ctl_head_copy(&s_c, c);
+    unsigned long long sums[kSumSlots];
+    if (!ctl_begin(&s_c, cur, fb.part, sums)) return;
+    if (threadIdx.x == 0) sssp_decide(&s_c, blockIdx.x == 0 ? trace : nullptr, sums);
     __syncthreads();
-    if (s_c.done) return;
-    sssp_control(fb, &s_c, c);
-    __syncthreads();
-    ctl_head_copy(c, &s_c);
+    ctl_end(&s_c, cur, nxt, fb, next_set);
+    compact_body(g, fb, &s_c, 1, dist, d16);
 }
 
 // Initialises distances, bitmaps and the control block for one run.
@@ -440,14 +443,15 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
         fb.bm[1][w] = 0;
         fb.bm[2][w] = 0;
     }
-    if (blockIdx.x == 0) clear_partials(fb.part);
+    if (blockIdx.x == 0)
+        for (int j = 0; j < kSumSets; j++) clear_partials(sum_set(fb.part, j));
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // With initial distances the changed set before epoch 0 is unknown:
         // that epoch gathers every neighbour.
         ctl->filter_knob = filter;
         ctl->filter = init_dist ? 0 : filter;
         for (int p = 0; p < 2; p++)
-            for (int j = 0; j < 5; j++) ctl->cnt[p][j] = 0;
+            for (int j = 0; j < 5; j++) fb.cnt[p][j] = 0;
         ctl->done = 0;
         ctl->epoch = 0;
         ctl->max_epochs = max_epochs;
@@ -473,11 +477,11 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
             int64_t d = int64_t(e - b);
             int cls = degree_class(d);
             if (cls == 3) {  // chunk descriptors: written by workgroup 0 below
-                ctl->cnt[0][3] = (uint32_t)((d + chunk - 1) / chunk);
-                ctl->cnt[0][4] = 1;
+                fb.cnt[0][3] = (uint32_t)((d + chunk - 1) / chunk);
+                fb.cnt[0][4] = 1;
             } else {
                 fb.q[0][cls][0] = src;
-                ctl->cnt[0][cls] = 1;
+                fb.cnt[0][cls] = 1;
             }
             ctl->in_n = 1;
             ctl->in_e = d;
@@ -696,20 +700,23 @@ struct SsspLaunch {
     int32_t lds_entries;
     int32_t push_cap;
     KernelTimer *kt;
-    static int launch(void *arg) {
+    // Epoch k: header slot k & 1, counter set k % 3.
+    static int launch(void *arg, long long k) {
         auto *a = static_cast<SsspLaunch *>(arg);
+        FrontierBufs fb = a->fb;
+        fb.part = sum_set(a->fb.part, k);
+        StepCtl *cur = a->ctl + (k & 1), *nxt = a->ctl + ((k + 1) & 1);
         a->kt->mark(a->s);
         if (a->d16)
             sssp_step_kernel<OffT, true><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t), a->s>>>(
-                a->plan, a->g, a->dist, a->d16, a->fb, a->ctl, a->nstage, a->push_cap);
+                a->plan, a->g, a->dist, a->d16, fb, cur, a->nstage, a->push_cap);
         else
             sssp_step_kernel<OffT, false><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t),
-                                            a->s>>>(a->plan, a->g, a->dist, nullptr, a->fb,
-                                                    a->ctl, a->nstage, a->push_cap);
+                                            a->s>>>(a->plan, a->g, a->dist, nullptr, fb, cur,
+                                                    a->nstage, a->push_cap);
         a->kt->mark(a->s);
-        sssp_ctl_kernel<<<1, kBlock, 0, a->s>>>(a->fb, a->ctl);
-        compact_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(a->g, a->fb, a->ctl, 1,
-                                                                          a->dist, a->d16);
+        sssp_between_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(
+            a->g, fb, cur, nxt, a->ctl, sum_set(a->fb.part, k + 1), a->dist, a->d16);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "SSSP epoch launch failed");
         return HC_OK;
     }
@@ -767,7 +774,7 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
         stats->epochs = fc.epoch;
         stats->push_epochs = fc.push_epochs;
         stats->pull_epochs = fc.pull_epochs;
-        stats->launches = (int32_t)(3 * launches);
+        stats->launches = (int32_t)(2 * launches);
         stats->reached = -1;
         stats->alg_bytes = (double)fc.alg_bytes;
         if (ws->f.kt.on) {

@@ -240,6 +240,10 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   bu.remaining           a bottom-up epoch after another sweeps only the
  *                          vertices the previous one left unvisited, with
  *                          this many lanes per vertex (1, 4 or 16; 0: off)
+ *   loop.batch             epochs the host enqueues between two reads of the
+ *                          loop's done flag (1..16; 2): more hides more host
+ *                          latency, and up to twice as many no-op epochs
+ *                          are queued past the end of a loop
  *   profile.kernel_events  1: time each launch of the loop's main kernel
  *                          (hc_run_stats_t.kernel_ms); 0 by default
  *   multi.graph            1: batched loops relaunch an epoch hipGraph
