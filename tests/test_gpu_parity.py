@@ -155,6 +155,37 @@ def test_bfs_bottom_up_variants_exact(hc, gpu, oracle, kron16, uniform16, first,
         hc.reset_tuning()
 
 
+@pytest.mark.parametrize("grid,batch", [(1, 1), (3, 5), (4096, 2)])
+def test_between_epoch_grid_and_batch_exact(hc, gpu, oracle, kron16, uniform16, grid, batch):
+    """Every workgroup of the between-epochs launch makes the controller's
+    decision on its own copy of the control header.  With one, three or
+    thousands of workgroups, and with the host enqueueing 1, 5 or 2 epochs
+    per poll: distances and depths bit-exact, the epoch trace records
+    bfs_do_cpu's direction sequence, and its counts add up to the run's."""
+    try:
+        hc.set_tuning("compact.grid", grid)
+        hc.set_tuning("loop.batch", batch)
+        for g in (kron16, uniform16):
+            with hc.DeviceGraph(g) as dg:
+                for s in map(int, g.pick_sources(2)):
+                    depth = oracle.bfs_depth(g.index, g.neighbors, s)
+                    b = hc.bfs(dg, s)
+                    assert np.array_equal(b.values, depth), s
+                    _, _, dirs = oracle.bfs_do(g.index, g.neighbors, s)
+                    tr = hc.epoch_trace(dg, "bfs")
+                    assert [e["mode"] == "pull" for e in tr] == [bool(x) for x in dirs]
+                    assert 1 + sum(e["frontier"] for e in tr) == b.stats["reached"]
+                    ref = oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, s)
+                    r = hc.sssp(dg, s)
+                    assert np.array_equal(r.values, ref), s
+                    tr = hc.epoch_trace(dg, "sssp")
+                    assert len(tr) == r.stats["epochs"]
+                    assert sum(e["mode"] == "pull" for e in tr) == r.stats["pull_epochs"]
+                    assert tr[-1]["frontier"] == 0
+    finally:
+        hc.reset_tuning()
+
+
 def test_sssp_dist16_mirror_exact(hc, gpu, oracle, kron16, uniform16):
     """Pull gathers through the 16-bit distance mirror: bit-exact on both
     scale-16 graphs in every direction, on the golden graphs, and on a
