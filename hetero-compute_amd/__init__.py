@@ -115,7 +115,7 @@ class ServerStats(C.Structure):
 
 class PrOpts(C.Structure):
     _fields_ = [("damping", C.c_float), ("tolerance", C.c_double),
-                ("max_iters", C.c_int32)]
+                ("max_iters", C.c_int32), ("rule", C.c_int32)]
 
 
 _lib = None
@@ -532,10 +532,13 @@ def bfs(g: DeviceGraph, source: int, alpha: int = 15, beta: int = 18,
 
 
 def pagerank(g: DeviceGraph, damping: float = 0.85, tolerance: float = 1e-4,
-             max_iters: int = 20) -> Result:
+             max_iters: int = 20, rule: int = 0) -> Result:
+    """Jacobi PageRank; rule 0 stops on the FP64 L1 residual, rule 1 on
+    pr_pull_cpu's per-vertex 5e-8 rule (stats['residual'] then counts the
+    vertices that moved in the last iteration)."""
     d = DeviceArray(g.num_nodes, np.float32)
     st = RunStats()
-    o = PrOpts(damping, tolerance, max_iters)
+    o = PrOpts(damping, tolerance, max_iters, rule)
     check(lib().hc_pagerank(g.h, d.ptr, C.byref(o), C.byref(st), None))
     return Result(d.get(), st.as_dict())
 

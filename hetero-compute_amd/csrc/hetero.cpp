@@ -453,6 +453,8 @@ int hc_pr_heterogeneous(hc_hetero_t h, const hc_pr_opts_t *opts, float *scores,
                         hc_run_stats_t *stats) {
     HC_CHECK_ARG(h && scores, "NULL argument");
     HC_CHECK_ARG(h->algo == kPr, "this schedule is not a PageRank schedule");
+    HC_CHECK_ARG(!opts || opts->rule == 0,
+                 "host+GPU PageRank stops on the L1 residual only (rule 0)");
     const float damping = opts && opts->damping > 0 ? opts->damping : 0.85f;
     double tol = opts ? opts->tolerance : 0.0;
     if (tol == 0.0) tol = 1e-4;

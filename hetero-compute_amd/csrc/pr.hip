@@ -10,7 +10,11 @@
 //   * the L1 residual sum |score' - score| in FP64: one partial per
 //     workgroup, summed in workgroup order by a one-workgroup controller
 //     kernel, which also finishes the chunked hub vertices and takes the
-//     stopping decision (residual < tol or max_iters reached).
+//     stopping decision (residual < tol or max_iters reached);
+//   * or, with rule 1, the stopping rule of pr_pull_cpu as written
+//     (src/kernels/cpu/pr.cuh:165-181) in Jacobi form: a vertex takes its
+//     new score only when it moves by more than 5e-8 (FP32 fabs), and the
+//     loop stops after an iteration in which no vertex moved.
 // Scores start at 1/N, as pr_pull_cpu does (src/kernels/cpu/pr.cuh:38-42).
 #include <algorithm>
 #include <cmath>
@@ -128,7 +132,7 @@ struct PrBufs {
 };
 
 struct PrCtl {
-    uint32_t unused;
+    int32_t rule;       // 0: FP64 L1 residual < tol; 1: no vertex moved > 5e-8
     int32_t done;
     int32_t iter;
     int32_t max_iters;
@@ -147,15 +151,27 @@ struct PrEpoch {
     float *c_out;
     float base;
     float damp;
+    int rule;  // PrCtl::rule
 };
 
+// Rule 0 returns |score' - score| (FP64); rule 1 keeps the old score unless
+// the new one moves it by more than 5e-8 (epoch_pr_pull_cpu_one_to_one,
+// src/kernels/cpu/pr.cuh:170-181) and returns 1 when it moved.
 __device__ __forceinline__ double pr_finalize(const PrEpoch &ep, const double *inv_deg,
                                               int32_t u, KSum k) {
     float sum = k.hi + k.lo;
     float s_new = __fadd_rn(ep.base, __fmul_rn(ep.damp, sum));
+    const float s_old = ep.s_in[u];
+    if (ep.rule) {
+        const bool moved = fabsf(__fsub_rn(s_new, s_old)) > 0.00000005f;
+        const float s = moved ? s_new : s_old;
+        ep.s_out[u] = s;
+        ep.c_out[u] = (float)((double)s * inv_deg[u]);
+        return moved ? 1.0 : 0.0;
+    }
     ep.s_out[u] = s_new;
     ep.c_out[u] = (float)((double)s_new * inv_deg[u]);
-    return fabs((double)s_new - (double)ep.s_in[u]);
+    return fabs((double)s_new - (double)s_old);
 }
 
 template <int G, typename OffT>
@@ -232,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void pr_step_kernel(
     if (ctl->done) return;
     const int it = ctl->iter;
     const bool odd = it & 1;
-    PrEpoch ep{odd ? s1 : s0, odd ? c1 : c0, odd ? s0 : s1, odd ? c0 : c1, base, damp};
+    PrEpoch ep{odd ? s1 : s0, odd ? c1 : c0, odd ? s0 : s1, odd ? c0 : c1, base, damp, ctl->rule};
     double r = block_sum_f64(pr_sweep(p, g, ep, b, nstage, s_stagef));
     if (threadIdx.x == 0) b.partials[blockIdx.x] = r;
 }
@@ -272,21 +288,23 @@ __global__ __launch_bounds__(kBlock) void pr_ctl_kernel(
     if (ctl->done) return;
     const int it = ctl->iter;
     const bool odd = it & 1;
-    PrEpoch ep{odd ? s1 : s0, odd ? c1 : c0, odd ? s0 : s1, odd ? c0 : c1, base, damp};
+    PrEpoch ep{odd ? s1 : s0, odd ? c1 : c0, odd ? s0 : s1, odd ? c0 : c1, base, damp, ctl->rule};
     const double r = pr_hubs_and_residual(p, ep, b, nparts);
     if (threadIdx.x == 0) {
+        // Rule 1: r counts the vertices that moved (exact in FP64).
+        const bool converged = ctl->rule ? r == 0.0 : r < ctl->tol;
         ctl->residual = r;
         ctl->iter = it + 1;
         ctl->alg_bytes += ctl->epoch_bytes;
         ctl->t_end = wall_clock64();
-        if (r < ctl->tol || it + 1 >= ctl->max_iters) ctl->done = 1;
+        if (converged || it + 1 >= ctl->max_iters) ctl->done = 1;
     }
 }
 
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void pr_init_kernel(
     GraphView<OffT> g, float *s0, float *c0, double *inv_deg, uint32_t *hub_cnt,
-    int32_t num_hubs, PrCtl *ctl, int max_iters, double tol, double epoch_bytes) {
+    int32_t num_hubs, PrCtl *ctl, int max_iters, double tol, double epoch_bytes, int rule) {
     const int stride = gridDim.x * kBlock;
     const float init = 1.0f / (float)g.n;
     for (int u = blockIdx.x * kBlock + threadIdx.x; u < g.n; u += stride) {
@@ -304,6 +322,7 @@ __global__ __launch_bounds__(kBlock) void pr_init_kernel(
         ctl->iter = 0;
         ctl->max_iters = max_iters;
         ctl->tol = tol;
+        ctl->rule = rule;
         ctl->residual = 0.0;
         ctl->alg_bytes = 0.0;
         ctl->epoch_bytes = epoch_bytes;
@@ -416,7 +435,7 @@ static long long pr_epoch_bytes(const hc_graph_s *g, const PlanHost *ph) {
 
 template <typename OffT>
 static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
-                  int max_iters, hc_run_stats_t *stats, hipStream_t s) {
+                  int max_iters, int rule, hc_run_stats_t *stats, hipStream_t s) {
     const PlanHost *ph = nullptr;
     HC_TRY(get_plan(g, 0, g->n, tuning().pull, &ph));
     PrWs *ws = nullptr;
@@ -429,7 +448,7 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     const float base = (1.0f - damp) / (float)g->n;
     pr_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, ws->s[0], ws->c[0], ws->inv_deg, ws->hub_cnt, ph->dev.num_hubs, ws->ctl,
-        max_iters, tol, (double)pr_epoch_bytes(g, ph));
+        max_iters, tol, (double)pr_epoch_bytes(g, ph), rule);
     HC_HIP(hipGetLastError());
     PrBufs b{ws->inv_deg, ws->hub_hi, ws->hub_lo, ws->hub_cnt, ws->hub_resid, ws->partials};
     hipEvent_t e0, e1;
@@ -495,13 +514,15 @@ int hc_pagerank(hc_graph_t g, float *scores, const hc_pr_opts_t *opts,
     // residual test (exactly max_iters iterations).
     double tol = opts && opts->tolerance != 0 ? opts->tolerance : 1e-4;
     int max_iters = opts && opts->max_iters > 0 ? opts->max_iters : 20;
+    const int rule = opts ? opts->rule : 0;
     HC_CHECK_ARG(damp < 1.0f, "damping must be in (0,1)");
+    HC_CHECK_ARG(rule == 0 || rule == 1, "rule must be 0 (L1 residual) or 1 (per vertex)");
     HC_CHECK_ARG(max_iters <= 100000, "max_iters too large");
     HC_TRY(ensure_init());
     std::lock_guard<std::mutex> lk(g->mu);
     hipStream_t s = pick_stream(stream);
-    return g->off64 ? run_pr<uint64_t>(g, scores, damp, tol, max_iters, stats, s)
-                    : run_pr<uint32_t>(g, scores, damp, tol, max_iters, stats, s);
+    return g->off64 ? run_pr<uint64_t>(g, scores, damp, tol, max_iters, rule, stats, s)
+                    : run_pr<uint32_t>(g, scores, damp, tol, max_iters, rule, stats, s);
 }
 
 int hc_pr_pull_epoch(hc_graph_t g, int32_t start_id, int32_t end_id,
@@ -526,7 +547,7 @@ int hc_pr_pull_epoch(hc_graph_t g, int32_t start_id, int32_t end_id,
     HC_TRY(get_pr_ws(g, ph->dev.cnt[0], ph->dev.num_hubs, &ws));
     PrBufs b{inv_degree, ws->hub_hi, ws->hub_lo, ws->hub_cnt, ws->hub_resid, ws->partials};
     PrEpoch ep{score_in, contrib, score_out, contrib_out,
-               (1.0f - damping) / (float)g->n, damping};
+               (1.0f - damping) / (float)g->n, damping, 0};
     const int nb = std::max(1, std::min(std::max(ph->total_units, 1), grid_blocks()));
     const int32_t ns = std::min<int32_t>(tuning().stage_pull, g->n);
     if (g->off64)

@@ -351,6 +351,11 @@ typedef struct hc_pr_opts {
     double tolerance;  /* stop when the FP64 L1 residual < tolerance (1e-4);
                           0 = default, < 0 = run exactly max_iters           */
     int32_t max_iters; /* 20                                                 */
+    int32_t rule;      /* 0: the residual rule above.  1: pr_pull_cpu's rule
+                          as written (src/kernels/cpu/pr.cuh:165-181): a
+                          vertex takes its new score only when it moves by
+                          more than 5e-8; stop after an iteration that moved
+                          no vertex (tolerance unused).  hc_pagerank only.  */
 } hc_pr_opts_t;
 
 /* PageRank pull, Jacobi iteration from 1/N (src/kernels/cpu/pr.cuh:33-65

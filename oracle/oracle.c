@@ -471,6 +471,45 @@ int32_t or_pr_gauss_seidel_f32(int32_t n, const int64_t *index,
     return sweeps;
 }
 
+/* The stopping rule of pr_pull_cpu as written (src/kernels/cpu/pr.cuh:33-65,
+ * 165-181) in Jacobi form: every vertex computes its new score from the
+ * previous sweep's scores; it takes the new score only when that moves it
+ * by more than 5e-8 (std::fabs in FP32), and the loop stops after a sweep in
+ * which no vertex moved, or after max_iters sweeps.  Returns the sweeps run. */
+int32_t or_pr_jacobi_rule_f32(int32_t n, const int64_t *index, const int32_t *nbr,
+                              float damping, int32_t max_iters, float *scores) {
+    const float eps = 0.00000005f;
+    float *cur = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    float *nxt = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    const float base = (1.0f - damping) / (float)n;
+    for (int32_t i = 0; i < n; i++) cur[i] = 1.0f / (float)n;
+    int32_t sweeps = 0;
+    long long updated = 1;
+    while (updated != 0 && sweeps < max_iters) {
+        updated = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : updated)
+        for (int32_t u = 0; u < n; u++) {
+            float total = 0.0f;
+            for (int64_t i = index[u]; i < index[u + 1]; i++) {
+                int32_t v = nbr[i];
+                total += cur[v] / (float)(index[v + 1] - index[v]);
+            }
+            const float ns = base + damping * total;
+            const int moved = fabsf(ns - cur[u]) > eps;
+            nxt[u] = moved ? ns : cur[u];
+            updated += moved;
+        }
+        float *t = cur;
+        cur = nxt;
+        nxt = t;
+        sweeps++;
+    }
+    memcpy(scores, cur, sizeof(float) * (size_t)n);
+    free(cur);
+    free(nxt);
+    return sweeps;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Segmentation                                                              */
 /* ------------------------------------------------------------------------ */

@@ -55,6 +55,8 @@ void or_pr_jacobi_f64(int32_t n, const int64_t *index, const int32_t *nbr,
 int32_t or_pr_gauss_seidel_f32(int32_t n, const int64_t *index,
                                const int32_t *nbr, int32_t max_iters,
                                float *scores);
+int32_t or_pr_jacobi_rule_f32(int32_t n, const int64_t *index, const int32_t *nbr,
+                              float damping, int32_t max_iters, float *scores);
 
 /* Segmentation */
 void or_equal_edge_ranges(int32_t n, const int64_t *index, int32_t segments,

@@ -30,6 +30,7 @@ _SIGS = {
     "or_pr_jacobi_f32": ([C.c_int32, _P, _P, C.c_float, C.c_int32, _P], None),
     "or_pr_jacobi_f64": ([C.c_int32, _P, _P, C.c_double, C.c_int32, _P], None),
     "or_pr_gauss_seidel_f32": ([C.c_int32, _P, _P, C.c_int32, _P], C.c_int32),
+    "or_pr_jacobi_rule_f32": ([C.c_int32, _P, _P, C.c_float, C.c_int32, _P], C.c_int32),
     "or_equal_edge_ranges": ([C.c_int32, _P, C.c_int32, C.c_int32, _P], None),
 }
 _lib = None
@@ -151,6 +152,17 @@ def pr_gauss_seidel_f32(index, nbr, max_iters):
     n = index.size - 1
     s = np.empty(n, np.float32)
     it = lib().or_pr_gauss_seidel_f32(n, _p(index), _p(nbr), max_iters, _p(s))
+    return s, int(it)
+
+
+def pr_jacobi_rule_f32(index, nbr, damping, max_iters):
+    """(scores, sweeps) of the Jacobi iteration under pr_pull_cpu's stopping
+    rule (src/kernels/cpu/pr.cuh:165-181): a vertex takes its new score only
+    when it moves by more than 5e-8; stop after a sweep that moved none."""
+    index, nbr, _ = _csr(index, nbr)
+    n = index.size - 1
+    s = np.empty(n, np.float32)
+    it = lib().or_pr_jacobi_rule_f32(n, _p(index), _p(nbr), damping, max_iters, _p(s))
     return s, int(it)
 
 

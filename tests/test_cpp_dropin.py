@@ -15,8 +15,8 @@ BIN = os.path.join(CPP, "bin", "verify")
 
 
 def _verify_binary():
-    if not os.path.exists(BIN):
-        subprocess.run(["make", "-C", CPP], check=True, capture_output=True)
+    # make rebuilds the program when a header it includes has changed.
+    subprocess.run(["make", "-C", CPP], check=True, capture_output=True)
     return BIN
 
 

@@ -169,6 +169,41 @@ def test_pagerank_gauss_seidel_reaches_jacobi_fixed_point(oracle, hc):
     assert np.abs(gs - fix).max() < 1e-4 * fix.max()
 
 
+def test_pagerank_per_vertex_rule(oracle, hc):
+    """pr_pull_cpu's stopping rule in Jacobi form: on a cycle every score is
+    already the fixed point 1/N (one sweep, nothing moves); on a Kronecker
+    graph the first sweep is the plain Jacobi sweep wherever a vertex moved
+    by more than 5e-8 and the start value elsewhere, and at the stop no
+    vertex is more than 5e-8 (plus FP32 rounding) from its FP64 update."""
+    n = 1000
+    idx = np.arange(0, 2 * n + 1, 2, dtype=np.int64)
+    nb = np.array([[(u - 1) % n, (u + 1) % n] for u in range(n)], np.int32).ravel()
+    s, sweeps = oracle.pr_jacobi_rule_f32(idx, nb, 0.85, 201)
+    assert sweeps == 1 and np.all(s == np.float32(1.0 / n))
+
+    g = hc.HostGraph.generate("kron", 14, 16)
+    idx, nb = g.index, g.neighbors
+    n = idx.size - 1
+    init = np.float32(1.0) / np.float32(n)
+    one, sweeps = oracle.pr_jacobi_rule_f32(idx, nb, 0.85, 1)
+    j1 = oracle.pr_jacobi_f32(idx, nb, 0.85, 1)
+    moved = np.abs(j1 - init) > np.float32(5e-8)
+    assert sweeps == 1 and 0 < (~moved).sum() < n
+    assert np.array_equal(one, np.where(moved, j1, init))
+
+    s, sweeps = oracle.pr_jacobi_rule_f32(idx, nb, 0.85, 201)
+    assert 1 < sweeps < 201
+    deg = np.diff(idx).astype(np.float64)
+    contrib = np.where(deg > 0, s.astype(np.float64) / np.maximum(deg, 1.0), 0.0)
+    rows = np.repeat(np.arange(n), np.diff(idx))
+    upd = 0.15 / n + 0.85 * np.bincount(rows, weights=contrib[nb], minlength=n)
+    assert np.abs(upd - s).max() <= 5e-8 + 1e-10
+    # The last sweep moved nothing; the one before it moved vertices.
+    s1, _ = oracle.pr_jacobi_rule_f32(idx, nb, 0.85, sweeps - 1)
+    s2, _ = oracle.pr_jacobi_rule_f32(idx, nb, 0.85, sweeps - 2)
+    assert np.array_equal(s1, s) and not np.array_equal(s2, s)
+
+
 def test_bfs_pull_epoch_first_neighbour(oracle, hc):
     """epoch_bfs_pull_cpu restatement vs an exhaustive scan."""
     g = hc.HostGraph.generate("kron", 10, 8, weighted=False)
