@@ -11,11 +11,11 @@
 //    neighbours with compare-and-swap; winners are appended to the next
 //    frontier (wavefront-aggregated) and their degrees summed (scout count).
 //  * Direction-optimizing loop, switching rule of bfs_do_cpu<alpha,beta>
-//    (src/kernels/cpu/bfs.cuh:48-114), evaluated on the device by a
-//    controller kernel between epochs.  Bottom-up epochs emit the next
-//    frontier as a bitmap only; top-down epochs emit bitmap and queues; the
-//    compaction kernel builds queues from the bitmap when the direction turns
-//    from bottom-up to top-down.
+//    (src/kernels/cpu/bfs.cuh:48-114), evaluated on the device by the
+//    between-epochs launch.  Bottom-up epochs emit the next frontier as a
+//    bitmap only; top-down epochs emit bitmap and queues; the same launch
+//    builds queues from the bitmap when the direction turns from bottom-up
+//    to top-down.
 //  * A bottom-up epoch lists the vertices it leaves unvisited (per-workgroup
 //    list segments, no global counter); the next bottom-up epoch sweeps only
 //    those.  One-lane rows read their first neighbour from a per-vertex
