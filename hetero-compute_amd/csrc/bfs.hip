@@ -201,7 +201,9 @@ __device__ __forceinline__ void bu_commit(const GraphView<OffT> &g,
             visited[u] = level;
             if (parents) parents[u] = par;
         }
-        if (wave_set_bit(next, u, found)) {
+        // Only u's own lane sets u's bit in a bottom-up epoch.
+        wave_or_bits(next, u, found);
+        if (found) {
             cnt++;
             edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
         }

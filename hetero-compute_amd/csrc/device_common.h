@@ -309,6 +309,26 @@ __device__ __forceinline__ bool wave_set_bit(uint32_t *bm, int32_t u, bool pred)
     return first;
 }
 
+// Sets bit u of bm for every lane with pred, lanes that share a word merged
+// into one atomicOr whose old value is not waited for.  For callers whose
+// lanes own their vertices in this launch (the bit is new exactly when pred).
+__device__ __forceinline__ void wave_or_bits(uint32_t *bm, int32_t u, bool pred) {
+    const uint32_t word = pred ? uint32_t(u) >> 5 : 0u;
+    const uint32_t bit = pred ? (1u << (u & 31)) : 0u;
+    uint64_t todo = __ballot(pred);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t w = __shfl(word, leader);
+        const bool mine = pred && word == w;
+        const uint64_t grp = __ballot(mine);
+        uint32_t bits = mine ? bit : 0u;
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) bits |= __shfl_xor(bits, s, 64);
+        if (lane_id() == leader) atomicOr(&bm[w], bits);
+        todo &= ~grp;
+    }
+}
+
 // Sum of a per-thread value over a 256-thread workgroup; valid in thread 0.
 __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
     __shared__ unsigned long long s_part[kWavesPerBlock];

@@ -222,8 +222,16 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
         } else {
             pull_group<1, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
         }
-        if (LOOP) {
+        if (LOOP && c == 0) {
+            // Several chunks may lower the same hub: count its bit once.
             if (wave_set_bit(next_bm, u, chg)) {
+                cnt++;
+                edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
+            }
+        } else if (LOOP) {
+            // Any other row is swept by one group only.
+            wave_or_bits(next_bm, u, chg);
+            if (chg) {
                 cnt++;
                 edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
             }
