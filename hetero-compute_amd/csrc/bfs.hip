@@ -111,17 +111,20 @@ __device__ __forceinline__ bool bu_scan(const GraphView<OffT> &g,
 // With `first` (one lane): the visited flag, the first neighbour and the row
 // bounds are read together, so most rows cost one round of coalesced loads
 // and a bitmap probe instead of a dependent gather into the edge array.
+// deg: the row's degree when its bounds were read here (-1 otherwise), so the
+// commit does not read them again.
 template <int G, bool CHECK, typename OffT>
 __device__ __forceinline__ void bu_vertex(const GraphView<OffT> &g,
                                           const int32_t *__restrict__ visited,
                                           const int32_t *__restrict__ first,
                                           const Staged<uint32_t> &frontier, int32_t uu,
                                           int lane, int gbase, bool &found, bool &open,
-                                          int32_t &par, uint32_t &scanned) {
+                                          int32_t &par, uint32_t &scanned, int64_t &deg) {
     if (G == 1 && first) {
         const int32_t vis = CHECK ? visited[uu] : -1;
         const int32_t f = first[uu];
         const OffT b = g.rbeg(uu), e = g.rend(uu);
+        deg = int64_t(e - b);
         if (vis < 0) {
             int32_t pr = -1;
             const bool fnd = bu_scan<1, true>(g, frontier, b, e, 0, 0, pr, scanned, f);
@@ -155,7 +158,9 @@ __device__ __forceinline__ void bu_vertex(const GraphView<OffT> &g,
     }
     if (!CHECK || visited[uu] < 0) {
         int32_t pr = -1;
-        bool f = bu_scan<G>(g, frontier, g.rbeg(uu), g.rend(uu), lane, gbase, pr, scanned);
+        const OffT b = g.rbeg(uu), e = g.rend(uu);
+        deg = int64_t(e - b);
+        bool f = bu_scan<G>(g, frontier, b, e, lane, gbase, pr, scanned);
         if (lane == 0) {
             found = f;
             open = !f;
@@ -171,7 +176,7 @@ __device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
                                          const int32_t *__restrict__ first,
                                          const Staged<uint32_t> &frontier,
                                          bool &found, bool &open, int32_t &u, int32_t &par,
-                                         uint32_t &scanned) {
+                                         uint32_t &scanned, int64_t &deg) {
     constexpr int PER = kBlock / G;
     const int gid = threadIdx.x / G, lane = threadIdx.x % G;
     const int gbase = lane_id() - lane;
@@ -180,7 +185,7 @@ __device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
         int32_t uu = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
         u = uu;
         bu_vertex<G, true>(g, visited, first, frontier, uu, lane, gbase, found, open, par,
-                           scanned);
+                           scanned, deg);
     }
 }
 
@@ -194,7 +199,7 @@ __device__ __forceinline__ void bu_commit(const GraphView<OffT> &g,
                                           int32_t *__restrict__ parents, int32_t level,
                                           uint32_t *__restrict__ next, int32_t *rem,
                                           uint32_t *rem_cnt, bool found, bool open,
-                                          int32_t u, int32_t par, uint32_t &cnt,
+                                          int32_t u, int32_t par, int64_t deg, uint32_t &cnt,
                                           unsigned long long &edges_acc) {
     if (LOOP) {
         if (found) {
@@ -205,7 +210,7 @@ __device__ __forceinline__ void bu_commit(const GraphView<OffT> &g,
         wave_or_bits(next, u, found);
         if (found) {
             cnt++;
-            edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
+            edges_acc += (unsigned long long)(deg >= 0 ? deg : int64_t(g.rend(u) - g.rbeg(u)));
         }
         if (rem) wave_append(rem, rem_cnt, open, u);
     } else if (found) {
@@ -234,16 +239,20 @@ __device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
         const int k = unit - p.unit_start[c];
         bool found = false, open = false;
         int32_t u = 0, par = -1;
+        int64_t deg = -1;
         if (c == 2)
-            bu_group<64>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned);
+            bu_group<64>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned,
+                         deg);
         else if (c == 3)
-            bu_group<16>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned);
+            bu_group<16>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned,
+                         deg);
         else if (c == 4)
-            bu_group<4>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned);
+            bu_group<4>(p, c, k, g, visited, first_g, frontier, found, open, u, par, scanned,
+                        deg);
         else if (c == 5)
-            bu_group<1>(p, c, k, g, visited, first, frontier, found, open, u, par, scanned);
-        bu_commit<LOOP>(g, visited, parents, level, next, rem, rem_cnt, found, open, u, par, cnt,
-                        edges_acc);
+            bu_group<1>(p, c, k, g, visited, first, frontier, found, open, u, par, scanned, deg);
+        bu_commit<LOOP>(g, visited, parents, level, next, rem, rem_cnt, found, open, u, par, deg,
+                        cnt, edges_acc);
     }
 }
 
@@ -274,15 +283,16 @@ __device__ void bu_list_sweep(const GraphView<OffT> &g, int32_t *__restrict__ de
         const uint32_t idx = base + gid;
         bool found = false, open = false;
         int32_t u = 0, par = -1;
+        int64_t deg = -1;
         if (idx < len) {
             u = list[idx];
             // Listed vertices are unvisited: only pull epochs ran since
             // they were listed, and each visits only the vertices it scans.
             bu_vertex<G, false>(g, depth, G == 1 ? first : nullptr, frontier, u, lane, gbase,
-                                found, open, par, scanned);
+                                found, open, par, scanned, deg);
         }
-        bu_commit<true>(g, depth, parents, level, next, rem, rem_cnt, found, open, u, par, cnt,
-                        edges_acc);
+        bu_commit<true>(g, depth, parents, level, next, rem, rem_cnt, found, open, u, par, deg,
+                        cnt, edges_acc);
     }
 }
 
@@ -302,12 +312,16 @@ __device__ __forceinline__ void td_range(const GraphView<OffT> &g,
         int32_t v = g.adj[i];
         scanned++;
         bool app = false;
-        if (depth[v] < 0 && atomicCAS(&depth[v], -1, level) == -1) {
+        // The neighbour's bounds are read with its depth (one round trip
+        // before the claim instead of one after it).
+        const int32_t dv = depth[v];
+        const OffT vb = g.rbeg(v), ve = g.rend(v);
+        if (dv < 0 && atomicCAS(&depth[v], -1, level) == -1) {
             app = true;
             if (parents) parents[v] = u;
             atomicOr(&next[v >> 5], 1u << (v & 31));
             cnt++;
-            edges_acc += (unsigned long long)(g.rend(v) - g.rbeg(v));
+            edges_acc += (unsigned long long)(ve - vb);
         }
         push_out_append(out, g, fb, ctl, nxt, app, v);
     }
