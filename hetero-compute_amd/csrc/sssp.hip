@@ -133,22 +133,27 @@ __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
                                            const DR &rd,
                                            const Staged<uint32_t> &changed,
                                            int32_t *__restrict__ dist, uint16_t *d16,
-                                           bool &chg, int32_t &u, uint32_t &gathered) {
+                                           bool &chg, int32_t &u, uint32_t &gathered,
+                                           int64_t &deg) {
     constexpr int PER = kBlock / G;
     const int gid = threadIdx.x / G, lane = threadIdx.x % G;
     const int idx = k * PER + gid;
     if (idx < p.cnt[c]) {
         int32_t uu = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
-        int32_t m = row_min<G, FILTER>(g, rd, changed, g.rbeg(uu), g.rend(uu), lane, gathered);
+        const OffT b = g.rbeg(uu), e = g.rend(uu);
+        // The row's own distance is read with its bounds, before the
+        // gathers: in a pull epoch only this group writes it.
+        const int32_t cur = lane == 0 ? dist[uu] : 0;
+        int32_t m = row_min<G, FILTER>(g, rd, changed, b, e, lane, gathered);
         if (G > 1) m = group_min<G>(m);
         if (lane == 0) {
-            int32_t cur = dist[uu];
             if (m < cur) {
                 dist[uu] = m;
                 if (d16) d16[uu] = dist16(m);
                 chg = true;
             }
             u = uu;
+            deg = int64_t(e - b);
         }
     }
 }
@@ -177,6 +182,7 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
         const int k = unit - p.unit_start[c];
         bool chg = false;
         int32_t u = 0;
+        int64_t deg = -1;  // the row's degree when its bounds were read here
         if (c <= 1) {
             int32_t uu;
             OffT b, e;
@@ -189,6 +195,7 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
                 uu = p.list[1] ? p.list[1][k] : p.beg[1] + k;
                 b = g.rbeg(uu);
                 e = g.rend(uu);
+                deg = int64_t(e - b);
             }
             int32_t m = block_min(row_min<kBlock, FILTER>(g, rd, changed, b, e, threadIdx.x,
                                                           gathered));
@@ -214,13 +221,13 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
                 u = uu;
             }
         } else if (c == 2) {
-            pull_group<64, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
+            pull_group<64, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered, deg);
         } else if (c == 3) {
-            pull_group<16, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
+            pull_group<16, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered, deg);
         } else if (c == 4) {
-            pull_group<4, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
+            pull_group<4, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered, deg);
         } else {
-            pull_group<1, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered);
+            pull_group<1, FILTER>(p, c, k, g, rd, changed, dist, d16, chg, u, gathered, deg);
         }
         if (LOOP && c == 0) {
             // Several chunks may lower the same hub: count its bit once.
@@ -233,7 +240,7 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
             wave_or_bits(next_bm, u, chg);
             if (chg) {
                 cnt++;
-                edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
+                edges_acc += (unsigned long long)(deg >= 0 ? deg : int64_t(g.rend(u) - g.rbeg(u)));
             }
         } else {
             cnt += chg ? 1u : 0u;
