@@ -169,6 +169,46 @@ __device__ __forceinline__ void bu_vertex(const GraphView<OffT> &g,
     }
 }
 
+// A one-lane row of the plan's last class read ahead of its use (visited
+// flag, first neighbour, bounds), and its bottom-up step once resolved:
+// the same probe and scan as bu_vertex<1> with first-neighbour probes.
+template <typename OffT>
+struct RowAhead {
+    int32_t u, vis, f;
+    OffT b, e;
+    bool ok;
+};
+
+template <typename OffT>
+__device__ __forceinline__ RowAhead<OffT> row_ahead(const PlanDev &p, int k,
+                                                    const GraphView<OffT> &g,
+                                                    const int32_t *__restrict__ visited,
+                                                    const int32_t *__restrict__ first) {
+    RowAhead<OffT> r;
+    const int idx = k * kBlock + threadIdx.x;
+    r.ok = idx < p.cnt[5];
+    r.u = r.ok ? (p.list[5] ? p.list[5][idx] : p.beg[5] + idx) : 0;
+    r.vis = r.ok ? visited[r.u] : 0;
+    r.f = r.ok ? first[r.u] : -1;
+    r.b = r.ok ? g.rbeg(r.u) : OffT(0);
+    r.e = r.ok ? g.rend(r.u) : OffT(0);
+    return r;
+}
+
+template <typename OffT>
+__device__ __forceinline__ void row_resolve(const RowAhead<OffT> &r, const GraphView<OffT> &g,
+                                            const Staged<uint32_t> &frontier, bool &found,
+                                            bool &open, int32_t &par, uint32_t &scanned,
+                                            int64_t &deg) {
+    deg = int64_t(r.e - r.b);
+    if (!r.ok || r.vis >= 0) return;
+    int32_t pr = -1;
+    const bool fnd = bu_scan<1, true>(g, frontier, r.b, r.e, 0, 0, pr, scanned, r.f);
+    found = fnd;
+    open = !fnd;
+    par = pr;
+}
+
 template <int G, typename OffT>
 __device__ __forceinline__ void bu_group(const PlanDev &p, int c, int k,
                                          const GraphView<OffT> &g,
@@ -237,6 +277,28 @@ __device__ void bu_sweep(const PlanDev &p, const GraphView<OffT> &g,
         int c = 0;
         while (unit >= p.unit_start[c + 1]) c++;
         const int k = unit - p.unit_start[c];
+        if (LOOP && c == 5 && first) {
+            // One-lane rows, the last class: this workgroup's remaining
+            // units in a pipeline that reads the next unit's rows before
+            // committing this unit's, so the commit's stores and atomics
+            // are not waited for before the next loads are in flight.
+            RowAhead<OffT> cur = row_ahead(p, k, g, visited, first);
+            for (;;) {
+                const int nunit = unit + (int)gridDim.x;
+                const bool more = nunit < total;
+                RowAhead<OffT> nx{0, 0, -1, OffT(0), OffT(0), false};
+                if (more) nx = row_ahead(p, nunit - p.unit_start[5], g, visited, first);
+                bool found = false, open = false;
+                int32_t par = -1;
+                int64_t deg = -1;
+                row_resolve(cur, g, frontier, found, open, par, scanned, deg);
+                bu_commit<LOOP>(g, visited, parents, level, next, rem, rem_cnt, found, open,
+                                cur.u, par, deg, cnt, edges_acc);
+                if (!more) return;
+                unit = nunit;
+                cur = nx;
+            }
+        }
         bool found = false, open = false;
         int32_t u = 0, par = -1;
         int64_t deg = -1;
@@ -279,6 +341,8 @@ __device__ void bu_list_sweep(const GraphView<OffT> &g, int32_t *__restrict__ de
     constexpr int PER = kBlock / G;
     const int gid = threadIdx.x / G, lane = threadIdx.x % G;
     const int gbase = lane_id() - lane;
+    // Listed vertices are unvisited: only pull epochs ran since they were
+    // listed, and each visits only the vertices it scans.
     for (uint32_t base = 0; base < len; base += PER) {
         const uint32_t idx = base + gid;
         bool found = false, open = false;
@@ -286,8 +350,6 @@ __device__ void bu_list_sweep(const GraphView<OffT> &g, int32_t *__restrict__ de
         int64_t deg = -1;
         if (idx < len) {
             u = list[idx];
-            // Listed vertices are unvisited: only pull epochs ran since
-            // they were listed, and each visits only the vertices it scans.
             bu_vertex<G, false>(g, depth, G == 1 ? first : nullptr, frontier, u, lane, gbase,
                                 found, open, par, scanned, deg);
         }
