@@ -166,7 +166,9 @@ struct KernelTimer {
         if (!on) return;
         if (used == (int)ev.size()) {
             hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) {
+            // Device-scope release: the timestamp is not delayed by a
+            // system-scope cache writeback after the kernel.
+            if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) {
                 on = false;
                 return;
             }
