@@ -402,9 +402,13 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
     const uint32_t total = u2 + (n2 + kSmallPerUnit - 1) / kSmallPerUnit;
     // Edges per unit: a hub chunk, a workgroup row (<= kHubDeg), wavefront
     // rows, or 8-lane rows.
-    const uint32_t unit_max = (uint32_t)max<int64_t>(ctl->chunk, kHubDeg);
+    // An upper bound of the staged count: the workgroup synchronizes to check
+    // for room only when the next unit could overfill the buffer.
+    uint32_t staged = 0;
     for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
-        if (unit != blockIdx.x) push_out_make_room(out, g, fb, ctl, nxt, unit_max);
+        const uint32_t umax = push_unit_max(unit, nch, ctl->chunk);
+        if (staged + umax > out.cap) staged = push_out_make_room(out, g, fb, ctl, nxt, umax);
+        staged += umax;
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
             td_range<kBlock>(g, depth, parents, level, next, fb, ctl, out, nxt, h.u,

@@ -289,18 +289,30 @@ __device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
 // Called by the whole workgroup between units of a push sweep: flushes the
 // staged output early when the next unit could overfill the buffer (a unit
 // appends at most `unit_max` entries: its edges), so the per-lane queue
-// atomics of the overflow path stay a last resort.
+// atomics of the overflow path stay a last resort.  Returns the staged count
+// afterwards (the same in every thread).  The second barrier keeps every
+// thread's read of the count ahead of the next unit's appends, so all
+// threads take the same branch.
 template <typename OffT>
-__device__ __forceinline__ void push_out_make_room(const PushOut &o, const GraphView<OffT> &g,
-                                                   const FrontierBufs &fb, StepCtl *ctl, int p,
-                                                   uint32_t unit_max) {
+__device__ __forceinline__ uint32_t push_out_make_room(const PushOut &o, const GraphView<OffT> &g,
+                                                       const FrontierBufs &fb, StepCtl *ctl,
+                                                       int p, uint32_t unit_max) {
     __syncthreads();
     const uint32_t n = *o.count;
-    if (n == 0 || n + unit_max <= o.cap) return;  // uniform: all threads read n after the barrier
+    __syncthreads();
+    if (n == 0 || n + unit_max <= o.cap) return n;
     push_out_flush(o, g, fb, ctl, p);
     __syncthreads();
     if (threadIdx.x == 0) *o.count = 0;
     __syncthreads();
+    return 0;
+}
+
+// Entries a push unit can append: a hub chunk's edges, or at most kHubDeg
+// (a workgroup row, four wavefront rows of at most kBigDeg, or 32 small rows
+// of at most kMidDeg).
+__device__ __forceinline__ uint32_t push_unit_max(uint32_t unit, uint32_t nch, int64_t chunk) {
+    return unit < nch ? (uint32_t)chunk : (uint32_t)kHubDeg;
 }
 
 // Chunk descriptors of a hub source (parity 0), written by one workgroup.

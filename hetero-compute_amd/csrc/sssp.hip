@@ -292,9 +292,13 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
     const uint32_t u1 = nch + n0;
     const uint32_t u2 = u1 + (n1 + kWavesPerBlock - 1) / kWavesPerBlock;
     const uint32_t total = u2 + (n2 + kSmallPerUnit - 1) / kSmallPerUnit;
-    const uint32_t unit_max = (uint32_t)max<int64_t>(ctl->chunk, kHubDeg);
+    // An upper bound of the staged count: the workgroup synchronizes to check
+    // for room only when the next unit could overfill the buffer.
+    uint32_t staged = 0;
     for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
-        if (unit != blockIdx.x) push_out_make_room(out, g, fb, ctl, nxt, unit_max);
+        const uint32_t umax = push_unit_max(unit, nch, ctl->chunk);
+        if (staged + umax > out.cap) staged = push_out_make_room(out, g, fb, ctl, nxt, umax);
+        staged += umax;
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
             push_range<kBlock>(g, dist, fb, ctl, out, nxt, wbm, h.u, OffT(h.b), OffT(h.e),
