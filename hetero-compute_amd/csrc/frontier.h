@@ -38,9 +38,7 @@
 namespace hc {
 
 constexpr int64_t kHubDeg = 1024;
-constexpr int64_t kMinChunkEdges = 128;
-constexpr int kTicketGroups = 64;   // first-level completion counters
-constexpr int kTicketStride = 32;   // words between counters (own cache lines)  // smallest push chunk (sizes the chunk lists)
+constexpr int64_t kMinChunkEdges = 128;  // smallest push chunk (sizes the chunk lists)
 constexpr int64_t kBigDeg = 256;
 constexpr int64_t kMidDeg = 16;
 constexpr int kSmallLanes = 8;
