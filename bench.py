@@ -74,9 +74,22 @@ def run_graph(hc, kind, scale, degree, steps, warmup, seed):
         sssp_ep.append(r.stats["epochs"])
     hc.synchronize()
     wall = time.time() - wall0
+    # With host buffers at the boundary (the reference-named drivers upload
+    # the graph and return host arrays): one query of each kind with its
+    # result copied back, timed on the host; the graph upload is upload_s.
+    t0 = time.perf_counter()
+    rb = hc.bfs(dg, int(sources[0]), want_parents=False)
+    t1 = time.perf_counter()
+    rs = hc.sssp(dg, int(sources[0]))
+    t2 = time.perf_counter()
+    host = {"graph_upload_s": round(upload_s, 3),
+            "csr_bytes": int(g.index.nbytes + g.neighbors.nbytes + g.weights.nbytes),
+            "bfs_with_result_ms": round((t1 - t0) * 1e3, 3), "bfs_loop_ms": round(rb.stats["ms"], 3),
+            "sssp_with_result_ms": round((t2 - t1) * 1e3, 3),
+            "sssp_loop_ms": round(rs.stats["ms"], 3)}
     out = {
         "g": g, "dg": dg, "n": n, "m": m, "sources": sources, "gen_s": gen_s,
-        "upload_s": upload_s, "wall_s": wall,
+        "upload_s": upload_s, "wall_s": wall, "host_buffers": host,
         "bfs_ms": np.array(bfs_ms), "sssp_ms": np.array(sssp_ms),
         "sssp_bytes": np.array(sssp_bytes), "sssp_launches": np.array(sssp_launch),
         "bfs_bytes": np.array(bfs_bytes), "bfs_launches": np.array(bfs_launch),
@@ -613,6 +626,7 @@ def main():
         "epochs_mean": {**{f"bfs_{k}": float(r["bfs_epochs"].mean()) for k, r in results.items()},
                         **{f"sssp_{k}": float(r["sssp_epochs"].mean())
                            for k, r in results.items()}},
+        "host_buffers": {k: r["host_buffers"] for k, r in results.items()},
         "pagerank": pr,
         "batched": batched,
         "serving": serving,
