@@ -155,6 +155,52 @@ def test_bfs_bottom_up_variants_exact(hc, gpu, oracle, kron16, uniform16, first,
         hc.reset_tuning()
 
 
+def _star_with_tail(leaves, tail):
+    """A hub joined to `leaves` leaves, one of which starts a path of `tail`
+    vertices; weights vary per edge (the same in both directions)."""
+    edges = [(0, i) for i in range(1, leaves + 1)]
+    prev = leaves
+    for j in range(tail):
+        edges.append((prev, leaves + 1 + j))
+        prev = leaves + 1 + j
+    n = leaves + 1 + tail
+    src = np.array([a for a, b in edges] + [b for a, b in edges], np.int32)
+    dst = np.array([b for a, b in edges] + [a for a, b in edges], np.int32)
+    w = np.array([(7 * a + 13 * b) % 255 + 1 for a, b in edges] * 2, np.int32)
+    return n, src, dst, w
+
+
+@pytest.mark.parametrize("leaves", [3000, 70000])
+def test_hub_source_and_tiny_graphs_exact(hc, gpu, oracle, leaves):
+    """Sources whose rows are split into hub chunks (a star centre of degree
+    3000 or 70000, past the hub and chunk-list thresholds), a leaf of the same
+    star, the far end of a path hanging off it, and graphs of one and two
+    vertices: distances and depths bit-exact in every direction."""
+    n, src, dst, w = _star_with_tail(leaves, 40)
+    order = np.lexsort((dst, src))
+    index = np.zeros(n + 1, np.int64)
+    np.add.at(index, src + 1, 1)
+    index = np.cumsum(index)
+    g = hc.HostGraph.from_csr(index, dst[order], w[order])
+    cases = [(g, [0, 1, leaves, n - 1])]
+    one = hc.HostGraph.from_csr(np.zeros(2, np.int64), np.zeros(0, np.int32),
+                                np.zeros(0, np.int32))
+    two = hc.HostGraph.from_csr(np.array([0, 1, 2], np.int64), np.array([1, 0], np.int32),
+                                np.array([9, 9], np.int32))
+    cases += [(one, [0]), (two, [0, 1])]
+    for gg, sources in cases:
+        with hc.DeviceGraph(gg) as dg:
+            for s in sources:
+                ref = oracle.sssp_dijkstra(gg.index, gg.neighbors, gg.weights, s)
+                depth = oracle.bfs_depth(gg.index, gg.neighbors, s)
+                for direction in (0, 1, 2):
+                    assert np.array_equal(hc.sssp(dg, s, direction=direction).values, ref), \
+                        (gg.num_nodes, s, direction)
+                    b = hc.bfs(dg, s, direction=direction)
+                    assert np.array_equal(b.values, depth), (gg.num_nodes, s, direction)
+                    assert oracle.verify_parents(depth, b.parents, s) == 0
+
+
 @pytest.mark.parametrize("grid,batch", [(1, 1), (3, 5), (4096, 2)])
 def test_between_epoch_grid_and_batch_exact(hc, gpu, oracle, kron16, uniform16, grid, batch):
     """Every workgroup of the between-epochs launch makes the controller's
