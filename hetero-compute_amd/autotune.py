@@ -48,6 +48,7 @@ KNOB_SPACE = {
         "sssp.alpha": [2, 4, 8, 16],
         "sssp.filter": [0, 1],
         "sssp.dist16": [0, 1],
+        "sssp.push_dedup": [0, 1],
         "sssp.stage": [0, 2048, 8192],
         "sssp.block_threshold": [1024, 4096, 16384],
         "pull.wave_threshold": [48, 96, 192],

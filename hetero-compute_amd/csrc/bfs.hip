@@ -385,7 +385,7 @@ __device__ __forceinline__ void td_range(const GraphView<OffT> &g,
             cnt++;
             edges_acc += (unsigned long long)(ve - vb);
         }
-        push_out_append(out, g, fb, ctl, nxt, app, v);
+        push_out_append(out, g, fb, ctl, nxt, app, v, cnt, edges_acc);
     }
 }
 
@@ -407,7 +407,8 @@ __device__ void td_sweep(const GraphView<OffT> &g, int32_t *__restrict__ depth,
     uint32_t staged = 0;
     for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
         const uint32_t umax = push_unit_max(unit, nch, ctl->chunk);
-        if (staged + umax > out.cap) staged = push_out_make_room(out, g, fb, ctl, nxt, umax);
+        if (staged + umax > out.cap)
+            staged = push_out_make_room(out, g, fb, ctl, nxt, umax, cnt, edges_acc);
         staged += umax;
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
@@ -495,7 +496,9 @@ __global__ __launch_bounds__(kBlock) void bfs_step_kernel(
         __shared__ uint32_t s_out;
         if (threadIdx.x == 0) s_out = 0;
         __syncthreads();
-        const PushOut out{reinterpret_cast<int32_t *>(s_words), &s_out, (uint32_t)push_cap};
+        // Claims by compare-and-swap are unique: no deduplication.
+        const PushOut out{reinterpret_cast<int32_t *>(s_words), &s_out, (uint32_t)push_cap,
+                          nullptr};
         td_sweep(g, depth, parents, level, next, fb, ctl, out, cur, nxt, cnt, eacc, scanned);
         push_out_flush(out, g, fb, ctl, nxt);
     }

@@ -101,6 +101,8 @@ struct Tuning {
     int sssp_dist16 = 2;   // SSSP pulls gather a 16-bit mirror of the distances
                            // (0 off, 1 on, 2: on for flat graphs)
     int sssp_filter = 0;   // SSSP pulls gather only neighbours changed last epoch
+    int sssp_push_dedup = 2;  // SSSP pushes deduplicate improved vertices when staging
+                              // them (1) or on the spot (0); 2: by graph shape
     int sssp_stage = 0;    // per-vertex entries staged in LDS by SSSP pull sweeps
     int64_t sssp_block_threshold = 4096;  // SSSP: degree above which a WG takes a row
     int64_t sssp_g16_threshold = 0;       // SSSP: ... 16 lanes take a row (0: by shape)

@@ -204,16 +204,30 @@ __device__ __forceinline__ void frontier_append(const GraphView<OffT> &g,
 // classified at the end of the sweep and reserved with one atomic per class
 // and workgroup.  Entries past `cap` go to the queues directly.
 
+// With `dedup` (SSSP: the next-frontier bitmap), staged entries are
+// candidates that may repeat (a vertex lowered by several edges of one
+// epoch); the flush keeps the entry that sets the vertex's bit first and
+// counts it, so the sweep's per-edge chain holds one returning atomic (the
+// minimum), not two.  Entries past `cap` are deduplicated on the spot.
 struct PushOut {
     int32_t *buf;      // LDS, cap entries
     uint32_t *count;   // LDS: entries appended (past cap: sent to the queues)
     uint32_t cap;
+    uint32_t *dedup;   // bitmap that deduplicates candidates, or nullptr
+};
+
+// Counts of the entries a deduplicating flush kept (frontier vertices and
+// their degree sum).
+struct PushKept {
+    uint32_t n;
+    unsigned long long e;
 };
 
 template <typename OffT>
 __device__ __forceinline__ void push_out_append(const PushOut &o, const GraphView<OffT> &g,
                                                 const FrontierBufs &fb, StepCtl *ctl, int p,
-                                                bool pred, int32_t v) {
+                                                bool pred, int32_t v, uint32_t &kept_n,
+                                                unsigned long long &kept_e) {
     const uint64_t m = __ballot(pred);
     if (m == 0) return;
     const int leader = __ffsll((unsigned long long)m) - 1;
@@ -223,15 +237,26 @@ __device__ __forceinline__ void push_out_append(const PushOut &o, const GraphVie
     const uint32_t pos = base + (uint32_t)__popcll(m & lanemask_lt());
     const bool fits = pos < o.cap;
     if (pred && fits) o.buf[pos] = v;
-    frontier_append(g, fb, ctl, p, pred && !fits, v);
+    bool direct = pred && !fits;
+    if (o.dedup && direct) {
+        const uint32_t bit = 1u << (v & 31);
+        direct = !(atomicOr(&o.dedup[v >> 5], bit) & bit);
+        if (direct) {
+            kept_n++;
+            kept_e += (unsigned long long)(g.rend(v) - g.rbeg(v));
+        }
+    }
+    frontier_append(g, fb, ctl, p, direct, v);
 }
 
 // Moves the staged entries into the queues of parity p (all threads of the
-// workgroup; called once, after the sweep).
+// workgroup; after the sweep, or earlier to make room).  Returns this
+// thread's share of the kept counts when deduplicating (zero otherwise).
 template <typename OffT>
-__device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
-                               const FrontierBufs &fb, StepCtl *ctl, int p) {
+__device__ PushKept push_out_flush(const PushOut &o, const GraphView<OffT> &g,
+                                   const FrontierBufs &fb, StepCtl *ctl, int p) {
     __shared__ uint32_t s_n[5], s_base[5];
+    PushKept kept{0u, 0ull};
     __syncthreads();
     const uint32_t n = min(*o.count, o.cap);
     if (threadIdx.x < 5) s_n[threadIdx.x] = 0;
@@ -240,6 +265,15 @@ __device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         const int32_t v = o.buf[i];
         const int64_t d = int64_t(g.rend(v) - g.rbeg(v));
+        if (o.dedup) {
+            const uint32_t bit = 1u << (v & 31);
+            if (atomicOr(&o.dedup[v >> 5], bit) & bit) {
+                o.buf[i] = -1;  // a repeat: dropped
+                continue;
+            }
+            kept.n++;
+            kept.e += (unsigned long long)d;
+        }
         const int c = degree_class(d);
         if (c < 3) {
             atomicAdd(&s_n[c], 1u);
@@ -262,7 +296,7 @@ __device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
         int32_t v = 0;
         int c = -1;
         OffT b = 0, e = 0;
-        if (i < n) {
+        if (i < n && o.buf[i] >= 0) {
             v = o.buf[i];
             b = g.rbeg(v);
             e = g.rend(v);
@@ -282,6 +316,7 @@ __device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
             }
         }
     }
+    return kept;
 }
 
 // Called by the whole workgroup between units of a push sweep: flushes the
@@ -294,12 +329,16 @@ __device__ void push_out_flush(const PushOut &o, const GraphView<OffT> &g,
 template <typename OffT>
 __device__ __forceinline__ uint32_t push_out_make_room(const PushOut &o, const GraphView<OffT> &g,
                                                        const FrontierBufs &fb, StepCtl *ctl,
-                                                       int p, uint32_t unit_max) {
+                                                       int p, uint32_t unit_max,
+                                                       uint32_t &kept_n,
+                                                       unsigned long long &kept_e) {
     __syncthreads();
     const uint32_t n = *o.count;
     __syncthreads();
     if (n == 0 || n + unit_max <= o.cap) return n;
-    push_out_flush(o, g, fb, ctl, p);
+    const PushKept k = push_out_flush(o, g, fb, ctl, p);
+    kept_n += k.n;
+    kept_e += k.e;
     __syncthreads();
     if (threadIdx.x == 0) *o.count = 0;
     __syncthreads();

@@ -88,6 +88,7 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"sssp.alpha", &t.sssp_alpha},
                {"sssp.filter", &t.sssp_filter},
                {"sssp.dist16", &t.sssp_dist16},
+               {"sssp.push_dedup", &t.sssp_push_dedup},
                {"sssp.stage", &t.sssp_stage},
                {"sssp.grid_per_cu", &t.sssp_grid_per_cu},
                {"pull.stage", &t.stage_pull},
@@ -560,6 +561,8 @@ int hc_set_tuning(const char *key, int64_t value) {
                           !strcmp(key, "sssp.filter");
         if (!strcmp(key, "sssp.dist16"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "sssp.dist16 must be 0, 1 or 2");
+        if (!strcmp(key, "sssp.push_dedup"))
+            HC_CHECK_ARG(value >= 0 && value <= 2, "sssp.push_dedup must be 0, 1 or 2");
         if (!strcmp(key, "bu.first"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "bu.first must be 0, 1 or 2");
         if (!strcmp(key, "loop.batch"))
@@ -571,7 +574,7 @@ int hc_set_tuning(const char *key, int64_t value) {
                          "bu.remaining must be 0, 1, 4 or 16");
         const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining") ||
                              !strcmp(key, "bu.first") || !strcmp(key, "sssp.alpha") ||
-                             !strcmp(key, "sssp.dist16") ||
+                             !strcmp(key, "sssp.dist16") || !strcmp(key, "sssp.push_dedup") ||
                              !strcmp(key, "bu.stage_min");
         HC_CHECK_ARG(value >= (zero_ok ? 0 : 1) && value <= (flag ? 1 : 1 << 20),
                      "%s out of range", key);

@@ -264,19 +264,20 @@ __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
     for (OffT i = b + OffT(lane); i < e; i += OffT(G)) {
         int32_t v = g.adj[i];
         int32_t nd = relax(du, g.wt[i]);
-        bool app = false;
-        if (nd < dist[v]) {
-            int32_t old = atomicMin(&dist[v], nd);
-            if (nd < old) {
-                uint32_t bit = 1u << (v & 31);
-                app = !(atomicOr(&wbm[v >> 5], bit) & bit);
+        bool lowered = false;
+        if (nd < dist[v]) lowered = nd < atomicMin(&dist[v], nd);
+        if (!out.dedup && lowered) {
+            // The first edge to set v's bit appends v, now.
+            const uint32_t bit = 1u << (v & 31);
+            lowered = !(atomicOr(&wbm[v >> 5], bit) & bit);
+            if (lowered) {
+                cnt++;
+                edges_acc += (unsigned long long)(g.rend(v) - g.rbeg(v));
             }
         }
-        if (app) {
-            cnt++;
-            edges_acc += (unsigned long long)(g.rend(v) - g.rbeg(v));
-        }
-        push_out_append(out, g, fb, ctl, nxt, app, v);
+        // Deferred: a lowered neighbour is a candidate; the staging keeps
+        // its first copy (bit `wbm`) and counts it there.
+        push_out_append(out, g, fb, ctl, nxt, lowered, v, cnt, edges_acc);
     }
 }
 
@@ -297,7 +298,8 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
     uint32_t staged = 0;
     for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
         const uint32_t umax = push_unit_max(unit, nch, ctl->chunk);
-        if (staged + umax > out.cap) staged = push_out_make_room(out, g, fb, ctl, nxt, umax);
+        if (staged + umax > out.cap)
+            staged = push_out_make_room(out, g, fb, ctl, nxt, umax, cnt, edges_acc);
         staged += umax;
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
@@ -336,7 +338,7 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
 template <typename OffT, bool D16>
 __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, uint16_t *d16, FrontierBufs fb,
-    StepCtl *ctl, int32_t nstage, int32_t push_cap) {
+    StepCtl *ctl, int32_t nstage, int32_t push_cap, int defer_dedup) {
     extern __shared__ int32_t s_stage[];
     if (ctl->done) return;
     const int mode = ctl->mode;
@@ -376,9 +378,11 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
         __shared__ uint32_t s_out;
         if (threadIdx.x == 0) s_out = 0;
         __syncthreads();
-        const PushOut out{s_stage, &s_out, (uint32_t)push_cap};
+        const PushOut out{s_stage, &s_out, (uint32_t)push_cap, defer_dedup ? wbm : nullptr};
         sssp_push_sweep(g, dist, fb, ctl, out, cur, nxt, wbm, cnt, eacc);
-        push_out_flush(out, g, fb, ctl, nxt);
+        const PushKept k = push_out_flush(out, g, fb, ctl, nxt);
+        cnt += k.n;
+        eacc += k.e;
     }
     write_partials(fb.part, cnt, eacc, gathered);
 }
@@ -718,6 +722,7 @@ struct SsspLaunch {
     int32_t nstage;
     int32_t lds_entries;
     int32_t push_cap;
+    int defer_dedup;
     KernelTimer *kt;
     // Epoch k: header slot k & 1, counter set k % 3.
     static int launch(void *arg, long long k) {
@@ -728,11 +733,12 @@ struct SsspLaunch {
         a->kt->mark(a->s);
         if (a->d16)
             sssp_step_kernel<OffT, true><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t), a->s>>>(
-                a->plan, a->g, a->dist, a->d16, fb, cur, a->nstage, a->push_cap);
+                a->plan, a->g, a->dist, a->d16, fb, cur, a->nstage, a->push_cap,
+                a->defer_dedup);
         else
             sssp_step_kernel<OffT, false><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t),
                                             a->s>>>(a->plan, a->g, a->dist, nullptr, fb, cur,
-                                                    a->nstage, a->push_cap);
+                                                    a->nstage, a->push_cap, a->defer_dedup);
         a->kt->mark(a->s);
         sssp_between_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(
             a->g, fb, cur, nxt, a->ctl, sum_set(a->fb.part, k + 1), a->dist, a->d16);
@@ -773,9 +779,13 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
         max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter, tuning().push_chunk, d16);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
+    // Push epochs deduplicate improved vertices when staging them (flat
+    // graphs: few repeats) or on the spot (skewed graphs: hubs' neighbours
+    // are lowered by many edges, and repeats would fill the staging).
+    const int defer_dedup = tuning().sssp_push_dedup == 2 ? (flat_graph(g) ? 1 : 0)
+                                                          : tuning().sssp_push_dedup;
     SsspLaunch<OffT> L{ph->dev, v, dist, d16, ws->f.fb, ws->f.ctl, grid, s, nstage, lds_entries,
-                       push_cap,
-                       &ws->f.kt};
+                       push_cap, defer_dedup, &ws->f.kt};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
     const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;
     float ms = 0;

@@ -237,6 +237,11 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   bu.first               bottom-up rows test their first neighbour from a
  *                          per-vertex array (n int32 per graph) before the
  *                          edge array: 1 one-lane rows, 2 all rows, 0 off (2)
+ *   sssp.push_dedup        SSSP push epochs deduplicate the vertices they lower
+ *                          while staging them in LDS (1: one returning atomic
+ *                          per lowering edge) or on the spot (0: two); 2
+ *                          (default): 1 on flat graphs, 0 on skewed ones,
+ *                          where hubs' neighbours repeat and fill the staging
  *   bu.remaining           a bottom-up epoch after another sweeps only the
  *                          vertices the previous one left unvisited, with
  *                          this many lanes per vertex (1, 4 or 16; 0: off)

@@ -232,6 +232,31 @@ def test_between_epoch_grid_and_batch_exact(hc, gpu, oracle, kron16, uniform16, 
         hc.reset_tuning()
 
 
+@pytest.mark.parametrize("dedup,stage", [(0, 8192), (1, 8192), (1, 64), (1, 0)])
+def test_sssp_push_dedup_exact(hc, gpu, oracle, kron16, uniform16, dedup, stage):
+    """Push epochs that deduplicate lowered vertices on the spot (0) or when
+    staging them (1), with the staging large, tiny (repeats overflow to the
+    on-the-spot path) or off: distances equal Dijkstra's, and every epoch's
+    frontier holds each vertex once (the trace's frontier sizes never exceed
+    the vertices reached).  Which epoch lowers a vertex depends on the order
+    of the atomics, so the epoch sequence itself is not compared."""
+    try:
+        hc.set_tuning("sssp.push_dedup", dedup)
+        hc.set_tuning("push.stage", stage)
+        for g in (kron16, uniform16):
+            with hc.DeviceGraph(g) as dg:
+                for s in map(int, g.pick_sources(3)):
+                    ref = oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, s)
+                    for direction in (0, 2):
+                        assert np.array_equal(hc.sssp(dg, s, direction=direction).values, ref), \
+                            (s, direction)
+                        reached = int((ref < oracle.INF).sum())
+                        for e in hc.epoch_trace(dg, "sssp"):
+                            assert e["frontier"] <= reached
+    finally:
+        hc.reset_tuning()
+
+
 def test_sssp_dist16_mirror_exact(hc, gpu, oracle, kron16, uniform16):
     """Pull gathers through the 16-bit distance mirror: bit-exact on both
     scale-16 graphs in every direction, on the golden graphs, and on a
