@@ -582,8 +582,10 @@ def main():
                                     seed=args.seed)
         unif = results["uniform"]["g"] if "uniform" in results else hc.HostGraph.generate(
             "uniform", args.scale, args.degree, weighted=True, seed=args.seed)
-        serving = run_serving(hc, oracle, big, unif, [1000, 5000, 20000], args.serve_queries,
-                              777, max(args.check, 4))
+        # 100 q/s sits below the loop's capacity (its latency is service time);
+        # 1k-20k q/s are the offered rates of the serving configuration.
+        serving = run_serving(hc, oracle, big, unif, [100, 1000, 5000, 20000],
+                              args.serve_queries, 777, max(args.check, 4))
         del big
 
     cpu = None

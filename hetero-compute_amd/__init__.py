@@ -96,7 +96,8 @@ class ScheduleBlock(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("memory_budget", C.c_int64)]
+    _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("memory_budget", C.c_int64),
+                ("min_batch_sssp", C.c_int32), ("min_batch_bfs", C.c_int32)]
 
 
 class QueryInfo(C.Structure):
@@ -851,9 +852,12 @@ class Server:
     """Resident/dynamic graphs and continuously batched BFS/SSSP queries
     (hc_server_*).  Host graphs added here are kept alive by the object."""
 
-    def __init__(self, slots: int = 4, max_batch: int = 64, memory_budget: int = 0):
+    def __init__(self, slots: int = 4, max_batch: int = 64, memory_budget: int = 0,
+                 min_batch_sssp: int = 0, min_batch_bfs: int = 0):
+        """min_batch_*: queries of one graph and kind that must wait before
+        they run as one batched sweep (0: the library's defaults, 8 and 64)."""
         self.h = _P()
-        o = ServerOpts(slots, max_batch, memory_budget)
+        o = ServerOpts(slots, max_batch, memory_budget, min_batch_sssp, min_batch_bfs)
         check(lib().hc_server_create(C.byref(o), C.byref(self.h)))
         self._graphs = []
 

@@ -236,12 +236,20 @@ void worker(hc_server_s *s, int slot) {
             continue;
         }
         // The oldest query and every waiting query of the same graph and
-        // kind, in arrival order, up to max_batch.
+        // kind, in arrival order, up to max_batch -- when enough of them
+        // wait for a batched sweep to beat single-source runs; otherwise
+        // the oldest query alone.
         const int64_t head = s->queue.front();
         const int gid = s->queries[head].graph, kind = s->queries[head].kind;
+        int waiting = 0;
+        for (auto it = s->queue.begin(); it != s->queue.end() && waiting < s->opts.max_batch;
+             ++it)
+            if (s->queries[*it].graph == gid && s->queries[*it].kind == kind) waiting++;
+        const int min_batch =
+            kind == HC_QUERY_BFS ? s->opts.min_batch_bfs : s->opts.min_batch_sssp;
+        const int take = waiting >= min_batch ? s->opts.max_batch : 1;
         std::vector<int64_t> batch;
-        for (auto it = s->queue.begin(); it != s->queue.end() &&
-                                         (int)batch.size() < s->opts.max_batch;) {
+        for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < take;) {
             const Query &q = s->queries[*it];
             if (q.graph == gid && q.kind == kind) {
                 batch.push_back(*it);
@@ -394,6 +402,8 @@ int hc_server_create(const hc_server_opts_t *opts, hc_server_t *out) {
     s->opts.slots = opts && opts->slots > 0 ? opts->slots : 4;
     s->opts.max_batch = opts && opts->max_batch > 0 ? opts->max_batch : kWidth;
     s->opts.memory_budget = opts ? opts->memory_budget : 0;
+    s->opts.min_batch_sssp = opts && opts->min_batch_sssp > 0 ? opts->min_batch_sssp : 8;
+    s->opts.min_batch_bfs = opts && opts->min_batch_bfs > 0 ? opts->min_batch_bfs : kWidth;
     if (s->opts.slots > 16 || s->opts.max_batch > kWidth) {
         delete s;
         return hc::fail(HC_ERR_INVALID, "slots must be <= 16 and max_batch <= %d", kWidth);

@@ -32,6 +32,12 @@ typedef struct hc_server_opts {
     int32_t max_batch;     /* queries per batched sweep, 1..64 (default 64)   */
     int64_t memory_budget; /* HBM bytes for results and dynamic graphs
                               (default: half of the free HBM)                 */
+    int32_t min_batch_sssp; /* a batched SSSP sweep runs when at least this
+                               many SSSP queries of one graph wait; fewer run
+                               one at a time (default 8: a 64-wide sweep costs
+                               about 8 single-source runs)                     */
+    int32_t min_batch_bfs;  /* the same for BFS (default 64: a batched BFS
+                               sweep pays only at full width, if at all)       */
 } hc_server_opts_t;
 
 typedef struct hc_query_info {

@@ -63,8 +63,9 @@ def test_mix_equals_alone(hc, gpu, oracle, graphs):
         t = srv.submit(gid, kind, src)
         alone.append(_answers(srv, [t])[0][0])
     srv.close()
-    # Mix: every query submitted at once to a batching multi-slot server.
-    srv, ids = _server(hc, graphs, slots=4, max_batch=64)
+    # Mix: every query submitted at once to a multi-slot server that
+    # batches any two or more waiting queries of one graph and kind.
+    srv, ids = _server(hc, graphs, slots=4, max_batch=64, min_batch_sssp=2, min_batch_bfs=2)
     tickets = [srv.submit(gid, kind, src) for gid, kind, src, _ in qs]
     mixed = _answers(srv, tickets)
     st = srv.stats()
@@ -86,7 +87,8 @@ def test_memory_budget_evicts_dynamic_graphs(hc, gpu, oracle, graphs):
     g_max = max(8 * (g.num_nodes + 1) + 8 * g.num_edges + 48 * g.num_nodes
                 for g in graphs["dyn"])
     budget = n_max * 4 * 64 + g_max + 65536
-    srv = hc.Server(slots=2, max_batch=64, memory_budget=budget)
+    srv = hc.Server(slots=2, max_batch=64, memory_budget=budget, min_batch_sssp=2,
+                    min_batch_bfs=2)
     dyn = [srv.add_graph(g, resident=False) for g in graphs["dyn"]]
     rng = np.random.RandomState(3)
     qs = []
@@ -104,6 +106,28 @@ def test_memory_budget_evicts_dynamic_graphs(hc, gpu, oracle, graphs):
         ref = (oracle.bfs_depth(g.index, g.neighbors, src) if kind == 0
                else oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
         assert np.array_equal(m, ref)
+
+
+def test_below_batch_threshold_runs_single_source(hc, gpu, oracle, graphs):
+    """Fewer waiting queries of one graph and kind than the batching
+    threshold run one at a time on the single-source loops: with the BFS
+    threshold at 64, twenty BFS queries submitted at once all answer from
+    runs of their own, exactly; SSSP queries (threshold 2) may share a
+    sweep and stay exact."""
+    srv = hc.Server(slots=2, max_batch=64, min_batch_sssp=2, min_batch_bfs=64)
+    g = graphs["unif"]
+    gid = srv.add_graph(g)
+    srcs = [int(x) for x in g.pick_sources(20, seed=11)]
+    tickets = [srv.submit(gid, 0, s) for s in srcs] + [srv.submit(gid, 1, s) for s in srcs[:8]]
+    res = _answers(srv, tickets)
+    srv.close()
+    for i, (m, info) in enumerate(res):
+        s = srcs[i % 20] if i < 20 else srcs[i - 20]
+        if i < 20:
+            assert info["batch_size"] == 1, info
+            assert np.array_equal(m, oracle.bfs_depth(g.index, g.neighbors, s))
+        else:
+            assert np.array_equal(m, oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, s))
 
 
 def test_server_rejects_bad_queries(hc, gpu, graphs):
