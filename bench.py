@@ -155,9 +155,9 @@ def class_sweep(hc, kind, scale, degree, seed, reps=3):
         deg = np.diff(g.index)
         t = hc.tuning()
         # The SSSP plan's class thresholds (sssp_plan_config).
-        wave = t["pull.wave_threshold"]
+        wave = t["sssp.wave_threshold"]
         block = max(t["sssp.block_threshold"], wave)
-        g16 = min(t["sssp.g16_threshold"], wave)
+        g16 = min(t["sssp.g16_threshold"] or 24, wave)
         thr = [("hub chunks", max(t["pull.hub_threshold"], block)), ("workgroup", block),
                ("wavefront", wave), ("16 lanes", g16),
                ("4 lanes", min(t["sssp.g4_threshold"], g16)), ("1 lane", 0)]

@@ -215,7 +215,7 @@ _SIGS = {
 }
 
 TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "sssp.filter", "sssp.stage", "sssp.grid_per_cu",
-               "sssp.block_threshold", "sssp.g16_threshold", "sssp.g4_threshold", "pull.stage", "bu.stage", "bu.remaining", "push.chunk", "push.stage", "bu.first", "sssp.dist16", "sssp.push_dedup", "compact.grid", "bu.stage_min", "loop.batch", "profile.kernel_events",
+               "sssp.block_threshold", "sssp.wave_threshold", "sssp.g16_threshold", "sssp.g4_threshold", "pull.stage", "bu.stage", "bu.remaining", "push.chunk", "push.stage", "bu.first", "sssp.dist16", "sssp.push_dedup", "compact.grid", "bu.stage_min", "loop.batch", "profile.kernel_events",
                "multi.graph", "multi.grid_per_cu", "pull.hub_threshold", "pull.hub_chunk",
                "pull.block_threshold", "pull.wave_threshold", "pull.g16_threshold",
                "pull.g4_threshold", "bu.wave_threshold", "bu.g16_threshold",

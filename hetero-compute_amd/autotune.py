@@ -51,8 +51,8 @@ KNOB_SPACE = {
         "sssp.push_dedup": [0, 1],
         "sssp.stage": [0, 2048, 8192],
         "sssp.block_threshold": [1024, 4096, 16384],
-        "pull.wave_threshold": [48, 96, 192],
-        "sssp.g16_threshold": [24, 48, 96],
+        "sssp.wave_threshold": [48, 96, 192],
+        "sssp.g16_threshold": [12, 24, 48],
         "sssp.g4_threshold": [5, 8, 16],
         "pull.hub_threshold": [16384, 65536],
         "sssp.grid_per_cu": [2, 4, 8, 16],

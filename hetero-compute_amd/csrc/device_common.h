@@ -105,7 +105,8 @@ struct Tuning {
                               // them (1) or on the spot (0); 2: by graph shape
     int sssp_stage = 0;    // per-vertex entries staged in LDS by SSSP pull sweeps
     int64_t sssp_block_threshold = 4096;  // SSSP: degree above which a WG takes a row
-    int64_t sssp_g16_threshold = 0;       // SSSP: ... 16 lanes take a row (0: by shape)
+    int64_t sssp_wave_threshold = 96;     // SSSP: ... a wavefront takes a row
+    int64_t sssp_g16_threshold = 0;       // SSSP: ... 16 lanes take a row (0: 24)
     int64_t sssp_g4_threshold = 8;        // SSSP: ... 4 lanes take a row (else 1)
     int stage_pull = 8192; // per-vertex entries staged in LDS by pull sweeps
     int stage_bu = 8192;   // frontier-bitmap words staged by bottom-up sweeps
@@ -138,14 +139,12 @@ struct Tuning {
     }
 };
 Tuning &tuning();
-// Plan of the SSSP pull sweeps: the shared pull classes with SSSP's own
-// workgroup threshold.
-// flat: a graph whose top 1% rows hold under 5% of the edges (sssp.g16
-// threshold 0 picks 24 for those, 48 for skewed graphs).
-inline PlanConfig sssp_plan_config(bool flat) {
+// Plan of the SSSP pull sweeps: the shared pull hub classes with SSSP's own
+// workgroup, wavefront and lane-group thresholds.
+inline PlanConfig sssp_plan_config() {
     PlanConfig c = tuning().pull;
-    const int64_t g16 = tuning().sssp_g16_threshold > 0 ? tuning().sssp_g16_threshold
-                                                         : (flat ? 24 : 48);
+    c.wave_threshold = tuning().sssp_wave_threshold;
+    const int64_t g16 = tuning().sssp_g16_threshold > 0 ? tuning().sssp_g16_threshold : 24;
     c.g16_threshold = std::min<int64_t>(g16, c.wave_threshold);
     c.g4_threshold = std::min<int64_t>(tuning().sssp_g4_threshold, c.g16_threshold);
     c.block_threshold = std::max<int64_t>(tuning().sssp_block_threshold, c.wave_threshold);

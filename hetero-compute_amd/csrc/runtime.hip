@@ -63,6 +63,7 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
         int64_t *p;
     } i64[] = {
         {"sssp.block_threshold", &t.sssp_block_threshold},
+        {"sssp.wave_threshold", &t.sssp_wave_threshold},
         {"sssp.g16_threshold", &t.sssp_g16_threshold},
         {"sssp.g4_threshold", &t.sssp_g4_threshold},
         {"pull.hub_threshold", &t.pull.hub_threshold},

@@ -685,7 +685,7 @@ static int launch_pull_epoch(hc_graph_s *g, int mapping, int32_t start,
             break;
         default: {
             const PlanHost *ph = nullptr;
-            HC_TRY(get_plan(g, start, end, sssp_plan_config(flat_graph(g)), &ph));
+            HC_TRY(get_plan(g, start, end, sssp_plan_config(), &ph));
             if (ph->total_units == 0) return HC_OK;
             int nb = std::max(1, std::min(ph->total_units, grid_blocks()));
             const int32_t ns = std::min<int32_t>(tuning().sssp_stage, g->n);
@@ -755,7 +755,7 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     SsspWs *ws = nullptr;
     HC_TRY(get_sssp_ws(g, &ws));
     const PlanHost *ph = nullptr;
-    HC_TRY(get_plan(g, 0, g->n, sssp_plan_config(flat_graph(g)), &ph));
+    HC_TRY(get_plan(g, 0, g->n, sssp_plan_config(), &ph));
     GraphView<OffT> v = view<OffT>(g);
     if (alpha <= 0) alpha = tuning().sssp_alpha > 0 ? tuning().sssp_alpha : (flat_graph(g) ? 2 : 4);
     uint16_t *d16 = nullptr;

@@ -212,10 +212,11 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *                          for flat graphs only (2)
  *   sssp.stage             distances staged in LDS by SSSP pull sweeps (0)
  *   sssp.block_threshold   SSSP: degree above which a workgroup takes a row
- *                          (4096; hub and wavefront classes are pull.*'s)
+ *                          (4096; the hub class is pull.*'s)
+ *   sssp.wave_threshold    SSSP: degree above which a wavefront takes a
+ *                          row (96)
  *   sssp.g16_threshold / sssp.g4_threshold   SSSP: degree above which 16 / 4
- *                          lanes take a row (0 / 8; 0: 48 for skewed graphs,
- *                          24 for flat ones)
+ *                          lanes take a row (0 / 8; 0: 24)
  *   pull.hub_threshold     degree above which a row is split into chunks
  *   pull.hub_chunk         edges per chunk (>= 256)
  *   pull.block_threshold   degree above which a workgroup takes a row
