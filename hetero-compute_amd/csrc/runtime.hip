@@ -402,6 +402,14 @@ int hc_get_device_info(hc_device_info_t *out) {
     std::memset(out, 0, sizeof(*out));
     std::snprintf(out->name, sizeof(out->name), "%s", prop.name);
     std::snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
+    if (out->name[0] == '\0') {
+        // The marketing name comes from a table that some driver installs lack;
+        // name the device by its ISA and CU count instead of leaving it blank.
+        const char *colon = std::strchr(prop.gcnArchName, ':');
+        int isa_len = colon ? (int)(colon - prop.gcnArchName) : (int)std::strlen(prop.gcnArchName);
+        std::snprintf(out->name, sizeof(out->name), "AMD %.*s (%d CUs)", isa_len,
+                      prop.gcnArchName, prop.multiProcessorCount);
+    }
     out->compute_units = prop.multiProcessorCount;
     out->wavefront_size = prop.warpSize;
     out->hbm_bytes = prop.totalGlobalMem;
