@@ -128,6 +128,7 @@ _SIGS = {
     "hc_init": ([C.c_int], C.c_int),
     "hc_get_device_info": ([C.POINTER(DeviceInfo)], C.c_int),
     "hc_device_synchronize": ([], C.c_int),
+    "hc_mem_get_info": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
     "hc_dmalloc": ([C.POINTER(_P), C.c_size_t], C.c_int),
     "hc_dfree": ([_P], C.c_int),
     "hc_hmalloc_pinned": ([C.POINTER(_P), C.c_size_t], C.c_int),
@@ -186,6 +187,8 @@ _SIGS = {
     "hc_server_fetch": ([_P, C.c_int64, _P], C.c_int),
     "hc_server_release": ([_P, C.c_int64], C.c_int),
     "hc_server_get_stats": ([_P, C.POINTER(ServerStats)], C.c_int),
+    "hc_server_graph_bytes": ([_P, C.POINTER(HostGraphStruct), C.c_int32,
+                               C.POINTER(C.c_int64)], C.c_int),
     "hc_cpu_epoch_ms": ([C.POINTER(HostGraphStruct), C.c_int32, C.c_int32, C.c_int32,
                          C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "hc_kernel_from_string": ([C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
@@ -273,6 +276,13 @@ def device_info() -> dict:
     return {"name": d.name.decode(), "arch": d.arch.decode(),
             "compute_units": d.compute_units, "wavefront_size": d.wavefront_size,
             "hbm_bytes": d.hbm_bytes, "lds_bytes_per_block": d.lds_bytes_per_block}
+
+
+def mem_get_info() -> tuple[int, int]:
+    """(free, total) device bytes of the current GPU."""
+    f, t = C.c_size_t(), C.c_size_t()
+    check(lib().hc_mem_get_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def synchronize():
@@ -889,6 +899,12 @@ class Server:
 
     def release(self, ticket: int):
         check(lib().hc_server_release(self.h, ticket))
+
+    def graph_bytes(self, g: HostGraph, resident: bool) -> int:
+        """Bytes the memory budget charges for g (hc_server_graph_bytes)."""
+        b = C.c_int64()
+        check(lib().hc_server_graph_bytes(self.h, g.struct_ref(), int(resident), C.byref(b)))
+        return b.value
 
     def stats(self) -> dict:
         st = ServerStats()

@@ -68,6 +68,24 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     return HC_OK;
 }
 
+int64_t share_workspace_bound(int64_t n, int64_t m) {
+    const int64_t words = (n + 31) / 32;
+    // frontier_ws_alloc: six queues, two hub-chunk lists (a chunk per
+    // kMinChunkEdges edges of every hub row, plus one per hub, bounded by
+    // m / kMinChunkEdges + m / kHubDeg + 1), three bitmaps, partials, control.
+    const int64_t chunks = m / kMinChunkEdges + m / kHubDeg + 1;
+    const int64_t frontier = 6 * 4 * n + 2 * (int64_t)sizeof(HubChunk) * chunks +
+                             3 * 4 * words + (int64_t)sizeof(unsigned long long) *
+                             kSumWords * kSumSets + 64 * 1024;
+    const int64_t sssp = frontier + 2 * n;                    // + 16-bit mirror
+    const int64_t bfs = frontier + 8 * n + 4 * (3 * (int64_t)grid_blocks() + 1);  // + depth, first
+    const int64_t multi = 3 * 4 * ((n + 3) / 4) + 4096;        // change flags
+    // Plans: id lists (graphs that are not degree-sorted) for the SSSP,
+    // bottom-up and batched sweeps, and their hub-chunk descriptors.
+    const int64_t plans = 3 * (4 * n + (int64_t)sizeof(HubChunk) * chunks + 4096);
+    return sssp + bfs + multi + plans;
+}
+
 void frontier_ws_free(FrontierWs *ws) {
     if (ws->dmem) hipFree(ws->dmem);
     if (ws->h_ctl) hipHostFree(ws->h_ctl);

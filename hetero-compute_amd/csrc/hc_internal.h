@@ -17,6 +17,13 @@ namespace hc {
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
 
+// Upper bound on the device bytes the workspaces of one graph handle can
+// hold once every single-source and batched algorithm has run on it (SSSP
+// and BFS frontier queues, bitmaps and hub-chunk lists, BFS depth and
+// first-neighbour arrays, the 16-bit distance mirror, batched-sweep flags,
+// plans), for a graph of n vertices and m directed edges.
+int64_t share_workspace_bound(int64_t n, int64_t m);
+
 }  // namespace hc
 
 #define HC_CHECK_ARG(cond, ...)                                   \

@@ -404,9 +404,13 @@ int hc_sssp_heterogeneous(hc_hetero_t h, const int32_t *init_dist, int32_t *dist
             HX_HIP(hipMemsetAsync(h->d_cnt + i, 0, 4, b.stream));
             HC_TRY(hc_sssp_pull_epoch(b.dg, b.mapping, b.start, b.end, h->d_state,
                                       h->d_cnt + i, b.sb.blocks, b.sb.threads, b.stream));
-            HX_HIP(hipMemcpyAsync(h->h_state + b.start, h->d_state + b.start,
-                                  size_t(b.end - b.start) * 4, hipMemcpyDeviceToHost,
-                                  b.stream));
+            // Host cores read the GPU-owned ranges in the next epoch only
+            // when the schedule has CPU blocks; otherwise the state comes
+            // back once, after the loop.
+            if (!h->cpu.empty())
+                HX_HIP(hipMemcpyAsync(h->h_state + b.start, h->d_state + b.start,
+                                      size_t(b.end - b.start) * 4, hipMemcpyDeviceToHost,
+                                      b.stream));
             HX_HIP(hipMemcpyAsync(h->h_cnt + i, h->d_cnt + i, 4, hipMemcpyDeviceToHost,
                                   b.stream));
         }
@@ -432,6 +436,8 @@ int hc_sssp_heterogeneous(hc_hetero_t h, const int32_t *init_dist, int32_t *dist
             HX_HIP(hipStreamSynchronize(cs));
         }
     } while (updated != 0);
+    if (ng && h->cpu.empty())
+        HX_HIP(hipMemcpy(h->h_state, h->d_state, size_t(n) * 4, hipMemcpyDeviceToHost));
     const double ms = now_ms() - t0;
     std::memcpy(dist, h->h_state, size_t(n) * 4);
     if (stats) {

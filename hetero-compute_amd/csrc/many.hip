@@ -14,6 +14,9 @@
 // frontier grows or exceeds n / beta), so depths and the number of push and
 // pull levels equal the single-graph run; parents are valid BFS parents.
 #include <algorithm>
+#include <chrono>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -41,6 +44,7 @@ struct hc_graph_batch_s {
     int32_t scratch_slots = 0;
     int32_t *queue = nullptr;  // work counter + statistics
     int32_t *d_src = nullptr;  // per-call sources
+    std::mutex mu;             // one call at a time per batch
 };
 
 struct Desc {
@@ -257,8 +261,8 @@ __global__ __launch_bounds__(kManyBlock) void bfs_many_kernel(
     const Desc *__restrict__ desc, const int32_t *__restrict__ order,
     const int32_t *__restrict__ sources, int num_graphs,
     const uint32_t *__restrict__ offsets, const int32_t *__restrict__ adjacency,
-    int32_t *__restrict__ depth, int32_t *__restrict__ parents, uint32_t *scratch,
-    int scratch_words, int *queue, unsigned long long *stats, int alpha, int beta) {
+    int32_t *__restrict__ depth, int32_t *__restrict__ parents, uint32_t *slots,
+    int slot_words, int *queue, unsigned long long *stats, int alpha, int beta) {
     extern __shared__ uint32_t lds_bm[];
     __shared__ int32_t heavy[kHeavyCap];
     __shared__ int s_nheavy, s_g;
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(kManyBlock) void bfs_many_kernel(
         else
             bfs_one_graph<true>(d, sources[gi], offsets + d.off_base, adjacency + d.edge_base,
                                 depth + d.node_base, par,
-                                scratch + (size_t)blockIdx.x * 3 * scratch_words, words, alpha,
+                                slots + (size_t)blockIdx.x * 3 * slot_words, words, alpha,
                                 beta, heavy, &s_nheavy, s_red, st);
     }
     // Levels are counted once per graph (thread 0); edges by every thread.
@@ -293,11 +297,88 @@ __global__ __launch_bounds__(kManyBlock) void bfs_many_kernel(
     }
 }
 
+// Clears the work counter and the statistics words of a call (a kernel, not
+// hipMemsetAsync, so that the launch sequence can be captured in a graph).
+__global__ void many_clear_kernel(int32_t *queue) {
+    if (threadIdx.x < 16) queue[threadIdx.x] = 0;
+}
+
 }  // namespace hc
 
 using namespace hc;
 
 namespace {
+
+struct ManyLaunch {
+    int32_t *d_src;        // device sources (num_graphs)
+    uint32_t *slots;       // bitmap slots, slot_words x 3 per workgroup
+    int slot_words;
+    int grid;
+    int32_t *queue;        // 16 words: counter, statistics
+};
+
+// Enqueues one persistent launch; with stats, waits for it and fills them
+// (ms: device time between events when `ev` are given, else the host-observed
+// time of the launch).
+int launch_many(hc_graph_batch_s *b, const ManyLaunch &L, const int32_t *sources,
+                int32_t *depth, int32_t *parents, int alpha, int beta, hc_run_stats_t *stats,
+                hipStream_t s, bool events) {
+    HC_HIP(hipMemcpyAsync(L.d_src, sources, sizeof(int32_t) * b->num_graphs,
+                          hipMemcpyHostToDevice, s));
+    many_clear_kernel<<<1, 64, 0, s>>>(L.queue);
+    HC_HIP(hipGetLastError());
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (events) {
+        HC_HIP(hipEventCreate(&e0));
+        HC_HIP(hipEventCreate(&e1));
+        HC_HIP(hipEventRecord(e0, s));
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    bfs_many_kernel<<<L.grid, kManyBlock, sizeof(uint32_t) * 3 * kLdsWords, s>>>(
+        b->desc, b->d_order, L.d_src, b->num_graphs, b->offsets, b->adj, depth, parents, L.slots,
+        L.slot_words, L.queue, reinterpret_cast<unsigned long long *>(L.queue + 2), alpha, beta);
+    HC_HIP(hipGetLastError());
+    if (!stats && !events) return HC_OK;
+    float ms = 0;
+    if (events) {
+        HC_HIP(hipEventRecord(e1, s));
+        HC_HIP(hipEventSynchronize(e1));
+        HC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    unsigned long long st[3] = {0, 0, 0};
+    HC_HIP(hipMemcpyAsync(st, L.queue + 2, sizeof(st), hipMemcpyDeviceToHost, s));
+    HC_HIP(hipStreamSynchronize(s));
+    if (!events)
+        ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                               t0).count();
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->ms = ms;
+        stats->push_epochs = (int32_t)st[0];
+        stats->pull_epochs = (int32_t)st[1];
+        stats->epochs = (int32_t)(st[0] + st[1]);
+        stats->launches = 1;
+        stats->reached = -1;
+        // Offsets (2 x 4 B per visited-check), ids of scanned edges, depth
+        // and parent initialisation and writes.
+        stats->alg_bytes = 4.0 * (double)st[2] + 16.0 * (double)b->total_nodes;
+        stats->kernel_ms = ms;
+        stats->kernel_launches = 1;
+    }
+    return HC_OK;
+}
+
+int check_many_args(hc_graph_batch_s *b, const int32_t *sources, const int32_t *depth) {
+    HC_CHECK_ARG(b && sources && depth, "NULL argument");
+    for (int i = 0; i < b->num_graphs; i++) {
+        const int32_t n = (int32_t)(b->node_base[i + 1] - b->node_base[i]);
+        HC_CHECK_ARG(sources[i] >= 0 && sources[i] < n, "graph %d: source %d outside [0,%d)", i,
+                     sources[i], n);
+    }
+    return HC_OK;
+}
 
 void free_batch(hc_graph_batch_s *b) {
     if (!b) return;
@@ -427,66 +508,27 @@ int hc_graph_batch_node_offsets(hc_graph_batch_t b, int64_t *out) {
 int hc_bfs_many(hc_graph_batch_t b, const int32_t *sources, int32_t *depth,
                 int32_t *parents, int32_t alpha, int32_t beta, hc_run_stats_t *stats,
                 hc_stream_t stream) {
-    HC_CHECK_ARG(b && sources && depth, "NULL argument");
+    HC_TRY(check_many_args(b, sources, depth));
     if (alpha <= 0) alpha = 15;
     if (beta <= 0) beta = 18;
-    for (int i = 0; i < b->num_graphs; i++) {
-        const int32_t n = (int32_t)(b->node_base[i + 1] - b->node_base[i]);
-        HC_CHECK_ARG(sources[i] >= 0 && sources[i] < n, "graph %d: source %d outside [0,%d)", i,
-                     sources[i], n);
-    }
     HC_TRY(ensure_init());
+    std::lock_guard<std::mutex> lk(b->mu);
     hipStream_t s = pick_stream(stream);
-    HC_HIP(hipMemcpyAsync(b->d_src, sources, sizeof(int32_t) * b->num_graphs,
-                          hipMemcpyHostToDevice, s));
     const int per_cu = 2;  // 48 KiB LDS, 512 threads and ~84 VGPRs per workgroup
     const int grid = std::max(1, std::min(b->num_graphs, device_state().cus * per_cu));
-    int scratch_words = 0;
+    int slot_words = 0;
     if (b->max_nodes > kLdsNodes) {
-        scratch_words = (b->max_nodes + 31) / 32;
+        slot_words = (b->max_nodes + 31) / 32;
         if (b->scratch_slots < grid) {
             if (b->scratch) hipFree(b->scratch);
             b->scratch = nullptr;
             b->scratch_slots = 0;
-            HC_HIP(hipMalloc(&b->scratch, sizeof(uint32_t) * 3 * (size_t)scratch_words * grid));
+            HC_HIP(hipMalloc(&b->scratch, sizeof(uint32_t) * 3 * (size_t)slot_words * grid));
             b->scratch_slots = grid;
         }
     }
-    // queue[0]: work counter; queue[2..7]: level/edge statistics.
-    HC_HIP(hipMemsetAsync(b->queue, 0, 64, s));
-    hipEvent_t e0, e1;
-    HC_HIP(hipEventCreate(&e0));
-    HC_HIP(hipEventCreate(&e1));
-    HC_HIP(hipEventRecord(e0, s));
-    bfs_many_kernel<<<grid, kManyBlock, sizeof(uint32_t) * 3 * kLdsWords, s>>>(
-        b->desc, b->d_order, b->d_src, b->num_graphs, b->offsets, b->adj, depth, parents, b->scratch,
-        scratch_words, b->queue, reinterpret_cast<unsigned long long *>(b->queue + 2), alpha,
-        beta);
-    HC_HIP(hipGetLastError());
-    HC_HIP(hipEventRecord(e1, s));
-    HC_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    HC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    unsigned long long st[3] = {0, 0, 0};
-    HC_HIP(hipMemcpyAsync(st, b->queue + 2, sizeof(st), hipMemcpyDeviceToHost, s));
-    HC_HIP(hipStreamSynchronize(s));
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->ms = ms;
-        stats->push_epochs = (int32_t)st[0];
-        stats->pull_epochs = (int32_t)st[1];
-        stats->epochs = (int32_t)(st[0] + st[1]);
-        stats->launches = 1;
-        stats->reached = -1;
-        // Offsets (2 x 4 B per visited-check), ids of scanned edges, depth
-        // and parent initialisation and writes.
-        stats->alg_bytes = 4.0 * (double)st[2] + 16.0 * (double)b->total_nodes;
-        stats->kernel_ms = ms;
-        stats->kernel_launches = 1;
-    }
-    return HC_OK;
+    const ManyLaunch L{b->d_src, b->scratch, slot_words, grid, b->queue};
+    return launch_many(b, L, sources, depth, parents, alpha, beta, stats, s, true);
 }
 
 }  // extern "C"

@@ -423,6 +423,13 @@ int hc_device_synchronize(void) {
     return HC_OK;
 }
 
+int hc_mem_get_info(size_t *free_bytes, size_t *total_bytes) {
+    HC_CHECK_ARG(free_bytes && total_bytes, "NULL argument");
+    HC_TRY(ensure_init());
+    HC_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return HC_OK;
+}
+
 int hc_dmalloc(void **ptr, size_t bytes) {
     HC_CHECK_ARG(ptr, "ptr is NULL");
     HC_TRY(ensure_init());

@@ -93,6 +93,17 @@ struct hc_server_s {
 
 namespace {
 
+// Budget charge of a graph: per-slot workspaces at their upper bound, plus
+// the CSR (device layout: 4- or 8-byte offsets, int32 ids and weights) for a
+// dynamic graph.
+int64_t graph_charge(const hc_server_s *s, const hc_host_graph_t *g, bool resident) {
+    const int64_t n = g->num_nodes, m = g->num_edges;
+    const int64_t off = (m < (int64_t(1) << 31) ? 4 : 8) * (n + 1);
+    const int64_t csr = off + 4 * m * (g->weights ? 2 : 1);
+    const int64_t ws = hc::share_workspace_bound(n, m) * s->opts.slots;
+    return resident ? ws : csr + ws;
+}
+
 int64_t node_bytes(const GraphEntry &g, int width) {
     return (int64_t)g.host.num_nodes * 4 * width;
 }
@@ -194,6 +205,11 @@ bool spill_one(hc_server_s *s, std::unique_lock<std::mutex> &lk) {
     lk.unlock();
     bool ok = true;
     {
+        // The result's lock is held until its device copy has gone back to
+        // the pool: a concurrent hc_server_fetch (which reads res->p under
+        // this lock) sees either the device copy before any reuse or the
+        // host copy.  Lock order: result, then server (fetch never holds
+        // both).
         std::lock_guard<std::mutex> rl(best->mu);
         try {
             best->host.resize(best->bytes / sizeof(int32_t));
@@ -202,18 +218,18 @@ bool spill_one(hc_server_s *s, std::unique_lock<std::mutex> &lk) {
         }
         ok = ok && hipMemcpy(best->host.data(), best->p, best->bytes, hipMemcpyDeviceToHost) ==
                        hipSuccess;
-    }
-    lk.lock();
-    best->spilling = false;
-    if (ok) {
-        s->pool.emplace(best->bytes, best->p);
-        s->pooled += (int64_t)best->bytes;
-        best->p = nullptr;
-        if (best->budgeted) s->in_use -= (int64_t)best->bytes;
-        best->budgeted = false;
-        s->st.spills++;
-    } else {
-        best->host.clear();
+        lk.lock();
+        best->spilling = false;
+        if (ok) {
+            s->pool.emplace(best->bytes, best->p);
+            s->pooled += (int64_t)best->bytes;
+            best->p = nullptr;
+            if (best->budgeted) s->in_use -= (int64_t)best->bytes;
+            best->budgeted = false;
+            s->st.spills++;
+        } else {
+            best->host.clear();
+        }
     }
     if (best->refs == 0) {  // released while spilling
         std::shared_ptr<Result> r = best;
@@ -466,15 +482,32 @@ int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t residen
     e.host = *g;
     e.resident = resident != 0;
     e.per_slot.assign(s->opts.slots, nullptr);
-    // Budgeted footprint of a dynamic graph: CSR plus per-slot workspaces
-    // (queues, bitmaps, flags), estimated at 24 bytes per vertex per slot.
-    const int64_t csr = 8ll * (g->num_nodes + 1) + 4ll * g->num_edges * (g->weights ? 2 : 1);
-    e.bytes = csr + 24ll * g->num_nodes * s->opts.slots;
+    e.bytes = graph_charge(s, g, e.resident);
+    if (e.resident) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        HC_CHECK_ARG(s->in_use + s->pooled + e.bytes <= s->opts.memory_budget,
+                     "the workspaces of a resident graph (%lld bytes) exceed the memory budget",
+                     (long long)e.bytes);
+    }
     if (e.resident) HC_TRY(hc_graph_upload(g, &e.dev));
     std::lock_guard<std::mutex> lk(s->mu);
-    if (e.resident) s->st.uploads++;
+    if (e.resident) {
+        // Resident graphs keep their per-slot workspaces for the server's
+        // lifetime: charged once, here.
+        s->in_use += e.bytes;
+        s->st.uploads++;
+        note_usage(s);
+    }
     s->graphs.push_back(std::move(e));
     *graph_id = (int32_t)s->graphs.size() - 1;
+    return HC_OK;
+}
+
+int hc_server_graph_bytes(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
+                          int64_t *bytes) {
+    HC_CHECK_ARG(s && g && bytes, "NULL argument");
+    HC_CHECK_ARG(g->num_nodes > 0 && g->index, "empty graph");
+    *bytes = graph_charge(s, g, resident != 0);
     return HC_OK;
 }
 

@@ -69,6 +69,7 @@ int hc_bfs_many(hc_graph_batch_t b, const int32_t *sources, int32_t *depth,
                 int32_t *parents, int32_t alpha, int32_t beta,
                 hc_run_stats_t *stats, hc_stream_t stream);
 
+
 #ifdef __cplusplus
 }
 #endif

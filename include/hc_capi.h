@@ -129,6 +129,8 @@ int hc_version(void);
 int hc_init(int device);
 int hc_get_device_info(hc_device_info_t *out);
 int hc_device_synchronize(void);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int hc_mem_get_info(size_t *free_bytes, size_t *total_bytes);
 int hc_dmalloc(void **ptr, size_t bytes);
 int hc_dfree(void *ptr);
 int hc_hmalloc_pinned(void **ptr, size_t bytes);

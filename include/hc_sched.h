@@ -120,9 +120,11 @@ int hc_hetero_free(hc_hetero_t h);
 
 /* sssp_pull_heterogeneous (src/kernels/heterogeneous/sssp.cuh:39-387): pull
  * epochs, GPU blocks on their streams while host cores run the CPU blocks
- * on pinned host state; GPU-owned ranges are copied to the host every epoch,
- * CPU-owned ranges to the device when another epoch follows; stop when no
- * vertex changed anywhere.  init_dist and dist are host arrays. */
+ * on pinned host state; when the schedule has CPU blocks, GPU-owned ranges
+ * are copied to the host every epoch and CPU-owned ranges to the device when
+ * another epoch follows (an all-GPU schedule copies the state back once, at
+ * the end); stop when no vertex changed anywhere.  init_dist and dist are
+ * host arrays. */
 int hc_sssp_heterogeneous(hc_hetero_t h, const int32_t *init_dist,
                           int32_t *dist, hc_run_stats_t *stats);
 /* pr_pull_heterogeneous (src/kernels/heterogeneous/pr.cuh:39-346) in the

@@ -74,6 +74,13 @@ int hc_server_destroy(hc_server_t s);
 int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
                         int32_t *graph_id);
 
+/* Bytes the budget charges for graph g: for a dynamic graph, its CSR plus
+ * the workspaces of one handle per slot (hc_graph_share), at their upper
+ * bound; for a resident graph (whose CSR is outside the budget), the
+ * per-slot workspaces only.  Resident graphs are charged when added. */
+int hc_server_graph_bytes(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
+                          int64_t *bytes);
+
 /* Enqueues a query; returns at once with a ticket. */
 int hc_server_submit(hc_server_t s, int32_t graph_id, int32_t kind, int32_t source,
                      int64_t *ticket);

@@ -382,6 +382,101 @@ double or_bfs_do(int32_t n, const int64_t *index, const int32_t *nbr,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Shortest-path certificate                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Checks k candidate distance vectors at once, in O(m k) with no search:
+ * column q of D (D[u * ld + q], u < n) claims the distances from src[q]
+ * under the pull relaxation d[u] = min over row u of d[v] + w (the rule of
+ * epoch_sssp_pull_cpu_one_to_one, src/kernels/cpu/sssp.cuh:155-182, and of
+ * the pull BFS epochs, src/kernels/cpu/bfs.cuh:305-335, with w = 1).  A
+ * column is accepted iff
+ *   (a) every entry is `unreached` or lies in [0, OR_INF), and d[src] = 0;
+ *   (b) no edge is violated: d[u] <= d[v] + w for every (v, w) of row u
+ *       (unreached counts as OR_INF, sums saturate at OR_INF);
+ *   (c) every reached u != src has a tight edge: some (v, w) of row u with
+ *       d[v] + w == d[u].
+ * With every weight >= 1 these are exactly the true distances: (b) makes d
+ * a feasible potential, so d <= the true distance; (c) gives each reached
+ * vertex a predecessor with a strictly smaller value, a chain that can only
+ * end at src, i.e. a path of length d[u].  Unreached vertices then have no
+ * reached neighbour (by (b)), so the reached set is src's component.
+ * wt NULL: unit weights (BFS depths).  bad[q] receives the number of
+ * vertices whose entry in column q breaks (a)-(c).  Returns the total, or
+ * -1 when a weight is below 1 or a source lies outside [0, n) (the proof
+ * above needs positive weights; graphs with zero weights are checked
+ * against Dijkstra instead). */
+int64_t or_certify_paths(int32_t n, const int64_t *index, const int32_t *nbr,
+                         const int32_t *wt, int32_t k, const int32_t *src,
+                         const int32_t *D, int64_t ld, int32_t unreached,
+                         int64_t *bad) {
+    for (int32_t q = 0; q < k; q++) {
+        bad[q] = 0;
+        if (src[q] < 0 || src[q] >= n) return -1;
+    }
+    const int64_t m = index[n];
+    if (wt) {
+        int64_t low = 0;
+#pragma omp parallel for reduction(+ : low)
+        for (int64_t i = 0; i < m; i++) low += wt[i] < 1;
+        if (low) return -1;
+    }
+    int64_t total = 0;
+#pragma omp parallel reduction(+ : total)
+    {
+        int32_t *du = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+        uint8_t *ok = (uint8_t *)malloc((size_t)k);   /* (a) holds, no violation yet */
+        uint8_t *tight = (uint8_t *)malloc((size_t)k);
+        int64_t *mine = (int64_t *)calloc((size_t)k, sizeof(int64_t));
+#pragma omp for schedule(dynamic, 512)
+        for (int32_t u = 0; u < n; u++) {
+            const int32_t *row = D + (int64_t)u * ld;
+            for (int32_t q = 0; q < k; q++) {
+                const int32_t x = row[q];
+                ok[q] = 1;
+                tight[q] = 0;
+                if (x == unreached) {
+                    du[q] = OR_INF;
+                } else if (x < 0 || x >= OR_INF) {
+                    ok[q] = 0;
+                    du[q] = OR_INF;
+                } else {
+                    du[q] = x;
+                }
+                if (u == src[q] && du[q] != 0) ok[q] = 0;
+            }
+            for (int64_t i = index[u]; i < index[u + 1]; i++) {
+                const int32_t *rv = D + (int64_t)nbr[i] * ld;
+                const int32_t w = wt ? wt[i] : 1;
+                for (int32_t q = 0; q < k; q++) {
+                    int32_t dv = rv[q];
+                    if (dv == unreached || dv < 0 || dv >= OR_INF) dv = OR_INF;
+                    const int32_t c = relax(dv, w);
+                    if (c < du[q]) ok[q] = 0;
+                    if (c == du[q]) tight[q] = 1;
+                }
+            }
+            for (int32_t q = 0; q < k; q++) {
+                const int reached_other = du[q] < OR_INF && u != src[q];
+                if (!ok[q] || (reached_other && !tight[q])) mine[q]++;
+            }
+        }
+        for (int32_t q = 0; q < k; q++) {
+            if (mine[q]) {
+#pragma omp atomic
+                bad[q] += mine[q];
+            }
+            total += mine[q];
+        }
+        free(du);
+        free(ok);
+        free(tight);
+        free(mine);
+    }
+    return total;
+}
+
+/* ------------------------------------------------------------------------ */
 /* PageRank                                                                  */
 /* ------------------------------------------------------------------------ */
 

@@ -47,6 +47,14 @@ double or_bfs_do(int32_t n, const int64_t *index, const int32_t *nbr,
                  int8_t *dirs, int32_t max_dirs, int32_t *ndirs,
                  int32_t nthreads);
 
+/* Certificate of k distance/depth vectors (column q of D, D[u*ld+q]) from
+ * src[q]: bad[q] = vertices breaking it; returns the total, -1 when a weight
+ * is < 1 or a source is out of range.  wt NULL = unit weights (BFS). */
+int64_t or_certify_paths(int32_t n, const int64_t *index, const int32_t *nbr,
+                         const int32_t *wt, int32_t k, const int32_t *src,
+                         const int32_t *D, int64_t ld, int32_t unreached,
+                         int64_t *bad);
+
 /* PageRank */
 void or_pr_jacobi_f32(int32_t n, const int64_t *index, const int32_t *nbr,
                       float damping, int32_t iters, float *scores);

@@ -32,6 +32,8 @@ _SIGS = {
     "or_pr_gauss_seidel_f32": ([C.c_int32, _P, _P, C.c_int32, _P], C.c_int32),
     "or_pr_jacobi_rule_f32": ([C.c_int32, _P, _P, C.c_float, C.c_int32, _P], C.c_int32),
     "or_equal_edge_ranges": ([C.c_int32, _P, C.c_int32, C.c_int32, _P], None),
+    "or_certify_paths": ([C.c_int32, _P, _P, _P, C.c_int32, _P, _P, C.c_int64, C.c_int32,
+                          _P], C.c_int64),
 }
 _lib = None
 
@@ -129,6 +131,39 @@ def bfs_do(index, nbr, src, alpha=15, beta=18, nthreads=0):
     ms = lib().or_bfs_do(n, _p(index), _p(nbr), src, alpha, beta, _p(parents), _p(dirs),
                          dirs.size, _p(nd), nthreads)
     return parents, ms, dirs[:int(nd[0])].copy()
+
+
+def certify_paths(index, nbr, wt, sources, D, unreached=INF):
+    """Certificate of distance (wt given) or depth (wt None) vectors:
+    D is a C-contiguous [n, ld] int32 matrix whose column q claims the
+    result from sources[q] (q < len(sources)).  Returns the per-column count
+    of vertices that break the certificate (all zero iff every column is the
+    exact answer; see or_certify_paths).  Raises ValueError for weights
+    below 1 or sources out of range, where the certificate proves nothing."""
+    index, nbr, wt = _csr(index, nbr, wt)
+    n = index.size - 1
+    D = np.ascontiguousarray(D, np.int32)
+    if D.ndim == 1:
+        D = D.reshape(n, 1)
+    src = np.ascontiguousarray(np.atleast_1d(sources), np.int32)
+    if D.shape[0] != n or D.shape[1] < src.size:
+        raise ValueError(f"D has shape {D.shape}, need ({n}, >= {src.size})")
+    bad = np.zeros(src.size, np.int64)
+    r = lib().or_certify_paths(n, _p(index), _p(nbr), _p(wt), src.size, _p(src), _p(D),
+                               D.shape[1], unreached, _p(bad))
+    if r < 0:
+        raise ValueError("certificate needs weights >= 1 and sources in range")
+    return bad
+
+
+def certify_sssp(index, nbr, wt, src, dist):
+    """Violations of one SSSP distance vector (HC_INF_DIST = unreachable)."""
+    return int(certify_paths(index, nbr, wt, [src], dist, INF)[0])
+
+
+def certify_bfs(index, nbr, src, depth):
+    """Violations of one BFS depth vector (-1 = unreachable)."""
+    return int(certify_paths(index, nbr, None, [src], depth, -1)[0])
 
 
 def pr_jacobi_f32(index, nbr, damping, iters):

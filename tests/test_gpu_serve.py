@@ -82,10 +82,16 @@ def test_mix_equals_alone(hc, gpu, oracle, graphs):
 
 def test_memory_budget_evicts_dynamic_graphs(hc, gpu, oracle, graphs):
     # Budget: room for one 64-wide result of the largest dynamic graph and
-    # little else, so dynamic graphs are evicted and re-uploaded.
+    # little else, so dynamic graphs are evicted and re-uploaded.  The
+    # device memory the server really holds (free-memory delta, taken after
+    # the queries with every uploaded graph and pooled buffer still held)
+    # must stay within the budget.
+    hc.synchronize()
+    free0, _ = hc.mem_get_info()
+    probe = hc.Server(slots=2)
+    g_max = max(probe.graph_bytes(g, False) for g in graphs["dyn"])
+    probe.close()
     n_max = max(g.num_nodes for g in graphs["dyn"])
-    g_max = max(8 * (g.num_nodes + 1) + 8 * g.num_edges + 48 * g.num_nodes
-                for g in graphs["dyn"])
     budget = n_max * 4 * 64 + g_max + 65536
     srv = hc.Server(slots=2, max_batch=64, memory_budget=budget, min_batch_sssp=2,
                     min_batch_bfs=2)
@@ -99,9 +105,13 @@ def test_memory_budget_evicts_dynamic_graphs(hc, gpu, oracle, graphs):
     tickets = [srv.submit(gid, kind, src) for gid, kind, src, _ in qs]
     res = _answers(srv, tickets)
     st = srv.stats()
+    hc.synchronize()
+    free1, _ = hc.mem_get_info()
     srv.close()
     assert st["evictions"] > 0 and st["uploads"] > len(graphs["dyn"])
     assert st["bytes_peak"] <= budget
+    # the server's streams and fetch buffer sit outside the budget: 8 MiB slack
+    assert free0 - free1 <= budget + (8 << 20), (free0 - free1, budget)
     for (gid, kind, src, g), (m, _) in zip(qs, res):
         ref = (oracle.bfs_depth(g.index, g.neighbors, src) if kind == 0
                else oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))

@@ -299,3 +299,84 @@ def test_equal_edge_ranges_restatement(oracle, hc):
             if exp[i] >= 0 and exp[i] % chunk:
                 exp[i] = (exp[i] // chunk + (exp[i] % chunk >= chunk // 2)) * chunk
         assert list(got) == exp
+
+
+# ---------------------------------------------------------------------------
+# bfs_do_cpu's push/pull switching rule, pinned by hand-derived sequences
+# ---------------------------------------------------------------------------
+# Each sequence below was worked out by hand from the rule of
+# src/kernels/cpu/bfs.cuh:63-101 (integer division; pull while
+# num_edges > edges_to_check / alpha; stay in pull while n > prev or
+# n > N / beta; num_edges = 1 after a pull phase; a push subtracts the
+# scout count before it runs), not by running a loop.  0 = push, 1 = pull.
+#
+# Star, centre + 8 leaves (N 9, m 16), from the centre: deg 8 > 16/15 = 1
+#   -> pull finds 8 (8 > 1: stay) -> pull finds 0 (0 <= 8, 0 <= 9/18 = 0).
+# Star, centre + 9 leaves (N 10, m 18), from a leaf: 1 > 18/15 = 1 is false
+#   -> push (scout 9) -> 9 > 17/15 -> pull 8 -> pull 0.
+# K_6 (m 30) from any vertex: 5 > 30/15 = 2 -> pull 5 -> pull 0.
+# Cycle C_12 from 0: 2 > 24/15 = 1 -> pull; N/18 = 0 keeps every pull that
+#   finds a vertex, so 7 pulls (levels 1..6, then an empty one).
+# Path P_50 from vertex 0 (m 98): pushes while 2 > e/15 fails; after push j
+#   e = 99 - 2j, first true at j = 35 (e = 29); the pull finds 1 node
+#   (1 <= 1 and 1 <= 50/18 = 2: leave); then 1 > 29/15 = 1 fails -> push
+#   (e 28), 2 > 1 -> pull, ... alternating for levels 37..48; push to 49
+#   (scout 1: 1 > 22/15 = 1 fails), and a last empty push.
+# Cycle C_40 from 0 (m 80, scout 4 per level after the first): pushes at
+#   e = 80, 78, 74, 70, 66, 62 (4 > e/15 false down to e = 62); at e = 58
+#   4 > 3 -> pull finds 2 (2 > 2 false, 2 > 40/18 = 2 false: leave); then
+#   push / pull alternate (e 57..52), the push at level 20 finds one vertex
+#   (scout 2 > 51/15 = 3 false) and the last push finds none.
+BFS_DO_SEQUENCES = {
+    "star8_centre": [1, 1],
+    "star9_leaf": [0, 1, 1],
+    "k6": [1, 1],
+    "cycle12": [1] * 7,
+    "path50": [0] * 35 + [1] + [0, 1] * 6 + [0, 0],
+    "cycle40": [0] * 6 + [1] + [0, 1] * 6 + [0, 0],
+}
+
+
+def _do_case(name):
+    if name == "star8_centre":
+        return csr_from_edges(9, [(0, i, 1) for i in range(1, 9)]), 0
+    if name == "star9_leaf":
+        return csr_from_edges(10, [(0, i, 1) for i in range(1, 10)]), 4
+    if name == "k6":
+        return csr_from_edges(6, [(i, j, 1) for i in range(6) for j in range(i + 1, 6)]), 3
+    if name == "cycle12":
+        return csr_from_edges(12, [(i, (i + 1) % 12, 1) for i in range(12)]), 0
+    if name == "path50":
+        return csr_from_edges(50, [(i, i + 1, 1) for i in range(49)]), 0
+    if name == "cycle40":
+        return csr_from_edges(40, [(i, (i + 1) % 40, 1) for i in range(40)]), 0
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", sorted(BFS_DO_SEQUENCES))
+def test_bfs_do_switching_sequence(oracle, name):
+    (idx, nb, _), src = _do_case(name)
+    for threads in (1, 4):
+        par, _, dirs = oracle.bfs_do(idx, nb, src, 15, 18, threads)
+        assert list(dirs) == BFS_DO_SEQUENCES[name], name
+        assert oracle.verify_parents(oracle.bfs_depth(idx, nb, src), par, src) == 0
+
+
+def test_bfs_do_switching_sequence_detects_slips(oracle):
+    """The same hand-derived sequences reject plausible slips of the rule:
+    alpha and beta swapped, and a push->pull test of >= instead of >
+    (equivalently, a threshold one lower)."""
+    (idx, nb, _), src = _do_case("path50")
+    _, _, swapped = oracle.bfs_do(idx, nb, src, 18, 15, 1)
+    assert list(swapped) != BFS_DO_SEQUENCES["path50"]
+    (idx, nb, _), src = _do_case("cycle40")
+    _, _, swapped = oracle.bfs_do(idx, nb, src, 18, 15, 1)
+    assert list(swapped) != BFS_DO_SEQUENCES["cycle40"]
+    # beta 17 keeps the first pull phase of C_40 going (2 > 40/17 = 2 is
+    # still false) but beta 21 (40/21 = 1) does: the sequence changes
+    _, _, b21 = oracle.bfs_do(idx, nb, src, 15, 21, 1)
+    assert list(b21) != BFS_DO_SEQUENCES["cycle40"]
+    # alpha 14 moves the first switch of P_50 (e/14 < 2 at e = 27, not 29)
+    (idx, nb, _), src = _do_case("path50")
+    _, _, a14 = oracle.bfs_do(idx, nb, src, 14, 18, 1)
+    assert list(a14) != BFS_DO_SEQUENCES["path50"]
