@@ -176,6 +176,8 @@ _SIGS = {
                              C.POINTER(C.c_int64), C.POINTER(C.c_size_t)], C.c_int),
     "hc_graph_batch_node_offsets": ([_P, _P], C.c_int),
     "hc_bfs_many": ([_P, _P, _P, _P, C.c_int32, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
+    "hc_bfs_many_noalloc": ([_P, _P, _P, _P, C.c_int32, C.c_int32, C.POINTER(RunStats), _P],
+                            C.c_int),
     "hc_graph_share": ([_P, C.POINTER(_P)], C.c_int),
     "hc_multi_extract": ([_P, C.c_int32, C.c_int32, C.c_int32, _P, _P], C.c_int),
     "hc_server_create": ([C.POINTER(ServerOpts), C.POINTER(_P)], C.c_int),
@@ -634,16 +636,21 @@ class GraphBatch:
 
     def bfs(self, sources, alpha: int = 15, beta: int = 18, want_parents: bool = True,
             depth: DeviceArray | None = None, parents: DeviceArray | None = None,
-            fetch: bool = True):
+            fetch: bool = True, noalloc: bool = False):
         """One BFS per graph in one launch; returns (depths, parents, stats)
-        with per-graph arrays split by node_offsets."""
+        with per-graph arrays split by node_offsets.  noalloc: run through
+        hc_bfs_many_noalloc, which uses the library's fixed scratch instead
+        of allocating bitmap slots."""
         src = np.ascontiguousarray(sources, np.int32)
+        if src.size != self.num_graphs:
+            raise ValueError(f"{src.size} sources for {self.num_graphs} graphs")
         dd = depth if depth is not None else DeviceArray(self.total_nodes, np.int32)
         dp = parents if parents is not None else (
             DeviceArray(self.total_nodes, np.int32) if want_parents else None)
         st = RunStats()
-        check(lib().hc_bfs_many(self.h, _ptr(src), dd.ptr, dp.ptr if dp else None, alpha,
-                                beta, C.byref(st), None))
+        fn = lib().hc_bfs_many_noalloc if noalloc else lib().hc_bfs_many
+        check(fn(self.h, _ptr(src), dd.ptr, dp.ptr if dp else None, alpha, beta,
+                 C.byref(st), None))
         if not fetch:
             return None, None, st.as_dict()
         o = self.node_offsets

@@ -389,6 +389,46 @@ void free_batch(hc_graph_batch_s *b) {
     delete b;
 }
 
+// Scratch of hc_bfs_many_noalloc, for callers that cannot allocate device
+// memory: the work counter and statistics words (padded to 256 bytes), the
+// sources of a call, and the global bitmap slots (three bitmaps per
+// workgroup) of graphs above the LDS limit.  Each GPU holds its own copy
+// from the first use of this file's kernels until the process ends; calls
+// that overlap on one GPU share it and race.
+constexpr int kNoAllocMaxGraphs = 1 << 16;
+constexpr size_t kNoAllocHeadBytes = 256;
+constexpr size_t kNoAllocSrcBytes = sizeof(int32_t) * kNoAllocMaxGraphs;
+constexpr size_t kNoAllocSlotBytes = size_t(32) << 20;
+constexpr size_t kNoAllocBytes = kNoAllocHeadBytes + kNoAllocSrcBytes + kNoAllocSlotBytes;
+constexpr int kNoAllocMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kNoAllocBytes / sizeof(uint32_t)];
+
+namespace {
+
+// Device address of `scratch` on the current GPU, looked up once per device.
+// The host-side name `scratch` is not a device address and is never passed
+// to a kernel or a copy.
+int noalloc_scratch(char **out) {
+    static std::mutex mu;
+    static char *ptrs[kNoAllocMaxDevices] = {};
+    int dev = 0;
+    HC_HIP(hipGetDevice(&dev));
+    HC_CHECK_ARG(dev >= 0 && dev < kNoAllocMaxDevices, "device %d above the %d supported", dev,
+                 kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[dev]) {
+        void *p = nullptr;
+        HC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        HC_CHECK_ARG(p != nullptr, "no device address for the scratch array");
+        ptrs[dev] = static_cast<char *>(p);
+    }
+    *out = ptrs[dev];
+    return HC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -529,6 +569,40 @@ int hc_bfs_many(hc_graph_batch_t b, const int32_t *sources, int32_t *depth,
     }
     const ManyLaunch L{b->d_src, b->scratch, slot_words, grid, b->queue};
     return launch_many(b, L, sources, depth, parents, alpha, beta, stats, s, true);
+}
+
+int hc_bfs_many_noalloc(hc_graph_batch_t b, const int32_t *sources, int32_t *depth,
+                        int32_t *parents, int32_t alpha, int32_t beta, hc_run_stats_t *stats,
+                        hc_stream_t stream) {
+    HC_TRY(check_many_args(b, sources, depth));
+    HC_CHECK_ARG(b->num_graphs <= kNoAllocMaxGraphs,
+                 "%d graphs exceed the %d that the allocation-free scratch holds",
+                 b->num_graphs, kNoAllocMaxGraphs);
+    if (alpha <= 0) alpha = 15;
+    if (beta <= 0) beta = 18;
+    HC_TRY(ensure_init());
+    char *base = nullptr;
+    HC_TRY(noalloc_scratch(&base));
+    std::lock_guard<std::mutex> lk(b->mu);
+    hipStream_t s = pick_stream(stream);
+    int grid = std::max(1, std::min(b->num_graphs, device_state().cus * 2));
+    int slot_words = 0;
+    if (b->max_nodes > kLdsNodes) {
+        // Fewer workgroups when the slots of a full grid do not fit: the
+        // persistent kernel takes graphs from its queue until none is left.
+        slot_words = (b->max_nodes + 31) / 32;
+        const size_t per_slot = sizeof(uint32_t) * 3 * (size_t)slot_words;
+        const size_t fit = kNoAllocSlotBytes / per_slot;
+        HC_CHECK_ARG(fit >= 1,
+                     "a graph of %d vertices needs %zu bytes of bitmap scratch, above the "
+                     "%zu-byte maximum of the allocation-free entry point",
+                     b->max_nodes, per_slot, kNoAllocSlotBytes);
+        grid = (int)std::min<size_t>((size_t)grid, fit);
+    }
+    const ManyLaunch L{reinterpret_cast<int32_t *>(base + kNoAllocHeadBytes),
+                       reinterpret_cast<uint32_t *>(base + kNoAllocHeadBytes + kNoAllocSrcBytes),
+                       slot_words, grid, reinterpret_cast<int32_t *>(base)};
+    return launch_many(b, L, sources, depth, parents, alpha, beta, stats, s, false);
 }
 
 }  // extern "C"

@@ -69,6 +69,23 @@ int hc_bfs_many(hc_graph_batch_t b, const int32_t *sources, int32_t *depth,
                 int32_t *parents, int32_t alpha, int32_t beta,
                 hc_run_stats_t *stats, hc_stream_t stream);
 
+/* hc_bfs_many without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as hc_bfs_many; its
+ * scratch (work counter, sources, bitmaps of graphs above 2^17 vertices) is
+ * a fixed 32.25 MiB array per GPU that the library holds until the process
+ * ends.  Limits: at most 65536 graphs, and a largest graph whose three
+ * bitmaps (12 bytes per 32 vertices) fit in 32 MiB (about 89 M vertices);
+ * above them the call fails with HC_ERR_INVALID.  Fewer workgroups run when
+ * the bitmaps of a full grid do not fit.  Calls that overlap on one GPU
+ * share the scratch and race: the caller runs one at a time per GPU.
+ * stats->ms is the host-observed time of the launch. */
+int hc_bfs_many_noalloc(hc_graph_batch_t b, const int32_t *sources,
+                        int32_t *depth, int32_t *parents, int32_t alpha,
+                        int32_t beta, hc_run_stats_t *stats,
+                        hc_stream_t stream);
+
 
 #ifdef __cplusplus
 }
