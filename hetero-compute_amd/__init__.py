@@ -164,6 +164,8 @@ _SIGS = {
                       _P], C.c_int),
     "hc_bfs": ([_P, C.c_int32, _P, _P, C.POINTER(BfsOpts), C.POINTER(RunStats), _P], C.c_int),
     "hc_pagerank": ([_P, _P, C.POINTER(PrOpts), C.POINTER(RunStats), _P], C.c_int),
+    "hc_pr_pull": ([_P, C.c_int, _P, _P, C.c_int, C.c_int, C.POINTER(PrOpts),
+                    C.POINTER(RunStats), _P], C.c_int),
     "hc_get_epoch_trace": ([_P, C.c_int, C.POINTER(EpochTrace)], C.c_int),
     "hc_set_tuning": ([C.c_char_p, C.c_int64], C.c_int),
     "hc_get_tuning": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
@@ -556,6 +558,25 @@ def pagerank(g: DeviceGraph, damping: float = 0.85, tolerance: float = 1e-4,
     return Result(d.get(), st.as_dict())
 
 
+def pr_pull(g: DeviceGraph, mapping: int = MAP_BUCKETED, init=None, blocks: int = 0,
+            threads: int = 0, damping: float = 0.85, tolerance: float = 1e-4,
+            max_iters: int = 20, rule: int = 0) -> Result:
+    """pr_pull_gpu's loop (hc_pr_pull): one epoch of the given mapping and
+    launch shape per iteration, from `init` (host scores, or 1/N)."""
+    d = DeviceArray(g.num_nodes, np.float32)
+    di = DeviceArray.from_host(np.asarray(init, np.float32)) if init is not None else None
+    st = RunStats()
+    o = PrOpts(damping, tolerance, max_iters, rule)
+    try:
+        check(lib().hc_pr_pull(g.h, mapping, di.ptr if di else None, d.ptr, blocks, threads,
+                               C.byref(o), C.byref(st), None))
+        return Result(d.get(), st.as_dict())
+    finally:
+        d.free()
+        if di:
+            di.free()
+
+
 # ---------------------------------------------------------------------------
 # Epoch API
 # ---------------------------------------------------------------------------
@@ -715,7 +736,8 @@ def gteps(num_edges_directed: int, ms: float) -> float:
 KERNELS = ("SSSP CPU one-to-one", "SSSP GPU one-to-one", "SSSP GPU warp-red",
            "SSSP GPU block-red", "SSSP GPU bucketed", "PR CPU one-to-one",
            "PR GPU one-to-one", "PR GPU warp-red", "PR GPU block-red", "PR GPU bucketed",
-           "BFS CPU pull", "BFS GPU one-to-one", "BFS GPU warp", "BFS GPU bucketed")
+           "BFS CPU pull", "BFS GPU one-to-one", "BFS GPU warp", "BFS GPU bucketed",
+           "BFS CPU push by node", "BFS CPU push by edge")
 DEVICES = ("Intel i7-9700K", "Intel Xeon E5-2686", "NVIDIA Quadro RTX 4000",
            "NVIDIA Tesla K80", "NVIDIA Tesla M60", "AMD Instinct MI355X", "Host CPU")
 DEV_CPU, DEV_GPU = 0, 1

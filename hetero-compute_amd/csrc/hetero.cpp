@@ -137,6 +137,7 @@ struct hc_hetero_s {
     uint32_t *d_front = nullptr, *d_next = nullptr;
     uint32_t *h_front = nullptr, *h_next = nullptr, *h_gnext = nullptr;
     int32_t *h_queue = nullptr;  // top-down window, N entries
+    int32_t push_kernel = HC_K_BFS_CPU_PUSH_BY_NODE;  // host top-down variant
     bool pinned = false;         // host state from hipHostMalloc
 
     ~hc_hetero_s() {
@@ -299,6 +300,28 @@ int hc_hetero_create(const hc_host_graph_t *g, hc_schedule_t s, hc_hetero_t *out
     std::vector<hc_schedule_block_t> sb(nb);
     for (int32_t i = 0; i < nb; i++) HC_TRY(hc_schedule_get_block(s, i, &sb[i]));
     const int algo = algo_of(sb[0].kernel);
+    // BFS push blocks (the top-down kernel the reference's push_pull_scheduler
+    // assigns to a push epoch, util/scheduler/scheduler/scheduler.py:402-451)
+    // pick the host-core top-down variant; they run over the whole frontier,
+    // so they take no part in the segment cover of the pull blocks.  A BFS
+    // schedule without pull blocks pulls on the host cores.
+    int32_t push_kernel = HC_K_BFS_CPU_PUSH_BY_NODE;
+    {
+        std::vector<hc_schedule_block_t> pull;
+        for (const hc_schedule_block_t &b : sb) {
+            if (b.kernel == HC_K_BFS_CPU_PUSH_BY_NODE || b.kernel == HC_K_BFS_CPU_PUSH_BY_EDGE)
+                push_kernel = b.kernel;
+            else
+                pull.push_back(b);
+        }
+        if (pull.size() != sb.size()) {
+            if (pull.empty())
+                pull.push_back(hc_schedule_block_t{HC_DEVICE_HOST_CPU, 0, 0, segs - 1,
+                                                   HC_K_BFS_CPU_PULL, 0, 0});
+            sb.swap(pull);
+            nb = (int32_t)sb.size();
+        }
+    }
     std::vector<int> owner(segs, -1);
     for (int32_t i = 0; i < nb; i++) {
         HC_CHECK_ARG(algo_of(sb[i].kernel) == algo,
@@ -320,6 +343,7 @@ int hc_hetero_create(const hc_host_graph_t *g, hc_schedule_t s, hc_hetero_t *out
     h->g = *g;
     if (algo != kSssp) h->g.weights = nullptr;
     h->algo = algo;
+    h->push_kernel = push_kernel;
     h->num_segments = segs;
     h->seg.resize(size_t(segs) + 1);
     HC_TRY(hc_equal_edge_ranges(g, segs, 1, h->seg.data()));
@@ -664,36 +688,49 @@ int hc_bfs_heterogeneous(hc_hetero_t h, int32_t source, int32_t alpha, int32_t b
                     q[w_end++] = (int32_t)(w * 32 + __builtin_ctz(bits));
             std::memset(h->h_front, 0, words * 4);
         } else {
-            // Top-down epoch on the host cores (epoch_bfs_push_cpu_by_node,
-            // :212-244).
+            // Top-down epoch on the host cores: frontier vertices split
+            // between threads (epoch_bfs_push_cpu_by_node, :212-244), or
+            // each vertex's edges split between them when the schedule names
+            // the by-edge kernel (epoch_bfs_push_cpu_by_edge, :255-288).
             edges_to_check -= num_edges;
             level++;
             int64_t ne = 0;
             std::atomic<int64_t> tail{w_end};
             const int64_t fs = w_start, fe = w_end;
+            const bool by_edge = h->push_kernel == HC_K_BFS_CPU_PUSH_BY_EDGE;
 #pragma omp parallel reduction(+ : ne)
             {
                 constexpr int kLocal = 4096;
                 int32_t loc[kLocal];
                 int lc = 0;
-#pragma omp for schedule(dynamic, 64) nowait
-                for (int64_t f = fs; f < fe; f++) {
-                    const int32_t u = q[f];
-                    for (int64_t i = g.index[u]; i < g.index[u + 1]; i++) {
-                        const int32_t v = g.neighbors[i];
-                        int32_t expect = -1;
-                        if (__atomic_load_n(&par[v], __ATOMIC_RELAXED) == -1 &&
-                            __atomic_compare_exchange_n(&par[v], &expect, u, false,
-                                                        __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
-                            if (lc == kLocal) {
-                                int64_t at = tail.fetch_add(lc);
-                                std::memcpy(q + at, loc, sizeof(int32_t) * lc);
-                                lc = 0;
-                            }
-                            loc[lc++] = v;
-                            if (depth) depth[v] = level;
-                            ne += g.index[v + 1] - g.index[v];
+                auto visit = [&](int32_t u, int64_t i) {
+                    const int32_t v = g.neighbors[i];
+                    int32_t expect = -1;
+                    if (__atomic_load_n(&par[v], __ATOMIC_RELAXED) == -1 &&
+                        __atomic_compare_exchange_n(&par[v], &expect, u, false,
+                                                    __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                        if (lc == kLocal) {
+                            int64_t at = tail.fetch_add(lc);
+                            std::memcpy(q + at, loc, sizeof(int32_t) * lc);
+                            lc = 0;
                         }
+                        loc[lc++] = v;
+                        if (depth) depth[v] = level;
+                        ne += g.index[v + 1] - g.index[v];
+                    }
+                };
+                if (by_edge) {
+                    for (int64_t f = fs; f < fe; f++) {
+                        const int32_t u = q[f];
+                        const int64_t b = g.index[u], e = g.index[u + 1];
+#pragma omp for schedule(static) nowait
+                        for (int64_t i = b; i < e; i++) visit(u, i);
+                    }
+                } else {
+#pragma omp for schedule(dynamic, 64) nowait
+                    for (int64_t f = fs; f < fe; f++) {
+                        const int32_t u = q[f];
+                        for (int64_t i = g.index[u]; i < g.index[u + 1]; i++) visit(u, i);
                     }
                 }
                 int64_t at = tail.fetch_add(lc);

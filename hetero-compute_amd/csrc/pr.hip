@@ -348,6 +348,86 @@ __global__ __launch_bounds__(kBlock) void pr_epoch_finish_kernel(PlanDev p, PrEp
 }
 
 // ---------------------------------------------------------------------------
+// Vertex-mapped epochs of the pr_pull_gpu driver (src/kernels/gpu/pr.cuh:
+// 42-118; epochs epoch_pr_pull_gpu_{one_to_one,warp_red,block_red}, :140-
+// 318): a thread, a wavefront or a workgroup per vertex over [0, n) with the
+// caller's launch shape, the Jacobi step and stopping measure of the fused
+// sweep (pr_finalize).  One residual partial per workgroup.
+// ---------------------------------------------------------------------------
+
+// Kahan sum of contrib over edges [b, e) taken `stride` apart from b + lane.
+template <typename OffT>
+__device__ __forceinline__ KSum row_ksum_strided(const GraphView<OffT> &g, const float *contrib,
+                                                 OffT b, OffT e, int lane, int stride) {
+    float sum = 0.0f, comp = 0.0f;
+    for (OffT i = b + OffT(lane); i < e; i += OffT(stride)) {
+        float y = contrib[g.adj[i]] - comp;
+        float t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    }
+    return KSum{sum, -comp};
+}
+
+template <int MAP, typename OffT>
+__global__ void pr_pull_mapped_kernel(GraphView<OffT> g, PrEpoch ep, const double *inv_deg,
+                                      double *partials) {
+    double racc = 0.0;
+    if (MAP == HC_MAP_THREAD) {
+        const int32_t stride = gridDim.x * blockDim.x;
+        for (int32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < g.n; u += stride)
+            racc += pr_finalize(ep, inv_deg, u,
+                                row_ksum<1>(g, unstaged(ep.c_in), g.rbeg(u), g.rend(u), 0));
+    } else if (MAP == HC_MAP_WAVE) {
+        const int32_t waves = gridDim.x * (blockDim.x / kWave);
+        for (int32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < g.n;
+             u += waves) {
+            KSum k = group_ksum<64>(
+                row_ksum<64>(g, unstaged(ep.c_in), g.rbeg(u), g.rend(u), lane_id()));
+            if (lane_id() == 0) racc += pr_finalize(ep, inv_deg, u, k);
+        }
+    } else {
+        for (int32_t u = blockIdx.x; u < g.n; u += gridDim.x) {
+            KSum k = block_ksum(row_ksum_strided(g, ep.c_in, g.rbeg(u), g.rend(u),
+                                                 (int)threadIdx.x, (int)blockDim.x));
+            if (threadIdx.x == 0) racc += pr_finalize(ep, inv_deg, u, k);
+        }
+    }
+    const double r = block_sum_f64(racc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// Initial state of a pr_pull_gpu run: scores from `init` (or 1/N), their
+// contributions and the FP64 inverse degrees.
+template <typename OffT>
+__global__ void pr_pull_init_kernel(GraphView<OffT> g, const float *init, float *s0, float *c0,
+                                    double *inv_deg) {
+    const int32_t stride = gridDim.x * blockDim.x;
+    const float uniform = 1.0f / (float)g.n;
+    for (int32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < g.n; u += stride) {
+        const int64_t d = int64_t(g.rend(u) - g.rbeg(u));
+        const double inv = d > 0 ? 1.0 / (double)d : 0.0;
+        const float x = init ? init[u] : uniform;
+        inv_deg[u] = inv;
+        s0[u] = x;
+        c0[u] = (float)((double)x * inv);
+    }
+}
+
+__global__ void pr_pull_zero_kernel(double *x) {
+    if (threadIdx.x == 0) *x = 0.0;
+}
+
+// Sum of the workgroup partials in workgroup order (one workgroup).
+__global__ __launch_bounds__(kBlock) void pr_pull_total_kernel(const double *partials,
+                                                              int nparts, double *total) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) v += partials[i];
+    const double t = block_sum_f64(v);
+    if (threadIdx.x == 0) *total = t;
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 
@@ -497,11 +577,139 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     return HC_OK;
 }
 
+// pr_pull_gpu's host loop: one mapped epoch per iteration, its stopping
+// measure read back, until it says stop or max_iters ran.  BUCKETED runs
+// the plan sweep of the epoch API.
+template <typename OffT>
+static int run_pr_pull(hc_graph_s *g, int mapping, const float *init, float *scores, float damp,
+                       double tol, int max_iters, int rule, int blocks, int threads,
+                       hc_run_stats_t *stats, hipStream_t s) {
+    const PlanHost *ph = nullptr;
+    if (mapping == HC_MAP_BUCKETED) HC_TRY(get_plan(g, 0, g->n, tuning().pull, &ph));
+    PrWs *ws = nullptr;
+    HC_TRY(get_pr_ws(g, ph ? ph->dev.cnt[0] : 0, ph ? ph->dev.num_hubs : 0, &ws));
+    GraphView<OffT> v = view<OffT>(g);
+    const int32_t ns = std::min<int32_t>(tuning().stage_pull, g->n);
+    int nb = blocks, nt = threads;
+    if (mapping == HC_MAP_BUCKETED) {
+        nb = std::max(1, std::min(std::max(ph->total_units, 1), grid_blocks()));
+        nt = kBlock;
+    }
+    double *dmem = nullptr;  // nb partials | the iteration's total
+    HC_HIP(hipMalloc(&dmem, sizeof(double) * (nb + 1)));
+    double *partials = dmem, *total = dmem + nb;
+    const int init_grid = std::max(1, std::min(grid_blocks(), (g->n + kBlock - 1) / kBlock));
+    pr_pull_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(v, init, ws->s[0], ws->c[0],
+                                                           ws->inv_deg);
+    int rc = hipGetLastError() == hipSuccess ? HC_OK : fail(HC_ERR_HIP, "PageRank init failed");
+    const float base = (1.0f - damp) / (float)g->n;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == HC_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+                        hipEventRecord(e0, s) != hipSuccess))
+        rc = fail(HC_ERR_HIP, "event setup failed");
+    int it = 0;
+    double measure = 0.0;
+    while (rc == HC_OK && it < max_iters) {
+        const int a = it & 1;
+        PrEpoch ep{ws->s[a], ws->c[a], ws->s[a ^ 1], ws->c[a ^ 1], base, damp, rule};
+        switch (mapping) {
+            case HC_MAP_THREAD:
+                pr_pull_mapped_kernel<HC_MAP_THREAD, OffT><<<nb, nt, 0, s>>>(v, ep, ws->inv_deg,
+                                                                             partials);
+                break;
+            case HC_MAP_WAVE:
+                pr_pull_mapped_kernel<HC_MAP_WAVE, OffT><<<nb, nt, 0, s>>>(v, ep, ws->inv_deg,
+                                                                           partials);
+                break;
+            case HC_MAP_BLOCK:
+                pr_pull_mapped_kernel<HC_MAP_BLOCK, OffT><<<nb, nt, 0, s>>>(v, ep, ws->inv_deg,
+                                                                            partials);
+                break;
+            default: {
+                PrBufs b{ws->inv_deg, ws->hub_hi, ws->hub_lo, ws->hub_cnt, ws->hub_resid, partials};
+                pr_epoch_kernel<OffT><<<nb, kBlock, ns * sizeof(float), s>>>(ph->dev, v, ep, b, ns);
+                break;
+            }
+        }
+        if (mapping == HC_MAP_BUCKETED) {
+            PrBufs b{ws->inv_deg, ws->hub_hi, ws->hub_lo, ws->hub_cnt, ws->hub_resid, partials};
+            pr_pull_zero_kernel<<<1, 64, 0, s>>>(total);
+            pr_epoch_finish_kernel<<<1, kBlock, 0, s>>>(ph->dev, ep, b, nb, total);
+        } else {
+            pr_pull_total_kernel<<<1, kBlock, 0, s>>>(partials, nb, total);
+        }
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&measure, total, sizeof(double), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = fail(HC_ERR_HIP, "PageRank epoch %d failed", it);
+            break;
+        }
+        it++;
+        // rule 0: FP64 L1 residual below tol; rule 1: no vertex moved.
+        if (rule ? measure == 0.0 : measure < tol) break;
+    }
+    float ms = 0.0f;
+    if (rc == HC_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                        hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = fail(HC_ERR_HIP, "event timing failed");
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (rc == HC_OK &&
+        (hipMemcpyAsync(scores, ws->s[it & 1], sizeof(float) * g->n, hipMemcpyDeviceToDevice,
+                        s) != hipSuccess ||
+         hipStreamSynchronize(s) != hipSuccess))
+        rc = fail(HC_ERR_HIP, "score copy failed");
+    hipFree(dmem);
+    if (rc == HC_OK && stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->ms = ms;
+        stats->epochs = it;
+        stats->pull_epochs = it;
+        stats->launches = it;
+        stats->reached = -1;
+        stats->residual = measure;
+    }
+    return rc;
+}
+
 }  // namespace hc
 
 using namespace hc;
 
 extern "C" {
+
+int hc_pr_pull(hc_graph_t g, int mapping, const float *init_score, float *scores, int blocks,
+               int threads, const hc_pr_opts_t *opts, hc_run_stats_t *stats,
+               hc_stream_t stream) {
+    HC_CHECK_ARG(g && scores, "NULL argument");
+    HC_CHECK_ARG(g->row0 == 0 && g->rows == g->n, "full runs need a whole-graph handle");
+    HC_CHECK_ARG(g->n > 0, "empty graph");
+    HC_CHECK_ARG(mapping >= HC_MAP_THREAD && mapping <= HC_MAP_BUCKETED, "unknown mapping %d",
+                 mapping);
+    HC_CHECK_ARG(blocks >= 0 && threads >= 0 && threads <= 1024,
+                 "launch shape must be >= 0 blocks and 0..1024 threads");
+    if (blocks == 0) blocks = 64;
+    if (threads == 0) threads = 1024;
+    if (mapping == HC_MAP_WAVE || mapping == HC_MAP_BLOCK)
+        HC_CHECK_ARG(threads % kWave == 0,
+                     "thread count must be a multiple of 64 for this mapping");
+    const float damp = opts && opts->damping > 0 ? opts->damping : 0.85f;
+    const double tol = opts && opts->tolerance != 0 ? opts->tolerance : 1e-4;
+    const int max_iters = opts && opts->max_iters > 0 ? opts->max_iters : 20;
+    const int rule = opts ? opts->rule : 0;
+    HC_CHECK_ARG(damp < 1.0f, "damping must be in (0,1)");
+    HC_CHECK_ARG(rule == 0 || rule == 1, "rule must be 0 (L1 residual) or 1 (per vertex)");
+    HC_CHECK_ARG(max_iters <= 100000, "max_iters too large");
+    HC_TRY(ensure_init());
+    std::lock_guard<std::mutex> lk(g->mu);
+    hipStream_t s = pick_stream(stream);
+    return g->off64 ? run_pr_pull<uint64_t>(g, mapping, init_score, scores, damp, tol,
+                                            max_iters, rule, blocks, threads, stats, s)
+                    : run_pr_pull<uint32_t>(g, mapping, init_score, scores, damp, tol,
+                                            max_iters, rule, blocks, threads, stats, s);
+}
 
 int hc_pagerank(hc_graph_t g, float *scores, const hc_pr_opts_t *opts,
                 hc_run_stats_t *stats, hc_stream_t stream) {

@@ -48,6 +48,7 @@ const KernelName kKernels[HC_K_COUNT] = {
     {"PR GPU block-red", HC_DEV_GPU},    {"PR GPU bucketed", HC_DEV_GPU},
     {"BFS CPU pull", HC_DEV_CPU},        {"BFS GPU one-to-one", HC_DEV_GPU},
     {"BFS GPU warp", HC_DEV_GPU},        {"BFS GPU bucketed", HC_DEV_GPU},
+    {"BFS CPU push by node", HC_DEV_CPU}, {"BFS CPU push by edge", HC_DEV_CPU},
 };
 
 const KernelName kDevices[HC_DEVICE_COUNT] = {

@@ -38,6 +38,13 @@ struct wnode_t {
 
 namespace hc {
 
+// d + w saturated at INF_WEIGHT (an unreached d stays unreached): the
+// relaxation of the library's integer SSSP (weights >= 0, d <= INF_WEIGHT).
+inline weight_t relax_host(weight_t d, weight_t w) {
+    const uint32_t c = (uint32_t)d + (uint32_t)w;
+    return c < (uint32_t)INF_WEIGHT ? (weight_t)c : INF_WEIGHT;
+}
+
 // Error behaviour of the reference (print and exit, src/cuda.cuh:14-21,
 // src/util.h:16-22) for the drop-in layer.
 inline void check_or_exit(int rc, const char *what) {

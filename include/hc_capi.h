@@ -371,6 +371,21 @@ typedef struct hc_pr_opts {
 int hc_pagerank(hc_graph_t g, float *scores, const hc_pr_opts_t *opts,
                 hc_run_stats_t *stats, hc_stream_t stream);
 
+/* pr_pull_gpu (src/kernels/gpu/pr.cuh:42-118) with its epoch kernel, start
+ * scores and launch shape honoured: one epoch per iteration of the given
+ * mapping (THREAD / WAVE / BLOCK: a thread, wavefront or workgroup per
+ * vertex, the counterparts of epoch_pr_pull_gpu_{one_to_one,warp_red,
+ * block_red}, :140-318, on `blocks` x `threads`, 0 = 64 x 1024, threads a
+ * multiple of 64 for WAVE and BLOCK; BUCKETED: the degree-bucketed sweep,
+ * library launch shape), the stopping measure read back after each
+ * iteration as the reference's host loop does.  Same Jacobi arithmetic and
+ * opts as hc_pagerank; init_score (device, may be NULL = 1/N) seeds the
+ * first iteration.  stats->ms: HIP events around the loop. */
+int hc_pr_pull(hc_graph_t g, int mapping, const float *init_score,
+               float *scores, int blocks, int threads,
+               const hc_pr_opts_t *opts, hc_run_stats_t *stats,
+               hc_stream_t stream);
+
 /* Per-epoch record of the last hc_sssp (algorithm 0) or hc_bfs (1) run on g:
  * the direction each epoch ran (0 push / top-down, 1 pull / bottom-up), the
  * size and degree sum of the frontier it produced and the edges it examined.

@@ -60,7 +60,9 @@ enum hc_kernel_id {
     HC_K_BFS_GPU_ONE_TO_ONE = 11, /* "BFS GPU one-to-one"   */
     HC_K_BFS_GPU_WARP = 12,       /* "BFS GPU warp"         */
     HC_K_BFS_GPU_BUCKETED = 13,   /* "BFS GPU bucketed"     */
-    HC_K_COUNT = 14
+    HC_K_BFS_CPU_PUSH_BY_NODE = 14, /* "BFS CPU push by node" */
+    HC_K_BFS_CPU_PUSH_BY_EDGE = 15, /* "BFS CPU push by edge" */
+    HC_K_COUNT = 16
 };
 
 /* Name lookups.  hc_kernel_from_string matches by prefix, as get_kernel_id
@@ -135,7 +137,12 @@ int hc_pr_heterogeneous(hc_hetero_t h, const hc_pr_opts_t *opts,
 /* bfs_do_heterogeneous (src/kernels/heterogeneous/bfs.cuh:19-223) with the
  * schedule's pull blocks for bottom-up epochs and host-core top-down epochs,
  * switched by the bfs_do_cpu rule (src/kernels/cpu/bfs.cuh:68-103) checked
- * after every epoch.  parents and depth are host arrays (depth may be NULL). */
+ * after every epoch.  A "BFS CPU push by node" / "... by edge" block (what
+ * the reference's push_pull_scheduler emits for push epochs) selects how the
+ * top-down epochs split their work between host threads (frontier vertices
+ * or each vertex's edges) and covers no segment; a schedule of push blocks
+ * only pulls on the host cores.  parents and depth are host arrays (depth
+ * may be NULL). */
 int hc_bfs_heterogeneous(hc_hetero_t h, int32_t source, int32_t alpha,
                          int32_t beta, int32_t *parents, int32_t *depth,
                          hc_run_stats_t *stats);

@@ -485,3 +485,36 @@ def test_pagerank_epoch_api(hc, gpu, oracle, kron16):
     got = so.get()
     assert np.abs(got.astype(np.float64) - ref).sum() <= PR_L1_TOL
     assert abs(resid - np.abs(got.astype(np.float64) - s0).sum()) <= 1e-9
+
+
+@pytest.mark.parametrize("mapping,blocks,threads", [
+    (0, 64, 1024), (0, 300, 128), (1, 64, 1024), (1, 97, 192), (2, 64, 256), (2, 512, 64),
+    (3, 0, 0)])
+def test_pr_pull_mappings(hc, gpu, oracle, kron16, mapping, blocks, threads):
+    """hc_pr_pull (pr_pull_gpu's loop) honours its mapping, launch shape and
+    start scores: 12 Jacobi iterations from 1/N match the FP32 reference
+    within 1e-6 L1; 7 more iterations started from those scores equal 19
+    from 1/N; and the per-vertex rule stops where the oracle's does."""
+    g = kron16
+    kw = dict(mapping=mapping, blocks=blocks, threads=threads)
+    with hc.DeviceGraph(g) as dg:
+        r12 = hc.pr_pull(dg, tolerance=-1.0, max_iters=12, **kw)
+        r19 = hc.pr_pull(dg, tolerance=-1.0, max_iters=19, **kw)
+        r7 = hc.pr_pull(dg, init=r12.values, tolerance=-1.0, max_iters=7, **kw)
+        rr = hc.pr_pull(dg, rule=1, max_iters=201, **kw)
+    assert r12.stats["epochs"] == 12 and r7.stats["epochs"] == 7
+    ref = oracle.pr_jacobi_f32(g.index, g.neighbors, 0.85, 12)
+    assert np.abs(r12.values.astype(np.float64) - ref).sum() <= PR_L1_TOL
+    assert np.abs(r7.values.astype(np.float64) - r19.values).sum() <= PR_L1_TOL
+    rref, sweeps = oracle.pr_jacobi_rule_f32(g.index, g.neighbors, 0.85, 201)
+    assert abs(rr.stats["epochs"] - sweeps) <= 1 and rr.stats["residual"] == 0.0
+    d = np.abs(rr.values.astype(np.float64) - rref)
+    assert d.sum() <= PR_L1_TOL and d.max() <= 1e-7
+
+
+def test_pr_pull_rejects_bad_launch(hc, gpu, kron16):
+    with hc.DeviceGraph(kron16) as dg:
+        for mapping, threads in ((1, 100), (2, 96 + 1), (0, 2048)):
+            with pytest.raises(hc.HcError) as e:
+                hc.pr_pull(dg, mapping=mapping, blocks=8, threads=threads)
+            assert e.value.code == -1

@@ -203,3 +203,28 @@ def test_cpu_sssp_epoch_export(hc, oracle):
         if upd[0] == 0:
             break
     assert np.array_equal(d, fx["sssp"][0])
+
+
+def test_bfs_push_kernel_names_parse_and_run(hc, oracle):
+    """The push-epoch kernel names the reference's push_pull_scheduler emits
+    ("BFS CPU push by node" / "... by edge") parse as CPU kernels, cover no
+    segment, and choose how host top-down epochs split their work; depths are
+    the same either way, and a schedule of push blocks only pulls on the host
+    cores."""
+    fx, g = _fixture_graph(hc, "kron10.npz")
+    for push in ("BFS CPU push by node", "BFS CPU push by edge"):
+        s = hc.Schedule.parse(f"Host CPU\n0 3\nBFS CPU pull\nIntel i7-9700K\n0 3\n{push}\n")
+        blocks = s.blocks
+        assert [b["kernel"] for b in blocks] == ["BFS CPU pull", push]
+        for sched in (s, hc.Schedule.parse(f"Host CPU\n0 1\n{push}\n")):
+            with hc.Hetero(g, sched) as h:
+                for i, src in enumerate(fx["sources"]):
+                    src = int(src)
+                    r = h.bfs(src)
+                    assert np.array_equal(r.values, fx["depth"][i]), (push, src)
+                    assert oracle.verify_parents(fx["depth"][i], r.parents, src) == 0
+                    _, _, dirs = oracle.bfs_do(fx["index"], fx["neighbors"], src)
+                    assert r.stats["push_epochs"] == int((dirs == 0).sum())
+    with pytest.raises(hc.HcError) as e:
+        hc.Schedule.parse("AMD Instinct MI355X\n0 1\nBFS CPU push by node\n")
+    assert "cannot run" in str(e.value)
