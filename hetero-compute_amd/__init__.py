@@ -227,7 +227,8 @@ TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "sssp.filter", "sssp.stage", "sssp.g
                "pull.block_threshold", "pull.wave_threshold", "pull.g16_threshold",
                "pull.g4_threshold", "bu.wave_threshold", "bu.g16_threshold",
                "bu.g4_threshold", "multi.hub_threshold", "multi.hub_chunk",
-               "multi.wave_threshold", "multi.g16_threshold")
+               "multi.wave_threshold", "multi.g16_threshold", "pull.col_shift",
+               "cb.min_slice", "cb.g4_slice", "cb.g16_slice", "cb.wave_slice", "cb.block_slice")
 
 
 def set_tuning(key: str, value: int):
@@ -478,6 +479,17 @@ class DeviceGraph:
             check(lib().hc_graph_upload_range(host.struct_ref(), start, end, C.byref(h)))
         self.h = h
         self.num_nodes = host.num_nodes
+
+    def share(self) -> "DeviceGraph":
+        """A handle on the same device arrays with its own workspaces
+        (hc_graph_share); this graph must outlive it."""
+        h = C.c_void_p()
+        check(lib().hc_graph_share(self.h, C.byref(h)))
+        out = DeviceGraph.__new__(DeviceGraph)
+        out.h = h
+        out.num_nodes = self.num_nodes
+        out._parent = self
+        return out
 
     def info(self) -> dict:
         i = GraphInfo()

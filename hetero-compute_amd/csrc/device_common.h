@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -83,11 +85,16 @@ struct PlanConfig {
     int64_t wave_threshold = 192;   // deg > this -> wavefront per vertex
     int64_t g16_threshold = 24;     // deg > this -> 16 lanes
     int64_t g4_threshold = 5;       // deg > this -> 4 lanes, else 1
+    // Only rows of degree in [min_degree, max_degree] (min_degree > 0 also
+    // drops degree-0 rows from the zero list).
+    int64_t min_degree = 0;
+    int64_t max_degree = INT64_MAX;
     bool operator<(const PlanConfig &o) const {
         return std::tie(hub_threshold, hub_chunk, block_threshold,
-                        wave_threshold, g16_threshold, g4_threshold) <
+                        wave_threshold, g16_threshold, g4_threshold, min_degree, max_degree) <
                std::tie(o.hub_threshold, o.hub_chunk, o.block_threshold,
-                        o.wave_threshold, o.g16_threshold, o.g4_threshold);
+                        o.wave_threshold, o.g16_threshold, o.g4_threshold, o.min_degree,
+                        o.max_degree);
     }
 };
 
@@ -121,6 +128,15 @@ struct Tuning {
     int kernel_events = 0; // 1: time every launch of a loop's main kernel
     int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
     int multi_grid_per_cu = 16;  // workgroups per CU of the batched sweep
+    int pull_col_shift = -1;     // pull sweeps gather column blocks of 2^this vertices
+                                 // (0: off; -1: automatic, flat graphs only; used
+                                 // when the graph has >= 2 blocks)
+    int64_t cb_min_slice = 4;    // rows of degree > this x blocks are swept by column
+                                 // block, lighter ones once over the plain CSR
+    int64_t cb_g4_slice = 2;     // column-blocked sweeps: mean slice degree (row
+    int64_t cb_g16_slice = 12;   // degree / blocks) above which 4 / 16 lanes, a
+    int64_t cb_wave_slice = 48;  // wavefront or a workgroup take a row's slice
+    int64_t cb_block_slice = 512;
     PlanConfig pull;       // SSSP and PageRank pull sweeps
     PlanConfig bu;         // BFS bottom-up sweeps (no chunk / WG classes)
     PlanConfig multi;      // batched sweeps: chunks, wave, 16-lane, 4-lane
@@ -149,6 +165,31 @@ inline PlanConfig sssp_plan_config() {
     c.g4_threshold = std::min<int64_t>(tuning().sssp_g4_threshold, c.g16_threshold);
     c.block_threshold = std::max<int64_t>(tuning().sssp_block_threshold, c.wave_threshold);
     if (c.hub_threshold < c.block_threshold) c.hub_threshold = c.block_threshold;
+    return c;
+}
+
+// Column-blocked pull sweeps over K blocks take the rows of degree above
+// cb_light_degree(K) block by block (plan cb_plan_config: classes by the
+// mean slice degree, row degree / K, no hub chunks); lighter rows, whose
+// slices would be mostly empty, are swept once over the plain CSR (plan
+// cb_light_config: the plain SSSP classes restricted to them).
+inline int64_t cb_light_degree(int K) { return tuning().cb_min_slice * K; }
+inline PlanConfig cb_plan_config(int K) {
+    const Tuning &t = tuning();
+    PlanConfig c;
+    c.hub_threshold = INT64_MAX;
+    c.hub_chunk = int64_t(1) << 40;
+    c.g4_threshold = t.cb_g4_slice * K;
+    c.g16_threshold = std::max(c.g4_threshold, t.cb_g16_slice * K);
+    c.wave_threshold = std::max(c.g16_threshold, t.cb_wave_slice * K);
+    c.block_threshold = std::max(c.wave_threshold, t.cb_block_slice * K);
+    c.min_degree = cb_light_degree(K) + 1;
+    return c;
+}
+inline PlanConfig cb_light_config(int K) {
+    PlanConfig c = sssp_plan_config();
+    c.min_degree = 1;
+    c.max_degree = cb_light_degree(K);
     return c;
 }
 
@@ -370,6 +411,30 @@ hipStream_t pick_stream(hc_stream_t s);
 }  // namespace hc
 
 // Host-side graph handle.
+namespace hc {
+// Column-blocked copy of a whole graph's CSR for pull sweeps: the edges of
+// row u whose neighbour lies in block k = v >> shift form the slice
+// [off[k][u], off[k][u+1]) of adj / wt, in CSR order.  A sweep that takes the
+// blocks one after another gathers per-vertex state from one block of 2^shift
+// vertices at a time, which one XCD's L2 holds.
+struct ColBlocks {
+    int32_t shift = 0;
+    int32_t K = 0;             // blocks: ceil(n / 2^shift)
+    void *off = nullptr;       // [K][n+1] offsets of the graph's offset type
+    int32_t *adj = nullptr;
+    int32_t *wt = nullptr;     // weighted graphs only
+    uint32_t *seen = nullptr;  // ceil(n/32) words: rows counted by an epoch-API sweep
+    void *dmem = nullptr;
+    size_t bytes = 0;
+};
+// One copy for a graph and every share of it, built on first use.
+struct ColBlockSlot {
+    std::mutex mu;
+    ColBlocks cb;
+    ~ColBlockSlot();
+};
+}  // namespace hc
+
 struct hc_graph_s {
     int32_t n = 0;          // global vertex count
     int32_t row0 = 0;       // first row held
@@ -394,6 +459,7 @@ struct hc_graph_s {
     void *ws_bfs = nullptr;
     void *ws_pr = nullptr;
     void *ws_multi = nullptr;
+    std::shared_ptr<hc::ColBlockSlot> cbslot = std::make_shared<hc::ColBlockSlot>();
 };
 
 namespace hc {
@@ -405,6 +471,13 @@ inline GraphView<OffT> view(const hc_graph_s *g) {
     return GraphView<OffT>{static_cast<const OffT *>(g->d_off), g->d_adj,
                            g->d_wt, g->row0, g->n};
 }
+// Column blocks of 2^shift vertices for a whole-graph handle (built on
+// stream s and waited for on first use); *out is NULL when the graph has
+// fewer than two blocks.
+int get_col_blocks(hc_graph_s *g, int shift, hipStream_t s, const ColBlocks **out);
+// Block shift a pull sweep of g uses under the current tuning (0: plain
+// sweep); flat: g has no hub rows (top 1% of rows hold < 5% of the edges).
+int col_shift_for(const hc_graph_s *g, bool flat);
 void free_sssp_ws(hc_graph_s *g);
 void free_bfs_ws(hc_graph_s *g);
 void free_pr_ws(hc_graph_s *g);

@@ -79,6 +79,11 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
         {"multi.hub_chunk", &t.multi.hub_chunk},
         {"multi.wave_threshold", &t.multi.wave_threshold},
         {"multi.g16_threshold", &t.multi.g16_threshold},
+        {"cb.min_slice", &t.cb_min_slice},
+        {"cb.g4_slice", &t.cb_g4_slice},
+        {"cb.g16_slice", &t.cb_g16_slice},
+        {"cb.wave_slice", &t.cb_wave_slice},
+        {"cb.block_slice", &t.cb_block_slice},
     };
     for (auto &x : i64)
         if (!strcmp(key, x.k)) return x.p;
@@ -103,7 +108,8 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"loop.batch", &t.loop_batch},
                {"profile.kernel_events", &t.kernel_events},
                {"multi.graph", &t.multi_graph},
-               {"multi.grid_per_cu", &t.multi_grid_per_cu}};
+               {"multi.grid_per_cu", &t.multi_grid_per_cu},
+               {"pull.col_shift", &t.pull_col_shift}};
     for (auto &x : i32)
         if (!strcmp(key, x.k)) {
             *is_int = true;
@@ -130,6 +136,7 @@ int get_plan(hc_graph_s *g, int32_t start, int32_t end, const PlanConfig &cfg,
     auto deg = [&](int32_t u) {
         return g->h_off[u - g->row0 + 1] - g->h_off[u - g->row0];
     };
+    const int64_t dmin = std::max<int64_t>(cfg.min_degree, 0), dmax = cfg.max_degree;
     auto cls_of = [&](int64_t d) {
         if (d == 0) return -1;
         if (d > cfg.hub_threshold) return 0;
@@ -147,29 +154,37 @@ int get_plan(hc_graph_s *g, int32_t start, int32_t end, const PlanConfig &cfg,
     std::vector<int32_t> hubs;
     std::vector<int32_t> zeros;
     if (g->sorted) {
+        // first index in [a, b) with deg <= t (degrees non-increasing)
+        auto first_le = [&](int32_t a, int32_t b, int64_t t) {
+            while (a < b) {
+                int32_t mid = a + (b - a) / 2;
+                if (deg(mid) <= t) b = mid; else a = mid + 1;
+            }
+            return a;
+        };
+        // The degree band [dmin, dmax] is a contiguous range of rows.
+        const int32_t band_beg = dmax == INT64_MAX ? start : first_le(start, end, dmax);
+        const int32_t band_end = dmin <= 1 ? end : first_le(band_beg, end, dmin - 1);
         // Contiguous classes: class c is [first u with deg <= T_{c-1},
         // first u with deg <= T_c).
         const int64_t thr[kNumClasses] = {cfg.hub_threshold, cfg.block_threshold,
                                           cfg.wave_threshold, cfg.g16_threshold,
                                           cfg.g4_threshold, 0};
-        int32_t lo = start;
+        int32_t lo = band_beg;
         for (int c = 0; c < kNumClasses; c++) {
-            // first index in [lo, end) with deg <= thr[c]
-            int32_t a = lo, b = end;
-            while (a < b) {
-                int32_t mid = a + (b - a) / 2;
-                if (deg(mid) <= thr[c]) b = mid; else a = mid + 1;
-            }
+            const int32_t a = first_le(lo, band_end, thr[c]);
             p.beg[c] = lo;
             p.cnt[c] = a - lo;
             lo = a;
         }
         p.zero_beg = lo;
-        p.zero_cnt = end - lo;
+        p.zero_cnt = dmin > 0 ? 0 : band_end - lo;
         for (int32_t u = p.beg[0]; u < p.beg[0] + p.cnt[0]; u++) hubs.push_back(u);
     } else {
         for (int32_t u = start; u < end; u++) {
-            int c = cls_of(deg(u));
+            const int64_t d = deg(u);
+            if (d > dmax || (d < dmin && !(d == 0 && dmin == 0))) continue;
+            int c = cls_of(d);
             if (c < 0) {
                 zeros.push_back(u);
                 continue;
@@ -560,6 +575,7 @@ int hc_graph_share(hc_graph_t g, hc_graph_t *out) {
     h->bytes = 0;
     h->h_off = g->h_off;
     h->shared = true;
+    h->cbslot = g->cbslot;  // one column-blocked copy for the graph and its shares
     *out = h;
     return HC_OK;
 }
@@ -581,6 +597,12 @@ int hc_set_tuning(const char *key, int64_t value) {
             HC_CHECK_ARG(value >= 0 && value <= 2, "sssp.push_dedup must be 0, 1 or 2");
         if (!strcmp(key, "bu.first"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "bu.first must be 0, 1 or 2");
+        if (!strcmp(key, "pull.col_shift")) {
+            HC_CHECK_ARG(value == -1 || value == 0 || (value >= 6 && value <= 30),
+                         "pull.col_shift must be -1 (automatic), 0 (off) or in [6, 30]");
+            *iref = (int)value;
+            return HC_OK;
+        }
         if (!strcmp(key, "loop.batch"))
             HC_CHECK_ARG(value >= 1 && value <= 16, "loop.batch must be in [1, 16]");
         if (!strcmp(key, "push.chunk"))
@@ -589,6 +611,7 @@ int hc_set_tuning(const char *key, int64_t value) {
             HC_CHECK_ARG(value == 0 || value == 1 || value == 4 || value == 16,
                          "bu.remaining must be 0, 1, 4 or 16");
         const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining") ||
+                             !strcmp(key, "pull.col_shift") ||
                              !strcmp(key, "bu.first") || !strcmp(key, "sssp.alpha") ||
                              !strcmp(key, "sssp.dist16") || !strcmp(key, "sssp.push_dedup") ||
                              !strcmp(key, "bu.stage_min");

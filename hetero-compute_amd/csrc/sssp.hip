@@ -127,7 +127,25 @@ __device__ __forceinline__ int32_t block_min(int32_t m) {
 // Fused degree-bucketed pull sweep
 // ---------------------------------------------------------------------------
 
-template <int G, bool FILTER, typename DR, typename OffT>
+// Lowers dist[u] to m when that is lower; chg: this call lowered it.  For
+// rows that several groups of one sweep may lower (hub chunks, the column
+// blocks of a row): atomicMin decides, a stale `cur` only costs an atomic.
+__device__ __forceinline__ void lower_shared_row(int32_t *dist, uint16_t *d16, int32_t u,
+                                                 int32_t cur, int32_t m, bool &chg) {
+    if (m < cur) {
+        const int32_t old = atomicMin(&dist[u], m);
+        chg = m < old;
+        // Two groups lowering one row may store their mirror values out of
+        // order; a stale (larger) entry only weakens this epoch, and the row
+        // is marked changed, so the next epoch starts from a rebuild.
+        if (chg && d16) d16[u] = dist16(m);
+    }
+}
+
+// One row of class c per group of G lanes.  SHARED: other groups of the
+// sweep may lower the same row (column-blocked sweeps), so the row is
+// lowered with atomicMin.
+template <int G, bool FILTER, bool SHARED = false, typename DR, typename OffT>
 __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
                                            const GraphView<OffT> &g,
                                            const DR &rd,
@@ -142,12 +160,14 @@ __device__ __forceinline__ void pull_group(const PlanDev &p, int c, int k,
         int32_t uu = p.list[c] ? p.list[c][idx] : p.beg[c] + idx;
         const OffT b = g.rbeg(uu), e = g.rend(uu);
         // The row's own distance is read with its bounds, before the
-        // gathers: in a pull epoch only this group writes it.
+        // gathers: in a pull epoch only this group writes it (unless SHARED).
         const int32_t cur = lane == 0 ? dist[uu] : 0;
         int32_t m = row_min<G, FILTER>(g, rd, changed, b, e, lane, gathered);
         if (G > 1) m = group_min<G>(m);
         if (lane == 0) {
-            if (m < cur) {
+            if (SHARED) {
+                lower_shared_row(dist, d16, uu, cur, m, chg);
+            } else if (m < cur) {
                 dist[uu] = m;
                 if (d16) d16[uu] = dist16(m);
                 chg = true;
@@ -201,15 +221,7 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
                                                           gathered));
             if (threadIdx.x == 0) {
                 if (c == 0) {
-                    if (m < dist[uu]) {
-                        int32_t old = atomicMin(&dist[uu], m);
-                        chg = m < old;
-                        // Two chunks lowering the same hub may store their
-                        // mirror values out of order; a stale (larger) entry
-                        // only weakens this epoch, and the hub is marked
-                        // changed, so the next epoch starts from a rebuild.
-                        if (chg && d16) d16[uu] = dist16(m);
-                    }
+                    lower_shared_row(dist, d16, uu, dist[uu], m, chg);
                 } else {
                     int32_t cur = dist[uu];
                     if (m < cur) {
@@ -244,6 +256,59 @@ __device__ void sssp_pull_sweep(const PlanDev &p, const GraphView<OffT> &g,
             }
         } else {
             cnt += chg ? 1u : 0u;
+        }
+    }
+}
+
+// One pull sweep over the column blocks of the graph (see colblock.hip):
+// units (block k, plan unit) in block-major order, so the grid gathers from
+// one block of distances at a time.  `cb` views the blocked arrays with the
+// offsets of block 0; block k's slices start (n+1) k offsets later.  Plan
+// classes are by mean slice degree and have no hub chunks.  A row may be
+// lowered by several blocks in one sweep: atomicMin, and `counted` (the
+// next-frontier bitmap, or the epoch API's seen bitmap) counts it once;
+// edges_acc adds its full degree.
+template <typename DR, typename OffT>
+__device__ void sssp_pull_sweep_cb(const PlanDev &p, const GraphView<OffT> &g,
+                                   const GraphView<OffT> &cb, int K, const DR &rd,
+                                   int32_t *__restrict__ dist, uint16_t *d16,
+                                   uint32_t *counted, uint32_t &cnt,
+                                   unsigned long long &edges_acc, uint32_t &gathered) {
+    const int U = p.unit_start[kNumClasses];
+    const int total = U * K;
+    const Staged<uint32_t> nochg = unstaged<uint32_t>(nullptr);
+    for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
+        const int k = unit / U;
+        const int pu = unit - k * U;
+        GraphView<OffT> gk = cb;
+        gk.off = cb.off + (size_t)k * ((size_t)g.n + 1);
+        int c = 1;
+        while (pu >= p.unit_start[c + 1]) c++;
+        const int kk = pu - p.unit_start[c];
+        bool chg = false;
+        int32_t u = 0;
+        int64_t deg = -1;
+        if (c == 1) {
+            const int32_t uu = p.list[1] ? p.list[1][kk] : p.beg[1] + kk;
+            const int32_t m = block_min(row_min<kBlock, false>(gk, rd, nochg, gk.rbeg(uu),
+                                                               gk.rend(uu), threadIdx.x,
+                                                               gathered));
+            if (threadIdx.x == 0) {
+                lower_shared_row(dist, d16, uu, dist[uu], m, chg);
+                u = uu;
+            }
+        } else if (c == 2) {
+            pull_group<64, false, true>(p, c, kk, gk, rd, nochg, dist, d16, chg, u, gathered, deg);
+        } else if (c == 3) {
+            pull_group<16, false, true>(p, c, kk, gk, rd, nochg, dist, d16, chg, u, gathered, deg);
+        } else if (c == 4) {
+            pull_group<4, false, true>(p, c, kk, gk, rd, nochg, dist, d16, chg, u, gathered, deg);
+        } else {
+            pull_group<1, false, true>(p, c, kk, gk, rd, nochg, dist, d16, chg, u, gathered, deg);
+        }
+        if (wave_set_bit(counted, u, chg)) {
+            cnt++;
+            edges_acc += (unsigned long long)(g.rend(u) - g.rbeg(u));
         }
     }
 }
@@ -335,10 +400,11 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
 // `nstage` hub distances are staged in dynamic LDS for pull epochs; push
 // epochs stage up to `push_cap` output entries there.  D16: pull
 // gathers read the 16-bit mirror `d16` instead.
-template <typename OffT, bool D16>
+template <typename OffT, bool D16, bool CB>
 __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     PlanDev p, GraphView<OffT> g, int32_t *__restrict__ dist, uint16_t *d16, FrontierBufs fb,
-    StepCtl *ctl, int32_t nstage, int32_t push_cap, int defer_dedup) {
+    StepCtl *ctl, int32_t nstage, int32_t push_cap, int defer_dedup, PlanDev pcb,
+    GraphView<OffT> cbv, int cbk) {
     extern __shared__ int32_t s_stage[];
     if (ctl->done) return;
     const int mode = ctl->mode;
@@ -353,7 +419,24 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
         cbm[w] = 0;
     uint32_t cnt = 0, gathered = 0;
     unsigned long long eacc = 0;
-    if (mode == kPull) {
+    if (mode == kPull && CB) {
+        // Column-blocked sweep (the host picks it only without filtering):
+        // the heavy rows of pcb block by block, then the light rows of plan
+        // p once over the plain CSR (heavy rows first, as in the plain
+        // sweep's class order, so light rows read their neighbours' values
+        // of this epoch).
+        if (D16) {
+            const Dist16Reader rd{d16, dist};
+            sssp_pull_sweep_cb(pcb, g, cbv, cbk, rd, dist, d16, wbm, cnt, eacc, gathered);
+            sssp_pull_sweep<true, false>(p, g, rd, unstaged(rbm), dist, d16, wbm, cnt, eacc,
+                                         gathered);
+        } else {
+            const Staged<int32_t> rd = stage(dist, nstage, s_stage);
+            sssp_pull_sweep_cb(pcb, g, cbv, cbk, rd, dist, nullptr, wbm, cnt, eacc, gathered);
+            sssp_pull_sweep<true, false>(p, g, rd, unstaged(rbm), dist, nullptr, wbm, cnt, eacc,
+                                         gathered);
+        }
+    } else if (mode == kPull) {
         // One gather path per instantiation: both in one kernel would make
         // it hold the registers of both (and spill).
         if (D16) {
@@ -591,6 +674,22 @@ __global__ __launch_bounds__(kBlock) void sssp_pull_fused_kernel(
     if (threadIdx.x == 0 && s) atomicAdd(updated, (int32_t)s);
 }
 
+// Epoch API sweep over the column blocks of a whole graph; `seen` (n bits,
+// zeroed by the host) counts each lowered row once.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void sssp_pull_cb_kernel(
+    PlanDev plight, PlanDev pcb, GraphView<OffT> g, GraphView<OffT> cbv, int cbk,
+    int32_t *__restrict__ dist, int32_t *updated, uint32_t *seen) {
+    uint32_t upd = 0, gathered = 0;
+    unsigned long long eacc = 0;
+    sssp_pull_sweep_cb(pcb, g, cbv, cbk, unstaged(dist), dist, nullptr, seen, upd, eacc,
+                       gathered);
+    sssp_pull_sweep<false, false>(plight, g, unstaged(dist), unstaged<uint32_t>(nullptr), dist,
+                                  nullptr, nullptr, upd, eacc, gathered);
+    unsigned long long s = block_sum_u64(upd);
+    if (threadIdx.x == 0 && s) atomicAdd(updated, (int32_t)s);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -685,6 +784,22 @@ static int launch_pull_epoch(hc_graph_s *g, int mapping, int32_t start,
             break;
         default: {
             const PlanHost *ph = nullptr;
+            // A whole-graph sweep gathers column block by column block.
+            const ColBlocks *cb = nullptr;
+            if (start == 0 && end == g->n) HC_TRY(get_col_blocks(g, col_shift_for(g, flat_graph(g)), s, &cb));
+            if (cb) {
+                const PlanHost *pl = nullptr;
+                HC_TRY(get_plan(g, 0, g->n, cb_plan_config(cb->K), &ph));
+                HC_TRY(get_plan(g, 0, g->n, cb_light_config(cb->K), &pl));
+                const int units = ph->total_units * cb->K + pl->total_units;
+                if (units == 0) return HC_OK;
+                GraphView<OffT> cbv{static_cast<const OffT *>(cb->off), cb->adj, cb->wt, 0, g->n};
+                HC_HIP(hipMemsetAsync(cb->seen, 0, sizeof(uint32_t) * ((g->n + 31) / 32), s));
+                const int nb = std::max(1, std::min(units, grid_blocks()));
+                sssp_pull_cb_kernel<OffT><<<nb, kBlock, 0, s>>>(pl->dev, ph->dev, v, cbv, cb->K,
+                                                                dist, updated, cb->seen);
+                break;
+            }
             HC_TRY(get_plan(g, start, end, sssp_plan_config(), &ph));
             if (ph->total_units == 0) return HC_OK;
             int nb = std::max(1, std::min(ph->total_units, grid_blocks()));
@@ -724,6 +839,14 @@ struct SsspLaunch {
     int32_t push_cap;
     int defer_dedup;
     KernelTimer *kt;
+    PlanDev cbplan;          // column-blocked pull sweeps (cbk > 0)
+    GraphView<OffT> cbv;
+    int cbk;
+    template <bool D16, bool CB>
+    void step(FrontierBufs fb, StepCtl *cur) {
+        sssp_step_kernel<OffT, D16, CB><<<grid, kBlock, lds_entries * sizeof(int32_t), s>>>(
+            plan, g, dist, d16, fb, cur, nstage, push_cap, defer_dedup, cbplan, cbv, cbk);
+    }
     // Epoch k: header slot k & 1, counter set k % 3.
     static int launch(void *arg, long long k) {
         auto *a = static_cast<SsspLaunch *>(arg);
@@ -731,14 +854,13 @@ struct SsspLaunch {
         fb.part = sum_set(a->fb.part, k);
         StepCtl *cur = a->ctl + (k & 1), *nxt = a->ctl + ((k + 1) & 1);
         a->kt->mark(a->s);
-        if (a->d16)
-            sssp_step_kernel<OffT, true><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t), a->s>>>(
-                a->plan, a->g, a->dist, a->d16, fb, cur, a->nstage, a->push_cap,
-                a->defer_dedup);
-        else
-            sssp_step_kernel<OffT, false><<<a->grid, kBlock, a->lds_entries * sizeof(int32_t),
-                                            a->s>>>(a->plan, a->g, a->dist, nullptr, fb, cur,
-                                                    a->nstage, a->push_cap, a->defer_dedup);
+        if (a->d16) {
+            if (a->cbk) a->template step<true, true>(fb, cur);
+            else a->template step<true, false>(fb, cur);
+        } else {
+            if (a->cbk) a->template step<false, true>(fb, cur);
+            else a->template step<false, false>(fb, cur);
+        }
         a->kt->mark(a->s);
         sssp_between_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(
             a->g, fb, cur, nxt, a->ctl, sum_set(a->fb.part, k + 1), a->dist, a->d16);
@@ -768,8 +890,23 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const int32_t nstage = std::min<int32_t>(tuning().sssp_stage, g->n);
     const int32_t push_cap = tuning().push_stage;
     const int32_t lds_entries = std::max<int32_t>(nstage, push_cap);
-    const void *step_fn = d16 ? reinterpret_cast<const void *>(&sssp_step_kernel<OffT, true>)
-                              : reinterpret_cast<const void *>(&sssp_step_kernel<OffT, false>);
+    // Pull epochs gather column block by column block (not with filtering,
+    // which tests a changed bit per neighbour before its gather).
+    const ColBlocks *cb = nullptr;
+    const PlanHost *phb = nullptr;
+    if (direction != 2 && !tuning().sssp_filter) {
+        HC_TRY(get_col_blocks(g, col_shift_for(g, flat_graph(g)), s, &cb));
+        if (cb) {
+            // Plain sweep for the light rows only.
+            HC_TRY(get_plan(g, 0, g->n, cb_light_config(cb->K), &ph));
+            HC_TRY(get_plan(g, 0, g->n, cb_plan_config(cb->K), &phb));
+        }
+    }
+    const void *step_fn =
+        d16 ? (cb ? reinterpret_cast<const void *>(&sssp_step_kernel<OffT, true, true>)
+                  : reinterpret_cast<const void *>(&sssp_step_kernel<OffT, true, false>))
+            : (cb ? reinterpret_cast<const void *>(&sssp_step_kernel<OffT, false, true>)
+                  : reinterpret_cast<const void *>(&sssp_step_kernel<OffT, false, false>));
     const int grid = resident_grid(step_fn,
                                    kBlock, lds_entries * sizeof(int32_t),
                                    std::min(ws->f.grid, grid_blocks(tuning().sssp_grid_per_cu)));
@@ -785,7 +922,11 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const int defer_dedup = tuning().sssp_push_dedup == 2 ? (flat_graph(g) ? 1 : 0)
                                                           : tuning().sssp_push_dedup;
     SsspLaunch<OffT> L{ph->dev, v, dist, d16, ws->f.fb, ws->f.ctl, grid, s, nstage, lds_entries,
-                       push_cap, defer_dedup, &ws->f.kt};
+                       push_cap, defer_dedup, &ws->f.kt, phb ? phb->dev : PlanDev{},
+                       cb ? GraphView<OffT>{static_cast<const OffT *>(cb->off), cb->adj, cb->wt,
+                                            0, g->n}
+                          : GraphView<OffT>{},
+                       cb ? cb->K : 0};
     // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
     const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;
     float ms = 0;

@@ -258,6 +258,19 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   multi.grid_per_cu      workgroups per CU of the batched sweep (8)
  *   multi.hub_threshold / multi.hub_chunk / multi.wave_threshold /
  *   multi.g16_threshold    degree classes of the batched sweep
+ *   pull.col_shift         pull sweeps of whole graphs with at least two
+ *                          blocks of 2^this vertices gather block by block
+ *                          from a column-blocked copy of the CSR (built on
+ *                          first use: as many bytes again as ids + weights,
+ *                          plus K (n+1) offsets; at most 32 blocks); 0 off,
+ *                          6..30 on every graph, -1 (default) automatic:
+ *                          flat graphs only, four blocks of >= 2^20 vertices
+ *   cb.min_slice           rows of degree above this x blocks are swept block
+ *                          by block, lighter rows once over the plain CSR (4)
+ *   cb.g4_slice / cb.g16_slice / cb.wave_slice / cb.block_slice   mean
+ *                          slice degree (degree / blocks) above which 4 / 16
+ *                          lanes, a wavefront, a workgroup take a row's slice
+ *                          in column-blocked sweeps (2 / 12 / 48 / 512)
  * Plans are rebuilt for new values; changing knobs while another thread
  * runs on the same graph is not supported. */
 int hc_set_tuning(const char *key, int64_t value);

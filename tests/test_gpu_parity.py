@@ -518,3 +518,51 @@ def test_pr_pull_rejects_bad_launch(hc, gpu, kron16):
             with pytest.raises(hc.HcError) as e:
                 hc.pr_pull(dg, mapping=mapping, blocks=8, threads=threads)
             assert e.value.code == -1
+
+
+@pytest.mark.parametrize("shift,dist16,min_slice", [(12, 0, 4), (12, 1, 0), (14, 0, 1),
+                                                     (6, 1, 2), (12, 0, 1000)])
+def test_sssp_column_blocked_sweeps_exact(hc, gpu, oracle, kron16, uniform16, shift, dist16,
+                                          min_slice):
+    """Pull epochs over the column-blocked copy of the CSR (pull.col_shift
+    forces blocks at scale 16: 2^12 vertices -> 16 blocks, 2^14 -> 4, 2^6 ->
+    clamped to 32 blocks): distances bit-exact in every direction, with and
+    without the 16-bit mirror, on both graphs and a share of a handle; one
+    whole-graph epoch-API sweep lies between the Jacobi epoch and the
+    fixpoint and counts each lowered row once.  cb.min_slice moves the
+    split between rows swept once (light) and block by block (heavy): 0
+    blocks every row, 1000 none."""
+    try:
+        hc.set_tuning("pull.col_shift", shift)
+        hc.set_tuning("sssp.dist16", dist16)
+        hc.set_tuning("cb.min_slice", min_slice)
+        for g in (kron16, uniform16):
+            idx, nb, w = g.index, g.neighbors, g.weights
+            srcs = list(map(int, g.pick_sources(3)))
+            with hc.DeviceGraph(g) as dg:
+                for s in srcs:
+                    ref = oracle.sssp_dijkstra(idx, nb, w, s)
+                    for direction in (0, 1):
+                        r = hc.sssp(dg, s, direction=direction)
+                        assert np.array_equal(r.values, ref), (s, direction)
+                        assert r.stats["pull_epochs"] >= 1
+                sh = dg.share()
+                try:
+                    assert np.array_equal(hc.sssp(sh, srcs[0], direction=1).values,
+                                          oracle.sssp_dijkstra(idx, nb, w, srcs[0]))
+                finally:
+                    sh.free()
+                d0 = np.full(g.num_nodes, hc.INF_DIST, np.int32)
+                d0[srcs[0]] = 0
+                fix = oracle.sssp_dijkstra(idx, nb, w, srcs[0])
+                dist = hc.DeviceArray.from_host(d0)
+                for _ in range(3):
+                    cur = dist.get()
+                    jac, _ = oracle.sssp_jacobi_epoch(idx, nb, w, 0, g.num_nodes, cur)
+                    upd = hc.sssp_pull_epoch(dg, dist, 0, g.num_nodes, mapping=hc.MAP_BUCKETED)
+                    new = dist.get()
+                    assert np.all(new <= jac) and np.all(new >= fix)
+                    assert upd == int((new != cur).sum())
+                dist.free()
+    finally:
+        hc.reset_tuning()
