@@ -129,11 +129,12 @@ def kernel_roofline(hc, res, algo, count):
             "sources_timed": count}
 
 
-def committed_traffic(kernel, kind, scale):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
-    calibration), or None when no summary for this workload exists."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{kind}{scale}_pmc.md")
+def committed_traffic(kernel, workload):
+    """Bytes per launch of `kernel` between L2 and the memory fabric, from the
+    committed rocprofv3 --pmc summary of the same workload (DRAM read and
+    write requests per dispatch, sized by request kind), or None when no
+    summary for this workload exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -203,6 +204,24 @@ def pull_sweeps(hc, kind, scale, degree, seed, reps):
                       "gedges_per_s": round(g.num_edges / min(ms) / 1e6, 1)}), flush=True)
 
 
+def gather_probe(nbytes, count, reps):
+    """`count` random 4-byte reads from a table of `nbytes` (torch's index
+    kernel; indices int64, results int32): with the table far above the
+    caches every read is a separate line, the reference point for sizing the
+    read requests of random gathers."""
+    import torch
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(5)
+    idx = torch.randint(0, n, (count,), device="cuda", generator=g)
+    for _ in range(reps):
+        y = x[idx]
+    torch.cuda.synchronize()
+    print(json.dumps({"gather_probe_table_bytes": n * 4, "reads": count, "reps": reps,
+                      "index_bytes": count * 8, "result_bytes": count * 4,
+                      "sum": int(y.sum().item())}), flush=True)
+
+
 def copy_probe(nbytes, reps):
     """A plain device copy of known size (torch's copy kernel): the reference
     point that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
@@ -228,10 +247,28 @@ def run_pagerank(hc, oracle, scale, degree, seed, check):
         # The stopping rule of pr_pull_cpu as written (per-vertex 5e-8,
         # cap 201 as in pr_pull_gpu), reported beside the residual rule.
         pw = hc.pagerank(dg, max_iters=201, rule=1)
+        # Roofline of the iteration kernel: the same run with HIP events
+        # around each launch, on the stream it runs on.
+        hc.set_tuning("profile.kernel_events", 1)
+        try:
+            pk = hc.pagerank(dg)
+        finally:
+            hc.set_tuning("profile.kernel_events", 0)
+    st = pk.stats
+    k_avg_s = st["kernel_ms"] / 1e3 / max(st["kernel_launches"], 1)
+    per_launch = st["alg_bytes"] / max(st["epochs"], 1)
+    achieved = per_launch / k_avg_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": committed_traffic("pr_step_kernel", f"kron{scale}"),
+            "kernel": "pr_step_kernel", "avg_launch_us": round(k_avg_s * 1e6, 2),
+            "alg_bytes_per_launch": round(per_launch, 1),
+            "launches_timed": st["kernel_launches"]}
     out = {"graph": f"kron{scale} ef{degree}", "n": g.num_nodes, "m": g.num_edges,
            "iters": p.stats["epochs"], "ms": p.stats["ms"], "residual": p.stats["residual"],
            "gteps": hc.gteps(g.num_edges, p.stats["ms"]),
-           "alg_gbs": p.stats["alg_bytes"] / (p.stats["ms"] / 1e3) / 1e9}
+           "alg_gbs": p.stats["alg_bytes"] / (p.stats["ms"] / 1e3) / 1e9,
+           "roofline": roof}
     if check > 0:
         got = p.values.astype(np.float64)
         ref64 = oracle.pr_jacobi_f64(g.index, g.neighbors, 0.85, p.stats["epochs"])
@@ -526,8 +563,9 @@ def main():
     ap.add_argument("--pagerank", type=int, default=1)
     ap.add_argument("--pr-scale", type=int, default=24,
                     help="Kronecker scale of the PageRank graph (unweighted)")
-    ap.add_argument("--roofline-sources", type=int, default=16,
-                    help="sources of the event-timed pass over the dominant kernel")
+    ap.add_argument("--roofline-sources", type=int, default=-1,
+                    help="sources of the event-timed pass over the step kernels "
+                         "(-1: the timed sources)")
     ap.add_argument("--batch", type=int, default=1,
                     help="run the batched forms (64-source SSSP, many-graph BFS)")
     ap.add_argument("--batch-scale", type=int, default=22)
@@ -541,6 +579,9 @@ def main():
                     help="only run --steps full SSSP pull sweeps on KIND at the fixpoint")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
+    ap.add_argument("--gather-probe", type=int, default=0, metavar="BYTES",
+                    help="only run 2^24 random 4-byte reads from a table of BYTES "
+                         "(counter calibration)")
     args = ap.parse_args()
 
     if args.check < 0:
@@ -553,6 +594,11 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe, 3)
         return
+    if args.gather_probe:
+        gather_probe(args.gather_probe, 1 << 24, 3)
+        return
+    if args.roofline_sources < 0:
+        args.roofline_sources = args.steps
 
     import __graft_entry__ as ge
     hc = ge.package()
@@ -595,15 +641,21 @@ def main():
             exact[kind] = {"sssp": s_ok, "bfs": b_ok, "sources_certified": args.check,
                            "vs_dijkstra_and_fifo_bfs": 1}
 
-    # Dominant kernel: the step kernel of the algorithm with the larger share
-    # of device time.  Its launches are timed with HIP events on the stream
-    # they run on, in a second pass over the same sources (events between
-    # kernels are kept out of the headline timing).
-    tot_sssp = sum(r["sssp_ms"].sum() for r in results.values())
-    tot_bfs = sum(r["bfs_ms"].sum() for r in results.values())
-    algo = "sssp" if tot_sssp >= tot_bfs else "bfs"
-    roofline = kernel_roofline(hc, R, algo, min(args.steps, args.roofline_sources))
-    roofline["traffic"] = committed_traffic(roofline["kernel"], main_kind, args.scale)
+    # Step kernels of SSSP and BFS on every graph: launches timed with HIP
+    # events on the stream they run on, in a second pass over the same
+    # sources as the headline timing (events between kernels are kept out of
+    # the headline itself).  The dominant kernel (the algorithm with the
+    # larger share of device time on the headline graph) is `roofline`.
+    rooflines = {}
+    for kind, r in results.items():
+        for a in ("sssp", "bfs"):
+            rf = kernel_roofline(hc, r, a, min(args.steps, args.roofline_sources))
+            rf["traffic"] = committed_traffic(rf["kernel"], f"{kind}{args.scale}")
+            rooflines[f"{rf['kernel']}_{kind}{args.scale}"] = rf
+    algo = "sssp" if R["sssp_ms"].sum() >= R["bfs_ms"].sum() else "bfs"
+    roofline = dict(rooflines[f"{algo}_step_kernel_{main_kind}{args.scale}"])
+    if pr is not None:
+        rooflines[f"pr_step_kernel_kron{args.pr_scale}"] = pr["roofline"]
 
     batched = None
     if args.batch:
@@ -684,6 +736,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "roofline": roofline,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
         "bit_exact": exact,
         "device": dev["name"],

@@ -880,11 +880,10 @@ struct BfsLaunch {
         FrontierBufs fb = a->fb;
         fb.part = sum_set(a->fb.part, k);
         StepCtl *cur = a->ctl + (k & 1), *nxt = a->ctl + ((k + 1) & 1);
-        a->kt->mark(a->s);
-        bfs_step_kernel<OffT><<<a->grid, kBlock, a->lds_words * sizeof(uint32_t), a->s>>>(
-            a->plan, a->g, a->depth, a->parents, a->first, fb, cur, a->rl, a->nstage,
-            a->stage_min, a->push_cap, a->first_all);
-        a->kt->mark(a->s);
+        timed_launch(a->kt, bfs_step_kernel<OffT>, dim3(a->grid), dim3(kBlock),
+                     uint32_t(a->lds_words * sizeof(uint32_t)), a->s, a->plan, a->g, a->depth,
+                     a->parents, static_cast<const int32_t *>(a->first), fb, cur, a->rl,
+                     a->nstage, a->stage_min, a->push_cap, a->first_all);
         bfs_between_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(
             a->g, fb, cur, nxt, a->ctl, sum_set(a->fb.part, k + 1), (int)sizeof(OffT));
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "BFS epoch launch failed");

@@ -6,6 +6,7 @@
 #ifndef HC_DEVICE_COMMON_H
 #define HC_DEVICE_COMMON_H
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -204,12 +205,26 @@ struct KernelTimer {
         on = tuning().kernel_events != 0;
         used = 0;
     }
+    // Two fresh events for the next timed launch (nullptr when off).
+    hipEvent_t *next_pair() {
+        if (!on) return nullptr;
+        while ((int)ev.size() < used + 2) {
+            hipEvent_t e;
+            // Device-scope release: the timestamp is not delayed by a
+            // system-scope cache writeback after the kernel.
+            if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) {
+                on = false;
+                return nullptr;
+            }
+            ev.push_back(e);
+        }
+        used += 2;
+        return &ev[used - 2];
+    }
     void mark(hipStream_t s) {
         if (!on) return;
         if (used == (int)ev.size()) {
             hipEvent_t e;
-            // Device-scope release: the timestamp is not delayed by a
-            // system-scope cache writeback after the kernel.
             if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) {
                 on = false;
                 return;
@@ -233,6 +248,21 @@ struct KernelTimer {
         for (hipEvent_t e : ev) hipEventDestroy(e);
     }
 };
+
+// Launches `kernel` on s.  With the timer on, the dispatch itself records
+// its start and stop events (hipExtLaunchKernelGGL), so the time between
+// them is the kernel's execution only: events recorded around a launch
+// would also count the wait for the host to enqueue it when the stream has
+// run dry.
+template <typename K, typename... Args>
+inline void timed_launch(KernelTimer *kt, K kernel, dim3 grid, dim3 block, uint32_t lds,
+                         hipStream_t s, Args... args) {
+    hipEvent_t *e = kt ? kt->next_pair() : nullptr;
+    if (e)
+        hipExtLaunchKernelGGL(kernel, grid, block, lds, s, e[0], e[1], 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+}
 
 // Host-side owner of a plan's device arrays.
 struct PlanHost {

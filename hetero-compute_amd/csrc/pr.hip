@@ -537,11 +537,9 @@ static int run_pr(hc_graph_s *g, float *scores, float damp, double tol,
     HC_HIP(hipEventRecord(e0, s));
     ws->kt.reset();
     for (int i = 0; i < max_iters; i++) {
-        ws->kt.mark(s);
-        pr_step_kernel<OffT><<<grid, kBlock, ns * sizeof(float), s>>>(
-            ph->dev, v, ws->s[0], ws->s[1], ws->c[0], ws->c[1], b, ws->ctl, base,
-            damp, ns);
-        ws->kt.mark(s);
+        timed_launch(&ws->kt, pr_step_kernel<OffT>, dim3(grid), dim3(kBlock),
+                     uint32_t(ns * sizeof(float)), s, ph->dev, v, ws->s[0], ws->s[1], ws->c[0],
+                     ws->c[1], b, ws->ctl, base, damp, ns);
         pr_ctl_kernel<<<1, kBlock, 0, s>>>(ph->dev, ws->s[0], ws->s[1], ws->c[0],
                                            ws->c[1], b, ws->ctl, base, damp, grid);
     }

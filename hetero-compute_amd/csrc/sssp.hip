@@ -844,8 +844,9 @@ struct SsspLaunch {
     int cbk;
     template <bool D16, bool CB>
     void step(FrontierBufs fb, StepCtl *cur) {
-        sssp_step_kernel<OffT, D16, CB><<<grid, kBlock, lds_entries * sizeof(int32_t), s>>>(
-            plan, g, dist, d16, fb, cur, nstage, push_cap, defer_dedup, cbplan, cbv, cbk);
+        timed_launch(kt, sssp_step_kernel<OffT, D16, CB>, dim3(grid), dim3(kBlock),
+                     uint32_t(lds_entries * sizeof(int32_t)), s, plan, g, dist, d16, fb, cur,
+                     nstage, push_cap, defer_dedup, cbplan, cbv, cbk);
     }
     // Epoch k: header slot k & 1, counter set k % 3.
     static int launch(void *arg, long long k) {
@@ -853,7 +854,6 @@ struct SsspLaunch {
         FrontierBufs fb = a->fb;
         fb.part = sum_set(a->fb.part, k);
         StepCtl *cur = a->ctl + (k & 1), *nxt = a->ctl + ((k + 1) & 1);
-        a->kt->mark(a->s);
         if (a->d16) {
             if (a->cbk) a->template step<true, true>(fb, cur);
             else a->template step<true, false>(fb, cur);
@@ -861,7 +861,6 @@ struct SsspLaunch {
             if (a->cbk) a->template step<false, true>(fb, cur);
             else a->template step<false, false>(fb, cur);
         }
-        a->kt->mark(a->s);
         sssp_between_kernel<OffT><<<compact_grid(a->grid), kBlock, 0, a->s>>>(
             a->g, fb, cur, nxt, a->ctl, sum_set(a->fb.part, k + 1), a->dist, a->d16);
         if (hipGetLastError() != hipSuccess) return fail(HC_ERR_HIP, "SSSP epoch launch failed");
