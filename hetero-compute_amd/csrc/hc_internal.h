@@ -24,6 +24,16 @@ void clear_error();
 // plans), for a graph of n vertices and m directed edges.
 int64_t share_workspace_bound(int64_t n, int64_t m);
 
+// Host -> device copies through the library's pinned staging ring (packed by
+// all host threads, sent asynchronously, overlapped; one upload at a time).
+// staged_copy_checked: count int32 values, each checked against [lo, hi);
+// returns HC_ERR_INVALID with *bad_index = the first offending index (no
+// message set), else -1.  staged_offsets: count int64 offsets narrowed to
+// uint32 (wide = false) or widened to uint64.
+int staged_copy_checked(void *dst, const int32_t *src, int64_t count, int32_t lo, int32_t hi,
+                        int64_t *bad_index);
+int staged_offsets(void *dst, const int64_t *off, int64_t count, bool wide);
+
 }  // namespace hc
 
 #define HC_CHECK_ARG(cond, ...)                                   \

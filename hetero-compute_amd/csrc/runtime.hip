@@ -315,24 +315,7 @@ static int upload_rows(const hc_host_graph_t *hg, int32_t start, int32_t end,
     g->max_degree = maxd;
     // For sorted rows, [row0, row0+nz) holds every non-zero-degree row.
     g->num_nonzero = sorted ? nz : rows;
-    // Validate neighbour ids and weights (the reference reads them unchecked).
     const int32_t *nb = hg->neighbors + e0;
-    for (int64_t e = 0; e < m; e++)
-        if (nb[e] < 0 || nb[e] >= hg->num_nodes) {
-            delete g;
-            return fail(HC_ERR_INVALID, "neighbour %lld = %d outside [0,%d)",
-                        (long long)(e0 + e), nb[e], hg->num_nodes);
-        }
-    if (g->weighted) {
-        const int32_t *w = hg->weights + e0;
-        for (int64_t e = 0; e < m; e++)
-            if (w[e] < 0 || w[e] >= HC_INF_DIST) {
-                delete g;
-                return fail(HC_ERR_INVALID,
-                            "weight %lld = %d outside [0, 2^30)",
-                            (long long)(e0 + e), w[e]);
-            }
-    }
     size_t ob = g->off64 ? 8 : 4;
     size_t off_bytes = ob * (size_t(rows) + 1);
     size_t adj_bytes = sizeof(int32_t) * size_t(std::max<int64_t>(m, 1));
@@ -350,23 +333,24 @@ static int upload_rows(const hc_host_graph_t *hg, int32_t start, int32_t end,
                     off_bytes + adj_bytes * (hg->weights ? 2 : 1));
     }
     g->bytes = off_bytes + adj_bytes * (g->weighted ? 2 : 1);
-    int rc = HC_OK;
-    if (g->off64) {
-        std::vector<uint64_t> o(g->h_off.begin(), g->h_off.end());
-        if (hipMemcpy(g->d_off, o.data(), off_bytes, hipMemcpyHostToDevice) != hipSuccess)
-            rc = fail(HC_ERR_HIP, "offset upload failed");
-    } else {
-        std::vector<uint32_t> o(g->h_off.begin(), g->h_off.end());
-        if (hipMemcpy(g->d_off, o.data(), off_bytes, hipMemcpyHostToDevice) != hipSuccess)
-            rc = fail(HC_ERR_HIP, "offset upload failed");
+    // Offsets narrowed on the way; neighbour ids and weights checked while
+    // they are packed into the pinned staging buffers (the reference reads
+    // them unchecked).
+    int rc = staged_offsets(g->d_off, g->h_off.data(), int64_t(rows) + 1, g->off64);
+    int64_t bad = -1;
+    if (rc == HC_OK && m > 0) {
+        rc = staged_copy_checked(g->d_adj, nb, m, 0, hg->num_nodes, &bad);
+        if (rc == HC_ERR_INVALID)
+            rc = fail(HC_ERR_INVALID, "neighbour %lld = %d outside [0,%d)",
+                      (long long)(e0 + bad), nb[bad], hg->num_nodes);
     }
-    if (rc == HC_OK && m > 0 &&
-        hipMemcpy(g->d_adj, nb, sizeof(int32_t) * m, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(HC_ERR_HIP, "neighbour upload failed");
-    if (rc == HC_OK && g->weighted && m > 0 &&
-        hipMemcpy(g->d_wt, hg->weights + e0, sizeof(int32_t) * m,
-                  hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(HC_ERR_HIP, "weight upload failed");
+    if (rc == HC_OK && g->weighted && m > 0) {
+        const int32_t *w = hg->weights + e0;
+        rc = staged_copy_checked(g->d_wt, w, m, 0, HC_INF_DIST, &bad);
+        if (rc == HC_ERR_INVALID)
+            rc = fail(HC_ERR_INVALID, "weight %lld = %d outside [0, 2^30)",
+                      (long long)(e0 + bad), w[bad]);
+    }
     if (rc != HC_OK) {
         hc_graph_free(g);
         return rc;

@@ -566,3 +566,39 @@ def test_sssp_column_blocked_sweeps_exact(hc, gpu, oracle, kron16, uniform16, sh
                 dist.free()
     finally:
         hc.reset_tuning()
+
+
+def test_upload_checks_every_chunk_and_copies_exactly(hc, gpu, oracle):
+    """The CSR travels through 32 MiB pinned staging buffers: a neighbour id
+    or a weight out of range in the third buffer is reported with its index
+    (and nothing leaks: the same upload corrected then succeeds), and a graph
+    spanning several buffers in every array (offsets of 9.5 M rows, 24 M
+    edges) gives the oracle's distances and depths."""
+    n = 9_500_000
+    rng = np.random.RandomState(11)
+    # Symmetric (as every graph of the path is): each random pair both ways.
+    a = rng.randint(0, n, size=12_000_000).astype(np.int32)
+    b = rng.randint(0, n, size=12_000_000).astype(np.int32)
+    w2 = rng.randint(1, 256, size=12_000_000).astype(np.int32)
+    src = np.concatenate([a, b])
+    order = np.argsort(src, kind="stable")
+    nb = np.concatenate([b, a])[order]
+    wt = np.concatenate([w2, w2])[order]
+    deg = np.bincount(src, minlength=n)
+    idx = np.zeros(n + 1, np.int64)
+    np.cumsum(deg, out=idx[1:])
+    m = int(idx[-1])
+    assert m > 20_000_000 and (n + 1) * 4 > (32 << 20)
+    bad = 17_000_000
+    for arr, val, what in ((nb, n, "neighbour"), (wt, 1 << 30, "weight")):
+        keep = int(arr[bad])
+        arr[bad] = val
+        with pytest.raises(hc.HcError) as e:
+            hc.DeviceGraph(hc.HostGraph.from_csr(idx, nb, wt))
+        assert e.value.code == -1 and f"{what} {bad} " in str(e.value), str(e.value)
+        arr[bad] = keep
+    g = hc.HostGraph.from_csr(idx, nb, wt)
+    with hc.DeviceGraph(g) as dg:
+        s = int(np.flatnonzero(deg > 3)[0])
+        assert np.array_equal(hc.sssp(dg, s).values, oracle.sssp_dijkstra(idx, nb, wt, s))
+        assert np.array_equal(hc.bfs(dg, s).values, oracle.bfs_depth(idx, nb, s))
