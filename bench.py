@@ -385,7 +385,10 @@ def run_many_graph_bfs(hc, oracle, count, seed, check):
 
 def run_serving(hc, oracle, big, unif, rates, queries, seed, check):
     """Poisson arrivals of 70% BFS / 30% SSSP queries; 20% target small
-    graphs uploaded on demand.  Latency: arrival -> result in HBM."""
+    graphs uploaded on demand.  Latency: arrival -> result in HBM.  Every
+    result on an uploaded graph is compared with the oracle; on the resident
+    graphs `check` results per graph and kind, spread over the run, are
+    certified exact (each certified column costs an O(m) pass on the host)."""
     import threading
     rng = np.random.RandomState(seed)
     dyn = [hc.HostGraph.generate("kron", int(s), 16, weighted=True, seed=seed + int(s))
@@ -417,10 +420,19 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check):
                 lag[i] = max(0.0, time.perf_counter() - start - arrive[i])
                 tickets[i] = srv.submit(gids[k], kind, src)
 
+        # Resident-graph results to certify: every n-th of each (graph, kind).
+        per_class = {(k, kind): 0 for k in (0, 1) for kind in (0, 1)}
+        for k, kind, _ in qs:
+            if k < 2:
+                per_class[(k, kind)] += 1
+        stride = {c: max(1, n // max(check, 1)) for c, n in per_class.items()}
+        seen = {c: 0 for c in per_class}
         th = threading.Thread(target=submitter)
         th.start()
         lat, edges, batches, checked, exact, errors = [], 0, [], 0, True, []
-        kept_small, kept_big, alone = [], [], 0
+        kept_small, kept_big = [], []
+        done_at = np.zeros(queries)
+        sweeps = {}
         for i in range(queries):
             while tickets[i] is None:
                 time.sleep(0.0001)
@@ -430,16 +442,21 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check):
                 srv.release(tickets[i])
                 continue
             lat.append(info["latency_ms"] + lag[i] * 1e3)
+            done_at[i] = time.perf_counter() - start
+            k_, kind_, _ = qs[i]
+            cls = ("kron" if k_ == 0 else "uniform" if k_ == 1 else "small") + \
+                ("_bfs" if kind_ == hc.QUERY_BFS else "_sssp")
+            sweeps.setdefault(cls, []).append((info["batch_size"], info["run_ms"]))
             edges += info["num_edges"]
             batches.append(info["batch_size"])
             k, kind, src = qs[i]
-            # Results kept for checking after the rate: every query on an
-            # uploaded small graph (against the CPU oracle) and the first
-            # `check` queries on the big graphs (certified exact, hence the
-            # same alone and in the mix).
-            if check > 0 and (k >= 2 or len(kept_big) < check):
-                got = srv.fetch(tickets[i])
-                (kept_small if k >= 2 else kept_big).append((k, kind, src, got))
+            if k >= 2:
+                kept_small.append((k, kind, src, srv.fetch(tickets[i])))
+            elif check > 0:
+                c = (k, kind)
+                if seen[c] % stride[c] == 0 and seen[c] // stride[c] < check:
+                    kept_big.append((k, kind, src, srv.fetch(tickets[i])))
+                seen[c] += 1
             srv.release(tickets[i])
         wall = time.perf_counter() - start
         th.join()
@@ -449,11 +466,13 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check):
                    else oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
             exact &= bool(np.array_equal(got, ref))
             checked += 1
+        kept_small = None
         # Results on the resident graphs: certified exact by the oracle's
         # shortest-path certificate, i.e. equal to the query run alone.
         groups = {}
         for k, kind, src, got in kept_big:
             groups.setdefault((k, kind), []).append((src, got))
+        certified = 0
         for (k, kind), items in groups.items():
             g = hosts[k]
             M = np.ascontiguousarray(np.stack([x[1] for x in items], axis=1))
@@ -462,23 +481,42 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check):
                                        [x[0] for x in items], M,
                                        oracle.INF if kind == hc.QUERY_SSSP else -1)
             exact &= bool(bad.sum() == 0)
-            alone += len(items)
+            certified += len(items)
             del M
+        kept_big = groups = None
         lat = np.array(lat)
+        # Steady state: the queries that arrived after the first fifth of the
+        # run (the queue has filled or drained to its working level by then).
+        steady = lat[queries // 5:]
         results.append({"rate_qps": rate, "queries": queries,
                         "p50_ms": round(float(np.percentile(lat, 50)), 3),
                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                        "steady_p50_ms": round(float(np.percentile(steady, 50)), 3),
+                        "steady_p99_ms": round(float(np.percentile(steady, 99)), 3),
                         "sustained_gteps": round(edges / 2 / wall / 1e9, 3),
                         "achieved_qps": round(queries / wall, 1),
                         "mean_batch": round(float(np.mean(batches)), 2),
-                        "checked_vs_oracle": checked, "certified_resident": alone,
+                        "mean_sssp_batch_resident": round(float(np.mean(
+                            [b for b, (k, kind, _) in zip(batches, qs)
+                             if k < 2 and kind == hc.QUERY_SSSP] or [0])), 2),
+                        # per class: queries, mean batch, slot-ms of sweeps per query
+                        "classes": {c: {"queries": len(v),
+                                        "mean_batch": round(float(np.mean([b for b, _ in v])), 1),
+                                        "run_ms_per_query": round(float(np.mean(
+                                            [r / b for b, r in v])), 3)}
+                                    for c, v in sorted(sweeps.items())},
+                        "checked_vs_oracle": checked, "certified_resident": certified,
                         "bit_exact": exact,
                         "failed": len(errors), "errors": errors[:3]})
+        log(f"[serve] {rate} q/s: achieved {results[-1]['achieved_qps']} q/s, "
+            f"p50 {results[-1]['p50_ms']} ms, p99 {results[-1]['p99_ms']} ms")
     st = srv.stats()
     srv.close()
     return {"graphs": {"kron": big.num_nodes, "uniform": unif.num_nodes,
                        "dynamic_scales": [int(np.log2(g.num_nodes)) for g in dyn]},
-            "rates": results, "uploads": st["uploads"], "evictions": st["evictions"],
+            "rates": results,
+            "saturation_qps": max(r["achieved_qps"] for r in results),
+            "uploads": st["uploads"], "evictions": st["evictions"],
             "spills": st["spills"],
             "batched_queries": st["batched_queries"]}
 
@@ -572,7 +610,9 @@ def main():
     ap.add_argument("--many-graphs", type=int, default=512)
     ap.add_argument("--serve", type=int, default=1, help="run the serving loop")
     ap.add_argument("--serve-scale", type=int, default=24)
-    ap.add_argument("--serve-queries", type=int, default=400)
+    ap.add_argument("--serve-queries", type=int, default=2000)
+    ap.add_argument("--serve-check", type=int, default=32,
+                    help="resident-graph results certified per graph, kind and rate")
     ap.add_argument("--class-sweep", default="", metavar="KIND",
                     help="only time the SSSP pull sweep per degree class on KIND (kron|uniform)")
     ap.add_argument("--pull-sweeps", default="", metavar="KIND",
@@ -675,7 +715,7 @@ def main():
         # 100 q/s sits below the loop's capacity (its latency is service time);
         # 1k-20k q/s are the offered rates of the serving configuration.
         serving = run_serving(hc, oracle, big, unif, [100, 1000, 5000, 20000],
-                              args.serve_queries, 777, max(args.check, 4))
+                              args.serve_queries, 777, args.serve_check if args.check else 0)
         del big
 
     cpu = None

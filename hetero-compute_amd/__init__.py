@@ -97,7 +97,8 @@ class ScheduleBlock(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("memory_budget", C.c_int64),
-                ("min_batch_sssp", C.c_int32), ("min_batch_bfs", C.c_int32)]
+                ("min_batch_sssp", C.c_int32), ("min_batch_bfs", C.c_int32),
+                ("max_wait_ms", C.c_int32)]
 
 
 class QueryInfo(C.Structure):
@@ -904,11 +905,14 @@ class Server:
     (hc_server_*).  Host graphs added here are kept alive by the object."""
 
     def __init__(self, slots: int = 4, max_batch: int = 64, memory_budget: int = 0,
-                 min_batch_sssp: int = 0, min_batch_bfs: int = 0):
+                 min_batch_sssp: int = 0, min_batch_bfs: int = 0, max_wait_ms: int = 0):
         """min_batch_*: queries of one graph and kind that must wait before
-        they run as one batched sweep (0: the library's defaults, 8 and 64)."""
+        they run as one batched sweep (0: the library's defaults, 8 and 64);
+        max_wait_ms: how long fast-arriving SSSP queries may wait to gather a
+        batch (0: the default, 500; -1: never)."""
         self.h = _P()
-        o = ServerOpts(slots, max_batch, memory_budget, min_batch_sssp, min_batch_bfs)
+        o = ServerOpts(slots, max_batch, memory_budget, min_batch_sssp, min_batch_bfs,
+                       max_wait_ms)
         check(lib().hc_server_create(C.byref(o), C.byref(self.h)))
         self._graphs = []
 

@@ -54,6 +54,13 @@ struct Query {
     int32_t column = -1;  // column of a batched result, -1 for a single one
 };
 
+// Arrival rate of one (graph, kind) class: moving average of the gaps
+// between its submissions.
+struct ClassRate {
+    double last = -1.0;  // ms of the last submission
+    double gap = -1.0;   // ms between submissions (-1: fewer than two yet)
+};
+
 struct GraphEntry {
     hc_host_graph_t host{};
     bool resident = false;
@@ -73,6 +80,7 @@ struct hc_server_s {
     std::condition_variable cv_work, cv_done, cv_mem;
     std::deque<int64_t> queue;
     std::map<int64_t, Query> queries;   // stable iterators across inserts
+    std::map<std::pair<int32_t, int32_t>, ClassRate> rates;  // by (graph, kind)
     std::deque<GraphEntry> graphs;      // stable references across inserts
     std::vector<std::thread> workers;
     std::vector<hipStream_t> streams;
@@ -239,6 +247,53 @@ bool spill_one(hc_server_s *s, std::unique_lock<std::mutex> &lk) {
     return ok;
 }
 
+// Chooses the next sweep (caller holds s->mu): classes (graph, kind) are
+// taken in the arrival order of their oldest waiting query.  A class runs
+// its oldest queries up to max_batch when at least min_batch of them wait,
+// else its oldest query alone -- except that an SSSP class whose arrivals
+// would gather 2 x min_batch queries within max_wait_ms of its oldest one's
+// arrival waits (other classes run meanwhile) until max_batch wait or the
+// oldest has waited max_wait_ms.  Returns false when every waiting class is
+// gathering; *wake is then the ms at which the first of them stops.
+bool pick_batch(hc_server_s *s, double now, std::vector<int64_t> &batch, double *wake) {
+    *wake = 1e300;
+    std::vector<std::pair<int32_t, int32_t>> seen;
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+        const Query &head = s->queries[*it];
+        const std::pair<int32_t, int32_t> cls{head.graph, head.kind};
+        if (std::find(seen.begin(), seen.end(), cls) != seen.end()) continue;
+        seen.push_back(cls);
+        int waiting = 0;
+        for (auto jt = it; jt != s->queue.end() && waiting < s->opts.max_batch; ++jt)
+            if (s->queries[*jt].graph == cls.first && s->queries[*jt].kind == cls.second)
+                waiting++;
+        const int min_batch =
+            cls.second == HC_QUERY_BFS ? s->opts.min_batch_bfs : s->opts.min_batch_sssp;
+        if (cls.second == HC_QUERY_SSSP && !s->stop && s->opts.max_wait_ms > 0 &&
+            waiting < s->opts.max_batch && s->opts.max_batch >= 2 * min_batch) {
+            const double gap = s->rates[cls].gap;
+            const double deadline = head.t_submit + s->opts.max_wait_ms;
+            if (gap > 0 && now < deadline &&
+                waiting + (deadline - now) / gap >= 2.0 * min_batch) {
+                *wake = std::min(*wake, deadline);
+                continue;  // gathering
+            }
+        }
+        const int take = waiting >= min_batch ? s->opts.max_batch : 1;
+        for (auto jt = it; jt != s->queue.end() && (int)batch.size() < take;) {
+            const Query &q = s->queries[*jt];
+            if (q.graph == cls.first && q.kind == cls.second) {
+                batch.push_back(*jt);
+                jt = s->queue.erase(jt);
+            } else {
+                ++jt;
+            }
+        }
+        return true;
+    }
+    return false;
+}
+
 void worker(hc_server_s *s, int slot) {
     hipStream_t stream = s->streams[slot];
     std::unique_lock<std::mutex> lk(s->mu);
@@ -251,29 +306,16 @@ void worker(hc_server_s *s, int slot) {
             if (s->stop) break;
             continue;
         }
-        // The oldest query and every waiting query of the same graph and
-        // kind, in arrival order, up to max_batch -- when enough of them
-        // wait for a batched sweep to beat single-source runs; otherwise
-        // the oldest query alone.
-        const int64_t head = s->queue.front();
-        const int gid = s->queries[head].graph, kind = s->queries[head].kind;
-        int waiting = 0;
-        for (auto it = s->queue.begin(); it != s->queue.end() && waiting < s->opts.max_batch;
-             ++it)
-            if (s->queries[*it].graph == gid && s->queries[*it].kind == kind) waiting++;
-        const int min_batch =
-            kind == HC_QUERY_BFS ? s->opts.min_batch_bfs : s->opts.min_batch_sssp;
-        const int take = waiting >= min_batch ? s->opts.max_batch : 1;
         std::vector<int64_t> batch;
-        for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < take;) {
-            const Query &q = s->queries[*it];
-            if (q.graph == gid && q.kind == kind) {
-                batch.push_back(*it);
-                it = s->queue.erase(it);
-            } else {
-                ++it;
-            }
+        double wake = 0;
+        if (!pick_batch(s, now_ms(), batch, &wake)) {
+            // Every waiting class is gathering: sleep until the first of them
+            // is due, or until a submission (or shutdown) wakes this slot.
+            s->cv_work.wait_for(lk, std::chrono::duration<double, std::milli>(
+                                        std::max(0.05, wake - now_ms())));
+            continue;
         }
+        const int gid = s->queries[batch[0]].graph, kind = s->queries[batch[0]].kind;
         GraphEntry &g = s->graphs[gid];
         g.in_use++;
         // Admission: result memory (and the graph, if it is not in HBM).
@@ -420,6 +462,7 @@ int hc_server_create(const hc_server_opts_t *opts, hc_server_t *out) {
     s->opts.memory_budget = opts ? opts->memory_budget : 0;
     s->opts.min_batch_sssp = opts && opts->min_batch_sssp > 0 ? opts->min_batch_sssp : 8;
     s->opts.min_batch_bfs = opts && opts->min_batch_bfs > 0 ? opts->min_batch_bfs : kWidth;
+    s->opts.max_wait_ms = opts && opts->max_wait_ms != 0 ? opts->max_wait_ms : 500;
     if (s->opts.slots > 16 || s->opts.max_batch > kWidth) {
         delete s;
         return hc::fail(HC_ERR_INVALID, "slots must be <= 16 and max_batch <= %d", kWidth);
@@ -528,6 +571,12 @@ int hc_server_submit(hc_server_t s, int32_t graph_id, int32_t kind, int32_t sour
     q.kind = kind;
     q.source = source;
     q.t_submit = now_ms();
+    ClassRate &r = s->rates[{graph_id, kind}];
+    if (r.last >= 0) {
+        const double g = q.t_submit - r.last;
+        r.gap = r.gap < 0 ? g : 0.875 * r.gap + 0.125 * g;
+    }
+    r.last = q.t_submit;
     s->queries.emplace(t, std::move(q));
     s->queue.push_back(t);
     s->st.submitted++;

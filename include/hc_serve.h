@@ -38,6 +38,13 @@ typedef struct hc_server_opts {
                                about 8 single-source runs)                     */
     int32_t min_batch_bfs;  /* the same for BFS (default 64: a batched BFS
                                sweep pays only at full width, if at all)       */
+    int32_t max_wait_ms;    /* SSSP queries of one graph arriving fast enough
+                               to gather 2 x min_batch_sssp of them within this
+                               many ms of the oldest one's arrival wait (while
+                               other queries run) until a full batch is there or
+                               the oldest has waited this long, then run as one
+                               sweep; slower classes run at once (default 500;
+                               < 0: never wait)                                */
 } hc_server_opts_t;
 
 typedef struct hc_query_info {

@@ -152,3 +152,43 @@ def test_server_rejects_bad_queries(hc, gpu, graphs):
     with pytest.raises(hc.HcError):
         srv.wait(12345)
     srv.close()
+
+
+def test_fast_sssp_arrivals_gather_batches(hc, gpu, oracle, graphs):
+    """SSSP queries arriving fast enough to gather 2 x min_batch within
+    max_wait_ms wait for a batch (while BFS queries keep running alone);
+    sparse arrivals run at once, one query per run.  Every answer exact."""
+    import time
+    g = graphs["kron"]
+    srv = hc.Server(slots=4, max_batch=64, max_wait_ms=300)
+    gid = srv.add_graph(g)
+    rng = np.random.RandomState(5)
+    fast = [(int(rng.rand() < 0.3), int(s)) for s in g.pick_sources(240, seed=9)]
+    t0 = time.perf_counter()
+    tickets = []
+    for i, (kind, src) in enumerate(fast):       # ~2000 queries/s
+        delay = t0 + i * 0.0005 - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(srv.submit(gid, kind, src))
+    res = _answers(srv, tickets)
+    sssp_batches = [info["batch_size"] for (kind, _), (_, info) in zip(fast, res) if kind == 1]
+    bfs_batches = [info["batch_size"] for (kind, _), (_, info) in zip(fast, res) if kind == 0]
+    assert np.mean(sssp_batches) >= 16, sssp_batches
+    assert max(bfs_batches) == 1
+    for (kind, src), (m, info) in zip(fast, res):
+        ref = (oracle.bfs_depth(g.index, g.neighbors, src) if kind == 0
+               else oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
+        assert np.array_equal(m, ref), (kind, src)
+    # Sparse arrivals (one SSSP query every 60 ms): no gathering.
+    slow = [int(s) for s in g.pick_sources(6, seed=21)]
+    tickets = []
+    for src in slow:
+        tickets.append(srv.submit(gid, 1, src))
+        time.sleep(0.06)
+    res = _answers(srv, tickets)
+    assert all(info["batch_size"] == 1 for _, info in res)
+    assert all(info["queue_ms"] < 200 for _, info in res), [info["queue_ms"] for _, info in res]
+    for src, (m, _) in zip(slow, res):
+        assert np.array_equal(m, oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
+    srv.close()
