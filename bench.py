@@ -521,6 +521,62 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check):
             "batched_queries": st["batched_queries"]}
 
 
+def run_coexec(hc, oracle, res, sources, segments=24):
+    """Host+GPU co-execution of SSSP pull epochs (the heterogeneous driver,
+    one runtime for any schedule) on the benchmark graph: the all-GPU
+    schedule, the schedule the autotuner derives from per-segment profiles
+    (host cores vs the best GPU kernel per segment, PCIe exchange priced in),
+    and a forced split with the two sparsest segments on the host cores.
+    Distances of every run are compared with the all-GPU ones and the first
+    source's with the oracle's Dijkstra."""
+    import importlib
+    at = importlib.import_module("hetero-compute_amd.autotune")
+    g = res["g"]
+    t0 = time.time()
+    prof = at.profile_segments(g, "sssp", segments)
+    blocks, makespan, plan = at.schedule_from_profile(prof)
+    prof_s = time.time() - t0
+    gpu_only = [{"device": "AMD Instinct MI355X", "start": 0, "end": segments - 1,
+                 "kernel": "SSSP GPU bucketed"}]
+    forced = [{"device": "AMD Instinct MI355X", "start": 0, "end": segments - 3,
+               "kernel": "SSSP GPU bucketed"},
+              {"device": "Host CPU", "start": segments - 2, "end": segments - 1,
+               "kernel": "SSSP CPU one-to-one"}]
+    out = {"segments": segments, "profile_s": round(prof_s, 1),
+           "tuned_cpu_segments": plan["cpu_segments"],
+           "tuned_makespan_estimate_ms": round(makespan, 3), "schedules": {}}
+    ref = None
+    exact = True
+    for name, bl in (("gpu_only", gpu_only), ("autotuned", blocks), ("forced_split", forced)):
+        with hc.Hetero(g, hc.Schedule.create(bl, segments)) as h:
+            ms, ep = [], []
+            for i, src in enumerate(sources):
+                r = h.sssp(int(src))
+                ms.append(r.stats["ms"])
+                ep.append(r.stats["epochs"])
+                if name == "gpu_only":
+                    if i == 0:
+                        exact &= bool(np.array_equal(r.values, oracle.sssp_dijkstra(
+                            g.index, g.neighbors, g.weights, int(src))))
+                    ref = (ref or []) + [r.values]
+                else:
+                    exact &= bool(np.array_equal(r.values, ref[i]))
+        out["schedules"][name] = {"blocks": len(bl), "ms_mean": round(float(np.mean(ms)), 3),
+                                  "pull_epochs_mean": float(np.mean(ep))}
+    best = min(out["schedules"].items(), key=lambda kv: kv[1]["ms_mean"])
+    out["fastest"] = best[0]
+    # gpu_only / autotuned: the gain of the profile-derived schedule (its
+    # host segments, if any, and per-segment GPU kernels); gpu_only /
+    # forced_split: what handing the two sparsest segments to the host
+    # cores gains (< 1: a loss).
+    sch = out["schedules"]
+    out["autotuned_gain"] = round(sch["gpu_only"]["ms_mean"] / sch["autotuned"]["ms_mean"], 3)
+    out["host_split_gain"] = round(sch["gpu_only"]["ms_mean"] / sch["forced_split"]["ms_mean"],
+                                   3)
+    out["bit_exact"] = exact
+    return out
+
+
 def check_exact(hc, oracle, res, count):
     """Exactness of the first `count` timed sources: every SSSP distance and
     BFS depth vector is certified by the oracle's O(m) shortest-path
@@ -609,6 +665,8 @@ def main():
     ap.add_argument("--batch-scale", type=int, default=22)
     ap.add_argument("--many-graphs", type=int, default=512)
     ap.add_argument("--serve", type=int, default=1, help="run the serving loop")
+    ap.add_argument("--coexec", type=int, default=1,
+                    help="host+GPU co-execution: all-GPU vs profile-derived vs forced split")
     ap.add_argument("--serve-scale", type=int, default=24)
     ap.add_argument("--serve-queries", type=int, default=2000)
     ap.add_argument("--serve-check", type=int, default=32,
@@ -705,6 +763,10 @@ def main():
         log("[batch] many-graph BFS")
         batched["bfs_many"] = run_many_graph_bfs(hc, oracle, args.many_graphs, 4242,
                                                  args.check)
+    coexec = None
+    if args.coexec:
+        log("[coexec] profiling segments")
+        coexec = run_coexec(hc, oracle, R, R["sources"][:2])
     serving = None
     if args.serve:
         log("[serve] generating graphs")
@@ -775,6 +837,7 @@ def main():
         "pagerank": pr,
         "batched": batched,
         "serving": serving,
+        "coexec": coexec,
         "roofline": roofline,
         "rooflines": rooflines,
         "cpu_baseline": cpu,
