@@ -171,6 +171,9 @@ _SIGS = {
     "hc_set_tuning": ([C.c_char_p, C.c_int64], C.c_int),
     "hc_get_tuning": ([C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
     "hc_reset_tuning": ([], C.c_int),
+    "hc_graph_set_tuning": ([_P, C.c_char_p, C.c_int64], C.c_int),
+    "hc_graph_get_tuning": ([_P, C.c_char_p, C.POINTER(C.c_int64)], C.c_int),
+    "hc_graph_reset_tuning": ([_P], C.c_int),
     "hc_sssp_multi": ([_P, _P, C.c_int32, _P, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
     "hc_bfs_multi": ([_P, _P, C.c_int32, _P, C.c_int32, C.POINTER(RunStats), _P], C.c_int),
     "hc_graph_batch_upload": ([C.POINTER(HostGraphStruct), C.c_int32, C.POINTER(_P)], C.c_int),
@@ -187,6 +190,7 @@ _SIGS = {
     "hc_server_destroy": ([_P], C.c_int),
     "hc_server_add_graph": ([_P, C.POINTER(HostGraphStruct), C.c_int32,
                              C.POINTER(C.c_int32)], C.c_int),
+    "hc_server_set_graph_tuning": ([_P, C.c_int32, C.c_char_p, C.c_int64], C.c_int),
     "hc_server_submit": ([_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64)], C.c_int),
     "hc_server_wait": ([_P, C.c_int64, C.POINTER(QueryInfo)], C.c_int),
     "hc_server_fetch": ([_P, C.c_int64, _P], C.c_int),
@@ -229,7 +233,9 @@ TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "sssp.filter", "sssp.stage", "sssp.g
                "pull.g4_threshold", "bu.wave_threshold", "bu.g16_threshold",
                "bu.g4_threshold", "multi.hub_threshold", "multi.hub_chunk",
                "multi.wave_threshold", "multi.g16_threshold", "pull.col_shift",
-               "cb.min_slice", "cb.g4_slice", "cb.g16_slice", "cb.wave_slice", "cb.block_slice")
+               "cb.min_slice", "cb.g4_slice", "cb.g16_slice", "cb.wave_slice", "cb.block_slice",
+               "multi.block", "many.block", "many.per_cu", "many.light_deg", "many.heavy_cap",
+               "many.lds_nodes")
 
 
 def set_tuning(key: str, value: int):
@@ -491,6 +497,18 @@ class DeviceGraph:
         out.num_nodes = self.num_nodes
         out._parent = self
         return out
+
+    def set_tuning(self, key: str, value: int):
+        """A knob of this graph and its shares only (hc_graph_set_tuning)."""
+        check(lib().hc_graph_set_tuning(self.h, key.encode(), int(value)))
+
+    def get_tuning(self, key: str) -> int:
+        v = C.c_int64()
+        check(lib().hc_graph_get_tuning(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    def reset_tuning(self):
+        check(lib().hc_graph_reset_tuning(self.h))
 
     def info(self) -> dict:
         i = GraphInfo()
@@ -921,6 +939,10 @@ class Server:
         check(lib().hc_server_add_graph(self.h, g.struct_ref(), int(resident), C.byref(gid)))
         self._graphs.append(g)
         return gid.value
+
+    def set_graph_tuning(self, graph: int, key: str, value: int):
+        """A knob for queries on one graph (hc_server_set_graph_tuning)."""
+        check(lib().hc_server_set_graph_tuning(self.h, graph, key.encode(), int(value)))
 
     def submit(self, graph: int, kind: int, source: int) -> int:
         t = C.c_int64()

@@ -8,10 +8,12 @@ a Python scheduler assigns segments to devices
 generator emits one driver per schedule (kernelgen/sssp_hetero.py).  Here
 the measurements drive the library directly:
 
-* ``tune_knobs`` searches the tuning knobs (workgroups per CU, degree-class
-  thresholds, LDS staging, SSSP push/pull ratio, batched-sweep classes) by
-  coordinate descent on measured device time, checking that every candidate
-  returns bit-identical results, and applies the best;
+* ``tune_knobs`` searches the tuning knobs (workgroup size and workgroups
+  per CU, degree-class thresholds, LDS staging, column blocking, SSSP
+  push/pull ratio, batched-sweep classes, the many-graph kernel's launch
+  shape and per-level thresholds) by coordinate descent on measured device
+  time, checking that every candidate returns bit-identical results, and
+  applies the best, process-wide or to the one graph (``per_graph``);
 * ``profile_segments`` times every equal-edge segment on the host cores and
   with every GPU epoch mapping and launch shape;
 * ``schedule_from_profile`` picks each segment's fastest GPU kernel and moves
@@ -19,7 +21,10 @@ the measurements drive the library directly:
   (GPU time + PCIe exchange vs host time), then merges neighbouring segments
   into blocks: the result is an ``hc`` Schedule (``.skd`` text) that
   ``Hetero`` runs;
-* results are cached in a JSON file keyed by device and graph signature.
+* results are cached in a JSON file keyed by device, algorithm and graph
+  signature; ``apply_cached`` sets a graph's cached knobs on its handle (or
+  on a server's graph), so callers running several graphs each get their
+  own.
 
 Run as a script to tune a generated graph:
     python hetero-compute_amd/autotune.py --kind kron --scale 20 --algorithm sssp
@@ -56,6 +61,9 @@ KNOB_SPACE = {
         "sssp.g4_threshold": [5, 8, 16],
         "pull.hub_threshold": [16384, 65536],
         "sssp.grid_per_cu": [2, 4, 8, 16],
+        "pull.col_shift": [-1, 0, 20, 21],
+        "cb.min_slice": [2, 4, 8],
+        "cb.g16_slice": [8, 12, 24],
     },
     "bfs": {
         "grid_per_cu": [2, 4, 8, 16],
@@ -77,11 +85,20 @@ KNOB_SPACE = {
         "pull.hub_threshold": [4096, 16384, 65536],
     },
     "multi": {
+        "multi.block": [128, 256, 512, 1024],
         "multi.grid_per_cu": [4, 8, 16],
         "multi.wave_threshold": [8, 16, 32, 64],
         "multi.g16_threshold": [2, 4, 8],
         "multi.hub_threshold": [2048, 4096, 16384],
         "multi.hub_chunk": [1024, 2048, 8192],
+    },
+    # The many-graph BFS (process-wide knobs; `dg` is a GraphBatch).
+    "many": {
+        "many.block": [256, 512, 1024],
+        "many.per_cu": [0, 1, 2, 4],
+        "many.light_deg": [8, 16, 32, 64, 128],
+        "many.heavy_cap": [128, 256, 512, 1024],
+        "many.lds_nodes": [0, 1 << 16, 1 << 17],
     },
 }
 
@@ -97,7 +114,14 @@ MAPPING = {"one-to-one": 0, "warp-red": 1, "warp": 1, "block-red": 2, "bucketed"
 
 
 def signature(g) -> str:
-    """Graph key for the cache: size and a hash of the degree sequence."""
+    """Graph key for the cache: size and a hash of the degree sequence (of
+    every graph, for a list of graphs: a batch)."""
+    if isinstance(g, (list, tuple)):
+        h = hashlib.sha1()
+        for x in g:
+            h.update(np.diff(x.index).astype(np.int64).tobytes())
+        return (f"batch{len(g)}-n{sum(x.num_nodes for x in g)}-"
+                f"m{sum(x.num_edges for x in g)}-{h.hexdigest()[:16]}")
     deg = np.diff(g.index).astype(np.int64)
     h = hashlib.sha1(deg.tobytes()).hexdigest()[:16]
     return f"n{g.num_nodes}-m{g.num_edges}-{h}"
@@ -117,6 +141,11 @@ def _run(dg, algorithm, sources):
             r = hc.bfs(dg, int(s), want_parents=False)
         elif algorithm == "pr":
             r = hc.pagerank(dg)
+        elif algorithm == "many":
+            d, _, st = dg.bfs(sources, want_parents=False)
+            ms.append(st["ms"])
+            outs.extend(d)
+            break
         else:
             r = hc.sssp_multi(dg, sources)
             ms.append(r.stats["ms"])
@@ -134,24 +163,30 @@ def _same(a, b, algorithm):
 
 
 def tune_knobs(dg, algorithm: str, sources, space: dict | None = None, repeats: int = 2,
-               log=None) -> dict:
-    """Coordinate descent over the knobs of `algorithm` ("sssp", "bfs", "pr"
-    or "multi" for the batched sweep).  Every candidate's results must match
-    the default configuration's exactly (PageRank: within 1e-6 L1); the best
-    setting found is applied and returned with its timings."""
+               log=None, per_graph: bool = False) -> dict:
+    """Coordinate descent over the knobs of `algorithm` ("sssp", "bfs", "pr",
+    "multi" for the batched sweep, or "many" for the many-graph BFS, where
+    `dg` is a GraphBatch and `sources` one source per graph).  Every
+    candidate's results must match the default configuration's exactly
+    (PageRank: within 1e-6 L1); the best setting found is applied and
+    returned with its timings.  per_graph: knobs are set on `dg` only
+    (DeviceGraph.set_tuning), not process-wide."""
     space = space or KNOB_SPACE[algorithm]
-    start = hc.tuning()
+    if per_graph and algorithm == "many":
+        raise ValueError("the many-graph knobs are process-wide")
+    setk = dg.set_tuning if per_graph else hc.set_tuning
+    getk = dg.get_tuning if per_graph else hc.get_tuning
     _run(dg, algorithm, sources)  # warm plans and workspaces
     base_ms, base_out = min((_run(dg, algorithm, sources) for _ in range(repeats)),
                             key=lambda t: t[0])
-    best = {k: start[k] for k in space}
+    best = {k: getk(k) for k in space}
     best_ms = base_ms
     trials = []
     for key, values in space.items():
         for v in values:
             if v == best[key]:
                 continue
-            hc.set_tuning(key, v)
+            setk(key, v)
             ms, out = min((_run(dg, algorithm, sources) for _ in range(repeats)),
                           key=lambda t: t[0])
             ok = _same(out, base_out, algorithm)
@@ -162,14 +197,37 @@ def tune_knobs(dg, algorithm: str, sources, space: dict | None = None, repeats: 
                 raise RuntimeError(f"{key}={v} changed the results of {algorithm}")
             if ms < best_ms:
                 best_ms, best[key] = ms, v
-        hc.set_tuning(key, best[key])
+        setk(key, best[key])
+    # The winner timed again, with every chosen knob in place.
+    final_ms, final_out = min((_run(dg, algorithm, sources) for _ in range(repeats)),
+                              key=lambda t: t[0])
+    if not _same(final_out, base_out, algorithm):
+        raise RuntimeError(f"the chosen knobs changed the results of {algorithm}")
     return {"algorithm": algorithm, "default_ms": round(base_ms, 4),
-            "tuned_ms": round(best_ms, 4), "knobs": best, "trials": trials}
+            "tuned_ms": round(final_ms, 4), "knobs": best, "trials": trials}
 
 
-def apply_knobs(knobs: dict):
+def apply_knobs(knobs: dict, target=None):
+    """Sets knobs process-wide, on a DeviceGraph, or on a server's graph
+    (target = (server, graph_id))."""
     for k, v in knobs.items():
-        hc.set_tuning(k, int(v))
+        if target is None:
+            hc.set_tuning(k, int(v))
+        elif isinstance(target, tuple):
+            target[0].set_graph_tuning(target[1], k, int(v))
+        else:
+            target.set_tuning(k, int(v))
+
+
+def apply_cached(path: str, g, algorithm: str, target=None) -> dict:
+    """Applies the knobs cached for graph `g` (a HostGraph, or a list of
+    them for "many") and `algorithm` to `target` (see apply_knobs); returns
+    them ({} when the cache holds none for this device and graph)."""
+    entry = load(path, cache_key(g, algorithm))
+    knobs = dict(entry["knobs"]) if entry else {}
+    if knobs:
+        apply_knobs(knobs, target)
+    return knobs
 
 
 # ---------------------------------------------------------------------------
@@ -429,13 +487,24 @@ def load(path: str, key: str) -> dict | None:
         return json.load(f).get(key)
 
 
+def many_graphs(count: int, seed: int = 4242):
+    """The many-graph workload: `count` Kronecker graphs of scale U{10..16}
+    and edge factor U{8..32}, graph i from seed + 1 + i."""
+    rng = np.random.RandomState(seed)
+    return [hc.HostGraph.generate("kron", int(rng.randint(10, 17)), int(rng.randint(8, 33)),
+                                  weighted=False, seed=seed + 1 + i) for i in range(count)]
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--kind", default="kron")
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--degree", type=int, default=16)
-    ap.add_argument("--algorithm", default="sssp", choices=["sssp", "bfs", "pr", "multi"])
+    ap.add_argument("--algorithm", default="sssp",
+                    choices=["sssp", "bfs", "pr", "multi", "many"])
+    ap.add_argument("--graphs", type=int, default=512,
+                    help="many: Kronecker graphs of scale U{10..16}, edge factor U{8..32}")
     ap.add_argument("--sources", type=int, default=4)
     ap.add_argument("--segments", type=int, default=0,
                     help="also build a host/GPU schedule over this many segments")
@@ -446,12 +515,20 @@ def main(argv=None):
                     help="write the segment/tree profiles here as the reference's YAML")
     args = ap.parse_args(argv)
     hc.check(hc.lib().hc_init(0))
+    if args.algorithm == "many":
+        graphs = many_graphs(args.graphs)
+        sources = [int(x.pick_sources(1)[0]) for x in graphs]
+        with hc.GraphBatch(graphs) as b:
+            res = tune_knobs(b, "many", sources, log=print)
+        print(json.dumps({k: v for k, v in res.items() if k != "trials"}))
+        save(args.cache, cache_key(graphs, "many"), res)
+        return
     g = hc.HostGraph.generate(args.kind, args.scale, args.degree,
                               weighted=args.algorithm in ("sssp", "multi"))
     n_src = 64 if args.algorithm == "multi" else args.sources
     sources = g.pick_sources(n_src)
     with hc.DeviceGraph(g) as dg:
-        res = tune_knobs(dg, args.algorithm, sources, log=print)
+        res = tune_knobs(dg, args.algorithm, sources, log=print, per_graph=True)
     print(json.dumps({k: v for k, v in res.items() if k != "trials"}))
     save(args.cache, cache_key(g, args.algorithm), res)
     if args.segments and args.algorithm != "multi":

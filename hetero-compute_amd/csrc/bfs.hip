@@ -967,7 +967,7 @@ int hc_bfs(hc_graph_t g, int32_t source, int32_t *depth, int32_t *parents,
     int direction = opts ? opts->direction : 0;
     HC_CHECK_ARG(direction >= 0 && direction <= 2, "direction must be 0, 1 or 2");
     HC_TRY(ensure_init());
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     return g->off64 ? run_bfs_loop<uint64_t>(g, source, depth, parents, direction, alpha, beta, stats, s)
                     : run_bfs_loop<uint32_t>(g, source, depth, parents, direction, alpha, beta, stats, s);
@@ -992,7 +992,7 @@ int hc_bfs_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
         HC_CHECK_ARG(threads % kWave == 0, "thread count must be a multiple of 64");
     HC_TRY(ensure_init());
     if (start_id == end_id) return HC_OK;
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     if (!threads) threads = 256;
     if (!blocks) blocks = grid_blocks();
@@ -1032,7 +1032,7 @@ int hc_bfs_push_epoch(hc_graph_t g, const int32_t *frontier,
                  "push epochs need a whole-graph handle");
     HC_TRY(ensure_init());
     if (frontier_size == 0) return HC_OK;
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     int blocks = std::max(1, std::min(grid_blocks(), (frontier_size + 3) / 4));
     if (g->off64)

@@ -14,6 +14,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -101,6 +102,12 @@ struct PlanConfig {
 
 // Process-wide tuning table (set through hc_set_tuning by the autotuner or a
 // caller; read when plans and workspaces are built).
+// Many-graph BFS limits: graphs of up to kManyLdsNodesMax vertices can keep
+// their three bitmaps in LDS (48 KiB); at most kManyHeavyCapMax heavy frontier
+// vertices are listed per level.
+constexpr int kManyLdsNodesMax = 1 << 17;
+constexpr int kManyHeavyCapMax = 1024;
+
 struct Tuning {
     int grid_per_cu = 8;   // workgroups per CU of the fused / step launches
     int sssp_alpha = 0;    // SSSP goes pull when frontier edges > m / alpha (0: by
@@ -129,6 +136,12 @@ struct Tuning {
     int kernel_events = 0; // 1: time every launch of a loop's main kernel
     int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
     int multi_grid_per_cu = 16;  // workgroups per CU of the batched sweep
+    int multi_block = 256;       // threads per workgroup of the batched sweep (128-1024)
+    int many_block = 512;        // many-graph BFS: threads per workgroup (256, 512, 1024)
+    int many_per_cu = 2;         // ... workgroups per CU (0: as many as the device holds)
+    int many_light_deg = 32;     // ... top-down: a thread expands vertices up to this degree
+    int many_heavy_cap = 512;    // ... heavier ones, up to this many per level, by wavefronts
+    int many_lds_nodes = 1 << 17;  // ... graphs up to this many vertices keep bitmaps in LDS
     int pull_col_shift = -1;     // pull sweeps gather column blocks of 2^this vertices
                                  // (0: off; -1: automatic, flat graphs only; used
                                  // when the graph has >= 2 blocks)
@@ -155,7 +168,10 @@ struct Tuning {
         multi.g4_threshold = 0;
     }
 };
+// The knobs in force on the calling thread: those of the graph whose call
+// it is running (a GraphCall), else the process-wide set.
 Tuning &tuning();
+Tuning &global_tuning();
 // Plan of the SSSP pull sweeps: the shared pull hub classes with SSSP's own
 // workgroup, wavefront and lane-group thresholds.
 inline PlanConfig sssp_plan_config() {
@@ -457,6 +473,12 @@ struct ColBlocks {
     void *dmem = nullptr;
     size_t bytes = 0;
 };
+// Knobs set on one graph (and every share of it): key/value pairs applied
+// over the process-wide set for the duration of each call on the graph.
+struct GraphTuning {
+    std::mutex mu;
+    std::vector<std::pair<std::string, int64_t>> over;
+};
 // One copy for a graph and every share of it, built on first use.
 struct ColBlockSlot {
     std::mutex mu;
@@ -490,7 +512,37 @@ struct hc_graph_s {
     void *ws_pr = nullptr;
     void *ws_multi = nullptr;
     std::shared_ptr<hc::ColBlockSlot> cbslot = std::make_shared<hc::ColBlockSlot>();
+    std::shared_ptr<hc::GraphTuning> tune = std::make_shared<hc::GraphTuning>();
 };
+
+namespace hc {
+// Applies key = value to t (validated as hc_set_tuning does).
+int set_knob(Tuning &t, const char *key, int64_t value);
+// Makes tuning() return `t` on this thread while alive (t NULL: unchanged).
+class TuningScope {
+  public:
+    explicit TuningScope(Tuning *t);
+    ~TuningScope();
+    TuningScope(const TuningScope &) = delete;
+    TuningScope &operator=(const TuningScope &) = delete;
+
+  private:
+    Tuning *prev_;
+    bool set_;
+};
+// A call on graph g: holds g->mu and, when the graph has knobs of its own,
+// runs under the process-wide knobs with the graph's applied over them.
+class GraphCall {
+  public:
+    explicit GraphCall(hc_graph_s *g);
+
+  private:
+    std::lock_guard<std::mutex> lk_;
+    Tuning eff_;
+    TuningScope scope_;
+    static Tuning *effective(hc_graph_s *g, Tuning *eff);
+};
+}  // namespace hc
 
 namespace hc {
 // Plan for rows [start, end) under cfg (cached on the graph).

@@ -24,6 +24,9 @@ void clear_error();
 // plans), for a graph of n vertices and m directed edges.
 int64_t share_workspace_bound(int64_t n, int64_t m);
 
+// HC_OK when key = value would be accepted by hc_set_tuning (nothing set).
+int check_knob(const char *key, int64_t value);
+
 // Host -> device copies through the library's pinned staging ring (packed by
 // all host threads, sent asynchronously, overlapped; one upload at a time).
 // staged_copy_checked: count int32 values, each checked against [lo, hi);

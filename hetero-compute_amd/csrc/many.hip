@@ -42,6 +42,7 @@ struct hc_graph_batch_s {
     // Global bitmaps for graphs above the LDS limit, one slot per workgroup.
     uint32_t *scratch = nullptr;
     int32_t scratch_slots = 0;
+    size_t scratch_words = 0;
     int32_t *queue = nullptr;  // work counter + statistics
     int32_t *d_src = nullptr;  // per-call sources
     std::mutex mu;             // one call at a time per batch
@@ -58,13 +59,18 @@ struct Desc {
 
 namespace hc {
 
-// Graphs with at most this many vertices keep their bitmaps in LDS
-// (3 x 4096 words = 48 KiB per workgroup).
-constexpr int kLdsNodes = 1 << 17;
-constexpr int kLdsWords = kLdsNodes / 32;
-constexpr int kManyBlock = 512;
-constexpr int kHeavyCap = 512;   // per-level list of high-degree frontier vertices
-constexpr int kLightDeg = 32;    // top-down: a thread handles up to this degree
+// Graphs with at most many.lds_nodes vertices (at most kManyLdsNodesMax:
+// 3 x 4096 words = 48 KiB per workgroup) keep their bitmaps in LDS; the LDS
+// of a launch is sized to the largest such graph of the batch.
+
+// Launch shape and per-level thresholds of one many-graph launch (knobs
+// many.block, many.per_cu, many.light_deg, many.heavy_cap, many.lds_nodes).
+struct ManyShape {
+    int block;      // threads per workgroup: 256, 512 or 1024
+    int light_deg;  // top-down: a thread expands frontier vertices up to this degree
+    int heavy_cap;  // heavier ones (at most this many per level) go to whole waves
+    int lds_words;  // bitmap words per LDS bitmap (graphs of <= 32x this many vertices)
+};
 
 struct ManyStats {
     unsigned long long pushes, pulls, scanned;
@@ -93,6 +99,7 @@ __device__ __forceinline__ bool bm_claim(uint32_t *b, int32_t v) {
 }
 
 // Block-wide sum of a per-thread 64-bit value (result valid in all threads).
+template <int BLOCK>
 __device__ __forceinline__ long long many_block_sum(long long v, long long *s_red) {
 #pragma unroll
     for (int s = kWave / 2; s > 0; s >>= 1) v += __shfl_xor(v, s, kWave);
@@ -100,27 +107,29 @@ __device__ __forceinline__ long long many_block_sum(long long v, long long *s_re
     if (lane_id() == 0) s_red[threadIdx.x / kWave] = v;
     __syncthreads();
     long long t = 0;
-    for (int i = 0; i < kManyBlock / kWave; i++) t += s_red[i];
+#pragma unroll
+    for (int i = 0; i < BLOCK / kWave; i++) t += s_red[i];
     return t;
 }
 
 // One direction-optimizing BFS of graph d from src by the whole workgroup;
 // bm holds three bitmaps of `words` words (LDS, or a global slot when
-// Global).
-template <bool Global>
+// Global).  Frontier vertices of degree above light_deg are expanded by
+// whole wavefronts, up to heavy_cap of them per level.
+template <int BLOCK, bool Global>
 __device__ __forceinline__ void bfs_one_graph(const Desc &d, int32_t src,
                               const uint32_t *__restrict__ off,
                               const int32_t *__restrict__ adj, int32_t *__restrict__ depth,
                               int32_t *__restrict__ parents, uint32_t *bm, int words,
-                              int alpha, int beta, int32_t *heavy, int *s_nheavy,
-                              long long *s_red, ManyStats &st) {
+                              int alpha, int beta, int light_deg, int heavy_cap,
+                              int32_t *heavy, int *s_nheavy, long long *s_red, ManyStats &st) {
     const int tid = threadIdx.x;
     const int n = d.n;
     uint32_t *vis = bm;
     uint32_t *front = bm + words;
     uint32_t *next = bm + 2 * words;
-    for (int i = tid; i < 3 * words; i += kManyBlock) bm[i] = 0;
-    for (int u = tid; u < n; u += kManyBlock) {
+    for (int i = tid; i < 3 * words; i += BLOCK) bm[i] = 0;
+    for (int u = tid; u < n; u += BLOCK) {
         depth[u] = -1;
         if (parents) parents[u] = -1;
     }
@@ -152,15 +161,15 @@ __device__ __forceinline__ void bfs_one_graph(const Desc &d, int32_t src,
             // expanded by their thread, heavy ones queued for whole waves.
             if (tid == 0) *s_nheavy = 0;
             __syncthreads();
-            for (int w = tid; w < words; w += kManyBlock) {
+            for (int w = tid; w < words; w += BLOCK) {
                 uint32_t bits = bm_word<Global>(front, w);
                 while (bits) {
                     const int32_t u = w * 32 + __ffs(bits) - 1;
                     bits &= bits - 1;
                     const uint32_t b = off[u], e = off[u + 1];
-                    if (e - b > (uint32_t)kLightDeg) {
+                    if (e - b > (uint32_t)light_deg) {
                         const int slot = atomicAdd(s_nheavy, 1);
-                        if (slot < kHeavyCap) {
+                        if (slot < heavy_cap) {
                             heavy[slot] = u;
                             continue;
                         }
@@ -179,8 +188,8 @@ __device__ __forceinline__ void bfs_one_graph(const Desc &d, int32_t src,
                 }
             }
             __syncthreads();
-            const int nh = min(*s_nheavy, kHeavyCap);
-            const int wave = tid / kWave, nwaves = kManyBlock / kWave, lane = lane_id();
+            const int nh = min(*s_nheavy, heavy_cap);
+            const int wave = tid / kWave, nwaves = BLOCK / kWave, lane = lane_id();
             for (int h = wave; h < nh; h += nwaves) {
                 const int32_t u = heavy[h];
                 const uint32_t b = off[u], e = off[u + 1];
@@ -200,7 +209,7 @@ __device__ __forceinline__ void bfs_one_graph(const Desc &d, int32_t src,
         } else {
             // Bottom-up: every unvisited vertex takes its first neighbour
             // (CSR order) in the frontier.
-            for (int u = tid; u < n; u += kManyBlock) {
+            for (int u = tid; u < n; u += BLOCK) {
                 if (bm_get<Global>(vis, u)) continue;
                 const uint32_t b = off[u], e = off[u + 1];
                 for (uint32_t i = b; i < e; i++) {
@@ -218,14 +227,14 @@ __device__ __forceinline__ void bfs_one_graph(const Desc &d, int32_t src,
             }
             st.pulls += (tid == 0);
         }
-        const long long q = many_block_sum(cnt, s_red);
-        const long long scq = many_block_sum(sc, s_red);
+        const long long q = many_block_sum<BLOCK>(cnt, s_red);
+        const long long scq = many_block_sum<BLOCK>(sc, s_red);
         st.scanned += (unsigned long long)scanned;
         // Next level's frontier; clear the old one for reuse.
         uint32_t *t = front;
         front = next;
         next = t;
-        for (int i = tid; i < words; i += kManyBlock) next[i] = 0;
+        for (int i = tid; i < words; i += BLOCK) next[i] = 0;
         fsize = q;
         // bfs_do_cpu's switching rule, evaluated identically by all threads.
         if (!pull) {
@@ -257,17 +266,20 @@ __device__ __forceinline__ void bfs_one_graph(const Desc &d, int32_t src,
     }
 }
 
-__global__ __launch_bounds__(kManyBlock) void bfs_many_kernel(
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void bfs_many_kernel(
     const Desc *__restrict__ desc, const int32_t *__restrict__ order,
     const int32_t *__restrict__ sources, int num_graphs,
     const uint32_t *__restrict__ offsets, const int32_t *__restrict__ adjacency,
     int32_t *__restrict__ depth, int32_t *__restrict__ parents, uint32_t *slots,
-    int slot_words, int *queue, unsigned long long *stats, int alpha, int beta) {
+    int slot_words, int *queue, unsigned long long *stats, int alpha, int beta,
+    ManyShape shape) {
     extern __shared__ uint32_t lds_bm[];
-    __shared__ int32_t heavy[kHeavyCap];
+    __shared__ int32_t heavy[kManyHeavyCapMax];
     __shared__ int s_nheavy, s_g;
-    __shared__ long long s_red[kManyBlock / kWave];
+    __shared__ long long s_red[BLOCK / kWave];
     ManyStats st{0, 0, 0};
+    const int lds_nodes = shape.lds_words * 32;
     for (;;) {
         if (threadIdx.x == 0) s_g = atomicAdd(queue, 1);
         __syncthreads();
@@ -278,18 +290,20 @@ __global__ __launch_bounds__(kManyBlock) void bfs_many_kernel(
         const Desc d = desc[gi];
         const int words = (d.n + 31) / 32;
         int32_t *par = parents ? parents + d.node_base : nullptr;
-        if (d.n <= kLdsNodes)
-            bfs_one_graph<false>(d, sources[gi], offsets + d.off_base, adjacency + d.edge_base,
-                                 depth + d.node_base, par, lds_bm, words, alpha, beta, heavy,
-                                 &s_nheavy, s_red, st);
+        if (d.n <= lds_nodes)
+            bfs_one_graph<BLOCK, false>(d, sources[gi], offsets + d.off_base,
+                                        adjacency + d.edge_base, depth + d.node_base, par, lds_bm,
+                                        words, alpha, beta, shape.light_deg, shape.heavy_cap,
+                                        heavy, &s_nheavy, s_red, st);
         else
-            bfs_one_graph<true>(d, sources[gi], offsets + d.off_base, adjacency + d.edge_base,
-                                depth + d.node_base, par,
-                                slots + (size_t)blockIdx.x * 3 * slot_words, words, alpha,
-                                beta, heavy, &s_nheavy, s_red, st);
+            bfs_one_graph<BLOCK, true>(d, sources[gi], offsets + d.off_base,
+                                       adjacency + d.edge_base, depth + d.node_base, par,
+                                       slots + (size_t)blockIdx.x * 3 * slot_words, words, alpha,
+                                       beta, shape.light_deg, shape.heavy_cap, heavy, &s_nheavy,
+                                       s_red, st);
     }
     // Levels are counted once per graph (thread 0); edges by every thread.
-    const long long sc = many_block_sum((long long)st.scanned, s_red);
+    const long long sc = many_block_sum<BLOCK>((long long)st.scanned, s_red);
     if (threadIdx.x == 0) {
         atomicAdd(&stats[0], st.pushes);
         atomicAdd(&stats[1], st.pulls);
@@ -315,7 +329,36 @@ struct ManyLaunch {
     int slot_words;
     int grid;
     int32_t *queue;        // 16 words: counter, statistics
+    ManyShape shape;
 };
+
+// Launch shape of a batch under the current knobs: LDS bitmaps sized to the
+// largest graph that fits the many.lds_nodes limit, global bitmap slots of
+// slot_words words for the graphs above it (0 when none is), and the
+// workgroups (at most one per graph) that the device holds at once with
+// that LDS, or many.per_cu per CU when set.
+ManyShape many_shape(const hc_graph_batch_s *b, int *grid, int *slot_words) {
+    const Tuning &t = tuning();
+    ManyShape sh{t.many_block, t.many_light_deg, t.many_heavy_cap, 0};
+    const int limit = std::min(t.many_lds_nodes, kManyLdsNodesMax);
+    int fit = 0;
+    for (int i = 0; i < b->num_graphs; i++) {
+        const int n = (int)(b->node_base[i + 1] - b->node_base[i]);
+        if (n <= limit) fit = std::max(fit, n);
+    }
+    sh.lds_words = (fit + 31) / 32;
+    *slot_words = b->max_nodes > sh.lds_words * 32 ? (b->max_nodes + 31) / 32 : 0;
+    const size_t lds = sizeof(uint32_t) * 3 * (size_t)sh.lds_words;
+    int per_cu = t.many_per_cu;
+    if (per_cu <= 0) {
+        const void *k = sh.block == 256   ? reinterpret_cast<const void *>(&bfs_many_kernel<256>)
+                        : sh.block == 512 ? reinterpret_cast<const void *>(&bfs_many_kernel<512>)
+                                          : reinterpret_cast<const void *>(&bfs_many_kernel<1024>);
+        per_cu = resident_grid(k, sh.block, lds, INT32_MAX) / std::max(1, device_state().cus);
+    }
+    *grid = std::max(1, std::min(b->num_graphs, device_state().cus * std::max(1, per_cu)));
+    return sh;
+}
 
 // Enqueues one persistent launch; with stats, waits for it and fills them
 // (ms: device time between events when `ev` are given, else the host-observed
@@ -334,9 +377,20 @@ int launch_many(hc_graph_batch_s *b, const ManyLaunch &L, const int32_t *sources
         HC_HIP(hipEventRecord(e0, s));
     }
     const auto t0 = std::chrono::steady_clock::now();
-    bfs_many_kernel<<<L.grid, kManyBlock, sizeof(uint32_t) * 3 * kLdsWords, s>>>(
-        b->desc, b->d_order, L.d_src, b->num_graphs, b->offsets, b->adj, depth, parents, L.slots,
-        L.slot_words, L.queue, reinterpret_cast<unsigned long long *>(L.queue + 2), alpha, beta);
+    const size_t lds = sizeof(uint32_t) * 3 * (size_t)L.shape.lds_words;
+    unsigned long long *st_words = reinterpret_cast<unsigned long long *>(L.queue + 2);
+#define HC_MANY_LAUNCH(B)                                                                      \
+    bfs_many_kernel<B><<<L.grid, B, lds, s>>>(b->desc, b->d_order, L.d_src, b->num_graphs,     \
+                                              b->offsets, b->adj, depth, parents, L.slots,     \
+                                              L.slot_words, L.queue, st_words, alpha, beta,    \
+                                              L.shape)
+    if (L.shape.block == 256)
+        HC_MANY_LAUNCH(256);
+    else if (L.shape.block == 512)
+        HC_MANY_LAUNCH(512);
+    else
+        HC_MANY_LAUNCH(1024);
+#undef HC_MANY_LAUNCH
     HC_HIP(hipGetLastError());
     if (!stats && !events) return HC_OK;
     float ms = 0;
@@ -554,20 +608,21 @@ int hc_bfs_many(hc_graph_batch_t b, const int32_t *sources, int32_t *depth,
     HC_TRY(ensure_init());
     std::lock_guard<std::mutex> lk(b->mu);
     hipStream_t s = pick_stream(stream);
-    const int per_cu = 2;  // 48 KiB LDS, 512 threads and ~84 VGPRs per workgroup
-    const int grid = std::max(1, std::min(b->num_graphs, device_state().cus * per_cu));
-    int slot_words = 0;
-    if (b->max_nodes > kLdsNodes) {
-        slot_words = (b->max_nodes + 31) / 32;
-        if (b->scratch_slots < grid) {
+    int grid = 1, slot_words = 0;
+    const ManyShape shape = many_shape(b, &grid, &slot_words);
+    if (slot_words > 0) {
+        const size_t need = 3 * (size_t)slot_words * grid;
+        if (b->scratch_slots < grid || b->scratch_words < need) {
             if (b->scratch) hipFree(b->scratch);
             b->scratch = nullptr;
             b->scratch_slots = 0;
-            HC_HIP(hipMalloc(&b->scratch, sizeof(uint32_t) * 3 * (size_t)slot_words * grid));
+            b->scratch_words = 0;
+            HC_HIP(hipMalloc(&b->scratch, sizeof(uint32_t) * need));
             b->scratch_slots = grid;
+            b->scratch_words = need;
         }
     }
-    const ManyLaunch L{b->d_src, b->scratch, slot_words, grid, b->queue};
+    const ManyLaunch L{b->d_src, b->scratch, slot_words, grid, b->queue, shape};
     return launch_many(b, L, sources, depth, parents, alpha, beta, stats, s, true);
 }
 
@@ -585,12 +640,11 @@ int hc_bfs_many_noalloc(hc_graph_batch_t b, const int32_t *sources, int32_t *dep
     HC_TRY(noalloc_scratch(&base));
     std::lock_guard<std::mutex> lk(b->mu);
     hipStream_t s = pick_stream(stream);
-    int grid = std::max(1, std::min(b->num_graphs, device_state().cus * 2));
-    int slot_words = 0;
-    if (b->max_nodes > kLdsNodes) {
+    int grid = 1, slot_words = 0;
+    const ManyShape shape = many_shape(b, &grid, &slot_words);
+    if (slot_words > 0) {
         // Fewer workgroups when the slots of a full grid do not fit: the
         // persistent kernel takes graphs from its queue until none is left.
-        slot_words = (b->max_nodes + 31) / 32;
         const size_t per_slot = sizeof(uint32_t) * 3 * (size_t)slot_words;
         const size_t fit = kNoAllocSlotBytes / per_slot;
         HC_CHECK_ARG(fit >= 1,
@@ -601,7 +655,7 @@ int hc_bfs_many_noalloc(hc_graph_batch_t b, const int32_t *sources, int32_t *dep
     }
     const ManyLaunch L{reinterpret_cast<int32_t *>(base + kNoAllocHeadBytes),
                        reinterpret_cast<uint32_t *>(base + kNoAllocHeadBytes + kNoAllocSrcBytes),
-                       slot_words, grid, reinterpret_cast<int32_t *>(base)};
+                       slot_words, grid, reinterpret_cast<int32_t *>(base), shape};
     return launch_many(b, L, sources, depth, parents, alpha, beta, stats, s, false);
 }
 

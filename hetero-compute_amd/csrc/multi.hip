@@ -178,8 +178,10 @@ __device__ __forceinline__ int commit(int32_t *dist, uint32_t *nxt, int32_t my_u
     return changed;
 }
 
-template <typename OffT>
-__global__ __launch_bounds__(kBlock) void multi_step_kernel(MultiArgs<OffT> a) {
+// BLOCK threads per workgroup (knob multi.block): the sweep works in
+// wavefront units, so the workgroup size only sets how many share a CU slot.
+template <typename OffT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void multi_step_kernel(MultiArgs<OffT> a) {
     MultiCtl *ctl = a.ctl;
     if (ctl->done) return;
     const int ep = ctl->epoch;
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void multi_step_kernel(MultiArgs<OffT> a) {
     uint32_t *nxt = a.chg[(ep + 1) % 3];
     uint32_t *clr = a.chg[(ep + 2) % 3];
     const int words = (a.n + 3) >> 2;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < words; i += gridDim.x * kBlock)
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < words; i += gridDim.x * BLOCK)
         clr[i] = 0;
 
     const PlanDev &p = a.p;
@@ -200,13 +202,13 @@ __global__ __launch_bounds__(kBlock) void multi_step_kernel(MultiArgs<OffT> a) {
     const int n3 = (p.cnt[3] + 3) / 4;
     const int n45 = (p.cnt[4] + p.cnt[5] + 15) / 16;
     const int total = n0 + n12 + n3 + n45;
-    const int wstride = gridDim.x * (kBlock / kWave);
+    const int wstride = gridDim.x * (BLOCK / kWave);
     unsigned long long bytes = 0;
     int changed = 0;
     auto vid = [&](int c, int i) -> int32_t {
         return p.list[c] ? p.list[c][i] : p.beg[c] + i;
     };
-    for (int unit = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; unit < total;
+    for (int unit = blockIdx.x * (BLOCK / kWave) + threadIdx.x / kWave; unit < total;
          unit += wstride) {
         if (unit < n0) {
             const HubChunk hc = p.chunks[unit];
@@ -322,6 +324,7 @@ struct MultiWs {
     MultiCtl *h_ctl = nullptr;  // pinned mirror for polling
     int words = 0;
     int grid = 0;
+    int block = 0;  // threads per workgroup of the step kernel
     const PlanHost *plan = nullptr;
     // Epoch graphs, [0] weighted, [1] unit weights.
     hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -346,7 +349,8 @@ void free_multi_ws(hc_graph_s *g) {
 static int get_multi_ws(hc_graph_s *g, MultiWs **out) {
     auto *ws = static_cast<MultiWs *>(g->ws_multi);
     const int grid = grid_blocks(tuning().multi_grid_per_cu);
-    if (ws && ws->grid != grid) {
+    const int block = tuning().multi_block;
+    if (ws && (ws->grid != grid || ws->block != block)) {  // epoch graphs bake both in
         free_multi_ws(g);
         ws = nullptr;
     }
@@ -355,6 +359,7 @@ static int get_multi_ws(hc_graph_s *g, MultiWs **out) {
         if (!ws) return fail(HC_ERR_OOM, "host allocation failed");
         ws->words = (g->n + 3) / 4;
         ws->grid = grid;
+        ws->block = block;
         hipError_t e1 = hipMalloc(&ws->flags, sizeof(uint32_t) * 3 * (size_t)std::max(ws->words, 1));
         hipError_t e2 = hipMalloc(&ws->ctl, sizeof(MultiCtl));
         hipError_t e3 = hipHostMalloc(&ws->h_ctl, sizeof(MultiCtl), hipHostMallocDefault);
@@ -369,11 +374,25 @@ static int get_multi_ws(hc_graph_s *g, MultiWs **out) {
 }
 
 template <typename OffT>
-static int launch_multi_epochs(const MultiArgs<OffT> &a, int grid, int epochs,
+static const void *multi_step_fn(int block) {
+    switch (block) {
+        case 128: return reinterpret_cast<const void *>(&multi_step_kernel<OffT, 128>);
+        case 512: return reinterpret_cast<const void *>(&multi_step_kernel<OffT, 512>);
+        case 1024: return reinterpret_cast<const void *>(&multi_step_kernel<OffT, 1024>);
+        default: return reinterpret_cast<const void *>(&multi_step_kernel<OffT, 256>);
+    }
+}
+
+template <typename OffT>
+static int launch_multi_epochs(const MultiArgs<OffT> &a, int grid, int block, int epochs,
                                unsigned long long sweep, hipStream_t s, KernelTimer *kt) {
+    const void *step = multi_step_fn<OffT>(block);
+    MultiArgs<OffT> args = a;
+    void *kargs[] = {(void *)&args};
     for (int i = 0; i < epochs; i++) {
         if (kt) kt->mark(s);
-        multi_step_kernel<OffT><<<grid, kBlock, 0, s>>>(a);
+        if (hipLaunchKernel(step, dim3(grid), dim3(block), kargs, 0, s) != hipSuccess)
+            return fail(HC_ERR_HIP, "batched SSSP launch failed");
         if (kt) kt->mark(s);
         multi_ctl_kernel<<<1, kWave, 0, s>>>(a.ctl, sweep);
     }
@@ -428,9 +447,9 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
         for (int i = 0; i < kEpochsPerGraph; i++) {
             hipKernelNodeParams kp{};
             void *step_args[] = {(void *)&a};
-            kp.func = reinterpret_cast<void *>(&multi_step_kernel<OffT>);
+            kp.func = const_cast<void *>(multi_step_fn<OffT>(ws->block));
             kp.gridDim = dim3(grid);
-            kp.blockDim = dim3(kBlock);
+            kp.blockDim = dim3(ws->block);
             kp.sharedMemBytes = 0;
             kp.kernelParams = step_args;
             kp.extra = nullptr;
@@ -469,7 +488,7 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
                 break;
             }
         } else {
-            rc = launch_multi_epochs(a, grid, kEpochsPerGraph, sweep, s, &ws->kt);
+            rc = launch_multi_epochs(a, grid, ws->block, kEpochsPerGraph, sweep, s, &ws->kt);
             if (rc != HC_OK) break;
         }
         launched += kEpochsPerGraph;
@@ -527,7 +546,7 @@ static int multi_entry(hc_graph_t g, const int32_t *sources, int32_t num_sources
         HC_CHECK_ARG(sources[q] >= 0 && sources[q] < g->n, "source %d = %d outside [0,%d)", q,
                      sources[q], g->n);
     HC_TRY(ensure_init());
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     return g->off64
                ? run_multi<uint64_t>(g, sources, num_sources, dist, max_epochs, unit, stats, s)

@@ -701,7 +701,7 @@ int hc_pr_pull(hc_graph_t g, int mapping, const float *init_score, float *scores
     HC_CHECK_ARG(rule == 0 || rule == 1, "rule must be 0 (L1 residual) or 1 (per vertex)");
     HC_CHECK_ARG(max_iters <= 100000, "max_iters too large");
     HC_TRY(ensure_init());
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     return g->off64 ? run_pr_pull<uint64_t>(g, mapping, init_score, scores, damp, tol,
                                             max_iters, rule, blocks, threads, stats, s)
@@ -725,7 +725,7 @@ int hc_pagerank(hc_graph_t g, float *scores, const hc_pr_opts_t *opts,
     HC_CHECK_ARG(rule == 0 || rule == 1, "rule must be 0 (L1 residual) or 1 (per vertex)");
     HC_CHECK_ARG(max_iters <= 100000, "max_iters too large");
     HC_TRY(ensure_init());
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     return g->off64 ? run_pr<uint64_t>(g, scores, damp, tol, max_iters, rule, stats, s)
                     : run_pr<uint32_t>(g, scores, damp, tol, max_iters, rule, stats, s);
@@ -745,7 +745,7 @@ int hc_pr_pull_epoch(hc_graph_t g, int32_t start_id, int32_t end_id,
     HC_CHECK_ARG(damping > 0 && damping < 1, "damping must be in (0,1)");
     HC_TRY(ensure_init());
     if (start_id == end_id) return HC_OK;
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     const PlanHost *ph = nullptr;
     HC_TRY(get_plan(g, start_id, end_id, tuning().pull, &ph));

@@ -23,10 +23,35 @@ int ensure_init() {
     return hc_init(0);
 }
 
-Tuning &tuning() {
+Tuning &global_tuning() {
     static Tuning t;
     return t;
 }
+
+namespace {
+thread_local Tuning *tls_tuning = nullptr;
+}  // namespace
+
+Tuning &tuning() { return tls_tuning ? *tls_tuning : global_tuning(); }
+
+TuningScope::TuningScope(Tuning *t) : prev_(tls_tuning), set_(t != nullptr) {
+    if (set_) tls_tuning = t;
+}
+
+TuningScope::~TuningScope() {
+    if (set_) tls_tuning = prev_;
+}
+
+Tuning *GraphCall::effective(hc_graph_s *g, Tuning *eff) {
+    GraphTuning &gt = *g->tune;
+    std::lock_guard<std::mutex> lk(gt.mu);
+    if (gt.over.empty()) return nullptr;
+    *eff = global_tuning();
+    for (const auto &kv : gt.over) (void)set_knob(*eff, kv.first.c_str(), kv.second);
+    return eff;
+}
+
+GraphCall::GraphCall(hc_graph_s *g) : lk_(g->mu), eff_(), scope_(effective(g, &eff_)) {}
 
 int grid_blocks(int per_cu) {
     if (per_cu <= 0) per_cu = tuning().grid_per_cu;
@@ -54,8 +79,7 @@ int resident_grid(const void *kernel, int block, size_t lds_bytes, int cap) {
 
 namespace {
 int64_t g_knob_tmp;
-int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
-    Tuning &t = tuning();
+int64_t *knob_ref(Tuning &t, const char *key, bool *is_int, int **iref) {
     *is_int = false;
     *iref = nullptr;
     struct {
@@ -109,6 +133,12 @@ int64_t *knob_ref(const char *key, bool *is_int, int **iref) {
                {"profile.kernel_events", &t.kernel_events},
                {"multi.graph", &t.multi_graph},
                {"multi.grid_per_cu", &t.multi_grid_per_cu},
+               {"multi.block", &t.multi_block},
+               {"many.block", &t.many_block},
+               {"many.per_cu", &t.many_per_cu},
+               {"many.light_deg", &t.many_light_deg},
+               {"many.heavy_cap", &t.many_heavy_cap},
+               {"many.lds_nodes", &t.many_lds_nodes},
                {"pull.col_shift", &t.pull_col_shift}};
     for (auto &x : i32)
         if (!strcmp(key, x.k)) {
@@ -560,15 +590,19 @@ int hc_graph_share(hc_graph_t g, hc_graph_t *out) {
     h->h_off = g->h_off;
     h->shared = true;
     h->cbslot = g->cbslot;  // one column-blocked copy for the graph and its shares
+    h->tune = g->tune;      // and one set of per-graph knobs
     *out = h;
     return HC_OK;
 }
 
-int hc_set_tuning(const char *key, int64_t value) {
+}  // extern "C"
+
+namespace hc {
+int set_knob(Tuning &t, const char *key, int64_t value) {
     HC_CHECK_ARG(key, "key is NULL");
     bool is_int = false;
     int *iref = nullptr;
-    int64_t *p = knob_ref(key, &is_int, &iref);
+    int64_t *p = knob_ref(t, key, &is_int, &iref);
     if (!p) return fail(HC_ERR_NOT_FOUND, "unknown tuning key '%s'", key);
     if (is_int) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
@@ -589,6 +623,28 @@ int hc_set_tuning(const char *key, int64_t value) {
         }
         if (!strcmp(key, "loop.batch"))
             HC_CHECK_ARG(value >= 1 && value <= 16, "loop.batch must be in [1, 16]");
+        if (!strcmp(key, "multi.block")) {
+            HC_CHECK_ARG(value == 128 || value == 256 || value == 512 || value == 1024,
+                         "multi.block must be 128, 256, 512 or 1024");
+            *iref = (int)value;
+            return HC_OK;
+        }
+        if (!strncmp(key, "many.", 5)) {
+            if (!strcmp(key, "many.block"))
+                HC_CHECK_ARG(value == 256 || value == 512 || value == 1024,
+                             "many.block must be 256, 512 or 1024");
+            if (!strcmp(key, "many.per_cu"))
+                HC_CHECK_ARG(value >= 0 && value <= 32, "many.per_cu must be in [0, 32]");
+            if (!strcmp(key, "many.heavy_cap"))
+                HC_CHECK_ARG(value >= 0 && value <= kManyHeavyCapMax,
+                             "many.heavy_cap must be in [0, %d]", kManyHeavyCapMax);
+            if (!strcmp(key, "many.lds_nodes"))
+                HC_CHECK_ARG(value >= 0 && value <= kManyLdsNodesMax,
+                             "many.lds_nodes must be in [0, %d]", kManyLdsNodesMax);
+            HC_CHECK_ARG(value >= 0 && value <= 1 << 20, "%s out of range", key);
+            *iref = (int)value;
+            return HC_OK;
+        }
         if (!strcmp(key, "push.chunk"))
             HC_CHECK_ARG(value >= kMinChunkEdges, "push.chunk must be >= %d", (int)kMinChunkEdges);
         if (!strcmp(key, "bu.remaining"))
@@ -614,18 +670,66 @@ int hc_set_tuning(const char *key, int64_t value) {
     return HC_OK;
 }
 
+int check_knob(const char *key, int64_t value) {
+    Tuning probe = global_tuning();
+    return set_knob(probe, key, value);
+}
+}  // namespace hc
+
+extern "C" {
+
+int hc_set_tuning(const char *key, int64_t value) {
+    return set_knob(global_tuning(), key, value);
+}
+
 int hc_get_tuning(const char *key, int64_t *value) {
     HC_CHECK_ARG(key && value, "NULL argument");
     bool is_int = false;
     int *iref = nullptr;
-    int64_t *p = knob_ref(key, &is_int, &iref);
+    int64_t *p = knob_ref(global_tuning(), key, &is_int, &iref);
     if (!p) return fail(HC_ERR_NOT_FOUND, "unknown tuning key '%s'", key);
     *value = is_int ? *iref : *p;
     return HC_OK;
 }
 
 int hc_reset_tuning(void) {
-    tuning() = Tuning();
+    global_tuning() = Tuning();
+    return HC_OK;
+}
+
+int hc_graph_set_tuning(hc_graph_t g, const char *key, int64_t value) {
+    HC_CHECK_ARG(g && key, "NULL argument");
+    HC_TRY(check_knob(key, value));  // a rejected value is not recorded
+    GraphTuning &gt = *g->tune;
+    std::lock_guard<std::mutex> lk(gt.mu);
+    for (auto &kv : gt.over)
+        if (kv.first == key) {
+            kv.second = value;
+            return HC_OK;
+        }
+    gt.over.emplace_back(key, value);
+    return HC_OK;
+}
+
+int hc_graph_get_tuning(hc_graph_t g, const char *key, int64_t *value) {
+    HC_CHECK_ARG(g && key && value, "NULL argument");
+    {
+        GraphTuning &gt = *g->tune;
+        std::lock_guard<std::mutex> lk(gt.mu);
+        for (const auto &kv : gt.over)
+            if (kv.first == key) {
+                *value = kv.second;
+                return HC_OK;
+            }
+    }
+    return hc_get_tuning(key, value);
+}
+
+int hc_graph_reset_tuning(hc_graph_t g) {
+    HC_CHECK_ARG(g, "NULL graph");
+    GraphTuning &gt = *g->tune;
+    std::lock_guard<std::mutex> lk(gt.mu);
+    gt.over.clear();
     return HC_OK;
 }
 

@@ -66,6 +66,7 @@ struct GraphEntry {
     bool resident = false;
     hc_graph_t dev = nullptr;             // owner of the device arrays
     std::vector<hc_graph_t> per_slot;     // shares, one per slot
+    std::vector<std::pair<std::string, int64_t>> knobs;  // applied at every upload
     int64_t bytes = 0;                    // budgeted bytes while uploaded
     bool uploading = false;
     int in_use = 0;
@@ -375,6 +376,8 @@ void worker(hc_server_s *s, int slot) {
             g.uploading = false;
             if (status == HC_OK) {
                 g.dev = d;
+                for (const auto &kv : g.knobs)  // checked when recorded
+                    (void)hc_graph_set_tuning(d, kv.first.c_str(), kv.second);
                 s->st.uploads++;
             } else {
                 s->in_use -= g.bytes;
@@ -543,6 +546,26 @@ int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t residen
     }
     s->graphs.push_back(std::move(e));
     *graph_id = (int32_t)s->graphs.size() - 1;
+    return HC_OK;
+}
+
+int hc_server_set_graph_tuning(hc_server_t s, int32_t graph_id, const char *key,
+                               int64_t value) {
+    HC_CHECK_ARG(s && key, "NULL argument");
+    HC_TRY(hc::check_knob(key, value));
+    std::lock_guard<std::mutex> lk(s->mu);
+    HC_CHECK_ARG(graph_id >= 0 && graph_id < (int32_t)s->graphs.size(), "unknown graph %d",
+                 graph_id);
+    GraphEntry &g = s->graphs[graph_id];
+    bool found = false;
+    for (auto &kv : g.knobs)
+        if (kv.first == key) {
+            kv.second = value;
+            found = true;
+        }
+    if (!found) g.knobs.emplace_back(key, value);
+    // The uploaded handle (and so every slot's share of it) takes it now.
+    if (g.dev) HC_TRY(hc_graph_set_tuning(g.dev, key, value));
     return HC_OK;
 }
 

@@ -972,7 +972,7 @@ int hc_sssp_pull_epoch(hc_graph_t g, int mapping, int32_t start_id,
     HC_TRY(check_launch(mapping, blocks, threads));
     HC_TRY(ensure_init());
     if (start_id == end_id) return HC_OK;
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     return g->off64 ? launch_pull_epoch<uint64_t>(g, mapping, start_id, end_id, dist, updated, blocks, threads, s)
                     : launch_pull_epoch<uint32_t>(g, mapping, start_id, end_id, dist, updated, blocks, threads, s);
@@ -992,7 +992,7 @@ int hc_sssp(hc_graph_t g, int32_t source, int32_t *dist,
     int max_epochs = opts ? opts->max_epochs : 0;
     HC_CHECK_ARG(direction >= 0 && direction <= 2, "direction must be 0, 1 or 2");
     HC_TRY(ensure_init());
-    std::lock_guard<std::mutex> lk(g->mu);
+    GraphCall call(g);
     hipStream_t s = pick_stream(stream);
     return g->off64 ? run_sssp_loop<uint64_t>(g, source, nullptr, dist, direction, alpha, max_epochs, stats, s)
                     : run_sssp_loop<uint32_t>(g, source, nullptr, dist, direction, alpha, max_epochs, stats, s);
@@ -1009,7 +1009,7 @@ int hc_sssp_pull(hc_graph_t g, int mapping, const int32_t *init_dist,
     HC_TRY(ensure_init());
     hipStream_t s = pick_stream(stream);
     if (mapping == HC_MAP_BUCKETED) {
-        std::lock_guard<std::mutex> lk(g->mu);
+        GraphCall call(g);
         return g->off64 ? run_sssp_loop<uint64_t>(g, 0, init_dist, dist, 1, 8, 0, stats, s)
                         : run_sssp_loop<uint32_t>(g, 0, init_dist, dist, 1, 8, 0, stats, s);
     }
@@ -1030,7 +1030,7 @@ int hc_sssp_pull(hc_graph_t g, int mapping, const int32_t *init_dist,
     while (rc == HC_OK && h_upd != 0) {
         hipMemsetAsync(d_upd, 0, sizeof(int32_t), s);
         {
-            std::lock_guard<std::mutex> lk(g->mu);
+            GraphCall call(g);
             rc = g->off64 ? launch_pull_epoch<uint64_t>(g, mapping, 0, g->n, dist, d_upd, blocks, threads, s)
                           : launch_pull_epoch<uint32_t>(g, mapping, 0, g->n, dist, d_upd, blocks, threads, s);
         }

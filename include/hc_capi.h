@@ -104,7 +104,15 @@ typedef struct hc_device_info {
 
 /* Statistics of a full traversal / iteration run. */
 typedef struct hc_run_stats {
-    double ms;             /* device time of the epoch loop (HIP events)      */
+    double ms;             /* time of the epoch loop.  Device-driven loops
+                              (hc_bfs, hc_sssp, hc_pagerank, hc_sssp_multi):
+                              the device's constant-rate wall clock, read by
+                              the init kernel and by the controller of the
+                              last epoch.  Host-driven loops (hc_sssp_pull,
+                              hc_pr_pull, hc_bfs_many): HIP events around the
+                              launches (hc_bfs_many_noalloc: host clock).
+                              Heterogeneous runs: host clock, copies
+                              included.                                    */
     int32_t epochs;        /* epochs executed                                 */
     int32_t push_epochs;   /* top-down / push epochs                          */
     int32_t pull_epochs;   /* bottom-up / pull epochs                         */
@@ -255,7 +263,9 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   profile.kernel_events  1: time each launch of the loop's main kernel
  *                          (hc_run_stats_t.kernel_ms); 0 by default
  *   multi.graph            1: batched loops relaunch an epoch hipGraph
- *   multi.grid_per_cu      workgroups per CU of the batched sweep (8)
+ *   multi.grid_per_cu      workgroups per CU of the batched sweep (16)
+ *   multi.block            threads per workgroup of the batched sweep (128,
+ *                          256, 512 or 1024; 256)
  *   multi.hub_threshold / multi.hub_chunk / multi.wave_threshold /
  *   multi.g16_threshold    degree classes of the batched sweep
  *   pull.col_shift         pull sweeps of whole graphs with at least two
@@ -271,11 +281,36 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *                          slice degree (degree / blocks) above which 4 / 16
  *                          lanes, a wavefront, a workgroup take a row's slice
  *                          in column-blocked sweeps (2 / 12 / 48 / 512)
+ *   many.block             threads per workgroup of hc_bfs_many (256, 512
+ *                          or 1024; 512)
+ *   many.per_cu            its workgroups per CU (at most one per graph; 0:
+ *                          as many as the device holds with the launch's
+ *                          LDS; 2)
+ *   many.light_deg         its top-down levels: a thread expands frontier
+ *                          vertices up to this degree (32) ...
+ *   many.heavy_cap         ... heavier ones, up to this many per level (at
+ *                          most 1024; 512), by whole wavefronts
+ *   many.lds_nodes         graphs of up to this many vertices (at most 2^17)
+ *                          keep their bitmaps in LDS, sized to the largest
+ *                          of them in the batch; larger ones use global
+ *                          bitmaps (2^17)
  * Plans are rebuilt for new values; changing knobs while another thread
  * runs on the same graph is not supported. */
 int hc_set_tuning(const char *key, int64_t value);
 int hc_get_tuning(const char *key, int64_t *value);
 int hc_reset_tuning(void);
+
+/* Knobs of one graph (no counterpart in the reference, whose tuned choices
+ * are baked into generated drivers per graph): hc_graph_set_tuning records
+ * key = value on g and every share of it (made before or after); each call
+ * on those handles then runs with the process-wide knobs and the graph's
+ * recorded ones applied over them, so calls on different graphs in other
+ * threads (a server's slots) use their own.  Values are checked as by
+ * hc_set_tuning.  hc_graph_get_tuning reads the graph's value, else the
+ * process-wide one; hc_graph_reset_tuning drops the graph's. */
+int hc_graph_set_tuning(hc_graph_t g, const char *key, int64_t value);
+int hc_graph_get_tuning(hc_graph_t g, const char *key, int64_t *value);
+int hc_graph_reset_tuning(hc_graph_t g);
 
 /* ------------------------------------------------------------------------ */
 /* Epoch kernels (one bulk-synchronous sweep of [start_id, end_id))         */

@@ -81,6 +81,13 @@ int hc_server_destroy(hc_server_t s);
 int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
                         int32_t *graph_id);
 
+/* Records knob key = value for queries on graph graph_id
+ * (hc_graph_set_tuning on its device handle and the slots' shares of it;
+ * dynamic graphs take it again at every upload), e.g. the autotuner's
+ * cached choice for that graph.  No counterpart in the reference. */
+int hc_server_set_graph_tuning(hc_server_t s, int32_t graph_id, const char *key,
+                               int64_t value);
+
 /* Bytes the budget charges for graph g: for a dynamic graph, its CSR plus
  * the workspaces of one handle per slot (hc_graph_share), at their upper
  * bound; for a resident graph (whose CSR is outside the budget), the

@@ -68,6 +68,60 @@ def test_cache_roundtrip(hc, at, tmp_path):
     assert at.load(p, "missing") is None
 
 
+class _Recorder:
+    """Stands in for a DeviceGraph (set_tuning) or a Server
+    (set_graph_tuning): records what apply_cached sets."""
+
+    def __init__(self):
+        self.calls = []
+
+    def set_tuning(self, key, value):
+        self.calls.append((key, value))
+
+    def set_graph_tuning(self, gid, key, value):
+        self.calls.append((gid, key, value))
+
+
+def test_apply_cached_by_signature(hc, at, tmp_path):
+    g = hc.HostGraph.generate("kron", 8, 8, weighted=True)
+    other = hc.HostGraph.generate("kron", 8, 8, weighted=True, seed=99)
+    assert at.signature(g) != at.signature(other)
+    p = str(tmp_path / "c.json")
+    at.save(p, at.cache_key(g, "sssp"), {"knobs": {"sssp.stage": 8192, "sssp.alpha": 2}})
+    at.save(p, at.cache_key(g, "multi"), {"knobs": {"multi.block": 512}})
+    dg = _Recorder()
+    assert at.apply_cached(p, g, "sssp", dg) == {"sssp.stage": 8192, "sssp.alpha": 2}
+    assert sorted(dg.calls) == [("sssp.alpha", 2), ("sssp.stage", 8192)]
+    srv = _Recorder()
+    assert at.apply_cached(p, g, "multi", (srv, 3)) == {"multi.block": 512}
+    assert srv.calls == [(3, "multi.block", 512)]
+    # Another graph, another algorithm, or no cache file: nothing applied.
+    none = _Recorder()
+    assert at.apply_cached(p, other, "sssp", none) == {}
+    assert at.apply_cached(p, g, "bfs", none) == {}
+    assert at.apply_cached(str(tmp_path / "absent.json"), g, "sssp", none) == {}
+    assert none.calls == []
+
+
+def test_batch_signature(hc, at):
+    gs = [hc.HostGraph.generate("kron", 6 + i, 8, seed=i) for i in range(3)]
+    s = at.signature(gs)
+    assert s.startswith("batch3-") and s == at.signature(list(gs))
+    assert s != at.signature(gs[:2]) and s != at.signature(gs[::-1])
+
+
+def test_many_graphs_workload(hc, at):
+    """The autotuner's many-graph workload is the benchmark's: scales
+    U{10..16}, edge factors U{8..32}, graph i from seed + 1 + i."""
+    gs = at.many_graphs(4, seed=4242)
+    rng = np.random.RandomState(4242)
+    for i, g in enumerate(gs):
+        sc, ef = int(rng.randint(10, 17)), int(rng.randint(8, 33))
+        ref = hc.HostGraph.generate("kron", sc, ef, weighted=False, seed=4243 + i)
+        assert np.array_equal(g.index, ref.index)
+        assert np.array_equal(g.neighbors, ref.neighbors)
+
+
 @pytest.mark.parametrize("alg", [0, 1, 2])
 def test_cpu_epoch_timing(hc, alg):
     g = hc.HostGraph.generate("kron", 12, 16, weighted=True)
