@@ -573,6 +573,16 @@ int32_t or_pr_gauss_seidel_f32(int32_t n, const int64_t *index,
  * which no vertex moved, or after max_iters sweeps.  Returns the sweeps run. */
 int32_t or_pr_jacobi_rule_f32(int32_t n, const int64_t *index, const int32_t *nbr,
                               float damping, int32_t max_iters, float *scores) {
+    return or_pr_jacobi_rule_f32_order(n, index, nbr, damping, max_iters, 0, scores);
+}
+
+/* The same with each vertex's FP32 sum taken over its neighbours in reverse
+ * CSR order when `reverse` is set: the rule's 5e-8 test is decided on FP32
+ * values, so another summation order (the device's compensated sums are
+ * one) moves some vertices across it. */
+int32_t or_pr_jacobi_rule_f32_order(int32_t n, const int64_t *index, const int32_t *nbr,
+                                    float damping, int32_t max_iters, int32_t reverse,
+                                    float *scores) {
     const float eps = 0.00000005f;
     float *cur = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
     float *nxt = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
@@ -585,8 +595,9 @@ int32_t or_pr_jacobi_rule_f32(int32_t n, const int64_t *index, const int32_t *nb
 #pragma omp parallel for schedule(dynamic, 1024) reduction(+ : updated)
         for (int32_t u = 0; u < n; u++) {
             float total = 0.0f;
-            for (int64_t i = index[u]; i < index[u + 1]; i++) {
-                int32_t v = nbr[i];
+            const int64_t b = index[u], e = index[u + 1];
+            for (int64_t k = 0; k < e - b; k++) {
+                const int32_t v = nbr[reverse ? e - 1 - k : b + k];
                 total += cur[v] / (float)(index[v + 1] - index[v]);
             }
             const float ns = base + damping * total;

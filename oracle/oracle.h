@@ -65,6 +65,9 @@ int32_t or_pr_gauss_seidel_f32(int32_t n, const int64_t *index,
                                float *scores);
 int32_t or_pr_jacobi_rule_f32(int32_t n, const int64_t *index, const int32_t *nbr,
                               float damping, int32_t max_iters, float *scores);
+int32_t or_pr_jacobi_rule_f32_order(int32_t n, const int64_t *index, const int32_t *nbr,
+                                    float damping, int32_t max_iters, int32_t reverse,
+                                    float *scores);
 
 /* Segmentation */
 void or_equal_edge_ranges(int32_t n, const int64_t *index, int32_t segments,

@@ -31,6 +31,8 @@ _SIGS = {
     "or_pr_jacobi_f64": ([C.c_int32, _P, _P, C.c_double, C.c_int32, _P], None),
     "or_pr_gauss_seidel_f32": ([C.c_int32, _P, _P, C.c_int32, _P], C.c_int32),
     "or_pr_jacobi_rule_f32": ([C.c_int32, _P, _P, C.c_float, C.c_int32, _P], C.c_int32),
+    "or_pr_jacobi_rule_f32_order": ([C.c_int32, _P, _P, C.c_float, C.c_int32, C.c_int32, _P],
+                                    C.c_int32),
     "or_equal_edge_ranges": ([C.c_int32, _P, C.c_int32, C.c_int32, _P], None),
     "or_certify_paths": ([C.c_int32, _P, _P, _P, C.c_int32, _P, _P, C.c_int64, C.c_int32,
                           _P], C.c_int64),
@@ -198,6 +200,17 @@ def pr_jacobi_rule_f32(index, nbr, damping, max_iters):
     n = index.size - 1
     s = np.empty(n, np.float32)
     it = lib().or_pr_jacobi_rule_f32(n, _p(index), _p(nbr), damping, max_iters, _p(s))
+    return s, int(it)
+
+
+def pr_jacobi_rule_f32_reversed(index, nbr, damping, max_iters):
+    """pr_jacobi_rule_f32 with each vertex's sum over its neighbours in reverse
+    CSR order (another FP32 summation order of the same rule)."""
+    index, nbr, _ = _csr(index, nbr)
+    n = index.size - 1
+    s = np.empty(n, np.float32)
+    it = lib().or_pr_jacobi_rule_f32_order(n, _p(index), _p(nbr), damping, max_iters, 1,
+                                           _p(s))
     return s, int(it)
 
 

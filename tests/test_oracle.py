@@ -204,6 +204,39 @@ def test_pagerank_per_vertex_rule(oracle, hc):
     assert np.array_equal(s1, s) and not np.array_equal(s2, s)
 
 
+def test_pagerank_per_vertex_rule_reversed_sums(oracle, hc):
+    """The rule with each FP32 sum taken in reverse neighbour order: its
+    first sweep equals a sequential FP32 loop over the reversed rows, which
+    differs from the forward order at some vertices; at the stop the same
+    5e-8 invariant holds."""
+    g = hc.HostGraph.generate("kron", 9, 16)
+    idx, nb = g.index, g.neighbors
+    n = idx.size - 1
+    deg = np.diff(idx)
+    cur = np.full(n, np.float32(1.0) / np.float32(n), np.float32)
+    want = np.empty(n, np.float32)
+    for u in range(n):
+        t = np.float32(0.0)
+        for i in range(idx[u + 1] - 1, idx[u] - 1, -1):
+            v = nb[i]
+            t = np.float32(t + np.float32(cur[v] / np.float32(deg[v])))
+        base = np.float32(np.float32(1.0) - np.float32(0.85)) / np.float32(n)
+        ns = np.float32(base + np.float32(np.float32(0.85) * t))
+        want[u] = ns if abs(np.float32(ns - cur[u])) > np.float32(5e-8) else cur[u]
+    one, sweeps = oracle.pr_jacobi_rule_f32_reversed(idx, nb, 0.85, 1)
+    assert sweeps == 1 and np.array_equal(one, want)
+    fwd = oracle.pr_jacobi_f32(idx, nb, 0.85, 1)
+    rev = oracle.pr_jacobi_rule_f32_reversed(idx, nb, 0.85, 1)[0]
+    assert not np.array_equal(fwd, rev)  # the order matters in FP32
+
+    s, sweeps = oracle.pr_jacobi_rule_f32_reversed(idx, nb, 0.85, 201)
+    assert 1 < sweeps < 201
+    contrib = np.where(deg > 0, s.astype(np.float64) / np.maximum(deg, 1), 0.0)
+    rows = np.repeat(np.arange(n), deg)
+    upd = 0.15 / n + 0.85 * np.bincount(rows, weights=contrib[nb], minlength=n)
+    assert np.abs(upd - s).max() <= 5e-8 + 1e-10
+
+
 def test_bfs_pull_epoch_first_neighbour(oracle, hc):
     """epoch_bfs_pull_cpu restatement vs an exhaustive scan."""
     g = hc.HostGraph.generate("kron", 10, 8, weighted=False)
