@@ -426,7 +426,8 @@ def run_many_graph_bfs(hc, oracle, count, seed, check, tuned=None):
     return out
 
 
-def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None):
+def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, slots=4,
+                min_batch_bfs=0):
     """Poisson arrivals of 70% BFS / 30% SSSP queries; 20% target small
     graphs uploaded on demand.  Latency: arrival -> result in HBM.  Every
     result on an uploaded graph is compared with the oracle; on the resident
@@ -436,7 +437,7 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None):
     rng = np.random.RandomState(seed)
     dyn = [hc.HostGraph.generate("kron", int(s), 16, weighted=True, seed=seed + int(s))
            for s in rng.randint(12, 19, size=8)]
-    srv = hc.Server(slots=4, max_batch=64)
+    srv = hc.Server(slots=slots, max_batch=64, min_batch_bfs=min_batch_bfs)
     gids = [srv.add_graph(big), srv.add_graph(unif)] + [srv.add_graph(g, resident=False)
                                                         for g in dyn]
     hosts = [big, unif] + dyn
@@ -558,8 +559,10 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None):
         log(f"[serve] {rate} q/s: achieved {results[-1]['achieved_qps']} q/s, "
             f"p50 {results[-1]['p50_ms']} ms, p99 {results[-1]['p99_ms']} ms")
     st = srv.stats()
+    costs = {name: srv.graph_costs(gid) for name, gid in (("kron", gids[0]),
+                                                           ("uniform", gids[1]))}
     srv.close()
-    return {"tuned_knobs": tuned_knobs,
+    return {"slots": slots, "tuned_knobs": tuned_knobs, "measured_costs_ms": costs,
             "graphs": {"kron": big.num_nodes, "uniform": unif.num_nodes,
                        "dynamic_scales": [int(np.log2(g.num_nodes)) for g in dyn]},
             "rates": results,
@@ -716,6 +719,13 @@ def main():
     ap.add_argument("--coexec", type=int, default=1,
                     help="host+GPU co-execution: all-GPU vs profile-derived vs forced split")
     ap.add_argument("--serve-scale", type=int, default=24)
+    ap.add_argument("--serve-min-batch-bfs", type=int, default=0,
+                    help="BFS queries of one graph that must wait before they run as one "
+                         "batched sweep (0: the server's default)")
+    ap.add_argument("--serve-rates", default="100,1000,5000,20000",
+                    help="offered query rates (q/s), comma-separated")
+    ap.add_argument("--serve-slots", type=int, default=4,
+                    help="server slots (concurrent batches, one stream each)")
     ap.add_argument("--serve-queries", type=int, default=2000)
     ap.add_argument("--serve-check", type=int, default=32,
                     help="resident-graph results certified per graph, kind and rate")
@@ -828,9 +838,9 @@ def main():
             "uniform", args.scale, args.degree, weighted=True, seed=args.seed)
         # 100 q/s sits below the loop's capacity (its latency is service time);
         # 1k-20k q/s are the offered rates of the serving configuration.
-        serving = run_serving(hc, oracle, big, unif, [100, 1000, 5000, 20000],
+        serving = run_serving(hc, oracle, big, unif, [int(x) for x in args.serve_rates.split(",")],
                               args.serve_queries, 777, args.serve_check if args.check else 0,
-                              args.tuned)
+                              args.tuned, args.serve_slots, args.serve_min_batch_bfs)
         del big
 
     cpu = None

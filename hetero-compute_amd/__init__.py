@@ -98,7 +98,7 @@ class ScheduleBlock(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("memory_budget", C.c_int64),
                 ("min_batch_sssp", C.c_int32), ("min_batch_bfs", C.c_int32),
-                ("max_wait_ms", C.c_int32)]
+                ("max_wait_ms", C.c_int32), ("calibrate", C.c_int32)]
 
 
 class QueryInfo(C.Structure):
@@ -191,6 +191,7 @@ _SIGS = {
     "hc_server_add_graph": ([_P, C.POINTER(HostGraphStruct), C.c_int32,
                              C.POINTER(C.c_int32)], C.c_int),
     "hc_server_set_graph_tuning": ([_P, C.c_int32, C.c_char_p, C.c_int64], C.c_int),
+    "hc_server_graph_costs": ([_P, C.c_int32, _P, _P], C.c_int),
     "hc_server_submit": ([_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64)], C.c_int),
     "hc_server_wait": ([_P, C.c_int64, C.POINTER(QueryInfo)], C.c_int),
     "hc_server_fetch": ([_P, C.c_int64, _P], C.c_int),
@@ -923,14 +924,16 @@ class Server:
     (hc_server_*).  Host graphs added here are kept alive by the object."""
 
     def __init__(self, slots: int = 4, max_batch: int = 64, memory_budget: int = 0,
-                 min_batch_sssp: int = 0, min_batch_bfs: int = 0, max_wait_ms: int = 0):
+                 min_batch_sssp: int = 0, min_batch_bfs: int = 0, max_wait_ms: int = 0,
+                 calibrate: int = 0):
         """min_batch_*: queries of one graph and kind that must wait before
-        they run as one batched sweep (0: the library's defaults, 8 and 64);
-        max_wait_ms: how long fast-arriving SSSP queries may wait to gather a
-        batch (0: the default, 500; -1: never)."""
+        they run as one batched sweep (0: from the costs measured when a
+        resident graph is added, else 8 and 64); max_wait_ms: how long
+        fast-arriving queries may wait to gather a batch (0: the default,
+        500; -1: never); calibrate -1: measure no costs."""
         self.h = _P()
         o = ServerOpts(slots, max_batch, memory_budget, min_batch_sssp, min_batch_bfs,
-                       max_wait_ms)
+                       max_wait_ms, calibrate)
         check(lib().hc_server_create(C.byref(o), C.byref(self.h)))
         self._graphs = []
 
@@ -939,6 +942,15 @@ class Server:
         check(lib().hc_server_add_graph(self.h, g.struct_ref(), int(resident), C.byref(gid)))
         self._graphs.append(g)
         return gid.value
+
+    def graph_costs(self, graph: int) -> dict:
+        """Device ms of a single query and of a 64-wide batch per kind,
+        measured when the graph was added (hc_server_graph_costs)."""
+        sm = (C.c_double * 2)()
+        bm = (C.c_double * 2)()
+        check(lib().hc_server_graph_costs(self.h, graph, sm, bm))
+        return {"bfs": {"single_ms": sm[0], "batch_ms": bm[0]},
+                "sssp": {"single_ms": sm[1], "batch_ms": bm[1]}}
 
     def set_graph_tuning(self, graph: int, key: str, value: int):
         """A knob for queries on one graph (hc_server_set_graph_tuning)."""

@@ -25,7 +25,8 @@ constexpr int kScanItems = 16;
 constexpr int kScanChunk = kBlock * kScanItems;
 
 ColBlockSlot::~ColBlockSlot() {
-    if (cb.dmem) (void)hipFree(cb.dmem);
+    for (auto &kv : by_shift)
+        if (kv.second.dmem) (void)hipFree(kv.second.dmem);
 }
 
 // cnt[k][u] = edges of row u whose neighbour is in block k; cnt[k][n] = 0.
@@ -207,22 +208,15 @@ int get_col_blocks(hc_graph_s *g, int shift, hipStream_t s, const ColBlocks **ou
     if (K < 2 || K > kMaxColBlocks || g->row0 != 0 || g->rows != g->n) return HC_OK;
     ColBlockSlot &slot = *g->cbslot;
     std::lock_guard<std::mutex> lk(slot.mu);
-    if (slot.cb.dmem && slot.cb.shift == shift) {
-        *out = &slot.cb;
+    auto it = slot.by_shift.find(shift);
+    if (it != slot.by_shift.end()) {
+        *out = &it->second;  // map nodes stay put as others are added
         return HC_OK;
-    }
-    if (slot.cb.dmem) {
-        // Another block size was asked for: rebuild (kernels of earlier
-        // sweeps on other streams are done, as each call waits for its run).
-        (void)hipDeviceSynchronize();
-        (void)hipFree(slot.cb.dmem);
-        slot.cb = ColBlocks{};
     }
     ColBlocks cb;
     HC_TRY(g->off64 ? build_col_blocks<uint64_t>(g, shift, K, s, &cb)
                     : build_col_blocks<uint32_t>(g, shift, K, s, &cb));
-    slot.cb = cb;
-    *out = &slot.cb;
+    *out = &slot.by_shift.emplace(shift, cb).first->second;
     return HC_OK;
 }
 

@@ -137,6 +137,13 @@ struct Tuning {
     int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
     int multi_grid_per_cu = 16;  // workgroups per CU of the batched sweep
     int multi_block = 256;       // threads per workgroup of the batched sweep (128-1024)
+    int msbfs = 1;               // batched BFS: bit-parallel levels (1) or the batched
+                                 // sweep with unit weights (0)
+    int msbfs_alpha = 32;        // bit-parallel BFS pushes a level while its frontier's
+                                 // edges < m / this, else pulls
+    int msbfs_col_shift = -1;    // its pulls gather column blocks of 2^this vertices
+                                 // (0: off; -1: four blocks on flat graphs, sixteen
+                                 // on skewed ones, at least 2^16 vertices each)
     int many_block = 512;        // many-graph BFS: threads per workgroup (256, 512, 1024)
     int many_per_cu = 2;         // ... workgroups per CU (0: as many as the device holds)
     int many_light_deg = 32;     // ... top-down: a thread expands vertices up to this degree
@@ -479,10 +486,12 @@ struct GraphTuning {
     std::mutex mu;
     std::vector<std::pair<std::string, int64_t>> over;
 };
-// One copy for a graph and every share of it, built on first use.
+// One copy per block size for a graph and every share of it, each built on
+// first use (SSSP sweeps and the bit-parallel batched BFS use different
+// sizes, and neither rebuilds the other's).
 struct ColBlockSlot {
     std::mutex mu;
-    ColBlocks cb;
+    std::map<int, ColBlocks> by_shift;
     ~ColBlockSlot();
 };
 }  // namespace hc
@@ -511,6 +520,7 @@ struct hc_graph_s {
     void *ws_bfs = nullptr;
     void *ws_pr = nullptr;
     void *ws_multi = nullptr;
+    void *ws_msbfs = nullptr;
     std::shared_ptr<hc::ColBlockSlot> cbslot = std::make_shared<hc::ColBlockSlot>();
     std::shared_ptr<hc::GraphTuning> tune = std::make_shared<hc::GraphTuning>();
 };
@@ -560,10 +570,19 @@ int get_col_blocks(hc_graph_s *g, int shift, hipStream_t s, const ColBlocks **ou
 // Block shift a pull sweep of g uses under the current tuning (0: plain
 // sweep); flat: g has no hub rows (top 1% of rows hold < 5% of the edges).
 int col_shift_for(const hc_graph_s *g, bool flat);
+// True when the top 1% of g's rows hold under 5% of its edges (cached; the
+// caller holds g->mu).
+bool flat_graph(hc_graph_s *g);
 void free_sssp_ws(hc_graph_s *g);
 void free_bfs_ws(hc_graph_s *g);
 void free_pr_ws(hc_graph_s *g);
 void free_multi_ws(hc_graph_s *g);
+void free_msbfs_ws(hc_graph_s *g);
+// Bit-parallel BFS from up to 64 sources (whole-graph handle, caller holds
+// the graph): depth[v * 64 + q] = level from sources[q], HC_INF_DIST beyond
+// max_levels (<= 0: no limit) or when unreachable.
+int run_msbfs_entry(hc_graph_s *g, const int32_t *sources, int num, int32_t *depth,
+                    int max_levels, hc_run_stats_t *stats, hipStream_t s);
 }  // namespace hc
 
 #endif

@@ -548,6 +548,8 @@ static int multi_entry(hc_graph_t g, const int32_t *sources, int32_t num_sources
     HC_TRY(ensure_init());
     GraphCall call(g);
     hipStream_t s = pick_stream(stream);
+    if (unit && tuning().msbfs)
+        return run_msbfs_entry(g, sources, num_sources, dist, max_epochs, stats, s);
     return g->off64
                ? run_multi<uint64_t>(g, sources, num_sources, dist, max_epochs, unit, stats, s)
                : run_multi<uint32_t>(g, sources, num_sources, dist, max_epochs, unit, stats, s);

@@ -134,6 +134,9 @@ int64_t *knob_ref(Tuning &t, const char *key, bool *is_int, int **iref) {
                {"multi.graph", &t.multi_graph},
                {"multi.grid_per_cu", &t.multi_grid_per_cu},
                {"multi.block", &t.multi_block},
+               {"msbfs.on", &t.msbfs},
+               {"msbfs.alpha", &t.msbfs_alpha},
+               {"msbfs.col_shift", &t.msbfs_col_shift},
                {"many.block", &t.many_block},
                {"many.per_cu", &t.many_per_cu},
                {"many.light_deg", &t.many_light_deg},
@@ -559,6 +562,7 @@ int hc_graph_free(hc_graph_t g) {
     free_bfs_ws(g);
     free_pr_ws(g);
     free_multi_ws(g);
+    free_msbfs_ws(g);
     for (auto &kv : g->plans) hipFree(kv.second.dmem);
     if (!g->shared) {
         hipFree(g->d_off);
@@ -608,13 +612,19 @@ int set_knob(Tuning &t, const char *key, int64_t value) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
                            !strcmp(key, "sssp.stage") || !strcmp(key, "push.stage");
         const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph") ||
-                          !strcmp(key, "sssp.filter");
+                          !strcmp(key, "sssp.filter") || !strcmp(key, "msbfs.on");
         if (!strcmp(key, "sssp.dist16"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "sssp.dist16 must be 0, 1 or 2");
         if (!strcmp(key, "sssp.push_dedup"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "sssp.push_dedup must be 0, 1 or 2");
         if (!strcmp(key, "bu.first"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "bu.first must be 0, 1 or 2");
+        if (!strcmp(key, "msbfs.col_shift")) {
+            HC_CHECK_ARG(value == -1 || value == 0 || (value >= 6 && value <= 30),
+                         "msbfs.col_shift must be -1 (automatic), 0 (off) or in [6, 30]");
+            *iref = (int)value;
+            return HC_OK;
+        }
         if (!strcmp(key, "pull.col_shift")) {
             HC_CHECK_ARG(value == -1 || value == 0 || (value >= 6 && value <= 30),
                          "pull.col_shift must be -1 (automatic), 0 (off) or in [6, 30]");

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -67,6 +68,10 @@ struct GraphEntry {
     hc_graph_t dev = nullptr;             // owner of the device arrays
     std::vector<hc_graph_t> per_slot;     // shares, one per slot
     std::vector<std::pair<std::string, int64_t>> knobs;  // applied at every upload
+    // Device ms of one single-source query and of one 64-wide batch, per
+    // kind, measured when a resident graph is added (0: not measured).
+    double single_ms[2] = {0, 0};
+    double batch_ms[2] = {0, 0};
     int64_t bytes = 0;                    // budgeted bytes while uploaded
     bool uploading = false;
     int in_use = 0;
@@ -77,6 +82,7 @@ struct GraphEntry {
 
 struct hc_server_s {
     hc_server_opts_t opts{};
+    bool user_min[2] = {false, false};  // min_batch_bfs / _sssp given by the caller
     std::mutex mu;
     std::condition_variable cv_work, cv_done, cv_mem;
     std::deque<int64_t> queue;
@@ -250,12 +256,23 @@ bool spill_one(hc_server_s *s, std::unique_lock<std::mutex> &lk) {
 
 // Chooses the next sweep (caller holds s->mu): classes (graph, kind) are
 // taken in the arrival order of their oldest waiting query.  A class runs
-// its oldest queries up to max_batch when at least min_batch of them wait,
-// else its oldest query alone -- except that an SSSP class whose arrivals
-// would gather 2 x min_batch queries within max_wait_ms of its oldest one's
-// arrival waits (other classes run meanwhile) until max_batch wait or the
-// oldest has waited max_wait_ms.  Returns false when every waiting class is
-// gathering; *wake is then the ms at which the first of them stops.
+// its oldest queries up to max_batch when at least its min_batch of them
+// wait (class_min_batch), else its oldest query alone -- except that a class
+// that batches at all and whose arrivals would gather 2 x min_batch queries
+// (at most max_batch) within max_wait_ms of its oldest one's arrival waits
+// (other classes run meanwhile) until max_batch wait or the oldest has
+// waited max_wait_ms.  Returns false when every waiting class is gathering;
+// *wake is then the ms at which the first of them stops.
+// Waiting queries of a class from which a batched sweep beats running them
+// one by one: the measured break-even count, or the fixed option; above
+// max_batch the class never batches.
+int class_min_batch(const hc_server_s *s, const GraphEntry &g, int kind) {
+    const int fixed = kind == HC_QUERY_BFS ? s->opts.min_batch_bfs : s->opts.min_batch_sssp;
+    if (s->user_min[kind] || !(g.single_ms[kind] > 0 && g.batch_ms[kind] > 0)) return fixed;
+    const double k = std::ceil(g.batch_ms[kind] / g.single_ms[kind]);
+    return (int)std::max(2.0, std::min(k, (double)s->opts.max_batch + 1));
+}
+
 bool pick_batch(hc_server_s *s, double now, std::vector<int64_t> &batch, double *wake) {
     *wake = 1e300;
     std::vector<std::pair<int32_t, int32_t>> seen;
@@ -268,19 +285,20 @@ bool pick_batch(hc_server_s *s, double now, std::vector<int64_t> &batch, double 
         for (auto jt = it; jt != s->queue.end() && waiting < s->opts.max_batch; ++jt)
             if (s->queries[*jt].graph == cls.first && s->queries[*jt].kind == cls.second)
                 waiting++;
-        const int min_batch =
-            cls.second == HC_QUERY_BFS ? s->opts.min_batch_bfs : s->opts.min_batch_sssp;
-        if (cls.second == HC_QUERY_SSSP && !s->stop && s->opts.max_wait_ms > 0 &&
-            waiting < s->opts.max_batch && s->opts.max_batch >= 2 * min_batch) {
+        const int min_batch = class_min_batch(s, s->graphs[cls.first], cls.second);
+        if (min_batch <= s->opts.max_batch && !s->stop && s->opts.max_wait_ms > 0 &&
+            waiting < s->opts.max_batch) {
             const double gap = s->rates[cls].gap;
             const double deadline = head.t_submit + s->opts.max_wait_ms;
-            if (gap > 0 && now < deadline &&
-                waiting + (deadline - now) / gap >= 2.0 * min_batch) {
+            const double target = std::min(2.0 * min_batch, (double)s->opts.max_batch);
+            if (gap > 0 && now < deadline && waiting + (deadline - now) / gap >= target) {
                 *wake = std::min(*wake, deadline);
                 continue;  // gathering
             }
         }
-        const int take = waiting >= min_batch ? s->opts.max_batch : 1;
+        const int take = waiting >= min_batch && min_batch <= s->opts.max_batch
+                             ? s->opts.max_batch
+                             : 1;
         for (auto jt = it; jt != s->queue.end() && (int)batch.size() < take;) {
             const Query &q = s->queries[*jt];
             if (q.graph == cls.first && q.kind == cls.second) {
@@ -463,9 +481,12 @@ int hc_server_create(const hc_server_opts_t *opts, hc_server_t *out) {
     s->opts.slots = opts && opts->slots > 0 ? opts->slots : 4;
     s->opts.max_batch = opts && opts->max_batch > 0 ? opts->max_batch : kWidth;
     s->opts.memory_budget = opts ? opts->memory_budget : 0;
+    s->user_min[HC_QUERY_SSSP] = opts && opts->min_batch_sssp > 0;
+    s->user_min[HC_QUERY_BFS] = opts && opts->min_batch_bfs > 0;
     s->opts.min_batch_sssp = opts && opts->min_batch_sssp > 0 ? opts->min_batch_sssp : 8;
     s->opts.min_batch_bfs = opts && opts->min_batch_bfs > 0 ? opts->min_batch_bfs : kWidth;
     s->opts.max_wait_ms = opts && opts->max_wait_ms != 0 ? opts->max_wait_ms : 500;
+    s->opts.calibrate = opts ? opts->calibrate : 0;
     if (s->opts.slots > 16 || s->opts.max_batch > kWidth) {
         delete s;
         return hc::fail(HC_ERR_INVALID, "slots must be <= 16 and max_batch <= %d", kWidth);
@@ -520,6 +541,70 @@ int hc_server_destroy(hc_server_t s) {
     return HC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Device ms of single-source queries and of a 64-wide batch of each kind on
+// a resident graph (e.dev), on slot 0's stream: each run once to build plans
+// and workspaces, then timed (singles averaged over three sources).  Sources
+// are vertices with edges, spread over the id range.  Leaves the costs at 0
+// when the graph has no edges or the scratch matrix cannot be allocated.
+void calibrate_graph(hc_server_s *s, GraphEntry &e) {
+    const int32_t n = e.host.num_nodes;
+    std::vector<int32_t> src;
+    const int64_t step = std::max<int64_t>(1, n / 257);
+    for (int64_t k = 0; k < n && (int)src.size() < kWidth; k++) {
+        const int32_t v = (int32_t)((k * step + k / 257) % n);
+        if (e.host.index[v + 1] > e.host.index[v]) src.push_back(v);
+    }
+    if (src.empty()) return;
+    for (size_t i = 0; (int)src.size() < kWidth; i++) {
+        const int32_t v = src[i];  // repeat the ones found
+        src.push_back(v);
+    }
+    void *buf = nullptr;
+    if (hipMalloc(&buf, sizeof(int32_t) * (size_t)n * kWidth) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    int32_t *d = static_cast<int32_t *>(buf);
+    hipStream_t st = s->streams[0];
+    hc_run_stats_t rs{};
+    for (int kind = 0; kind < 2; kind++) {
+        if (kind == HC_QUERY_SSSP && !e.host.weights) continue;
+        auto single = [&](int32_t v) {
+            if (kind == HC_QUERY_BFS) {
+                hc_bfs_opts_t o{15, 18, 0};
+                return hc_bfs(e.dev, v, d, nullptr, &o, &rs, st);
+            }
+            return hc_sssp(e.dev, v, d, nullptr, &rs, st);
+        };
+        auto batch = [&]() {
+            return kind == HC_QUERY_BFS ? hc_bfs_multi(e.dev, src.data(), kWidth, d, 0, &rs, st)
+                                        : hc_sssp_multi(e.dev, src.data(), kWidth, d, 0, &rs, st);
+        };
+        if (single(src[0]) != HC_OK) continue;
+        double sum = 0;
+        int runs = 0;
+        for (int i = 1; i <= 3 && i < (int)src.size(); i++)
+            if (single(src[i * 7 % src.size()]) == HC_OK) {
+                sum += rs.ms;
+                runs++;
+            }
+        if (runs == 0 || batch() != HC_OK || batch() != HC_OK) continue;
+        e.single_ms[kind] = sum / runs;
+        e.batch_ms[kind] = rs.ms;
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(buf);
+    hc::clear_error();
+}
+
+}  // namespace
+
+extern "C" {
+
 int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
                         int32_t *graph_id) {
     HC_CHECK_ARG(s && g && graph_id, "NULL argument");
@@ -536,6 +621,7 @@ int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t residen
                      (long long)e.bytes);
     }
     if (e.resident) HC_TRY(hc_graph_upload(g, &e.dev));
+    if (e.resident && s->opts.calibrate >= 0) calibrate_graph(s, e);
     std::lock_guard<std::mutex> lk(s->mu);
     if (e.resident) {
         // Resident graphs keep their per-slot workspaces for the server's
@@ -546,6 +632,20 @@ int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t residen
     }
     s->graphs.push_back(std::move(e));
     *graph_id = (int32_t)s->graphs.size() - 1;
+    return HC_OK;
+}
+
+int hc_server_graph_costs(hc_server_t s, int32_t graph_id, double *single_ms,
+                          double *batch_ms) {
+    HC_CHECK_ARG(s && single_ms && batch_ms, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    HC_CHECK_ARG(graph_id >= 0 && graph_id < (int32_t)s->graphs.size(), "unknown graph %d",
+                 graph_id);
+    const GraphEntry &g = s->graphs[graph_id];
+    for (int k = 0; k < 2; k++) {
+        single_ms[k] = g.single_ms[k];
+        batch_ms[k] = g.batch_ms[k];
+    }
     return HC_OK;
 }
 

@@ -717,7 +717,7 @@ static double hub_share(hc_graph_s *g) {
     return g->hub_share = double(top) / double(g->m);
 }
 
-static bool flat_graph(hc_graph_s *g) { return hub_share(g) < 0.05; }
+bool flat_graph(hc_graph_s *g) { return hub_share(g) < 0.05; }
 
 struct SsspWs {
     FrontierWs f;

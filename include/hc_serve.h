@@ -34,17 +34,23 @@ typedef struct hc_server_opts {
                               (default: half of the free HBM)                 */
     int32_t min_batch_sssp; /* a batched SSSP sweep runs when at least this
                                many SSSP queries of one graph wait; fewer run
-                               one at a time (default 8: a 64-wide sweep costs
-                               about 8 single-source runs)                     */
-    int32_t min_batch_bfs;  /* the same for BFS (default 64: a batched BFS
-                               sweep pays only at full width, if at all)       */
-    int32_t max_wait_ms;    /* SSSP queries of one graph arriving fast enough
-                               to gather 2 x min_batch_sssp of them within this
-                               many ms of the oldest one's arrival wait (while
-                               other queries run) until a full batch is there or
-                               the oldest has waited this long, then run as one
-                               sweep; slower classes run at once (default 500;
-                               < 0: never wait)                                */
+                               one at a time.  0 (default): from the graph's
+                               measured costs (see calibrate), else 8          */
+    int32_t min_batch_bfs;  /* the same for BFS (0: measured, else 64)         */
+    int32_t max_wait_ms;    /* queries of one graph and kind arriving fast
+                               enough to gather 2 x its min_batch of them (at
+                               most max_batch) within this many ms of the
+                               oldest one's arrival wait (while other queries
+                               run) until a full batch is there or the oldest
+                               has waited this long, then run as one sweep;
+                               slower classes run at once (default 500; < 0:
+                               never wait)                                     */
+    int32_t calibrate;      /* 0 (default): when a resident graph is added,
+                               time one single-source query and one 64-wide
+                               batch of each kind on it; a class then batches
+                               from ceil(batch ms / single ms) waiting queries
+                               (never, when that exceeds max_batch).  < 0: no
+                               timing, the fixed defaults above                */
 } hc_server_opts_t;
 
 typedef struct hc_query_info {
@@ -80,6 +86,13 @@ int hc_server_destroy(hc_server_t s);
  * needed; the caller keeps g's arrays alive until hc_server_destroy. */
 int hc_server_add_graph(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
                         int32_t *graph_id);
+
+/* Device ms of one single-source query and of one 64-wide batched sweep of
+ * each kind (index HC_QUERY_BFS / HC_QUERY_SSSP) measured when graph_id was
+ * added; 0 where not measured (dynamic graphs, calibrate < 0, unweighted
+ * SSSP). */
+int hc_server_graph_costs(hc_server_t s, int32_t graph_id, double *single_ms,
+                          double *batch_ms);
 
 /* Records knob key = value for queries on graph graph_id
  * (hc_graph_set_tuning on its device handle and the slots' shares of it;

@@ -208,3 +208,127 @@ def test_bfs_multi_disconnected_golden(hc, gpu):
         r = hc.bfs_multi(dg, fx["sources"])
     for q in range(len(fx["sources"])):
         assert np.array_equal(r.values[q], fx["depth"][q])
+
+
+# Bit-parallel batched BFS (msbfs.on = 1, the default) against the batched
+# sweep with unit weights (msbfs.on = 0) and the oracle, with each level
+# direction forced in turn (msbfs.alpha 1: push while the frontier has fewer
+# edges than the graph; 2^20: pull from the first level on).
+
+@pytest.fixture()
+def reset_knobs(hc):
+    yield
+    hc.reset_tuning()
+
+
+def _bfs_multi_both(hc, dg, sources, max_epochs=0):
+    hc.set_tuning("msbfs.on", 0)
+    try:
+        old = hc.bfs_multi(dg, sources, max_epochs=max_epochs)
+    finally:
+        hc.set_tuning("msbfs.on", 1)
+    return hc.bfs_multi(dg, sources, max_epochs=max_epochs), old
+
+
+@pytest.mark.parametrize("alpha", [1, 8, 1 << 20])
+@pytest.mark.parametrize("kind,scale", [("kron", 14), ("uniform", 13)])
+def test_bfs_multi_bitwise_levels(hc, gpu, oracle, reset_knobs, alpha, kind, scale):
+    g = hc.HostGraph.generate(kind, scale, 16, weighted=False, seed=3)
+    sources = g.pick_sources(64, seed=alpha % 97)
+    hc.set_tuning("msbfs.alpha", alpha)
+    with hc.DeviceGraph(g) as dg:
+        new, old = _bfs_multi_both(hc, dg, sources)
+    assert np.array_equal(new.values, old.values)
+    for q in (0, 9, 63):
+        assert np.array_equal(new.values[q], oracle.bfs_depth(g.index, g.neighbors,
+                                                               int(sources[q]))), q
+    st = new.stats
+    assert st["epochs"] == st["push_epochs"] + st["pull_epochs"]
+    # The last level finds nothing: levels = largest depth + 1.
+    assert st["epochs"] == int(new.values.max()) + 1
+    if alpha == 1:
+        # Pull only when a level's frontier holds every edge (the 64
+        # frontiers can cover the whole graph at once).
+        assert st["pull_epochs"] <= 1
+    if alpha == 1 << 20:
+        assert st["push_epochs"] == 0
+
+
+@pytest.mark.parametrize("count", [1, 2, 7, 33])
+def test_bfs_multi_bitwise_partial_and_duplicates(hc, gpu, oracle, reset_knobs, count):
+    g = hc.HostGraph.generate("kron", 13, 8, weighted=False, seed=17)
+    sources = g.pick_sources(count, seed=count)
+    if count > 2:
+        sources[2] = sources[0]  # one vertex in two columns
+    with hc.DeviceGraph(g) as dg:
+        d = hc.DeviceArray(g.num_nodes * hc.MULTI_WIDTH, np.int32)
+        r = hc.bfs_multi(dg, sources, out=d)
+        raw = d.get().reshape(g.num_nodes, hc.MULTI_WIDTH)
+        d.free()
+    for q, s in enumerate(sources):
+        assert np.array_equal(r.values[q], oracle.bfs_depth(g.index, g.neighbors, int(s))), q
+    # Columns past num_sources stay unreached.
+    assert np.all(raw[:, count:] == hc.INF_DIST)
+
+
+def test_bfs_multi_bitwise_isolated_sources_and_limit(hc, gpu, oracle, reset_knobs):
+    g = hc.HostGraph.generate("kron", 12, 4, weighted=False, seed=5)
+    deg = np.diff(g.index)
+    iso = np.flatnonzero(deg == 0)[:3]
+    live = g.pick_sources(5)
+    sources = np.concatenate([iso, live]).astype(np.int32)
+    with hc.DeviceGraph(g) as dg:
+        full = hc.bfs_multi(dg, sources)
+        lim, old_lim = _bfs_multi_both(hc, dg, sources, max_epochs=2)
+    for q, s in enumerate(sources):
+        ref = oracle.bfs_depth(g.index, g.neighbors, int(s))
+        assert np.array_equal(full.values[q], ref), q
+        # At most two levels: depths up to 2 exact, farther vertices unreached.
+        want = np.where(ref <= 2, ref, -1)
+        assert np.array_equal(lim.values[q], want), q
+    assert lim.stats["epochs"] == 2
+
+
+def test_bfs_multi_bitwise_wide_offsets_and_unsorted(hc, gpu, oracle, reset_knobs,
+                                                     monkeypatch):
+    g = hc.HostGraph.generate("kron", 12, 16, weighted=False, seed=8)
+    rng = np.random.default_rng(2)
+    perm = rng.permutation(g.num_nodes)  # new id of each old vertex
+    inv = np.argsort(perm)
+    deg = np.diff(g.index)[inv]
+    idx = np.zeros(g.num_nodes + 1, np.int64)
+    np.cumsum(deg, out=idx[1:])
+    nb = np.concatenate([np.sort(perm[g.neighbors[g.index[o]:g.index[o + 1]]])
+                         for o in inv]).astype(np.int32)
+    h = hc.HostGraph.from_csr(idx, nb)
+    sources = h.pick_sources(64, seed=4)
+    monkeypatch.setenv("HC_FORCE_OFF64", "1")
+    with hc.DeviceGraph(h) as dg:
+        info = dg.info()
+        assert info["offset_bytes"] == 8 and not info["degree_sorted"]
+        new, old = _bfs_multi_both(hc, dg, sources)
+    assert np.array_equal(new.values, old.values)
+    for q in (0, 31, 63):
+        assert np.array_equal(new.values[q], oracle.bfs_depth(idx, nb, int(sources[q]))), q
+
+
+@pytest.mark.parametrize("shift", [9, 10, 12])
+@pytest.mark.parametrize("kind,scale", [("kron", 14), ("uniform", 13)])
+def test_bfs_multi_bitwise_column_blocks(hc, gpu, oracle, reset_knobs, shift, kind, scale):
+    """Pull levels over column blocks (forced here: automatic only on flat
+    graphs above 2^19 vertices), every level pulled, against the batched
+    sweep and the oracle."""
+    g = hc.HostGraph.generate(kind, scale, 16, weighted=False, seed=shift)
+    sources = g.pick_sources(64, seed=shift)
+    hc.set_tuning("msbfs.alpha", 1 << 20)
+    hc.set_tuning("msbfs.col_shift", shift)
+    with hc.DeviceGraph(g) as dg:
+        new, old = _bfs_multi_both(hc, dg, sources)
+        hc.set_tuning("msbfs.col_shift", 0)
+        plain = hc.bfs_multi(dg, sources)
+    assert new.stats["push_epochs"] == 0
+    assert np.array_equal(new.values, old.values)
+    assert np.array_equal(new.values, plain.values)
+    for q in (0, 32, 63):
+        assert np.array_equal(new.values[q], oracle.bfs_depth(g.index, g.neighbors,
+                                                               int(sources[q]))), q

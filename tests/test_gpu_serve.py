@@ -156,11 +156,13 @@ def test_server_rejects_bad_queries(hc, gpu, graphs):
 
 def test_fast_sssp_arrivals_gather_batches(hc, gpu, oracle, graphs):
     """SSSP queries arriving fast enough to gather 2 x min_batch within
-    max_wait_ms wait for a batch (while BFS queries keep running alone);
-    sparse arrivals run at once, one query per run.  Every answer exact."""
+    max_wait_ms wait for a batch; BFS queries, whose fixed threshold is a
+    full batch, run alone or as full batches; sparse arrivals run at once,
+    one query per run.  Every answer exact.  (Fixed thresholds: no cost
+    calibration.)"""
     import time
     g = graphs["kron"]
-    srv = hc.Server(slots=4, max_batch=64, max_wait_ms=300)
+    srv = hc.Server(slots=4, max_batch=64, max_wait_ms=300, calibrate=-1)
     gid = srv.add_graph(g)
     rng = np.random.RandomState(5)
     fast = [(int(rng.rand() < 0.3), int(s)) for s in g.pick_sources(240, seed=9)]
@@ -175,7 +177,7 @@ def test_fast_sssp_arrivals_gather_batches(hc, gpu, oracle, graphs):
     sssp_batches = [info["batch_size"] for (kind, _), (_, info) in zip(fast, res) if kind == 1]
     bfs_batches = [info["batch_size"] for (kind, _), (_, info) in zip(fast, res) if kind == 0]
     assert np.mean(sssp_batches) >= 16, sssp_batches
-    assert max(bfs_batches) == 1
+    assert all(b == 1 or b == 64 for b in bfs_batches), bfs_batches
     for (kind, src), (m, info) in zip(fast, res):
         ref = (oracle.bfs_depth(g.index, g.neighbors, src) if kind == 0
                else oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
@@ -192,3 +194,34 @@ def test_fast_sssp_arrivals_gather_batches(hc, gpu, oracle, graphs):
     for src, (m, _) in zip(slow, res):
         assert np.array_equal(m, oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, src))
     srv.close()
+
+
+def test_resident_graph_costs_measured(hc, gpu, oracle):
+    """Adding a resident graph times a single query and a 64-wide batch of
+    each kind; calibrate=-1 skips it; queries stay exact either way."""
+    g = hc.HostGraph.generate("kron", 13, 16, weighted=True, seed=12)
+    u = hc.HostGraph.generate("kron", 12, 8, weighted=False, seed=13)
+    with hc.Server(slots=2) as srv:
+        a = srv.add_graph(g)
+        b = srv.add_graph(u)
+        d = srv.add_graph(hc.HostGraph.generate("kron", 11, 8, weighted=True, seed=14),
+                          resident=False)
+        ca, cb, cd = srv.graph_costs(a), srv.graph_costs(b), srv.graph_costs(d)
+        for k in ("bfs", "sssp"):
+            assert ca[k]["single_ms"] > 0 and ca[k]["batch_ms"] > 0
+        assert cb["bfs"]["single_ms"] > 0 and cb["sssp"]["single_ms"] == 0  # unweighted
+        assert all(v == 0 for c in cd.values() for v in c.values())  # dynamic: unmeasured
+        tickets = [(gid, host, kind, int(s), srv.submit(gid, kind, int(s)))
+                   for gid, host in ((a, g), (b, u))
+                   for kind in ((hc.QUERY_BFS, hc.QUERY_SSSP) if host.weights is not None
+                                else (hc.QUERY_BFS,))
+                   for s in host.pick_sources(20, seed=kind)]
+        for gid, host, kind, s, t in tickets:
+            got = srv.fetch(t)
+            srv.release(t)
+            ref = (oracle.bfs_depth(host.index, host.neighbors, s) if kind == hc.QUERY_BFS
+                   else oracle.sssp_dijkstra(host.index, host.neighbors, host.weights, s))
+            assert np.array_equal(got, ref), (gid, kind, s)
+    with hc.Server(slots=1, calibrate=-1) as srv:
+        a = srv.add_graph(g)
+        assert all(v == 0 for c in srv.graph_costs(a).values() for v in c.values())
