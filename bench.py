@@ -379,6 +379,39 @@ def run_batched_sssp(hc, oracle, scale, degree, seed, check, tuned=None):
     return out
 
 
+def run_batched_bfs(hc, oracle, g, dg, label, check):
+    """64 BFS sources of a resident graph in one bit-parallel batch vs the
+    same 64 sources run one after another and vs the batched sweep with
+    unit weights (device time of each); every column certified."""
+    sources = g.pick_sources(64)
+    buf = hc.DeviceArray(g.num_nodes * hc.MULTI_WIDTH, np.int32)
+    hc.bfs_multi(dg, sources, out=buf, fetch=False)  # warm-up (plans, column blocks)
+    r = hc.bfs_multi(dg, sources, out=buf, fetch=False)
+    M = buf.get().reshape(g.num_nodes, hc.MULTI_WIDTH) if check else None
+    hc.set_tuning("msbfs.on", 0)
+    try:
+        hc.bfs_multi(dg, sources, out=buf, fetch=False)
+        old = hc.bfs_multi(dg, sources, out=buf, fetch=False).stats["ms"]
+    finally:
+        hc.set_tuning("msbfs.on", 1)
+    buf.free()
+    one = hc.DeviceArray(g.num_nodes, np.int32)
+    hc.bfs(dg, int(sources[0]), depth_out=one, want_parents=False, fetch=False)
+    seq = sum(hc.bfs(dg, int(s), depth_out=one, want_parents=False, fetch=False).stats["ms"]
+              for s in sources)
+    one.free()
+    out = {"graph": label, "batch_ms": round(r.stats["ms"], 3),
+           "levels": {"push": r.stats["push_epochs"], "pull": r.stats["pull_epochs"]},
+           "unit_weight_sweep_ms": round(old, 3), "sequential_ms": round(seq, 3),
+           "speedup_vs_sequential": round(seq / r.stats["ms"], 2),
+           "batch_gteps": round(hc.gteps(64 * g.num_edges, r.stats["ms"]), 3)}
+    if check:
+        bad = oracle.certify_paths(g.index, g.neighbors, None, sources, M, oracle.INF)
+        out["columns_certified"] = 64
+        out["bit_exact"] = bool(bad.sum() == 0)
+    return out
+
+
 def run_many_graph_bfs(hc, oracle, count, seed, check, tuned=None):
     """One BFS on each of `count` Kronecker graphs (scale U{10..16}, edge
     factor U{8..32}) in one persistent launch vs one hc_bfs call per graph."""
@@ -822,6 +855,18 @@ def main():
         log("[batch] 64-source SSSP")
         batched = {"sssp64": run_batched_sssp(hc, oracle, args.batch_scale, args.degree,
                                               args.seed, args.check, args.tuned)}
+        log("[batch] 64-source BFS")
+        gb = hc.HostGraph.generate("kron", args.batch_scale, args.degree, weighted=False,
+                                   seed=args.seed)
+        with hc.DeviceGraph(gb) as dgb:
+            batched["bfs64"] = {f"kron{args.batch_scale}": run_batched_bfs(
+                hc, oracle, gb, dgb, f"kron{args.batch_scale} ef{args.degree}", args.check)}
+        del gb
+        if "uniform" in results:
+            ru = results["uniform"]
+            batched["bfs64"][f"uniform{args.scale}"] = run_batched_bfs(
+                hc, oracle, ru["g"], ru["dg"], f"uniform{args.scale} ef{args.degree}",
+                args.check)
         log("[batch] many-graph BFS")
         batched["bfs_many"] = run_many_graph_bfs(hc, oracle, args.many_graphs, 4242,
                                                  args.check, args.tuned)
