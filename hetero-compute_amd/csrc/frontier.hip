@@ -80,10 +80,22 @@ int64_t share_workspace_bound(int64_t n, int64_t m) {
     const int64_t sssp = frontier + 2 * n;                    // + 16-bit mirror
     const int64_t bfs = frontier + 8 * n + 4 * (3 * (int64_t)grid_blocks() + 1);  // + depth, first
     const int64_t multi = 3 * 4 * ((n + 3) / 4) + 4096;        // change flags
+    // Bit-parallel batched BFS: seen and three frontier masks (8 bytes per
+    // vertex each), three vertex lists, counters.
+    const int64_t msbfs = 4 * 8 * n + 3 * 4 * n + 4096;
     // Plans: id lists (graphs that are not degree-sorted) for the SSSP,
-    // bottom-up and batched sweeps, and their hub-chunk descriptors.
-    const int64_t plans = 3 * (4 * n + (int64_t)sizeof(HubChunk) * chunks + 4096);
-    return sssp + bfs + multi + plans;
+    // bottom-up and batched sweeps and the column-blocked sweeps (heavy and
+    // light rows, the batched BFS's), and their hub-chunk descriptors.
+    const int64_t plans = 6 * (4 * n + (int64_t)sizeof(HubChunk) * chunks + 4096);
+    return sssp + bfs + multi + msbfs + plans;
+}
+
+int64_t graph_copy_bound(int64_t n, int64_t m, bool weighted) {
+    // Column-blocked copies (at most two block sizes: the SSSP sweeps' and
+    // the batched BFS's): ids (and weights), at most 32 blocks of offsets
+    // of up to 8 bytes, and the epoch API's row bitmap.
+    const int64_t one = 4 * m * (weighted ? 2 : 1) + 32 * 8 * (n + 1) + 4 * ((n + 31) / 32) + 4096;
+    return 2 * one;
 }
 
 void frontier_ws_free(FrontierWs *ws) {

@@ -23,6 +23,9 @@ void clear_error();
 // first-neighbour arrays, the 16-bit distance mirror, batched-sweep flags,
 // plans), for a graph of n vertices and m directed edges.
 int64_t share_workspace_bound(int64_t n, int64_t m);
+// Upper bound on the device bytes of the copies one graph (with all its
+// shares) builds on first use: its column-blocked CSR copies.
+int64_t graph_copy_bound(int64_t n, int64_t m, bool weighted);
 
 // HC_OK when key = value would be accepted by hc_set_tuning (nothing set).
 int check_knob(const char *key, int64_t value);

@@ -115,7 +115,8 @@ int64_t graph_charge(const hc_server_s *s, const hc_host_graph_t *g, bool reside
     const int64_t n = g->num_nodes, m = g->num_edges;
     const int64_t off = (m < (int64_t(1) << 31) ? 4 : 8) * (n + 1);
     const int64_t csr = off + 4 * m * (g->weights ? 2 : 1);
-    const int64_t ws = hc::share_workspace_bound(n, m) * s->opts.slots;
+    const int64_t ws = hc::share_workspace_bound(n, m) * s->opts.slots +
+                       hc::graph_copy_bound(n, m, g->weights != nullptr);
     return resident ? ws : csr + ws;
 }
 

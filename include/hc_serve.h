@@ -102,9 +102,10 @@ int hc_server_set_graph_tuning(hc_server_t s, int32_t graph_id, const char *key,
                                int64_t value);
 
 /* Bytes the budget charges for graph g: for a dynamic graph, its CSR plus
- * the workspaces of one handle per slot (hc_graph_share), at their upper
- * bound; for a resident graph (whose CSR is outside the budget), the
- * per-slot workspaces only.  Resident graphs are charged when added. */
+ * the workspaces of one handle per slot (hc_graph_share) and the copies the
+ * graph builds on first use (column-blocked CSR), at their upper bounds;
+ * for a resident graph (whose CSR is outside the budget), the workspaces
+ * and copies only.  Resident graphs are charged when added. */
 int hc_server_graph_bytes(hc_server_t s, const hc_host_graph_t *g, int32_t resident,
                           int64_t *bytes);
 
