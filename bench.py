@@ -379,11 +379,12 @@ def run_batched_sssp(hc, oracle, scale, degree, seed, check, tuned=None):
     return out
 
 
-def run_batched_bfs(hc, oracle, g, dg, label, check):
+def run_batched_bfs(hc, oracle, g, dg, label, check, tuned=None):
     """64 BFS sources of a resident graph in one bit-parallel batch vs the
     same 64 sources run one after another and vs the batched sweep with
     unit weights (device time of each); every column certified."""
     sources = g.pick_sources(64)
+    knobs = apply_tuned(tuned, g, ("bfs_multi",), dg)
     buf = hc.DeviceArray(g.num_nodes * hc.MULTI_WIDTH, np.int32)
     hc.bfs_multi(dg, sources, out=buf, fetch=False)  # warm-up (plans, column blocks)
     r = hc.bfs_multi(dg, sources, out=buf, fetch=False)
@@ -400,7 +401,7 @@ def run_batched_bfs(hc, oracle, g, dg, label, check):
     seq = sum(hc.bfs(dg, int(s), depth_out=one, want_parents=False, fetch=False).stats["ms"]
               for s in sources)
     one.free()
-    out = {"graph": label, "batch_ms": round(r.stats["ms"], 3),
+    out = {"graph": label, "tuned_knobs": knobs, "batch_ms": round(r.stats["ms"], 3),
            "levels": {"push": r.stats["push_epochs"], "pull": r.stats["pull_epochs"]},
            "unit_weight_sweep_ms": round(old, 3), "sequential_ms": round(seq, 3),
            "speedup_vs_sequential": round(seq / r.stats["ms"], 2),
@@ -476,7 +477,7 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
     hosts = [big, unif] + dyn
     # Per-graph knobs the autotuner cached (by graph signature), for the
     # single-source and batched sweeps of each graph.
-    tuned_knobs = {name: apply_tuned(tuned, g, ("bfs", "sssp", "multi"), (srv, gid))
+    tuned_knobs = {name: apply_tuned(tuned, g, ("bfs", "sssp", "multi", "bfs_multi"), (srv, gid))
                    for name, g, gid in (("kron", big, gids[0]), ("uniform", unif, gids[1]))}
     results = []
     for rate in rates:
@@ -860,13 +861,14 @@ def main():
                                    seed=args.seed)
         with hc.DeviceGraph(gb) as dgb:
             batched["bfs64"] = {f"kron{args.batch_scale}": run_batched_bfs(
-                hc, oracle, gb, dgb, f"kron{args.batch_scale} ef{args.degree}", args.check)}
+                hc, oracle, gb, dgb, f"kron{args.batch_scale} ef{args.degree}", args.check,
+                args.tuned)}
         del gb
         if "uniform" in results:
             ru = results["uniform"]
             batched["bfs64"][f"uniform{args.scale}"] = run_batched_bfs(
                 hc, oracle, ru["g"], ru["dg"], f"uniform{args.scale} ef{args.degree}",
-                args.check)
+                args.check, args.tuned)
         log("[batch] many-graph BFS")
         batched["bfs_many"] = run_many_graph_bfs(hc, oracle, args.many_graphs, 4242,
                                                  args.check, args.tuned)

@@ -92,6 +92,11 @@ KNOB_SPACE = {
         "multi.hub_threshold": [2048, 4096, 16384],
         "multi.hub_chunk": [1024, 2048, 8192],
     },
+    # The 64-source bit-parallel BFS.
+    "bfs_multi": {
+        "msbfs.alpha": [8, 16, 32, 64, 128],
+        "msbfs.col_shift": [-1, 0, 18, 19, 20, 21],
+    },
     # The many-graph BFS (process-wide knobs; `dg` is a GraphBatch).
     "many": {
         "many.block": [256, 512, 1024],
@@ -146,6 +151,11 @@ def _run(dg, algorithm, sources):
             ms.append(st["ms"])
             outs.extend(d)
             break
+        elif algorithm == "bfs_multi":
+            r = hc.bfs_multi(dg, sources)
+            ms.append(r.stats["ms"])
+            outs.append(r.values)
+            break
         else:
             r = hc.sssp_multi(dg, sources)
             ms.append(r.stats["ms"])
@@ -165,7 +175,8 @@ def _same(a, b, algorithm):
 def tune_knobs(dg, algorithm: str, sources, space: dict | None = None, repeats: int = 2,
                log=None, per_graph: bool = False) -> dict:
     """Coordinate descent over the knobs of `algorithm` ("sssp", "bfs", "pr",
-    "multi" for the batched sweep, or "many" for the many-graph BFS, where
+    "multi" for the batched SSSP sweep, "bfs_multi" for the 64-source
+    bit-parallel BFS, or "many" for the many-graph BFS, where
     `dg` is a GraphBatch and `sources` one source per graph).  Every
     candidate's results must match the default configuration's exactly
     (PageRank: within 1e-6 L1); the best setting found is applied and
@@ -502,7 +513,7 @@ def main(argv=None):
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--degree", type=int, default=16)
     ap.add_argument("--algorithm", default="sssp",
-                    choices=["sssp", "bfs", "pr", "multi", "many"])
+                    choices=["sssp", "bfs", "pr", "multi", "bfs_multi", "many"])
     ap.add_argument("--graphs", type=int, default=512,
                     help="many: Kronecker graphs of scale U{10..16}, edge factor U{8..32}")
     ap.add_argument("--sources", type=int, default=4)
@@ -525,7 +536,7 @@ def main(argv=None):
         return
     g = hc.HostGraph.generate(args.kind, args.scale, args.degree,
                               weighted=args.algorithm in ("sssp", "multi"))
-    n_src = 64 if args.algorithm == "multi" else args.sources
+    n_src = 64 if args.algorithm in ("multi", "bfs_multi") else args.sources
     sources = g.pick_sources(n_src)
     with hc.DeviceGraph(g) as dg:
         res = tune_knobs(dg, args.algorithm, sources, log=print, per_graph=True)
