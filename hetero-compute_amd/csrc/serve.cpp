@@ -547,10 +547,11 @@ int hc_server_destroy(hc_server_t s) {
 namespace {
 
 // Device ms of single-source queries and of a 64-wide batch of each kind on
-// a resident graph (e.dev), on slot 0's stream: each run once to build plans
-// and workspaces, then timed (singles averaged over three sources).  Sources
-// are vertices with edges, spread over the id range.  Leaves the costs at 0
-// when the graph has no edges or the scratch matrix cannot be allocated.
+// a resident graph (e.dev), on a stream of its own (slots may be serving
+// other graphs meanwhile): each run once to build plans and workspaces, then
+// timed (singles averaged over three sources).  Sources are vertices with
+// edges, spread over the id range.  Leaves the costs at 0 when the graph has
+// no edges or the scratch matrix or stream cannot be allocated.
 void calibrate_graph(hc_server_s *s, GraphEntry &e) {
     const int32_t n = e.host.num_nodes;
     std::vector<int32_t> src;
@@ -570,7 +571,12 @@ void calibrate_graph(hc_server_s *s, GraphEntry &e) {
         return;
     }
     int32_t *d = static_cast<int32_t *>(buf);
-    hipStream_t st = s->streams[0];
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(buf);
+        return;
+    }
     hc_run_stats_t rs{};
     for (int kind = 0; kind < 2; kind++) {
         if (kind == HC_QUERY_SSSP && !e.host.weights) continue;
@@ -598,6 +604,7 @@ void calibrate_graph(hc_server_s *s, GraphEntry &e) {
         e.batch_ms[kind] = rs.ms;
     }
     (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
     (void)hipFree(buf);
     hc::clear_error();
 }

@@ -29,11 +29,17 @@ int hc_sssp_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources,
                   int32_t *dist, int32_t max_epochs, hc_run_stats_t *stats,
                   hc_stream_t stream);
 
-/* Breadth-first depths from up to 64 sources at once: the same batched
- * sweep with every edge weighing 1 (weights, if any, are ignored).
+/* Breadth-first depths from up to 64 sources at once (bfs_gpu,
+ * src/kernels/gpu/bfs.cuh:21-95, batched), level-synchronous over 64-bit
+ * masks per vertex: the sources that reached it and, per level, those whose
+ * frontier it is on; each level pushes from the frontier or pulls over all
+ * rows by the frontier's edge count (knob msbfs.on = 0: the batched SSSP
+ * sweep with every edge weighing 1 instead).  Weights, if any, are ignored.
  * depth[v * 64 + q] is the BFS level of v from sources[q], HC_INF_DIST when
- * unreachable (hc_bfs reports -1 there); identical to hc_bfs's depths
- * otherwise. */
+ * unreachable (hc_bfs reports -1 there) or, with max_epochs > 0, deeper
+ * than max_epochs levels; identical to hc_bfs's depths otherwise.  Columns
+ * q >= num_sources stay HC_INF_DIST.  stats->ms: HIP events around the
+ * levels, the fill of the matrix included. */
 int hc_bfs_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources,
                  int32_t *depth, int32_t max_epochs, hc_run_stats_t *stats,
                  hc_stream_t stream);

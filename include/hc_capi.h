@@ -109,8 +109,9 @@ typedef struct hc_run_stats {
                               the device's constant-rate wall clock, read by
                               the init kernel and by the controller of the
                               last epoch.  Host-driven loops (hc_sssp_pull,
-                              hc_pr_pull, hc_bfs_many): HIP events around the
-                              launches (hc_bfs_many_noalloc: host clock).
+                              hc_pr_pull, hc_bfs_many, hc_bfs_multi): HIP
+                              events around the launches
+                              (hc_bfs_many_noalloc: host clock).
                               Heterogeneous runs: host clock, copies
                               included.                                    */
     int32_t epochs;        /* epochs executed                                 */
@@ -266,6 +267,13 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   multi.grid_per_cu      workgroups per CU of the batched sweep (16)
  *   multi.block            threads per workgroup of the batched sweep (128,
  *                          256, 512 or 1024; 256)
+ *   msbfs.on               hc_bfs_multi runs bit-parallel levels (1, default)
+ *                          or the batched sweep with unit weights (0)
+ *   msbfs.alpha            ... a bit-parallel level pushes while its frontier
+ *                          holds fewer than m / this edges, else pulls (32)
+ *   msbfs.col_shift        ... its pulls gather column blocks of 2^this
+ *                          vertices (0 off; -1 default: four blocks on flat
+ *                          graphs, sixteen on skewed ones, >= 2^16 each)
  *   multi.hub_threshold / multi.hub_chunk / multi.wave_threshold /
  *   multi.g16_threshold    degree classes of the batched sweep
  *   pull.col_shift         pull sweeps of whole graphs with at least two
