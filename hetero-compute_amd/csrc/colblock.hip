@@ -30,10 +30,13 @@ ColBlockSlot::~ColBlockSlot() {
 }
 
 // cnt[k][u] = edges of row u whose neighbour is in block k; cnt[k][n] = 0.
+// Per-thread counters in dynamic LDS, K x kBlock words (K blocks only, so
+// few blocks leave room for more workgroups per CU).
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void cb_count_kernel(GraphView<OffT> g, int shift, int K,
                                                           OffT *cnt) {
-    __shared__ uint32_t c[kMaxColBlocks][kBlock];
+    extern __shared__ uint32_t c_dyn[];
+    uint32_t(*c)[kBlock] = reinterpret_cast<uint32_t(*)[kBlock]>(c_dyn);
     const int t = threadIdx.x;
     const size_t row_stride = (size_t)g.n + 1;
     const int stride = gridDim.x * kBlock;
@@ -65,12 +68,30 @@ __global__ __launch_bounds__(kBlock) void cb_scan_reduce_kernel(const OffT *x, s
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-// Exclusive scan of the chunk sums in place (one thread: a few thousand).
+// Exclusive scan of the chunk sums in place, by one workgroup: each thread
+// sums a contiguous run, the run totals are scanned in LDS, then each thread
+// rewrites its run.
 template <typename OffT>
-__global__ void cb_scan_partials_kernel(OffT *part, int nparts) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    OffT run = 0;
-    for (int i = 0; i < nparts; i++) {
+__global__ __launch_bounds__(kBlock) void cb_scan_partials_kernel(OffT *part, int nparts) {
+    __shared__ OffT s_run[kBlock];
+    const int t = threadIdx.x;
+    const int per = (nparts + kBlock - 1) / kBlock;
+    const int b = std::min(nparts, t * per), e = std::min(nparts, b + per);
+    OffT tot = 0;
+    for (int i = b; i < e; i++) tot += part[i];
+    s_run[t] = tot;
+    __syncthreads();
+    if (t == 0) {
+        OffT run = 0;
+        for (int i = 0; i < kBlock; i++) {
+            const OffT x = s_run[i];
+            s_run[i] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    OffT run = s_run[t];
+    for (int i = b; i < e; i++) {
         const OffT v = part[i];
         part[i] = run;
         run += v;
@@ -109,12 +130,14 @@ __global__ __launch_bounds__(kBlock) void cb_scan_apply_kernel(OffT *x, size_t l
     }
 }
 
-// Copies each row's edges into their block slices, in CSR order.
+// Copies each row's edges into their block slices, in CSR order (per-thread
+// slice cursors in dynamic LDS, K x kBlock words).
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void cb_scatter_kernel(GraphView<OffT> g, int shift, int K,
                                                             const OffT *off, int32_t *adj,
                                                             int32_t *wt) {
-    __shared__ uint32_t c[kMaxColBlocks][kBlock];
+    extern __shared__ uint32_t c_dyn[];
+    uint32_t(*c)[kBlock] = reinterpret_cast<uint32_t(*)[kBlock]>(c_dyn);
     const int t = threadIdx.x;
     const size_t row_stride = (size_t)g.n + 1;
     const int stride = gridDim.x * kBlock;
@@ -155,12 +178,13 @@ static int build_col_blocks(hc_graph_s *g, int shift, int K, hipStream_t s, ColB
     uint32_t *seen = reinterpret_cast<uint32_t *>(base + sz_off + sz_adj + sz_wt);
     const GraphView<OffT> v = view<OffT>(g);
     const int rgrid = std::max(1, std::min(grid_blocks(), (int)((n + kBlock - 1) / kBlock)));
-    cb_count_kernel<OffT><<<std::max(rgrid, (K + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+    const size_t lds = sizeof(uint32_t) * (size_t)K * kBlock;
+    cb_count_kernel<OffT><<<std::max(rgrid, (K + kBlock - 1) / kBlock), kBlock, lds, s>>>(
         v, shift, K, off);
     cb_scan_reduce_kernel<OffT><<<nparts, kBlock, 0, s>>>(off, len, part);
-    cb_scan_partials_kernel<OffT><<<1, 64, 0, s>>>(part, nparts);
+    cb_scan_partials_kernel<OffT><<<1, kBlock, 0, s>>>(part, nparts);
     cb_scan_apply_kernel<OffT><<<nparts, kBlock, 0, s>>>(off, len, part);
-    cb_scatter_kernel<OffT><<<rgrid, kBlock, 0, s>>>(v, shift, K, off, adj, wt);
+    cb_scatter_kernel<OffT><<<rgrid, kBlock, lds, s>>>(v, shift, K, off, adj, wt);
     int rc = HC_OK;
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
         rc = fail(HC_ERR_HIP, "column-block build failed");
