@@ -466,12 +466,15 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
     graphs uploaded on demand.  Latency: arrival -> result in HBM.  Every
     result on an uploaded graph is compared with the oracle; on the resident
     graphs `check` results per graph and kind, spread over the run, are
-    certified exact (each certified column costs an O(m) pass on the host)."""
+    certified exact (each certified column costs an O(m) pass on the host),
+    and every result's device checksum (summed by its slot after the sweep)
+    is compared, after the rate, with that of the same query run alone."""
     import threading
     rng = np.random.RandomState(seed)
     dyn = [hc.HostGraph.generate("kron", int(s), 16, weighted=True, seed=seed + int(s))
            for s in rng.randint(12, 19, size=8)]
-    srv = hc.Server(slots=slots, max_batch=64, min_batch_bfs=min_batch_bfs)
+    srv = hc.Server(slots=slots, max_batch=64, min_batch_bfs=min_batch_bfs,
+                    checksum=check > 0)
     gids = [srv.add_graph(big), srv.add_graph(unif)] + [srv.add_graph(g, resident=False)
                                                         for g in dyn]
     hosts = [big, unif] + dyn
@@ -480,6 +483,7 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
     tuned_knobs = {name: apply_tuned(tuned, g, ("bfs", "sssp", "multi", "bfs_multi"), (srv, gid))
                    for name, g, gid in (("kron", big, gids[0]), ("uniform", unif, gids[1]))}
     results = []
+    alone = None
     for rate in rates:
         qs = []
         for _ in range(queries):
@@ -512,7 +516,7 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
         th = threading.Thread(target=submitter)
         th.start()
         lat, edges, batches, checked, exact, errors = [], 0, [], 0, True, []
-        kept_small, kept_big = [], []
+        kept_small, kept_big, sums = [], [], []
         done_at = np.zeros(queries)
         sweeps = {}
         for i in range(queries):
@@ -539,9 +543,32 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
                 if seen[c] % stride[c] == 0 and seen[c] // stride[c] < check:
                     kept_big.append((k, kind, src, srv.fetch(tickets[i])))
                 seen[c] += 1
+                sums.append((k, kind, src, srv.checksum(tickets[i])))
             srv.release(tickets[i])
         wall = time.perf_counter() - start
         th.join()
+        # Every resident result against the same query run alone, on a
+        # server that never batches (single-source path, default knobs),
+        # with at most 64 queries outstanding.
+        alone_runs = alone_equal = 0
+        if sums:
+            if alone is None:
+                alone = hc.Server(slots=slots, max_batch=1, calibrate=-1, checksum=True)
+                alone_ids = [alone.add_graph(big), alone.add_graph(unif)]
+            window = []
+            for j, (k, kind, src, h) in enumerate(sums):
+                window.append((alone.submit(alone_ids[k], kind, src), h))
+                if len(window) == 64 or j == len(sums) - 1:
+                    for t, h0 in window:
+                        info = alone.wait(t)
+                        ok = info["status"] == 0 and info["batch_size"] == 1 and \
+                            alone.checksum(t) == h0
+                        alone.release(t)
+                        alone_runs += 1
+                        alone_equal += int(ok)
+                    window = []
+        exact &= alone_equal == alone_runs
+        sums = None
         for k, kind, src, got in kept_small:
             g = hosts[k]
             ref = (oracle.bfs_depth(g.index, g.neighbors, src) if kind == hc.QUERY_BFS
@@ -588,6 +615,8 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
                                             [r / b for b, r in v])), 3)}
                                     for c, v in sorted(sweeps.items())},
                         "checked_vs_oracle": checked, "certified_resident": certified,
+                        "resident_checked_vs_alone": alone_runs,
+                        "resident_equal_to_alone": alone_equal,
                         "bit_exact": exact,
                         "failed": len(errors), "errors": errors[:3]})
         log(f"[serve] {rate} q/s: achieved {results[-1]['achieved_qps']} q/s, "
@@ -596,6 +625,8 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
     costs = {name: srv.graph_costs(gid) for name, gid in (("kron", gids[0]),
                                                            ("uniform", gids[1]))}
     srv.close()
+    if alone is not None:
+        alone.close()
     return {"slots": slots, "tuned_knobs": tuned_knobs, "measured_costs_ms": costs,
             "graphs": {"kron": big.num_nodes, "uniform": unif.num_nodes,
                        "dynamic_scales": [int(np.log2(g.num_nodes)) for g in dyn]},

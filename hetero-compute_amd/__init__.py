@@ -98,7 +98,8 @@ class ScheduleBlock(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("memory_budget", C.c_int64),
                 ("min_batch_sssp", C.c_int32), ("min_batch_bfs", C.c_int32),
-                ("max_wait_ms", C.c_int32), ("calibrate", C.c_int32)]
+                ("max_wait_ms", C.c_int32), ("calibrate", C.c_int32),
+                ("checksum", C.c_int32)]
 
 
 class QueryInfo(C.Structure):
@@ -186,6 +187,9 @@ _SIGS = {
                             C.c_int),
     "hc_graph_share": ([_P, C.POINTER(_P)], C.c_int),
     "hc_multi_extract": ([_P, C.c_int32, C.c_int32, C.c_int32, _P, _P], C.c_int),
+    "hc_checksum_i32": ([_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                         C.POINTER(C.c_uint64), _P], C.c_int),
+    "hc_checksum_multi": ([_P, C.c_int32, C.c_int32, C.c_int32, _P, _P], C.c_int),
     "hc_server_create": ([C.POINTER(ServerOpts), C.POINTER(_P)], C.c_int),
     "hc_server_destroy": ([_P], C.c_int),
     "hc_server_add_graph": ([_P, C.POINTER(HostGraphStruct), C.c_int32,
@@ -196,6 +200,7 @@ _SIGS = {
     "hc_server_wait": ([_P, C.c_int64, C.POINTER(QueryInfo)], C.c_int),
     "hc_server_fetch": ([_P, C.c_int64, _P], C.c_int),
     "hc_server_release": ([_P, C.c_int64], C.c_int),
+    "hc_server_checksum": ([_P, C.c_int64, C.POINTER(C.c_uint64)], C.c_int),
     "hc_server_get_stats": ([_P, C.POINTER(ServerStats)], C.c_int),
     "hc_server_graph_bytes": ([_P, C.POINTER(HostGraphStruct), C.c_int32,
                                C.POINTER(C.c_int64)], C.c_int),
@@ -746,6 +751,27 @@ def bfs_multi(g: DeviceGraph, sources, out: DeviceArray | None = None,
     return Result(vals, st.as_dict())
 
 
+def checksum(d: DeviceArray, num_nodes: int, stride: int = 1, column: int = 0,
+             unreached_minus1: bool = False) -> int:
+    """hc_checksum_i32: order-independent 64-bit checksum of num_nodes int32
+    values d[v * stride + column] (stride 1, or MULTI_WIDTH for a column of a
+    batched result), computed on the device."""
+    h = C.c_uint64()
+    check(lib().hc_checksum_i32(d.ptr, int(num_nodes), int(stride), int(column),
+                                int(unreached_minus1), C.byref(h), None))
+    return h.value
+
+
+def checksum_multi(d: DeviceArray, num_nodes: int, num_columns: int,
+                   unreached_minus1: bool = False) -> np.ndarray:
+    """hc_checksum_multi: the checksum() of columns 0..num_columns-1 of a
+    batched [num_nodes][MULTI_WIDTH] result, in one pass."""
+    out = np.zeros(int(num_columns), np.uint64)
+    check(lib().hc_checksum_multi(d.ptr, int(num_nodes), int(num_columns),
+                                  int(unreached_minus1), _ptr(out), None))
+    return out
+
+
 def epoch_trace(g: DeviceGraph, algorithm: str) -> list[dict]:
     """Per-epoch record of the last sssp/bfs run on g."""
     t = EpochTrace()
@@ -925,15 +951,16 @@ class Server:
 
     def __init__(self, slots: int = 4, max_batch: int = 64, memory_budget: int = 0,
                  min_batch_sssp: int = 0, min_batch_bfs: int = 0, max_wait_ms: int = 0,
-                 calibrate: int = 0):
+                 calibrate: int = 0, checksum: bool = False):
         """min_batch_*: queries of one graph and kind that must wait before
         they run as one batched sweep (0: from the costs measured when a
         resident graph is added, else 8 and 64); max_wait_ms: how long
         fast-arriving queries may wait to gather a batch (0: the default,
-        500; -1: never); calibrate -1: measure no costs."""
+        500; -1: never); calibrate -1: measure no costs; checksum: each
+        slot sums every result of its sweep on the device (checksum())."""
         self.h = _P()
         o = ServerOpts(slots, max_batch, memory_budget, min_batch_sssp, min_batch_bfs,
-                       max_wait_ms, calibrate)
+                       max_wait_ms, calibrate, int(checksum))
         check(lib().hc_server_create(C.byref(o), C.byref(self.h)))
         self._graphs = []
 
@@ -975,6 +1002,13 @@ class Server:
         out = np.empty(info["num_nodes"], np.int32)
         check(lib().hc_server_fetch(self.h, ticket, _ptr(out)))
         return out
+
+    def checksum(self, ticket: int) -> int:
+        """hc_checksum_i32 of the query's result as fetch() returns it,
+        computed where the result lies (hc_server_checksum)."""
+        h = C.c_uint64()
+        check(lib().hc_server_checksum(self.h, ticket, C.byref(h)))
+        return h.value
 
     def release(self, ticket: int):
         check(lib().hc_server_release(self.h, ticket))

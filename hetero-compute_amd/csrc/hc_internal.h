@@ -27,6 +27,20 @@ int64_t share_workspace_bound(int64_t n, int64_t m);
 // shares) builds on first use: its column-blocked CSR copies.
 int64_t graph_copy_bound(int64_t n, int64_t m, bool weighted);
 
+// Mix of (vertex, value) for result checksums (splitmix64's finaliser,
+// Vigna): the checksum of an array is the wrapping sum of mix over its
+// entries, so any order of summation gives the same value.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline uint64_t checksum_mix(uint64_t v, int32_t x) {
+    uint64_t z = (v << 32) ^ (uint64_t)(uint32_t)x;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
 // HC_OK when key = value would be accepted by hc_set_tuning (nothing set).
 int check_knob(const char *key, int64_t value);
 

@@ -24,6 +24,8 @@
 // kEpochsPerGraph epochs that the host relaunches until the controller
 // reports convergence.
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "../../include/hc_batch.h"
 #include "device_common.h"
@@ -535,6 +537,101 @@ __global__ void multi_extract_kernel(const int32_t *__restrict__ mat, int32_t n,
     }
 }
 
+// Order-independent checksum of one column of a vertex-major matrix (stride
+// 64) or of a plain array (stride 1): the 64-bit wrapping sum over v of
+// mix(v, value), value -1 for HC_INF_DIST when minus1 is set (BFS depths
+// of a batch, so that they match a single-source run's).
+__global__ void checksum_kernel(const int32_t *__restrict__ data, int32_t n, int stride,
+                                int col, int minus1, unsigned long long *out) {
+    unsigned long long h = 0;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < (size_t)n; v += step) {
+        int32_t x = data[v * (size_t)stride + col];
+        if (minus1 && x == HC_INF_DIST) x = -1;
+        h += checksum_mix((uint64_t)v, x);
+    }
+    const unsigned long long t = block_sum_u64(h);
+    if (threadIdx.x == 0) atomicAdd(out, t);
+}
+
+// The same sums for every column of a vertex-major [n][64] matrix in one
+// pass: lane l of each wave reads column l of kSumRows consecutive rows
+// (256 contiguous bytes a row), the waves' sums meet in LDS and one atomic
+// add per column and workgroup lands in out.
+constexpr int kSumRows = 4;
+
+__global__ void checksum_cols_kernel(const int32_t *__restrict__ m, int32_t n, int ncols,
+                                     int minus1, unsigned long long *out) {
+    __shared__ unsigned long long part[kWavesPerBlock][kQ];
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    unsigned long long h = 0;
+    const size_t step = (size_t)gridDim.x * kWavesPerBlock * kSumRows;
+    for (size_t r0 = ((size_t)blockIdx.x * kWavesPerBlock + w) * kSumRows; r0 < (size_t)n;
+         r0 += step) {
+        int32_t x[kSumRows];
+#pragma unroll
+        for (int i = 0; i < kSumRows; i++)
+            x[i] = r0 + i < (size_t)n ? m[(r0 + i) * kQ + lane] : 0;
+#pragma unroll
+        for (int i = 0; i < kSumRows; i++) {
+            if (r0 + i >= (size_t)n) break;
+            const int32_t v = minus1 && x[i] == HC_INF_DIST ? -1 : x[i];
+            h += checksum_mix((uint64_t)(r0 + i), v);
+        }
+    }
+    part[w][lane] = h;
+    __syncthreads();
+    if (threadIdx.x < ncols) {
+        unsigned long long t = 0;
+#pragma unroll
+        for (int i = 0; i < kWavesPerBlock; i++) t += part[i][threadIdx.x];
+        atomicAdd(&out[threadIdx.x], t);
+    }
+}
+
+__global__ void zero_words_kernel(unsigned long long *p, int count) {
+    if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
+}
+
+// Device accumulators of the checksum entry points: 64 words per call,
+// taken from a per-device free list (allocated on first need, kept until
+// the process ends), so concurrent calls on different streams do not wait
+// for each other.
+struct SumWords {
+    unsigned long long *p = nullptr;
+    int dev = -1;
+    static std::mutex mu;
+    static std::vector<unsigned long long *> free_list[64];
+
+    int take() {
+        HC_HIP(hipGetDevice(&dev));
+        HC_CHECK_ARG(dev >= 0 && dev < 64, "device %d above the 64 supported", dev);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!free_list[dev].empty()) {
+                p = free_list[dev].back();
+                free_list[dev].pop_back();
+                return HC_OK;
+            }
+        }
+        HC_HIP(hipMalloc(reinterpret_cast<void **>(&p), kQ * sizeof(unsigned long long)));
+        return HC_OK;
+    }
+    // Copies count sums to the host and waits for them.
+    int fetch(uint64_t *dst, int count, hipStream_t s) {
+        HC_HIP(hipMemcpyAsync(dst, p, sizeof(uint64_t) * count, hipMemcpyDeviceToHost, s));
+        HC_HIP(hipStreamSynchronize(s));
+        return HC_OK;
+    }
+    ~SumWords() {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu);
+        free_list[dev].push_back(p);
+    }
+};
+std::mutex SumWords::mu;
+std::vector<unsigned long long *> SumWords::free_list[64];
+
 static int multi_entry(hc_graph_t g, const int32_t *sources, int32_t num_sources,
                        int32_t *dist, int32_t max_epochs, int unit, hc_run_stats_t *stats,
                        hc_stream_t stream) {
@@ -565,6 +662,46 @@ int hc_sssp_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources, int
 int hc_bfs_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources, int32_t *depth,
                  int32_t max_epochs, hc_run_stats_t *stats, hc_stream_t stream) {
     return multi_entry(g, sources, num_sources, depth, max_epochs, 1, stats, stream);
+}
+
+int hc_checksum_i32(const int32_t *data, int32_t num_nodes, int32_t stride, int32_t column,
+                    int32_t unreached_minus1, uint64_t *sum, hc_stream_t stream) {
+    HC_CHECK_ARG(data && sum, "NULL argument");
+    HC_CHECK_ARG(num_nodes >= 0 && (stride == 1 || stride == kQ), "stride must be 1 or %d", kQ);
+    HC_CHECK_ARG(column >= 0 && column < stride, "bad column %d", column);
+    HC_TRY(ensure_init());
+    *sum = 0;
+    if (num_nodes == 0) return HC_OK;
+    SumWords acc;
+    HC_TRY(acc.take());
+    hipStream_t s = pick_stream(stream);
+    const int grid = std::max(1, std::min(grid_blocks(), (num_nodes + kBlock - 1) / kBlock));
+    zero_words_kernel<<<1, kQ, 0, s>>>(acc.p, 1);
+    checksum_kernel<<<grid, kBlock, 0, s>>>(data, num_nodes, stride, column, unreached_minus1,
+                                            acc.p);
+    HC_HIP(hipGetLastError());
+    return acc.fetch(sum, 1, s);
+}
+
+int hc_checksum_multi(const int32_t *matrix, int32_t num_nodes, int32_t num_columns,
+                      int32_t unreached_minus1, uint64_t *sums, hc_stream_t stream) {
+    HC_CHECK_ARG(matrix && sums, "NULL argument");
+    HC_CHECK_ARG(num_nodes >= 0 && num_columns >= 1 && num_columns <= kQ,
+                 "num_columns must be in 1..%d", kQ);
+    HC_TRY(ensure_init());
+    for (int c = 0; c < num_columns; c++) sums[c] = 0;
+    if (num_nodes == 0) return HC_OK;
+    SumWords acc;
+    HC_TRY(acc.take());
+    hipStream_t s = pick_stream(stream);
+    const int rows_per_block = kWavesPerBlock * kSumRows;
+    const int grid =
+        std::max(1, std::min(grid_blocks(), (num_nodes + rows_per_block - 1) / rows_per_block));
+    zero_words_kernel<<<1, kQ, 0, s>>>(acc.p, num_columns);
+    checksum_cols_kernel<<<grid, kBlock, 0, s>>>(matrix, num_nodes, num_columns,
+                                                 unreached_minus1, acc.p);
+    HC_HIP(hipGetLastError());
+    return acc.fetch(sums, num_columns, s);
 }
 
 int hc_multi_extract(const int32_t *matrix, int32_t num_nodes, int32_t column,

@@ -53,6 +53,8 @@ struct Query {
     std::string error;  // message of a failed run
     std::shared_ptr<Result> res;
     int32_t column = -1;  // column of a batched result, -1 for a single one
+    bool has_sum = false;  // sum computed after the sweep (opts.checksum)
+    uint64_t sum = 0;
 };
 
 // Arrival rate of one (graph, kind) class: moving average of the gaps
@@ -438,6 +440,17 @@ void worker(hc_server_s *s, int slot) {
                 status = hc::fail(HC_ERR_HIP, "stream synchronisation failed");
         }
         const double t_done = now_ms();
+        // Result checksums, on the slot's stream while the slot still
+        // holds the sweep.
+        uint64_t sums[kWidth] = {};
+        const bool summed = status == HC_OK && s->opts.checksum;
+        if (summed) {
+            const int minus1 = kind == HC_QUERY_BFS ? 1 : 0;
+            status = width == 1 ? hc_checksum_i32(buf, (int32_t)g.host.num_nodes, 1, 0, 0, sums,
+                                                  stream)
+                                : hc_checksum_multi(buf, (int32_t)g.host.num_nodes,
+                                                    (int32_t)batch.size(), minus1, sums, stream);
+        }
 
         lk.lock();
         s->running--;
@@ -460,6 +473,8 @@ void worker(hc_server_s *s, int slot) {
             if (status == HC_OK) {
                 q.res = res;
                 q.column = width == 1 ? -1 : (int32_t)i;
+                q.has_sum = summed;
+                q.sum = sums[i];
             }
         }
         s->st.batches++;
@@ -488,6 +503,7 @@ int hc_server_create(const hc_server_opts_t *opts, hc_server_t *out) {
     s->opts.min_batch_bfs = opts && opts->min_batch_bfs > 0 ? opts->min_batch_bfs : kWidth;
     s->opts.max_wait_ms = opts && opts->max_wait_ms != 0 ? opts->max_wait_ms : 500;
     s->opts.calibrate = opts ? opts->calibrate : 0;
+    s->opts.checksum = opts ? opts->checksum : 0;
     if (s->opts.slots > 16 || s->opts.max_batch > kWidth) {
         delete s;
         return hc::fail(HC_ERR_INVALID, "slots must be <= 16 and max_batch <= %d", kWidth);
@@ -793,6 +809,47 @@ int hc_server_fetch(hc_server_t s, int64_t ticket, int32_t *host_out) {
         hipStreamSynchronize(s->fetch_stream) != hipSuccess)
         return hc::fail(HC_ERR_HIP, "result copy failed");
     return HC_OK;
+}
+
+int hc_server_checksum(hc_server_t s, int64_t ticket, uint64_t *sum) {
+    HC_CHECK_ARG(s && sum, "NULL argument");
+    std::shared_ptr<Result> res;
+    int32_t column = -1, kind = 0;
+    int64_t n = 0;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->queries.find(ticket);
+        if (it == s->queries.end())
+            return hc::fail(HC_ERR_NOT_FOUND, "unknown ticket %lld", (long long)ticket);
+        s->cv_done.wait(lk, [&] { return it->second.done; });
+        if (it->second.status != HC_OK)
+            return hc::fail(it->second.status, "query %lld failed: %s", (long long)ticket,
+                            it->second.error.c_str());
+        if (it->second.has_sum) {
+            *sum = it->second.sum;
+            return HC_OK;
+        }
+        res = it->second.res;
+        column = it->second.column;
+        kind = it->second.kind;
+        n = s->graphs[it->second.graph].host.num_nodes;
+    }
+    const bool minus1 = kind == HC_QUERY_BFS;
+    std::lock_guard<std::mutex> rl(res->mu);
+    if (!res->p) {  // spilled to host: the same sum on the CPU
+        uint64_t h = 0;
+        for (int64_t v = 0; v < n; v++) {
+            int32_t x = column >= 0 ? res->host[(size_t)v * kWidth + column] : res->host[v];
+            if (column >= 0 && minus1 && x == HC_INF_DIST) x = -1;
+            h += hc::checksum_mix((uint64_t)v, x);
+        }
+        *sum = h;
+        return HC_OK;
+    }
+    std::lock_guard<std::mutex> fl(s->fetch_mu);
+    return hc_checksum_i32(res->p, (int32_t)n, column >= 0 ? kWidth : 1,
+                           column >= 0 ? column : 0, column >= 0 && minus1 ? 1 : 0, sum,
+                           s->fetch_stream);
 }
 
 int hc_server_release(hc_server_t s, int64_t ticket) {

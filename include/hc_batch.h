@@ -50,6 +50,22 @@ int hc_bfs_multi(hc_graph_t g, const int32_t *sources, int32_t num_sources,
 int hc_multi_extract(const int32_t *matrix, int32_t num_nodes, int32_t column,
                      int32_t unreached_minus1, int32_t *out, hc_stream_t stream);
 
+/* Order-independent 64-bit checksum of a result (no counterpart in the
+ * reference; for checking results against a run of the same query alone
+ * without copying them to the host): the wrapping sum over v < num_nodes of
+ * mix(v, data[v * stride + column]), stride 1 (a single-source array) or
+ * 64 (a column of a batched matrix), HC_INF_DIST read as -1 when
+ * unreached_minus1 is set; mix is splitmix64's finaliser of
+ * (v << 32) ^ (uint32_t)value.  Waits for the result. */
+int hc_checksum_i32(const int32_t *data, int32_t num_nodes, int32_t stride, int32_t column,
+                    int32_t unreached_minus1, uint64_t *sum, hc_stream_t stream);
+
+/* hc_checksum_i32 of columns 0 .. num_columns-1 of a batched [num_nodes][64]
+ * result, all in one pass over the matrix (sums[c] for column c).  Waits
+ * for the sums.  No counterpart in the reference. */
+int hc_checksum_multi(const int32_t *matrix, int32_t num_nodes, int32_t num_columns,
+                      int32_t unreached_minus1, uint64_t *sums, hc_stream_t stream);
+
 /* Many graphs resident in HBM as one batch: concatenated 4-byte offsets and
  * neighbour ids (each graph below 2^32 edges), descriptors and a
  * longest-first work order.  Graph i's vertex v is batch vertex

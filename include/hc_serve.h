@@ -51,6 +51,12 @@ typedef struct hc_server_opts {
                                from ceil(batch ms / single ms) waiting queries
                                (never, when that exceeds max_batch).  < 0: no
                                timing, the fixed defaults above                */
+    int32_t checksum;       /* 1: after each sweep, before its slot takes more
+                               work, the slot computes the hc_checksum_i32 of
+                               every result of the sweep on the device (one
+                               pass over a batched result) and keeps it for
+                               hc_server_checksum; the sweep's latency_ms
+                               ends before this pass (default 0)               */
 } hc_server_opts_t;
 
 typedef struct hc_query_info {
@@ -117,6 +123,12 @@ int hc_server_wait(hc_server_t s, int64_t ticket, hc_query_info_t *info);
 /* Copies the finished query's result to host memory: num_nodes int32 (BFS
  * depth, -1 unreached; SSSP distance, HC_INF_DIST unreachable). */
 int hc_server_fetch(hc_server_t s, int64_t ticket, int32_t *host_out);
+/* The hc_checksum_i32 checksum of the finished query's result, as
+ * hc_server_fetch would return it (BFS unreached read as -1): the value the
+ * slot computed after the sweep when opts.checksum is set, else computed
+ * now where the result lies, without copying it to the host.  Waits for the
+ * query.  No counterpart in the reference. */
+int hc_server_checksum(hc_server_t s, int64_t ticket, uint64_t *sum);
 /* Frees the query's result; the ticket becomes invalid. */
 int hc_server_release(hc_server_t s, int64_t ticket);
 int hc_server_get_stats(hc_server_t s, hc_server_stats_t *out);

@@ -48,3 +48,21 @@ def golden_files():
 
 def load_golden(name):
     return dict(np.load(os.path.join(GOLDEN_DIR, name), allow_pickle=False))
+
+
+def _mix64(z):
+    """splitmix64's output function (Vigna) on a uint64 array, wrapping."""
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def result_checksum(values) -> int:
+    """The library's result checksum restated: the wrapping 64-bit sum over
+    v of splitmix64-mix((v << 32) ^ uint32(values[v]))."""
+    x = np.ascontiguousarray(values, np.int32).view(np.uint32).astype(np.uint64)
+    v = np.arange(x.size, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return int(_mix64((v << np.uint64(32)) ^ x).sum(dtype=np.uint64))

@@ -332,3 +332,32 @@ def test_bfs_multi_bitwise_column_blocks(hc, gpu, oracle, reset_knobs, shift, ki
     for q in (0, 32, 63):
         assert np.array_equal(new.values[q], oracle.bfs_depth(g.index, g.neighbors,
                                                                int(sources[q]))), q
+
+
+def test_checksum_kernels_match_restatement(hc, gpu):
+    """hc_checksum_i32 (stride 1 and a column of a 64-wide matrix) and
+    hc_checksum_multi against the restated sum, HC_INF_DIST read as -1 where
+    asked; row counts not a multiple of the kernel's row groups."""
+    from conftest import result_checksum
+    rng = np.random.RandomState(4)
+    for n in (1, 5, 1023, 70001):
+        a = rng.randint(-3, 1 << 20, size=n).astype(np.int32)
+        a[rng.rand(n) < 0.2] = hc.INF_DIST
+        d = hc.DeviceArray.from_host(a)
+        assert hc.checksum(d, n) == result_checksum(a), n
+        m = rng.randint(0, 1 << 12, size=(n, hc.MULTI_WIDTH)).astype(np.int32)
+        m[rng.rand(n, hc.MULTI_WIDTH) < 0.3] = hc.INF_DIST
+        dm = hc.DeviceArray.from_host(m.ravel())
+        cols = 64 if n % 2 else 37
+        got = hc.checksum_multi(dm, n, cols, unreached_minus1=True)
+        ref = [result_checksum(np.where(m[:, c] == hc.INF_DIST, -1, m[:, c])) for c in range(cols)]
+        assert [int(x) for x in got] == ref, n
+        raw = hc.checksum_multi(dm, n, 3)
+        assert [int(x) for x in raw] == [result_checksum(m[:, c]) for c in range(3)]
+        for c in (0, cols - 1):
+            assert hc.checksum(dm, n, stride=hc.MULTI_WIDTH, column=c,
+                               unreached_minus1=True) == ref[c], (n, c)
+    empty = hc.DeviceArray(1, np.int32)
+    assert hc.checksum(empty, 0) == 0
+    with pytest.raises(hc.HcError):
+        hc.checksum(empty, 1, stride=2)

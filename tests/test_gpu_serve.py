@@ -6,6 +6,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from conftest import result_checksum
+
 pytestmark = pytest.mark.gpu
 
 
@@ -45,11 +47,16 @@ def _server(hc, graphs, **kw):
 
 
 def _answers(srv, tickets):
+    """Fetches every result; its device checksum (hc_server_checksum, kept
+    from the sweep or computed where the result lies, spilled or not) must
+    match the restatement's over the fetched values."""
     out = []
     for t in tickets:
         info = srv.wait(t)
         assert info["status"] == 0, info
-        out.append((srv.fetch(t), info))
+        got = srv.fetch(t)
+        assert srv.checksum(t) == result_checksum(got), (t, info)
+        out.append((got, info))
         srv.release(t)
     return out
 
@@ -65,7 +72,8 @@ def test_mix_equals_alone(hc, gpu, oracle, graphs):
     srv.close()
     # Mix: every query submitted at once to a multi-slot server that
     # batches any two or more waiting queries of one graph and kind.
-    srv, ids = _server(hc, graphs, slots=4, max_batch=64, min_batch_sssp=2, min_batch_bfs=2)
+    srv, ids = _server(hc, graphs, slots=4, max_batch=64, min_batch_sssp=2, min_batch_bfs=2,
+                       checksum=True)
     tickets = [srv.submit(gid, kind, src) for gid, kind, src, _ in qs]
     mixed = _answers(srv, tickets)
     st = srv.stats()
@@ -225,3 +233,26 @@ def test_resident_graph_costs_measured(hc, gpu, oracle):
     with hc.Server(slots=1, calibrate=-1) as srv:
         a = srv.add_graph(g)
         assert all(v == 0 for c in srv.graph_costs(a).values() for v in c.values())
+
+
+def test_checksums_mix_vs_alone_without_fetch(hc, gpu, graphs):
+    """The bench's check of every resident result: checksums kept after each
+    sweep of a batching server equal those of the same queries run one at a
+    time (single-source path), with no result copied to the host."""
+    g = graphs["kron"]
+    srcs = [int(x) for x in g.pick_sources(40, seed=31)]
+    with hc.Server(slots=3, max_batch=64, min_batch_sssp=2, min_batch_bfs=2, max_wait_ms=-1,
+                   checksum=True) as srv:
+        gid = srv.add_graph(g)
+        tickets = [(kind, s, srv.submit(gid, kind, s)) for s in srcs for kind in (0, 1)]
+        mixed = {}
+        for kind, s, t in tickets:
+            info = srv.wait(t)
+            mixed[(kind, s)] = (srv.checksum(t), info["batch_size"])
+            srv.release(t)
+        assert max(b for _, b in mixed.values()) > 1
+        for (kind, s), (h, _) in mixed.items():
+            t = srv.submit(gid, kind, s)
+            assert srv.wait(t)["batch_size"] == 1
+            assert srv.checksum(t) == h, (kind, s)
+            srv.release(t)
