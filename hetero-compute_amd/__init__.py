@@ -240,7 +240,8 @@ TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "sssp.filter", "sssp.stage", "sssp.g
                "bu.g4_threshold", "multi.hub_threshold", "multi.hub_chunk",
                "multi.wave_threshold", "multi.g16_threshold", "pull.col_shift",
                "cb.min_slice", "cb.g4_slice", "cb.g16_slice", "cb.wave_slice", "cb.block_slice",
-               "multi.block", "many.block", "many.per_cu", "many.light_deg", "many.heavy_cap",
+               "multi.block", "multi.delta", "multi.push_div", "many.block", "many.per_cu", "many.light_deg",
+               "many.heavy_cap",
                "many.lds_nodes", "msbfs.on", "msbfs.alpha", "msbfs.col_shift")
 
 

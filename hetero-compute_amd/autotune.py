@@ -91,6 +91,8 @@ KNOB_SPACE = {
         "multi.g16_threshold": [2, 4, 8],
         "multi.hub_threshold": [2048, 4096, 16384],
         "multi.hub_chunk": [1024, 2048, 8192],
+        "multi.push_div": [0, 4, 16, 64],
+        "multi.delta": [-1, 0, 48, 96, 192],
     },
     # The 64-source bit-parallel BFS.
     "bfs_multi": {

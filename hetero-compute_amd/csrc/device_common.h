@@ -137,6 +137,12 @@ struct Tuning {
     int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
     int multi_grid_per_cu = 16;  // workgroups per CU of the batched sweep
     int multi_block = 256;       // threads per workgroup of the batched sweep (128-1024)
+    int multi_push_div = 16;     // batched sweep: an epoch pushes from the flagged
+                                 // vertices when they hold <= m / this edges (0: never)
+    int multi_delta = -1;        // batched SSSP: improvements at or above a threshold
+                                 // that rises in steps of this wait (delta-stepping
+                                 // buckets; 0: off; -1: 12 x max weight / mean degree
+                                 // on flat graphs, off on skewed ones)
     int msbfs = 1;               // batched BFS: bit-parallel levels (1) or the batched
                                  // sweep with unit weights (0)
     int msbfs_alpha = 32;        // bit-parallel BFS pushes a level while its frontier's
@@ -507,6 +513,7 @@ struct hc_graph_s {
     bool shared = false;    // device arrays belong to another handle
     int32_t num_nonzero = 0;
     int64_t max_degree = 0;
+    int32_t max_weight = 0;  // largest weight of an even sample of the edges
     void *d_off = nullptr;
     int32_t *d_adj = nullptr;
     int32_t *d_wt = nullptr;

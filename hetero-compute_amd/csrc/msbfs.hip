@@ -262,7 +262,6 @@ __global__ __launch_bounds__(kBlock) void ms_pull_kernel(MsBufs b, GraphView<Off
     ms_clear_old(b, L);
     unsigned long long deg_acc = 0, edges = 0;
     const int total = p.unit_start[kNumClasses];
-    const int lane = lane_id();
     for (int unit = blockIdx.x; unit < total; unit += gridDim.x) {
         int cls = 0;
         while (unit >= p.unit_start[cls + 1]) cls++;

@@ -23,6 +23,7 @@
 // The loop is device-driven: step + controller kernels form a hipGraph of
 // kEpochsPerGraph epochs that the host relaunches until the controller
 // reports convergence.
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -39,12 +40,36 @@ struct MultiCtl {
     int32_t done;
     int32_t epoch;
     int32_t max_epochs;
-    int32_t pad;
+    int32_t promote;                // this launch promotes waiting vertices
     int32_t *dist;                  // vertex-major [n][64]
     unsigned long long changed[2];  // vertices changed in epoch e (slot e&1)
     unsigned long long bytes;       // algorithmic bytes of the epochs run
     long long t_start, t_end;
+    // Bucketed propagation (delta > 0): an improvement to a value below
+    // thresh flags the vertex for the next epoch, one at or above it marks
+    // the vertex as waiting; when an epoch changes nothing, thresh rises by
+    // delta and a promotion launch flags the waiting groups now below it.
+    int32_t thresh;
+    int32_t delta;
+    int32_t next_min;               // smallest waiting value left by a promotion
+    int32_t pad;
+    unsigned long long waits;       // waiting marks set since the last promotion
+    unsigned long long left;        // vertices still waiting after a promotion
+    unsigned long long promotions;  // promotion launches run
+    // Push epochs: when the vertices flagged for the next epoch have at most
+    // push_edges edges together and none more than kPushMaxDeg, the next
+    // epoch pushes from them (atomicMin into their neighbours' rows) instead
+    // of sweeping every row for flagged neighbours.
+    int32_t push;                   // this launch pushes
+    int32_t pad2;
+    unsigned long long push_edges;  // 0: never push
+    unsigned long long pushes;      // push launches run
+    unsigned long long fedges[2];   // edges of the vertices flagged in epoch e
+    unsigned long long fmaxdeg[2];  // largest degree among them
 };
+
+// Largest degree a push epoch walks with one wavefront per vertex.
+constexpr unsigned long long kPushMaxDeg = 1024;
 
 // Byte of vertex u in a word-packed flag array.
 __device__ __forceinline__ uint32_t flag_of(const uint32_t *f, int32_t u) {
@@ -64,6 +89,7 @@ struct MultiArgs {
     PlanDev p;
     GraphView<OffT> g;
     uint32_t *chg[3];
+    uint32_t *wait;  // waiting groups per vertex (bytes, as chg)
     MultiCtl *ctl;
     int32_t n;
     int32_t unit;  // 1: every edge weighs 1 (breadth-first depths)
@@ -151,33 +177,158 @@ __device__ __forceinline__ uint32_t relax_group(const MultiArgs<OffT> &a,
     return touched;
 }
 
+// Per-wavefront counts of one launch, summed per workgroup at its end.
+struct EpochAcc {
+    unsigned long long bytes = 0;
+    unsigned long long fedges = 0;  // edges of the vertices flagged for the next epoch
+    unsigned long long fmax = 0;    // their largest degree
+    int changed = 0;                // vertices flagged for the next epoch
+    int waits = 0;                  // vertices marked waiting (promotion: left waiting)
+    int32_t lo = HC_INF_DIST;       // promotion: smallest value left waiting
+};
+
+// Flags the groups of vertex u (degree deg) whose sources improved (imp):
+// groups with a new value below thresh in nxt, the others in wait.
+__device__ __forceinline__ void flag_improved(uint32_t *nxt, uint32_t *wait, int32_t u,
+                                              unsigned long long deg, bool imp, int32_t value,
+                                              int32_t thresh, EpochAcc &acc) {
+    const uint64_t ib = __ballot(imp);
+    if (!ib) return;
+    const uint64_t now = __ballot(imp && value < thresh);
+    const uint64_t later = ib & ~now;
+    const int sh = (u & 3) * 8;
+    if (now) {
+        if (lane_id() == 0) atomicOr(&nxt[u >> 2], group_mask(now) << sh);
+        acc.changed++;
+        acc.fedges += deg;
+        acc.fmax = deg > acc.fmax ? deg : acc.fmax;
+    }
+    if (later) {
+        if (lane_id() == 0) atomicOr(&wait[u >> 2], group_mask(later) << sh);
+        acc.waits++;
+    }
+}
+
 // Commits best[k] for the V sub-vertices (vertex of sub-vertex k is lane
-// k*G's my_u), flags improved groups in nxt, and returns the number of
-// vertices changed.  Plain row update: the unit owns these vertices.
+// k*G's my_u, its degree lane k*G's my_deg) and flags improved groups.
+// Plain row update: the unit owns these vertices.
 template <int V>
-__device__ __forceinline__ int commit(int32_t *dist, uint32_t *nxt, int32_t my_u,
-                                      const int32_t *best, uint32_t touched,
-                                      unsigned long long &bytes) {
+__device__ __forceinline__ void commit(int32_t *dist, uint32_t *nxt, uint32_t *wait,
+                                       int32_t thresh, int32_t my_u, int32_t my_deg,
+                                       const int32_t *best, uint32_t touched, EpochAcc &acc) {
     constexpr int G = kQ / V;
     const int lane = lane_id();
-    int changed = 0;
 #pragma unroll
     for (int k = 0; k < V; k++) {
         if (!((touched >> k) & 1u)) continue;  // wave-uniform
         const int32_t u = __builtin_amdgcn_readlane(my_u, k * G);
+        const int32_t deg = __builtin_amdgcn_readlane(my_deg, k * G);
         int32_t *row = dist + (size_t)u * kQ;
         const int32_t old = row[lane];
         const bool imp = best[k] < old;
         if (imp) row[lane] = best[k];
-        const uint64_t ib = __ballot(imp);
-        bytes += 256ull + 4ull * (unsigned long long)__popcll(ib);
-        if (ib) {
-            const uint32_t gm = group_mask(ib);
-            if (lane == 0) atomicOr(&nxt[u >> 2], gm << ((u & 3) * 8));
-            changed++;
+        acc.bytes += 256ull + 4ull * (unsigned long long)__popcll(__ballot(imp));
+        flag_improved(nxt, wait, u, (unsigned long long)deg, imp, best[k], thresh, acc);
+    }
+}
+
+// Promotion launch: every waiting group of a vertex whose smallest value in
+// the group is now below thresh is flagged for the next epoch and stops
+// waiting; the others keep waiting (acc.waits counts their vertices, acc.lo
+// is their smallest value).  A wavefront owns a whole word of 4 vertices.
+template <typename OffT>
+__device__ __forceinline__ void promote_waiting(const MultiArgs<OffT> &a, uint32_t *nxt,
+                                                const int32_t *dist, int32_t thresh,
+                                                int wave0, int wstride, EpochAcc &acc) {
+    const int lane = lane_id();
+    const int words = (a.n + 3) >> 2;
+    for (int w = wave0; w < words; w += wstride) {
+        const uint32_t word = a.wait[w];
+        acc.bytes += 4;
+        if (!word) continue;  // wave-uniform
+        uint32_t keep_w = 0, now_w = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const uint32_t m = (word >> (8 * b)) & 0xFFu;
+            if (!m) continue;
+            const int32_t u = 4 * w + b;
+            int32_t x = dist[(size_t)u * kQ + lane];
+            acc.bytes += 256;
+            // smallest value of each 8-lane group
+#pragma unroll
+            for (int s = 1; s < 8; s <<= 1) x = min(x, __shfl_xor(x, s, kWave));
+            const uint32_t now = group_mask(__ballot(x < thresh)) & m;
+            const uint32_t keep = m & ~now;
+            if ((keep >> (lane >> 3)) & 1u) acc.lo = min(acc.lo, x);
+            keep_w |= keep << (8 * b);
+            now_w |= now << (8 * b);
+            if (now) {
+                const unsigned long long deg =
+                    (unsigned long long)(a.g.rend(u) - a.g.rbeg(u));
+                acc.changed++;
+                acc.fedges += deg;
+                acc.fmax = deg > acc.fmax ? deg : acc.fmax;
+            }
+            acc.waits += keep != 0;
+        }
+        if (lane == 0) {
+            a.wait[w] = keep_w;
+            if (now_w) nxt[w] = now_w;  // cleared by the launch before; only writer
         }
     }
-    return changed;
+}
+
+// Push launch: every vertex v flagged in cur lowers, for each of its flagged
+// groups' sources, its neighbours' rows to dist[v] + w by atomicMin (lane q
+// = source q, four edges in flight), and flags the neighbours it improved.
+// A wavefront owns a word of 4 flagged vertices; the controller chose this
+// launch only when those vertices have few edges, none above kPushMaxDeg.
+template <typename OffT>
+__device__ __forceinline__ void push_flagged(const MultiArgs<OffT> &a, const uint32_t *cur,
+                                             uint32_t *nxt, int32_t *dist, int32_t thresh,
+                                             int wave0, int wstride, EpochAcc &acc) {
+    const int lane = lane_id();
+    const int words = (a.n + 3) >> 2;
+    for (int w = wave0; w < words; w += wstride) {
+        const uint32_t word = cur[w];
+        acc.bytes += 4;
+        if (!word) continue;  // wave-uniform
+#pragma unroll 1
+        for (int b = 0; b < 4; b++) {
+            const uint32_t m = (word >> (8 * b)) & 0xFFu;
+            if (!m) continue;
+            const int32_t v = 4 * w + b;
+            const int32_t dv = dist[(size_t)v * kQ + lane];
+            const bool mine = ((m >> (lane >> 3)) & 1u) && dv != HC_INF_DIST;
+            const OffT eb = a.g.rbeg(v), ee = a.g.rend(v);
+            acc.bytes += 256ull + 8ull * (unsigned long long)(ee - eb) * (1ull + 8ull * __popc(m));
+            for (OffT e = eb; e < ee; e += OffT(4)) {
+                int32_t u[4], c[4], old[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const bool ok = e + OffT(r) < ee;  // wave-uniform
+                    u[r] = ok ? a.g.adj[e + r] : -1;
+                    const int32_t wt = ok ? (a.unit ? 1 : a.g.wt[e + r]) : 0;
+                    c[r] = relax(dv, wt);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    old[r] = HC_INF_DIST;
+                    if (u[r] >= 0 && mine && c[r] < HC_INF_DIST)
+                        old[r] = atomicMin(&dist[(size_t)u[r] * kQ + lane], c[r]);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (u[r] < 0) continue;  // wave-uniform
+                    const bool imp = mine && c[r] < old[r];
+                    if (!__ballot(imp)) continue;
+                    const unsigned long long deg =
+                        (unsigned long long)(a.g.rend(u[r]) - a.g.rbeg(u[r]));
+                    flag_improved(nxt, a.wait, u[r], deg, imp, c[r], thresh, acc);
+                }
+            }
+        }
+    }
 }
 
 // BLOCK threads per workgroup (knob multi.block): the sweep works in
@@ -197,76 +348,104 @@ __global__ __launch_bounds__(BLOCK) void multi_step_kernel(MultiArgs<OffT> a) {
 
     const PlanDev &p = a.p;
     const int lane = lane_id();
-    // Wave units: chunks, then vertices of classes 1-2, then groups of 4
-    // (class 3), then groups of 16 (classes 4-5).
-    const int n0 = p.cnt[0];
-    const int n12 = p.cnt[1] + p.cnt[2];
-    const int n3 = (p.cnt[3] + 3) / 4;
-    const int n45 = (p.cnt[4] + p.cnt[5] + 15) / 16;
-    const int total = n0 + n12 + n3 + n45;
+    const int32_t thresh = ctl->thresh;
     const int wstride = gridDim.x * (BLOCK / kWave);
-    unsigned long long bytes = 0;
-    int changed = 0;
-    auto vid = [&](int c, int i) -> int32_t {
-        return p.list[c] ? p.list[c][i] : p.beg[c] + i;
-    };
-    for (int unit = blockIdx.x * (BLOCK / kWave) + threadIdx.x / kWave; unit < total;
-         unit += wstride) {
-        if (unit < n0) {
-            const HubChunk hc = p.chunks[unit];
-            int32_t best[1] = {HC_INF_DIST};
-            uint32_t t = relax_group<kQ>(a, cur, dist, OffT(hc.b), OffT(hc.e), best, bytes);
-            if (t) {
-                const int32_t old = atomicMin(&dist[(size_t)hc.u * kQ + lane], best[0]);
-                const uint64_t ib = __ballot(best[0] < old);
-                bytes += 256ull;
-                if (ib) {
-                    if (lane == 0) atomicOr(&nxt[hc.u >> 2], group_mask(ib) << ((hc.u & 3) * 8));
-                    changed++;
+    const int wave0 = blockIdx.x * (BLOCK / kWave) + threadIdx.x / kWave;
+    EpochAcc acc;
+    if (ctl->promote) {
+        promote_waiting(a, nxt, dist, thresh, wave0, wstride, acc);
+    } else if (ctl->push) {
+        push_flagged(a, cur, nxt, dist, thresh, wave0, wstride, acc);
+    } else {
+        // Wave units: chunks, then vertices of classes 1-2, then groups of 4
+        // (class 3), then groups of 16 (classes 4-5).
+        const int n0 = p.cnt[0];
+        const int n12 = p.cnt[1] + p.cnt[2];
+        const int n3 = (p.cnt[3] + 3) / 4;
+        const int n45 = (p.cnt[4] + p.cnt[5] + 15) / 16;
+        const int total = n0 + n12 + n3 + n45;
+        auto vid = [&](int c, int i) -> int32_t {
+            return p.list[c] ? p.list[c][i] : p.beg[c] + i;
+        };
+        for (int unit = wave0; unit < total; unit += wstride) {
+            if (unit < n0) {
+                const HubChunk hc = p.chunks[unit];
+                int32_t best[1] = {HC_INF_DIST};
+                uint32_t t =
+                    relax_group<kQ>(a, cur, dist, OffT(hc.b), OffT(hc.e), best, acc.bytes);
+                if (t) {
+                    const int32_t old = atomicMin(&dist[(size_t)hc.u * kQ + lane], best[0]);
+                    acc.bytes += 256ull;
+                    // a hub: its degree keeps the next epoch a sweep
+                    flag_improved(nxt, a.wait, hc.u, ~0ull >> 1, best[0] < old, best[0], thresh,
+                                  acc);
                 }
-            }
-        } else if (unit < n0 + n12) {
-            const int i = unit - n0;
-            const int32_t u = i < p.cnt[1] ? vid(1, i) : vid(2, i - p.cnt[1]);
-            int32_t best[1] = {HC_INF_DIST};
-            uint32_t t = relax_group<kQ>(a, cur, dist, a.g.rbeg(u), a.g.rend(u), best, bytes);
-            changed += commit<1>(dist, nxt, u, best, t, bytes);
-        } else if (unit < n0 + n12 + n3) {
-            const int i = (unit - n0 - n12) * 4 + lane / 16;
-            const bool ok = i < p.cnt[3];
-            const int32_t u = ok ? vid(3, i) : 0;
-            const OffT b = ok ? a.g.rbeg(u) : OffT(0), e = ok ? a.g.rend(u) : OffT(0);
-            int32_t best[4] = {HC_INF_DIST, HC_INF_DIST, HC_INF_DIST, HC_INF_DIST};
-            uint32_t t = relax_group<16>(a, cur, dist, b, e, best, bytes);
-            changed += commit<4>(dist, nxt, u, best, t, bytes);
-        } else {
-            const int c4 = p.cnt[4];
-            const int i = (unit - n0 - n12 - n3) * 16 + lane / 4;
-            const bool ok = i < c4 + p.cnt[5];
-            const int32_t u = ok ? (i < c4 ? vid(4, i) : vid(5, i - c4)) : 0;
-            const OffT b = ok ? a.g.rbeg(u) : OffT(0), e = ok ? a.g.rend(u) : OffT(0);
-            int32_t best[16];
+            } else if (unit < n0 + n12) {
+                const int i = unit - n0;
+                const int32_t u = i < p.cnt[1] ? vid(1, i) : vid(2, i - p.cnt[1]);
+                const OffT b = a.g.rbeg(u), e = a.g.rend(u);
+                int32_t best[1] = {HC_INF_DIST};
+                uint32_t t = relax_group<kQ>(a, cur, dist, b, e, best, acc.bytes);
+                commit<1>(dist, nxt, a.wait, thresh, u, (int32_t)min(e - b, OffT(INT32_MAX)), best,
+                          t, acc);
+            } else if (unit < n0 + n12 + n3) {
+                const int i = (unit - n0 - n12) * 4 + lane / 16;
+                const bool ok = i < p.cnt[3];
+                const int32_t u = ok ? vid(3, i) : 0;
+                const OffT b = ok ? a.g.rbeg(u) : OffT(0), e = ok ? a.g.rend(u) : OffT(0);
+                int32_t best[4] = {HC_INF_DIST, HC_INF_DIST, HC_INF_DIST, HC_INF_DIST};
+                uint32_t t = relax_group<16>(a, cur, dist, b, e, best, acc.bytes);
+                commit<4>(dist, nxt, a.wait, thresh, u, (int32_t)(e - b), best, t, acc);
+            } else {
+                const int c4 = p.cnt[4];
+                const int i = (unit - n0 - n12 - n3) * 16 + lane / 4;
+                const bool ok = i < c4 + p.cnt[5];
+                const int32_t u = ok ? (i < c4 ? vid(4, i) : vid(5, i - c4)) : 0;
+                const OffT b = ok ? a.g.rbeg(u) : OffT(0), e = ok ? a.g.rend(u) : OffT(0);
+                int32_t best[16];
 #pragma unroll
-            for (int k = 0; k < 16; k++) best[k] = HC_INF_DIST;
-            uint32_t t = relax_group<4>(a, cur, dist, b, e, best, bytes);
-            changed += commit<16>(dist, nxt, u, best, t, bytes);
+                for (int k = 0; k < 16; k++) best[k] = HC_INF_DIST;
+                uint32_t t = relax_group<4>(a, cur, dist, b, e, best, acc.bytes);
+                commit<16>(dist, nxt, a.wait, thresh, u, (int32_t)(e - b), best, t, acc);
+            }
         }
     }
-    // One pair of atomics per workgroup.
-    __shared__ unsigned long long s_bytes, s_changed;
+    // One set of atomics per workgroup.  In a promotion launch acc.waits
+    // counts the vertices left waiting and acc.lo is their smallest value.
+    __shared__ unsigned long long s_bytes, s_changed, s_waits, s_fedges, s_fmax;
+    __shared__ int32_t s_lo;
     if (threadIdx.x == 0) {
         s_bytes = 0;
         s_changed = 0;
+        s_waits = 0;
+        s_fedges = 0;
+        s_fmax = 0;
+        s_lo = HC_INF_DIST;
     }
     __syncthreads();
+    int32_t lo = acc.lo;
+#pragma unroll
+    for (int s = kWave / 2; s > 0; s >>= 1) lo = min(lo, __shfl_xor(lo, s, kWave));
     if (lane == 0) {
-        atomicAdd(&s_bytes, bytes);
-        atomicAdd(&s_changed, (unsigned long long)changed);
+        atomicAdd(&s_bytes, acc.bytes);
+        if (acc.changed) {
+            atomicAdd(&s_changed, (unsigned long long)acc.changed);
+            atomicAdd(&s_fedges, acc.fedges);
+            atomicMax(&s_fmax, acc.fmax);
+        }
+        if (acc.waits) atomicAdd(&s_waits, (unsigned long long)acc.waits);
+        if (lo < HC_INF_DIST) atomicMin(&s_lo, lo);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicAdd(&ctl->changed[ep & 1], s_changed);
         atomicAdd(&ctl->bytes, s_bytes);
+        if (s_changed) {
+            atomicAdd(&ctl->changed[ep & 1], s_changed);
+            atomicAdd(&ctl->fedges[ep & 1], s_fedges);
+            atomicMax(&ctl->fmaxdeg[ep & 1], s_fmax);
+        }
+        if (s_waits) atomicAdd(ctl->promote ? &ctl->left : &ctl->waits, s_waits);
+        if (s_lo < HC_INF_DIST) atomicMin(&ctl->next_min, s_lo);
     }
 }
 
@@ -274,24 +453,50 @@ __global__ void multi_ctl_kernel(MultiCtl *ctl, unsigned long long sweep_bytes) 
     if (threadIdx.x != 0 || ctl->done) return;
     const int ep = ctl->epoch;
     const unsigned long long c = ctl->changed[ep & 1];
+    const unsigned long long fe = ctl->fedges[ep & 1], fm = ctl->fmaxdeg[ep & 1];
     ctl->changed[(ep + 1) & 1] = 0;
+    ctl->fedges[(ep + 1) & 1] = 0;
+    ctl->fmaxdeg[(ep + 1) & 1] = 0;
     ctl->bytes += sweep_bytes;  // offsets and the flag clear, per epoch
     ctl->epoch = ep + 1;
     ctl->t_end = wall_clock64();
-    if (c == 0 || ep + 1 >= ctl->max_epochs) ctl->done = 1;
+    if (ep + 1 >= ctl->max_epochs) {
+        ctl->done = 1;
+        return;
+    }
+    const bool promoted = ctl->promote != 0;
+    ctl->promote = 0;
+    // The next epoch relaxes the flagged vertices: by pushing from them when
+    // they are few and light, else by a sweep over every row.
+    ctl->push = c != 0 && fe <= ctl->push_edges && fm <= kPushMaxDeg;
+    if (ctl->push) ctl->pushes++;
+    if (c != 0) return;
+    // Nothing flagged: raise the threshold past the waiting values, or stop
+    // when none wait.  After a promotion that flagged nothing, the next
+    // bucket starts at the smallest value still waiting.
+    const unsigned long long waiting = promoted ? ctl->left : ctl->left + ctl->waits;
+    if (waiting == 0) {
+        ctl->done = 1;
+        return;
+    }
+    const long long base = promoted ? (long long)ctl->next_min : (long long)ctl->thresh;
+    ctl->thresh = (int32_t)min(base + (long long)ctl->delta, (long long)HC_INF_DIST);
+    ctl->promote = 1;
+    ctl->promotions++;
+    ctl->waits = 0;
+    ctl->left = 0;
+    ctl->next_min = HC_INF_DIST;
 }
 
-__global__ void multi_fill_kernel(int32_t *dist, size_t count, uint32_t *f0, uint32_t *f1,
-                                  uint32_t *f2, int words) {
+// Fills dist with HC_INF_DIST and clears the flag arrays (3) and the
+// waiting array, stored back to back: 4 x words words.
+__global__ void multi_fill_kernel(int32_t *dist, size_t count, uint32_t *flags, int words) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += stride)
         dist[i] = HC_INF_DIST;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)words;
-         i += stride) {
-        f0[i] = 0;
-        f1[i] = 0;
-        f2[i] = 0;
-    }
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < 4 * (size_t)words;
+         i += stride)
+        flags[i] = 0;
 }
 
 struct MultiSources {
@@ -299,7 +504,8 @@ struct MultiSources {
 };
 
 __global__ void multi_seed_kernel(MultiCtl *ctl, int32_t *dist, uint32_t *f0,
-                                  MultiSources src, int num, int max_epochs) {
+                                  MultiSources src, int num, int max_epochs, int delta,
+                                  unsigned long long push_edges, int first_push) {
     const int q = threadIdx.x;
     if (q < num) {
         const int32_t s = src.v[q];
@@ -314,6 +520,19 @@ __global__ void multi_seed_kernel(MultiCtl *ctl, int32_t *dist, uint32_t *f0,
         ctl->changed[0] = 0;
         ctl->changed[1] = 0;
         ctl->bytes = 0;
+        ctl->promote = 0;
+        ctl->delta = delta;
+        ctl->thresh = delta > 0 ? delta : HC_INF_DIST;
+        ctl->next_min = HC_INF_DIST;
+        ctl->waits = 0;
+        ctl->left = 0;
+        ctl->promotions = 0;
+        // The sources are flagged for epoch 0: push from them when allowed.
+        ctl->push = first_push;
+        ctl->push_edges = push_edges;
+        ctl->pushes = first_push;
+        ctl->fedges[0] = ctl->fedges[1] = 0;
+        ctl->fmaxdeg[0] = ctl->fmaxdeg[1] = 0;
         ctl->t_start = wall_clock64();
         ctl->t_end = ctl->t_start;
     }
@@ -321,7 +540,7 @@ __global__ void multi_seed_kernel(MultiCtl *ctl, int32_t *dist, uint32_t *f0,
 
 // Per-graph workspace: flags, controller, and the captured epoch graph.
 struct MultiWs {
-    uint32_t *flags = nullptr;  // 3 arrays of `words` words
+    uint32_t *flags = nullptr;  // 3 flag arrays and the waiting array, `words` words each
     MultiCtl *ctl = nullptr;
     MultiCtl *h_ctl = nullptr;  // pinned mirror for polling
     int words = 0;
@@ -362,7 +581,7 @@ static int get_multi_ws(hc_graph_s *g, MultiWs **out) {
         ws->words = (g->n + 3) / 4;
         ws->grid = grid;
         ws->block = block;
-        hipError_t e1 = hipMalloc(&ws->flags, sizeof(uint32_t) * 3 * (size_t)std::max(ws->words, 1));
+        hipError_t e1 = hipMalloc(&ws->flags, sizeof(uint32_t) * 4 * (size_t)std::max(ws->words, 1));
         hipError_t e2 = hipMalloc(&ws->ctl, sizeof(MultiCtl));
         hipError_t e3 = hipHostMalloc(&ws->h_ctl, sizeof(MultiCtl), hipHostMallocDefault);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -402,6 +621,18 @@ static int launch_multi_epochs(const MultiArgs<OffT> &a, int grid, int block, in
                                            : fail(HC_ERR_HIP, "batched SSSP launch failed");
 }
 
+// Bucket width of the batched SSSP sweep (knob multi.delta): 12 x the
+// sampled largest weight / mean degree on flat graphs (uniform 23, weights
+// 1..255, degree 32: 96, measured best among 32..128), off on skewed ones
+// (Kronecker 24: every width tried was slower).
+static int multi_delta(hc_graph_s *g) {
+    const int d = tuning().multi_delta;
+    if (d >= 0) return d;
+    if (!g->weighted || g->m == 0 || g->max_weight <= 0 || !flat_graph(g)) return 0;
+    const double mean_deg = (double)g->m / std::max(1, g->rows);
+    return (int)std::max(1.0, std::ceil(12.0 * g->max_weight / mean_deg));
+}
+
 template <typename OffT>
 static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *dist,
                      int max_epochs, int unit, hc_run_stats_t *stats, hipStream_t s) {
@@ -420,8 +651,11 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
         }
         ws->plan = ph;
     }
-    uint32_t *f[3] = {ws->flags, ws->flags + ws->words, ws->flags + 2 * (size_t)ws->words};
-    MultiArgs<OffT> a{ws->plan->dev, view<OffT>(g), {f[0], f[1], f[2]}, ws->ctl, g->n, unit};
+    uint32_t *f[4] = {ws->flags, ws->flags + ws->words, ws->flags + 2 * (size_t)ws->words,
+                      ws->flags + 3 * (size_t)ws->words};
+    MultiArgs<OffT> a{ws->plan->dev, view<OffT>(g), {f[0], f[1], f[2]}, f[3], ws->ctl, g->n,
+                      unit};
+    const int delta = unit ? 0 : multi_delta(g);
     const int grid = ws->grid;
     const unsigned long long sweep =
         (unsigned long long)sizeof(OffT) * (g->n + 1ull) + 4ull * ws->words;
@@ -429,9 +663,27 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
     for (int q = 0; q < num; q++) src.v[q] = sources[q];
     const size_t cells = (size_t)g->n * kQ;
     const int fill_grid = std::max(1, std::min(grid, (int)((cells + kBlock - 1) / kBlock)));
-    multi_fill_kernel<<<fill_grid, kBlock, 0, s>>>(dist, cells, f[0], f[1], f[2], ws->words);
+    multi_fill_kernel<<<fill_grid, kBlock, 0, s>>>(dist, cells, ws->flags, ws->words);
+    // Without a limit from the caller the controller runs to convergence
+    // (Bellman-Ford: at most n + 1 epochs; with buckets the host's cap below
+    // bounds the loop) and the host reports a loop that hits its cap.
+    // Push epochs while the flagged vertices hold at most m / multi.push_div
+    // edges; epoch 0 pushes from the sources under the same rule.
+    const int push_div = tuning().multi_push_div;
+    const unsigned long long push_edges =
+        push_div > 0 ? std::max<unsigned long long>(1, (unsigned long long)g->m / push_div) : 0;
+    unsigned long long src_edges = 0, src_max = 0;
+    for (int q = 0; q < num; q++) {
+        const int64_t r = sources[q] - g->row0;
+        const unsigned long long d =
+            (r >= 0 && r < g->rows) ? (unsigned long long)(g->h_off[r + 1] - g->h_off[r]) : 0;
+        src_edges += d;
+        src_max = std::max(src_max, d);
+    }
+    const bool first_push = push_edges > 0 && src_edges <= push_edges && src_max <= kPushMaxDeg;
     multi_seed_kernel<<<1, kQ, 0, s>>>(ws->ctl, dist, f[0], src, num,
-                                       max_epochs > 0 ? max_epochs : g->n + 1);
+                                       max_epochs > 0 ? max_epochs : INT32_MAX, delta,
+                                       push_edges, first_push ? 1 : 0);
     HC_HIP(hipGetLastError());
 
     ws->kt.reset();
@@ -479,7 +731,10 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
     HC_HIP(hipEventCreateWithFlags(&evb[1], hipEventDisableTiming));
     MultiCtl *h = ws->h_ctl;
     h->done = 0;
-    const long long cap = (long long)(max_epochs > 0 ? max_epochs : g->n + 1) + 2 * kEpochsPerGraph;
+    const long long bound = max_epochs > 0 ? max_epochs
+                            : delta > 0    ? 64LL * (g->n + 1) + 4096
+                                           : (long long)g->n + 1;
+    const long long cap = bound + 2 * kEpochsPerGraph;
     long long launched = 0;
     bool done = false;
     int rc = HC_OK;
@@ -512,7 +767,9 @@ static int run_multi(hc_graph_s *g, const int32_t *sources, int num, int32_t *di
         std::memset(stats, 0, sizeof(*stats));
         stats->ms = double(c.t_end - c.t_start) / device_state().clock_khz;
         stats->epochs = c.epoch;
-        stats->pull_epochs = c.epoch;
+        const int32_t pushes = (int32_t)c.pushes;
+        stats->push_epochs = pushes;
+        stats->pull_epochs = c.epoch - (int32_t)c.promotions - pushes;
         stats->launches = (int32_t)(2 * launched + 2);
         stats->reached = num;
         stats->alg_bytes = (double)c.bytes;

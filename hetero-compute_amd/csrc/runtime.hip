@@ -134,6 +134,8 @@ int64_t *knob_ref(Tuning &t, const char *key, bool *is_int, int **iref) {
                {"multi.graph", &t.multi_graph},
                {"multi.grid_per_cu", &t.multi_grid_per_cu},
                {"multi.block", &t.multi_block},
+               {"multi.delta", &t.multi_delta},
+               {"multi.push_div", &t.multi_push_div},
                {"msbfs.on", &t.msbfs},
                {"msbfs.alpha", &t.msbfs_alpha},
                {"msbfs.col_shift", &t.msbfs_col_shift},
@@ -346,6 +348,15 @@ static int upload_rows(const hc_host_graph_t *hg, int32_t start, int32_t end,
     }
     g->sorted = sorted;
     g->max_degree = maxd;
+    if (g->weighted && m > 0) {
+        // Largest weight of an even sample of up to 2^16 edges (bucket width
+        // of the batched sweep).
+        const int32_t *wt = hg->weights + e0;
+        const int64_t step = std::max<int64_t>(1, m >> 16);
+        int32_t wmax = 0;
+        for (int64_t i = 0; i < m; i += step) wmax = std::max(wmax, wt[i]);
+        g->max_weight = wmax;
+    }
     // For sorted rows, [row0, row0+nz) holds every non-zero-degree row.
     g->num_nonzero = sorted ? nz : rows;
     const int32_t *nb = hg->neighbors + e0;
@@ -587,6 +598,7 @@ int hc_graph_share(hc_graph_t g, hc_graph_t *out) {
     h->off64 = g->off64;
     h->num_nonzero = g->num_nonzero;
     h->max_degree = g->max_degree;
+    h->max_weight = g->max_weight;
     h->d_off = g->d_off;
     h->d_adj = g->d_adj;
     h->d_wt = g->d_wt;
@@ -633,6 +645,12 @@ int set_knob(Tuning &t, const char *key, int64_t value) {
         }
         if (!strcmp(key, "loop.batch"))
             HC_CHECK_ARG(value >= 1 && value <= 16, "loop.batch must be in [1, 16]");
+        if (!strcmp(key, "multi.delta")) {
+            HC_CHECK_ARG(value >= -1 && value <= 1 << 30,
+                         "multi.delta must be -1 (automatic), 0 (off) or in [1, 2^30]");
+            *iref = (int)value;
+            return HC_OK;
+        }
         if (!strcmp(key, "multi.block")) {
             HC_CHECK_ARG(value == 128 || value == 256 || value == 512 || value == 1024,
                          "multi.block must be 128, 256, 512 or 1024");
@@ -661,6 +679,7 @@ int set_knob(Tuning &t, const char *key, int64_t value) {
             HC_CHECK_ARG(value == 0 || value == 1 || value == 4 || value == 16,
                          "bu.remaining must be 0, 1, 4 or 16");
         const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining") ||
+                             !strcmp(key, "multi.push_div") ||
                              !strcmp(key, "pull.col_shift") ||
                              !strcmp(key, "bu.first") || !strcmp(key, "sssp.alpha") ||
                              !strcmp(key, "sssp.dist16") || !strcmp(key, "sssp.push_dedup") ||

@@ -267,6 +267,17 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *   multi.grid_per_cu      workgroups per CU of the batched sweep (16)
  *   multi.block            threads per workgroup of the batched sweep (128,
  *                          256, 512 or 1024; 256)
+ *   multi.delta            batched SSSP: a distance improvement propagates at
+ *                          once only while below a threshold that rises by
+ *                          this much whenever nothing below it changes
+ *                          (delta-stepping buckets over the 64 sources);
+ *                          0 off; -1 (default) 12 x the graph's sampled
+ *                          largest weight / mean degree on flat graphs, off
+ *                          on skewed ones.  Results do not depend on it
+ *   multi.push_div         batched sweep: an epoch whose flagged vertices
+ *                          hold at most m / this edges (none more than
+ *                          1024) pushes from them with atomicMin instead of
+ *                          sweeping every row (16; 0 never)
  *   msbfs.on               hc_bfs_multi runs bit-parallel levels (1, default)
  *                          or the batched sweep with unit weights (0)
  *   msbfs.alpha            ... a bit-parallel level pushes while its frontier
