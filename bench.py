@@ -518,6 +518,7 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
         lat, edges, batches, checked, exact, errors = [], 0, [], 0, True, []
         kept_small, kept_big, sums = [], [], []
         done_at = np.zeros(queries)
+        fin = np.full(queries, np.nan)  # completion time (s from start) by the server's clock
         sweeps = {}
         for i in range(queries):
             while tickets[i] is None:
@@ -529,6 +530,7 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
                 continue
             lat.append(info["latency_ms"] + lag[i] * 1e3)
             done_at[i] = time.perf_counter() - start
+            fin[i] = arrive[i] + lag[i] + info["latency_ms"] / 1e3
             k_, kind_, _ = qs[i]
             cls = ("kron" if k_ == 0 else "uniform" if k_ == 1 else "small") + \
                 ("_bfs" if kind_ == hc.QUERY_BFS else "_sssp")
@@ -597,6 +599,12 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
         # Steady state: the queries that arrived after the first fifth of the
         # run (the queue has filled or drained to its working level by then).
         steady = lat[queries // 5:]
+        # Throughput while queries still arrive, after the first fifth: the
+        # results completed in [arrival of query n/5, last arrival] per
+        # second (equals the offered rate when the server keeps up; the
+        # whole-run rate also counts the drain after the last arrival).
+        t0, t1 = float(arrive[queries // 5]), float(arrive[-1])
+        steady_qps = float(np.sum((fin >= t0) & (fin <= t1))) / max(t1 - t0, 1e-9)
         results.append({"rate_qps": rate, "queries": queries,
                         "p50_ms": round(float(np.percentile(lat, 50)), 3),
                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -604,6 +612,7 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
                         "steady_p99_ms": round(float(np.percentile(steady, 99)), 3),
                         "sustained_gteps": round(edges / 2 / wall / 1e9, 3),
                         "achieved_qps": round(queries / wall, 1),
+                        "steady_qps": round(steady_qps, 1),
                         "mean_batch": round(float(np.mean(batches)), 2),
                         "mean_sssp_batch_resident": round(float(np.mean(
                             [b for b, (k, kind, _) in zip(batches, qs)
@@ -619,7 +628,8 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
                         "resident_equal_to_alone": alone_equal,
                         "bit_exact": exact,
                         "failed": len(errors), "errors": errors[:3]})
-        log(f"[serve] {rate} q/s: achieved {results[-1]['achieved_qps']} q/s, "
+        log(f"[serve] {rate} q/s: achieved {results[-1]['achieved_qps']} q/s "
+            f"(steady {results[-1]['steady_qps']}), "
             f"p50 {results[-1]['p50_ms']} ms, p99 {results[-1]['p99_ms']} ms")
     st = srv.stats()
     costs = {name: srv.graph_costs(gid) for name, gid in (("kron", gids[0]),
