@@ -240,7 +240,7 @@ TUNING_KEYS = ("grid_per_cu", "sssp.alpha", "sssp.filter", "sssp.stage", "sssp.g
                "bu.g4_threshold", "multi.hub_threshold", "multi.hub_chunk",
                "multi.wave_threshold", "multi.g16_threshold", "pull.col_shift",
                "cb.min_slice", "cb.g4_slice", "cb.g16_slice", "cb.wave_slice", "cb.block_slice",
-               "multi.block", "multi.delta", "multi.push_div", "many.block", "many.per_cu", "many.light_deg",
+               "multi.block", "multi.delta", "multi.push_div", "sssp.delta", "many.block", "many.per_cu", "many.light_deg",
                "many.heavy_cap",
                "many.lds_nodes", "msbfs.on", "msbfs.alpha", "msbfs.col_shift")
 
@@ -777,7 +777,8 @@ def epoch_trace(g: DeviceGraph, algorithm: str) -> list[dict]:
     """Per-epoch record of the last sssp/bfs run on g."""
     t = EpochTrace()
     check(lib().hc_get_epoch_trace(g.h, 0 if algorithm == "sssp" else 1, C.byref(t)))
-    return [{"mode": "pull" if t.mode[i] else "push", "frontier": t.frontier[i],
+    names = {0: "push", 1: "pull", 2: "promote"}
+    return [{"mode": names.get(t.mode[i], str(t.mode[i])), "frontier": t.frontier[i],
              "frontier_edges": t.frontier_edges[i], "scanned": t.scanned[i],
              "us": round(t.us[i], 2)}
             for i in range(t.count)]

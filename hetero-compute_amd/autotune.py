@@ -64,6 +64,7 @@ KNOB_SPACE = {
         "pull.col_shift": [-1, 0, 20, 21],
         "cb.min_slice": [2, 4, 8],
         "cb.g16_slice": [8, 12, 24],
+        "sssp.delta": [-1, 0, 8, 32],
     },
     "bfs": {
         "grid_per_cu": [2, 4, 8, 16],

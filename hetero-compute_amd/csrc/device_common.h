@@ -137,6 +137,9 @@ struct Tuning {
     int multi_graph = 1;   // batched SSSP/BFS: relaunch an epoch hipGraph
     int multi_grid_per_cu = 16;  // workgroups per CU of the batched sweep
     int multi_block = 256;       // threads per workgroup of the batched sweep (128-1024)
+    int sssp_delta = -1;         // single-source SSSP: bucket width (push epochs only;
+                                 // 0 off; -1: 2 x max weight / mean degree on flat
+                                 // graphs, off on skewed ones)
     int multi_push_div = 16;     // batched sweep: an epoch pushes from the flagged
                                  // vertices when they hold <= m / this edges (0: never)
     int multi_delta = -1;        // batched SSSP: improvements at or above a threshold

@@ -45,7 +45,7 @@ constexpr int kSmallLanes = 8;
 constexpr int kSmallPerUnit = kBlock / kSmallLanes;
 constexpr int kTraceLen = 128;
 
-enum StepMode : int32_t { kPush = 0, kPull = 1 };
+enum StepMode : int32_t { kPush = 0, kPull = 1, kPromote = 2 };
 
 struct StepCtl {
     int32_t done;
@@ -76,6 +76,16 @@ struct StepCtl {
     int32_t rem_valid;             // BFS: the next pull reads the unvisited lists
     int32_t rem_knob;              // BFS: lanes per listed vertex, 0: no lists
     int32_t chunk;                 // edges per hub chunk of push epochs
+    // SSSP buckets (delta > 0, push epochs only): a vertex lowered to a value
+    // at or above thresh goes to the far bitmap instead of the next
+    // frontier; when a frontier comes out empty the threshold rises (by
+    // delta, doubled after a promotion that found nothing) and a promotion
+    // epoch moves the far vertices now below it into the frontier.
+    int32_t thresh;
+    int32_t delta;
+    int32_t far_step;              // the last rise of thresh
+    int32_t promotions;            // promotion epochs run
+    long long far_live;            // upper bound of the far vertices
     // Device wall clock (constant rate): start of the loop, end of the last
     // epoch, and end of each traced epoch.
     long long t_start;
@@ -114,6 +124,7 @@ struct FrontierBufs {
     int32_t *q[2][3];
     HubChunk *ch[2];
     uint32_t *bm[3];
+    uint32_t *far;             // SSSP buckets: vertices waiting for a higher threshold
     uint32_t (*cnt)[5];        // queue fill [parity][q0, q1, q2, chunks, hubs]
     int32_t words;             // bitmap words
     Partials part;             // counter set of the current epoch

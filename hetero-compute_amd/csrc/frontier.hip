@@ -26,7 +26,7 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
     size_t pbytes = al(sizeof(unsigned long long) * kSumWords * kSumSets);
     size_t nbytes = al(sizeof(uint32_t) * 2 * 5);
     size_t total =
-        6 * qbytes + 2 * cbytes + 3 * bbytes + pbytes + nbytes + al(2 * sizeof(StepCtl));
+        6 * qbytes + 2 * cbytes + 4 * bbytes + pbytes + nbytes + al(2 * sizeof(StepCtl));
     char *base = nullptr;
     if (hipMalloc(reinterpret_cast<void **>(&base), total) != hipSuccess)
         return fail(HC_ERR_OOM, "frontier workspace of %zu bytes failed", total);
@@ -44,6 +44,8 @@ int frontier_ws_alloc(hc_graph_s *g, FrontierWs *ws) {
         ws->fb.bm[p] = reinterpret_cast<uint32_t *>(base + o);
         o += bbytes;
     }
+    ws->fb.far = reinterpret_cast<uint32_t *>(base + o);
+    o += bbytes;
     ws->fb.part.acc = reinterpret_cast<unsigned long long *>(base + o);
     o += pbytes;
     ws->fb.cnt = reinterpret_cast<uint32_t (*)[5]>(base + o);
@@ -72,10 +74,10 @@ int64_t share_workspace_bound(int64_t n, int64_t m) {
     const int64_t words = (n + 31) / 32;
     // frontier_ws_alloc: six queues, two hub-chunk lists (a chunk per
     // kMinChunkEdges edges of every hub row, plus one per hub, bounded by
-    // m / kMinChunkEdges + m / kHubDeg + 1), three bitmaps, partials, control.
+    // m / kMinChunkEdges + m / kHubDeg + 1), four bitmaps, partials, control.
     const int64_t chunks = m / kMinChunkEdges + m / kHubDeg + 1;
     const int64_t frontier = 6 * 4 * n + 2 * (int64_t)sizeof(HubChunk) * chunks +
-                             3 * 4 * words + (int64_t)sizeof(unsigned long long) *
+                             4 * 4 * words + (int64_t)sizeof(unsigned long long) *
                              kSumWords * kSumSets + 64 * 1024;
     const int64_t sssp = frontier + 2 * n;                    // + 16-bit mirror
     const int64_t bfs = frontier + 8 * n + 4 * (3 * (int64_t)grid_blocks() + 1);  // + depth, first

@@ -135,6 +135,7 @@ int64_t *knob_ref(Tuning &t, const char *key, bool *is_int, int **iref) {
                {"multi.grid_per_cu", &t.multi_grid_per_cu},
                {"multi.block", &t.multi_block},
                {"multi.delta", &t.multi_delta},
+               {"sssp.delta", &t.sssp_delta},
                {"multi.push_div", &t.multi_push_div},
                {"msbfs.on", &t.msbfs},
                {"msbfs.alpha", &t.msbfs_alpha},
@@ -645,9 +646,9 @@ int set_knob(Tuning &t, const char *key, int64_t value) {
         }
         if (!strcmp(key, "loop.batch"))
             HC_CHECK_ARG(value >= 1 && value <= 16, "loop.batch must be in [1, 16]");
-        if (!strcmp(key, "multi.delta")) {
+        if (!strcmp(key, "multi.delta") || !strcmp(key, "sssp.delta")) {
             HC_CHECK_ARG(value >= -1 && value <= 1 << 30,
-                         "multi.delta must be -1 (automatic), 0 (off) or in [1, 2^30]");
+                         "%s must be -1 (automatic), 0 (off) or in [1, 2^30]", key);
             *iref = (int)value;
             return HC_OK;
         }

@@ -15,6 +15,7 @@
 //    frontier's degree sum exceeds m/alpha) and stops the loop when nothing
 //    changed, so the host only enqueues launches.
 #include <algorithm>
+#include <cmath>
 #include <functional>
 #include <vector>
 #include <cstring>
@@ -317,6 +318,9 @@ __device__ void sssp_pull_sweep_cb(const PlanDev &p, const GraphView<OffT> &g,
 // Push expansion of the frontier
 // ---------------------------------------------------------------------------
 
+// Relaxes the out-edges [b, e) of u (G lanes); a lowered neighbour joins
+// the next frontier, or, when its new value is at or above `thresh`
+// (buckets), the far bitmap (`nfar` counts those settings).
 template <int G, typename OffT>
 __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
                                            int32_t *__restrict__ dist,
@@ -324,13 +328,19 @@ __device__ __forceinline__ void push_range(const GraphView<OffT> &g,
                                            const PushOut &out,
                                            int nxt, uint32_t *wbm, int32_t u, OffT b,
                                            OffT e, int lane, uint32_t &cnt,
-                                           unsigned long long &edges_acc) {
+                                           unsigned long long &edges_acc, int32_t thresh,
+                                           uint32_t &nfar) {
     const int32_t du = dist[u];
     for (OffT i = b + OffT(lane); i < e; i += OffT(G)) {
         int32_t v = g.adj[i];
         int32_t nd = relax(du, g.wt[i]);
         bool lowered = false;
         if (nd < dist[v]) lowered = nd < atomicMin(&dist[v], nd);
+        if (lowered && nd >= thresh) {
+            atomicOr(&fb.far[v >> 5], 1u << (v & 31));
+            nfar++;
+            lowered = false;
+        }
         if (!out.dedup && lowered) {
             // The first edge to set v's bit appends v, now.
             const uint32_t bit = 1u << (v & 31);
@@ -352,7 +362,8 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
                                 const FrontierBufs &fb, StepCtl *ctl,
                                 const PushOut &out, int cur,
                                 int nxt, uint32_t *wbm, uint32_t &cnt,
-                                unsigned long long &edges_acc) {
+                                unsigned long long &edges_acc, uint32_t &nfar) {
+    const int32_t thresh = ctl->thresh;
     const uint32_t nch = fb.cnt[cur][3], n0 = fb.cnt[cur][0],
                    n1 = fb.cnt[cur][1], n2 = fb.cnt[cur][2];
     const uint32_t u1 = nch + n0;
@@ -369,17 +380,17 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
         if (unit < nch) {
             HubChunk h = fb.ch[cur][unit];
             push_range<kBlock>(g, dist, fb, ctl, out, nxt, wbm, h.u, OffT(h.b), OffT(h.e),
-                               threadIdx.x, cnt, edges_acc);
+                               threadIdx.x, cnt, edges_acc, thresh, nfar);
         } else if (unit < u1) {
             int32_t u = fb.q[cur][0][unit - nch];
             push_range<kBlock>(g, dist, fb, ctl, out, nxt, wbm, u, g.rbeg(u), g.rend(u),
-                               threadIdx.x, cnt, edges_acc);
+                               threadIdx.x, cnt, edges_acc, thresh, nfar);
         } else if (unit < u2) {
             uint32_t idx = (unit - u1) * kWavesPerBlock + threadIdx.x / kWave;
             if (idx < n1) {
                 int32_t u = fb.q[cur][1][idx];
                 push_range<kWave>(g, dist, fb, ctl, out, nxt, wbm, u, g.rbeg(u), g.rend(u),
-                                  lane_id(), cnt, edges_acc);
+                                  lane_id(), cnt, edges_acc, thresh, nfar);
             }
         } else {
             uint32_t idx = (unit - u2) * kSmallPerUnit + threadIdx.x / kSmallLanes;
@@ -387,8 +398,39 @@ __device__ void sssp_push_sweep(const GraphView<OffT> &g,
                 int32_t u = fb.q[cur][2][idx];
                 push_range<kSmallLanes>(g, dist, fb, ctl, out, nxt, wbm, u, g.rbeg(u),
                                         g.rend(u), threadIdx.x % kSmallLanes, cnt,
-                                        edges_acc);
+                                        edges_acc, thresh, nfar);
             }
+        }
+    }
+}
+
+// Promotion epoch of the buckets: every far vertex whose distance is now
+// below `thresh` leaves the far bitmap and is set in the changed bitmap
+// `wbm` (compacted into the next frontier's queues by the launch after);
+// `cnt` / `edges_acc` count those, `left` the vertices still far.  A thread
+// owns one word of both bitmaps.
+template <typename OffT>
+__device__ void sssp_promote_sweep(const GraphView<OffT> &g, const int32_t *dist,
+                                   const FrontierBufs &fb, int32_t thresh, uint32_t *wbm,
+                                   uint32_t &cnt, unsigned long long &edges_acc, uint32_t &left) {
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < fb.words; w += gridDim.x * kBlock) {
+        const uint32_t word = fb.far[w];
+        if (!word) continue;
+        uint32_t now = 0;
+        for (uint32_t x = word; x; x &= x - 1) {
+            const int b = __ffs(x) - 1;
+            const int32_t v = w * 32 + b;
+            if (dist[v] < thresh) {
+                now |= 1u << b;
+                cnt++;
+                edges_acc += (unsigned long long)(g.rend(v) - g.rbeg(v));
+            } else {
+                left++;
+            }
+        }
+        if (now) {
+            fb.far[w] = word & ~now;
+            wbm[w] = now;  // cleared by the launch before; no other writer
         }
     }
 }
@@ -456,13 +498,17 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
                 sssp_pull_sweep<true, false>(p, g, rd, unstaged(rbm), dist, nullptr, wbm, cnt,
                                              eacc, gathered);
         }
+    } else if (mode == kPromote) {
+        // (`gathered` carries the vertices left far)
+        sssp_promote_sweep(g, dist, fb, ctl->thresh, wbm, cnt, eacc, gathered);
     } else {
         // Output staged in the dynamic LDS (unused by push epochs).
         __shared__ uint32_t s_out;
         if (threadIdx.x == 0) s_out = 0;
         __syncthreads();
         const PushOut out{s_stage, &s_out, (uint32_t)push_cap, defer_dedup ? wbm : nullptr};
-        sssp_push_sweep(g, dist, fb, ctl, out, cur, nxt, wbm, cnt, eacc);
+        // (`gathered` carries the far settings)
+        sssp_push_sweep(g, dist, fb, ctl, out, cur, nxt, wbm, cnt, eacc, gathered);
         const PushKept k = push_out_flush(out, g, fb, ctl, nxt);
         cnt += k.n;
         eacc += k.e;
@@ -477,11 +523,18 @@ __device__ void sssp_decide(StepCtl *c, StepCtl *trace,
     const unsigned long long out_n = sums[0], out_e = sums[1], gathered = sums[2];
     const int k = c->epoch;
     const int ran = c->mode;
-    if (ran == kPush) {
+    if (ran == kPromote) {
+        // far bitmap words, plus dist[v] and a bitmap entry per far vertex
+        c->alg_bytes += 4ull * (unsigned long long)((c->n + 31) / 32) + 8ull * (out_n + gathered);
+        c->promotions++;
+        c->far_live = (long long)gathered;
+        trace_epoch(c, trace, k, ran, out_n, out_e, 0);
+    } else if (ran == kPush) {
         // offsets + dist[u] per frontier vertex; id + weight + dist[v] per
         // edge; dist write + bitmap/queue entry per improved vertex.
         c->alg_bytes += 12ull * c->in_n + 12ull * c->in_e + 8ull * out_n;
         c->push_epochs++;
+        c->far_live += (long long)gathered;  // far settings (may repeat a vertex)
         trace_epoch(c, trace, k, ran, out_n, out_e, c->in_e);
     } else {
         // Offsets, ids and own distances always; weight + distance per
@@ -497,16 +550,27 @@ __device__ void sssp_decide(StepCtl *c, StepCtl *trace,
     c->in_e = (long long)out_e;
     c->filter = c->filter_knob;  // the changed set of this epoch is complete
     int next = ran;
-    if (out_n == 0 || (c->max_epochs > 0 && k + 1 >= c->max_epochs)) {
+    if (c->max_epochs > 0 && k + 1 >= c->max_epochs) {
         c->done = 1;
+    } else if (out_n == 0) {
+        if (c->delta > 0 && c->far_live > 0) {
+            // Raise the threshold and promote; after a promotion that found
+            // nothing below it, rise twice as far.
+            const long long step = ran == kPromote ? 2ll * c->far_step : (long long)c->delta;
+            c->far_step = (int32_t)min(step, (long long)HC_INF_DIST);
+            c->thresh = (int32_t)min((long long)c->thresh + step, (long long)HC_INF_DIST);
+            next = kPromote;
+        } else {
+            c->done = 1;
+        }
+    } else if (c->delta > 0 || c->direction == 2) {
+        next = kPush;
     } else if (c->direction == 1) {
         next = kPull;
-    } else if (c->direction == 2) {
-        next = kPush;
     } else {
         next = (long long)out_e > c->m / c->alpha ? kPull : kPush;
     }
-    c->need_compact = (!c->done && ran == kPull && next == kPush) ? 1 : 0;
+    c->need_compact = (!c->done && ran != kPush && next == kPush) ? 1 : 0;
     c->mode = next;
 }
 
@@ -536,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
     GraphView<OffT> g, int32_t *__restrict__ dist,
     const int32_t *__restrict__ init_dist, FrontierBufs fb, StepCtl *ctl,
     int32_t src, int direction, int alpha, int max_epochs, long long m,
-    long long sweep_bytes, int grid, int filter, int chunk, uint16_t *d16) {
+    long long sweep_bytes, int grid, int filter, int chunk, uint16_t *d16, int delta) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < g.n; i += stride) {
         const int32_t d = init_dist ? init_dist[i] : (i == src ? 0 : HC_INF_DIST);
@@ -548,6 +612,7 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
         fb.bm[0][w] = (!init_dist && w == (src >> 5)) ? (1u << (src & 31)) : 0u;
         fb.bm[1][w] = 0;
         fb.bm[2][w] = 0;
+        fb.far[w] = 0;
     }
     if (blockIdx.x == 0)
         for (int j = 0; j < kSumSets; j++) clear_partials(sum_set(fb.part, j));
@@ -570,6 +635,11 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
         ctl->m = m;
         ctl->sweep_bytes = sweep_bytes;
         ctl->push_epochs = ctl->pull_epochs = 0;
+        ctl->delta = delta;
+        ctl->thresh = delta > 0 ? delta : HC_INF_DIST;
+        ctl->far_step = delta;
+        ctl->promotions = 0;
+        ctl->far_live = 0;
         ctl->reached = 0;
         ctl->alg_bytes = 0;
         ctl->t_start = wall_clock64();
@@ -592,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
             ctl->in_n = 1;
             ctl->in_e = d;
             ctl->mode = direction == 1 ? kPull
-                        : direction == 2 ? kPush
+                        : (direction == 2 || delta > 0) ? kPush
                         : (d > m / alpha ? kPull : kPush);
         }
     }
@@ -869,6 +939,18 @@ struct SsspLaunch {
 };
 
 
+// Bucket width of the single-source loop (knob sssp.delta): 2 x the sampled
+// largest weight / mean degree on flat graphs (uniform 23, weights 1..255,
+// degree 32: 16; 8 and 16 measured best, 20.7 -> 10.5 ms), off on skewed
+// ones (Kronecker 23: 5.1 -> 8.3..9.2 ms with widths 16..256).
+static int sssp_delta(hc_graph_s *g) {
+    const int d = tuning().sssp_delta;
+    if (d >= 0) return d;
+    if (!g->weighted || g->m == 0 || g->max_weight <= 0 || !flat_graph(g)) return 0;
+    const double mean_deg = (double)g->m / std::max(1, g->rows);
+    return (int)std::max(1.0, std::ceil(2.0 * g->max_weight / mean_deg));
+}
+
 template <typename OffT>
 static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
                          int32_t *dist, int direction, int alpha,
@@ -891,9 +973,11 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const int32_t lds_entries = std::max<int32_t>(nstage, push_cap);
     // Pull epochs gather column block by column block (not with filtering,
     // which tests a changed bit per neighbour before its gather).
+    // Buckets (sssp.delta): push epochs only.
+    const int delta = init_dist || direction == 1 ? 0 : sssp_delta(g);
     const ColBlocks *cb = nullptr;
     const PlanHost *phb = nullptr;
-    if (direction != 2 && !tuning().sssp_filter) {
+    if (direction != 2 && delta == 0 && !tuning().sssp_filter) {
         HC_TRY(get_col_blocks(g, col_shift_for(g, flat_graph(g)), s, &cb));
         if (cb) {
             // Plain sweep for the light rows only.
@@ -912,7 +996,8 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const int init_grid = std::max(1, std::min(grid, (g->n + kBlock - 1) / kBlock));
     sssp_init_kernel<OffT><<<init_grid, kBlock, 0, s>>>(
         v, dist, init_dist, ws->f.fb, ws->f.ctl, src, direction, alpha,
-        max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter, tuning().push_chunk, d16);
+        max_epochs, (long long)g->m, sweep, grid, tuning().sssp_filter, tuning().push_chunk, d16,
+        delta);
     HC_HIP(hipGetLastError());
     ws->f.kt.reset();
     // Push epochs deduplicate improved vertices when staging them (flat
@@ -926,8 +1011,10 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
                                             0, g->n}
                           : GraphView<OffT>{},
                        cb ? cb->K : 0};
-    // Bellman-Ford needs at most n epochs; the cap bounds a faulty loop.
-    const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, g->n) : g->n) + 2;
+    // Bellman-Ford needs at most n epochs (buckets add promotion epochs and
+    // re-pushes: a generous bound); the cap bounds a faulty loop.
+    const long long bound = delta > 0 ? 64LL * (g->n + 1) + 4096 : (long long)g->n;
+    const long long cap = (max_epochs > 0 ? std::min<long long>(max_epochs, bound) : bound) + 2;
     float ms = 0;
     long long launches = 0;
     StepCtl fc;

@@ -274,6 +274,16 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *                          0 off; -1 (default) 12 x the graph's sampled
  *                          largest weight / mean degree on flat graphs, off
  *                          on skewed ones.  Results do not depend on it
+ *   sssp.delta             hc_sssp: a vertex lowered to a value at or above
+ *                          a threshold waits in a far set instead of the next
+ *                          frontier; when a frontier comes out empty the
+ *                          threshold rises by this much (doubling after a
+ *                          promotion that found nothing) and the far
+ *                          vertices below it become the frontier (near-far
+ *                          buckets; push epochs only, no pulls); 0 off; -1
+ *                          (default) 2 x the graph's sampled largest weight
+ *                          / mean degree on flat graphs, off on skewed ones
+ *                          and with direction 1.  Results do not depend on it
  *   multi.push_div         batched sweep: an epoch whose flagged vertices
  *                          hold at most m / this edges (none more than
  *                          1024) pushes from them with atomicMin instead of
@@ -460,7 +470,7 @@ int hc_pr_pull(hc_graph_t g, int mapping, const float *init_score,
  * per-iteration prints showed (src/kernels/cpu/bfs.cuh:82-83, 99-100). */
 typedef struct hc_epoch_trace {
     int32_t count;
-    int8_t mode[128];
+    int8_t mode[128];      /* 0 push, 1 pull, 2 promotion (SSSP buckets)      */
     uint32_t frontier[128];
     uint64_t frontier_edges[128];
     uint64_t scanned[128];

@@ -90,6 +90,7 @@ def test_sssp_filter_and_direction_exact(hc, gpu, oracle, kron16, uniform16, fil
     only) and other push/pull switch points give the same distances as
     Dijkstra, on both scale-16 graphs and every golden graph."""
     try:
+        hc.set_tuning("sssp.delta", 0)  # pulls on the uniform graph too
         hc.set_tuning("sssp.filter", filt)
         hc.set_tuning("sssp.alpha", alpha)
         for g in (kron16, uniform16):
@@ -272,6 +273,7 @@ def test_sssp_dist16_mirror_exact(hc, gpu, oracle, kron16, uniform16):
         idx[u + 1] = len(nb)
     path = hc.HostGraph.from_csr(idx, np.array(nb, np.int32), np.array(wt, np.int32))
     try:
+        hc.set_tuning("sssp.delta", 0)  # direction 0 pulls on the uniform graph too
         hc.set_tuning("sssp.dist16", 1)
         for g in (kron16, uniform16, path):
             srcs = [0, n // 2] if g is path else list(map(int, g.pick_sources(3)))
@@ -288,6 +290,55 @@ def test_sssp_dist16_mirror_exact(hc, gpu, oracle, kron16, uniform16):
             with hc.DeviceGraph(_graph(hc, fx)) as dg:
                 for i, s in enumerate(fx["sources"]):
                     r = hc.sssp(dg, int(s), direction=1)
+                    assert np.array_equal(r.values, fx["sssp"][i]), (name, int(s))
+    finally:
+        hc.reset_tuning()
+
+
+@pytest.mark.parametrize("delta", [1, 3, 16, 64, 1 << 20])
+def test_sssp_buckets_exact(hc, gpu, oracle, kron16, uniform16, delta):
+    """Near-far buckets (sssp.delta): vertices lowered to a value at or
+    above the threshold wait in the far set and come back by promotion
+    epochs; push epochs only.  Distances equal Dijkstra's on both scale-16
+    graphs (the automatic width is off on the Kronecker graph, forced here),
+    on the golden graphs and on a weighted path (many empty promotions at
+    small widths: the rise doubles); small widths
+    take promotion epochs, none pull; the trace's counts add up."""
+    n = 600
+    idx = np.concatenate([[0], np.cumsum([1 if u in (0, n - 1) else 2 for u in range(n)])])
+    nb, wt = [], []
+    for u in range(n):
+        for v in (u - 1, u + 1):
+            if 0 <= v < n:
+                nb.append(v)
+                wt.append(255 if (u + v) % 3 else 7)
+    path = hc.HostGraph.from_csr(idx.astype(np.int64), np.array(nb, np.int32),
+                                 np.array(wt, np.int32))
+    try:
+        hc.set_tuning("sssp.delta", delta)
+        for g in (kron16, uniform16, path):
+            srcs = [0, n // 2] if g is path else list(map(int, g.pick_sources(3)))
+            with hc.DeviceGraph(g) as dg:
+                for s in srcs:
+                    ref = oracle.sssp_dijkstra(g.index, g.neighbors, g.weights, s)
+                    r = hc.sssp(dg, s)
+                    assert np.array_equal(r.values, ref), (g.num_nodes, s)
+                    st = r.stats
+                    assert st["pull_epochs"] == 0, st
+                    promotions = st["epochs"] - st["push_epochs"]
+                    tr = hc.epoch_trace(dg, "sssp")
+                    assert sum(e["mode"] == "promote" for e in tr) == promotions or \
+                        st["epochs"] > len(tr)
+                    if delta <= 3 and g is not kron16:
+                        assert promotions > 0, st
+                    # direction 1 (pull only) ignores the buckets
+                    assert np.array_equal(hc.sssp(dg, s, direction=1).values, ref)
+        hc.set_tuning("sssp.delta", min(delta, 2))
+        for name in golden_files():
+            fx = load_golden(name)
+            with hc.DeviceGraph(_graph(hc, fx)) as dg:
+                for i, s in enumerate(fx["sources"]):
+                    r = hc.sssp(dg, int(s))
                     assert np.array_equal(r.values, fx["sssp"][i]), (name, int(s))
     finally:
         hc.reset_tuning()
@@ -533,6 +584,7 @@ def test_sssp_column_blocked_sweeps_exact(hc, gpu, oracle, kron16, uniform16, sh
     split between rows swept once (light) and block by block (heavy): 0
     blocks every row, 1000 none."""
     try:
+        hc.set_tuning("sssp.delta", 0)  # direction 0 pulls on the uniform graph too
         hc.set_tuning("pull.col_shift", shift)
         hc.set_tuning("sssp.dist16", dist16)
         hc.set_tuning("cb.min_slice", min_slice)
