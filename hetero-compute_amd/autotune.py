@@ -194,6 +194,7 @@ def tune_knobs(dg, algorithm: str, sources, space: dict | None = None, repeats: 
     base_ms, base_out = min((_run(dg, algorithm, sources) for _ in range(repeats)),
                             key=lambda t: t[0])
     best = {k: getk(k) for k in space}
+    base_knobs = dict(best)
     best_ms = base_ms
     trials = []
     for key, values in space.items():
@@ -212,11 +213,18 @@ def tune_knobs(dg, algorithm: str, sources, space: dict | None = None, repeats: 
             if ms < best_ms:
                 best_ms, best[key] = ms, v
         setk(key, best[key])
-    # The winner timed again, with every chosen knob in place.
+    # The winner timed again, with every chosen knob in place; when that is
+    # not faster than the starting configuration (single-knob gains that do
+    # not add up, or timing noise), the starting knobs are kept.
     final_ms, final_out = min((_run(dg, algorithm, sources) for _ in range(repeats)),
                               key=lambda t: t[0])
     if not _same(final_out, base_out, algorithm):
         raise RuntimeError(f"the chosen knobs changed the results of {algorithm}")
+    if final_ms > base_ms:
+        for key in space:
+            best[key] = base_knobs[key]
+            setk(key, best[key])
+        final_ms = base_ms
     return {"algorithm": algorithm, "default_ms": round(base_ms, 4),
             "tuned_ms": round(final_ms, 4), "knobs": best, "trials": trials}
 
