@@ -461,7 +461,7 @@ def run_many_graph_bfs(hc, oracle, count, seed, check, tuned=None):
 
 
 def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, slots=4,
-                min_batch_bfs=0):
+                min_batch_bfs=0, max_wait_ms=0):
     """Poisson arrivals of 70% BFS / 30% SSSP queries; 20% target small
     graphs uploaded on demand.  Latency: arrival -> result in HBM.  Every
     result on an uploaded graph is compared with the oracle; on the resident
@@ -473,7 +473,7 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
     rng = np.random.RandomState(seed)
     dyn = [hc.HostGraph.generate("kron", int(s), 16, weighted=True, seed=seed + int(s))
            for s in rng.randint(12, 19, size=8)]
-    srv = hc.Server(slots=slots, max_batch=64, min_batch_bfs=min_batch_bfs,
+    srv = hc.Server(slots=slots, max_batch=64, min_batch_bfs=min_batch_bfs, max_wait_ms=max_wait_ms,
                     checksum=check > 0)
     gids = [srv.add_graph(big), srv.add_graph(unif)] + [srv.add_graph(g, resident=False)
                                                         for g in dyn]
@@ -637,7 +637,8 @@ def run_serving(hc, oracle, big, unif, rates, queries, seed, check, tuned=None, 
     srv.close()
     if alone is not None:
         alone.close()
-    return {"slots": slots, "tuned_knobs": tuned_knobs, "measured_costs_ms": costs,
+    return {"slots": slots, "max_wait_ms": max_wait_ms or 500, "tuned_knobs": tuned_knobs,
+            "measured_costs_ms": costs,
             "graphs": {"kron": big.num_nodes, "uniform": unif.num_nodes,
                        "dynamic_scales": [int(np.log2(g.num_nodes)) for g in dyn]},
             "rates": results,
@@ -801,6 +802,9 @@ def main():
                     help="offered query rates (q/s), comma-separated")
     ap.add_argument("--serve-slots", type=int, default=4,
                     help="server slots (concurrent batches, one stream each)")
+    ap.add_argument("--serve-max-wait-ms", type=int, default=0,
+                    help="how long fast-arriving queries may wait to gather a batch "
+                         "(0: the server's default, 500; -1: never)")
     ap.add_argument("--serve-queries", type=int, default=2000)
     ap.add_argument("--serve-check", type=int, default=32,
                     help="resident-graph results certified per graph, kind and rate")
@@ -928,7 +932,8 @@ def main():
         # 1k-20k q/s are the offered rates of the serving configuration.
         serving = run_serving(hc, oracle, big, unif, [int(x) for x in args.serve_rates.split(",")],
                               args.serve_queries, 777, args.serve_check if args.check else 0,
-                              args.tuned, args.serve_slots, args.serve_min_batch_bfs)
+                              args.tuned, args.serve_slots, args.serve_min_batch_bfs,
+                              args.serve_max_wait_ms)
         del big
 
     cpu = None
