@@ -13,7 +13,8 @@ and the FP64 oracle at the same iteration count.
   C2  Kronecker 23 and uniform 23 (ef 16, weights U[1,255]): BFS and SSSP from
       the first 64 SourcePicker sources, all certified; parents verified.
   C3  PageRank on Kronecker 24: L1 <= 1e-6 against the FP32 reference.
-  C4a 64-source batched SSSP on Kronecker 22: all 64 columns certified.
+  C4a 64-source batched SSSP on Kronecker 22, and on uniform 23 (push epochs
+      and buckets): all 64 columns certified.
   C4b 512 Kronecker graphs, scale U{10..16}, ef U{8..32}: every depth vector
       equal to the oracle's, every parent array valid.
   C5  serving over resident Kronecker 24 + uniform 23 and uploaded Kronecker
@@ -118,6 +119,27 @@ def test_c4a_batched_sssp_kron22(hc, gpu, oracle):
     bad = oracle.certify_paths(g.index, g.neighbors, g.weights, srcs, M)
     assert bad.sum() == 0, np.flatnonzero(bad)
     assert np.array_equal(M[:, 5], single)
+    assert np.array_equal(M[:, 0], oracle.sssp_dijkstra(g.index, g.neighbors, g.weights,
+                                                        int(srcs[0])))
+
+
+def test_c4a_batched_sssp_uniform23_buckets(hc, gpu, oracle, c2):
+    """The batched sweep on the serving's flat graph (uniform 23: push epochs
+    and delta-stepping buckets by default): every column certified, one
+    equal to the single-source run (near-far buckets) and one to Dijkstra."""
+    g, dg = c2["uniform"]
+    srcs = g.pick_sources(64, seed=99)
+    buf = hc.DeviceArray(g.num_nodes * hc.MULTI_WIDTH, np.int32)
+    r = hc.sssp_multi(dg, srcs, out=buf, fetch=False)
+    M = buf.get().reshape(g.num_nodes, hc.MULTI_WIDTH)
+    buf.free()
+    st = r.stats
+    assert st["push_epochs"] > 0 and st["epochs"] > st["push_epochs"] + st["pull_epochs"], st
+    bad = oracle.certify_paths(g.index, g.neighbors, g.weights, srcs, M)
+    assert bad.sum() == 0, np.flatnonzero(bad)
+    single = hc.sssp(dg, int(srcs[7]))
+    assert single.stats["pull_epochs"] == 0  # near-far: push and promotion epochs only
+    assert np.array_equal(M[:, 7], single.values)
     assert np.array_equal(M[:, 0], oracle.sssp_dijkstra(g.index, g.neighbors, g.weights,
                                                         int(srcs[0])))
 
