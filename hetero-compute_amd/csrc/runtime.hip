@@ -625,7 +625,7 @@ int set_knob(Tuning &t, const char *key, int64_t value) {
         const bool stage = !strcmp(key, "pull.stage") || !strcmp(key, "bu.stage") ||
                            !strcmp(key, "sssp.stage") || !strcmp(key, "push.stage");
         const bool flag = !strcmp(key, "profile.kernel_events") || !strcmp(key, "multi.graph") ||
-                          !strcmp(key, "sssp.filter") || !strcmp(key, "msbfs.on");
+                          !strcmp(key, "msbfs.on");
         if (!strcmp(key, "sssp.dist16"))
             HC_CHECK_ARG(value >= 0 && value <= 2, "sssp.dist16 must be 0, 1 or 2");
         if (!strcmp(key, "sssp.push_dedup"))
@@ -680,7 +680,7 @@ int set_knob(Tuning &t, const char *key, int64_t value) {
             HC_CHECK_ARG(value == 0 || value == 1 || value == 4 || value == 16,
                          "bu.remaining must be 0, 1, 4 or 16");
         const bool zero_ok = stage || flag || !strcmp(key, "bu.remaining") ||
-                             !strcmp(key, "multi.push_div") ||
+                             !strcmp(key, "multi.push_div") || !strcmp(key, "sssp.filter") ||
                              !strcmp(key, "pull.col_shift") ||
                              !strcmp(key, "bu.first") || !strcmp(key, "sssp.alpha") ||
                              !strcmp(key, "sssp.dist16") || !strcmp(key, "sssp.push_dedup") ||

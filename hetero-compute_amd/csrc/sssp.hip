@@ -462,7 +462,8 @@ __global__ __launch_bounds__(kBlock) void sssp_step_kernel(
     uint32_t cnt = 0, gathered = 0;
     unsigned long long eacc = 0;
     if (mode == kPull && CB) {
-        // Column-blocked sweep (the host picks it only without filtering):
+        // Column-blocked sweep (the host picks it only when filtering is
+        // not forced; its sweeps gather every neighbour):
         // the heavy rows of pcb block by block, then the light rows of plan
         // p once over the plain CSR (heavy rows first, as in the plain
         // sweep's class order, so light rows read their neighbours' values
@@ -548,7 +549,11 @@ __device__ void sssp_decide(StepCtl *c, StepCtl *trace,
     c->epoch = k + 1;
     c->in_n = (long long)out_n;
     c->in_e = (long long)out_e;
-    c->filter = c->filter_knob;  // the changed set of this epoch is complete
+    // The changed set of this epoch is complete: the next pull may gather
+    // only from it (knob 1: always; k >= 2: when it holds under n / k
+    // vertices, so that most gathers are skipped).
+    c->filter = c->filter_knob == 1 ||
+                (c->filter_knob >= 2 && (long long)out_n * c->filter_knob < (long long)c->n);
     int next = ran;
     if (c->max_epochs > 0 && k + 1 >= c->max_epochs) {
         c->done = 1;
@@ -620,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void sssp_init_kernel(
         // With initial distances the changed set before epoch 0 is unknown:
         // that epoch gathers every neighbour.
         ctl->filter_knob = filter;
-        ctl->filter = init_dist ? 0 : filter;
+        ctl->filter = init_dist ? 0 : (filter == 1 || (filter >= 2 && filter < g.n));
         for (int p = 0; p < 2; p++)
             for (int j = 0; j < 5; j++) fb.cnt[p][j] = 0;
         ctl->done = 0;
@@ -977,7 +982,7 @@ static int run_sssp_loop(hc_graph_s *g, int32_t src, const int32_t *init_dist,
     const int delta = init_dist || direction == 1 ? 0 : sssp_delta(g);
     const ColBlocks *cb = nullptr;
     const PlanHost *phb = nullptr;
-    if (direction != 2 && delta == 0 && !tuning().sssp_filter) {
+    if (direction != 2 && delta == 0 && tuning().sssp_filter != 1) {
         HC_TRY(get_col_blocks(g, col_shift_for(g, flat_graph(g)), s, &cb));
         if (cb) {
             // Plain sweep for the light rows only.

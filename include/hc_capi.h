@@ -217,7 +217,9 @@ int hc_graph_get_info(hc_graph_t g, hc_graph_info_t *out);
  *                          edges (skewed), else 2 (0)
  *   sssp.grid_per_cu       cap on workgroups per CU of the SSSP loop (4)
  *   sssp.filter            1: SSSP pull epochs gather only neighbours whose
- *                          distance changed in the previous epoch (0)
+ *                          distance changed in the previous epoch; k >= 2:
+ *                          only when fewer than n / k vertices changed
+ *                          (column-blocked pulls never filter); 0: never (0)
  *   sssp.dist16            1: SSSP pull epochs gather a 16-bit mirror of the
  *                          distances (rebuilt before each pull; exact); 2: on
  *                          for flat graphs only (2)

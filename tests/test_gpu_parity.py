@@ -84,11 +84,12 @@ def test_sssp_bfs_scale16(hc, gpu, oracle, request, which):
             assert b.stats["reached"] == int((depth >= 0).sum())
 
 
-@pytest.mark.parametrize("filt,alpha", [(1, 8), (1, 2), (0, 2), (0, 16)])
+@pytest.mark.parametrize("filt,alpha", [(1, 8), (1, 2), (0, 2), (0, 16), (8, 4), (2, 16)])
 def test_sssp_filter_and_direction_exact(hc, gpu, oracle, kron16, uniform16, filt, alpha):
     """Change-filtered pull epochs (neighbours changed in the previous epoch
-    only) and other push/pull switch points give the same distances as
-    Dijkstra, on both scale-16 graphs and every golden graph."""
+    only: always, or when fewer than n / k changed) and other push/pull
+    switch points give the same distances as Dijkstra, on both scale-16
+    graphs and every golden graph."""
     try:
         hc.set_tuning("sssp.delta", 0)  # pulls on the uniform graph too
         hc.set_tuning("sssp.filter", filt)
