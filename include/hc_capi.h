@@ -114,7 +114,9 @@ typedef struct hc_run_stats {
                               (hc_bfs_many_noalloc: host clock).
                               Heterogeneous runs: host clock, copies
                               included.                                    */
-    int32_t epochs;        /* epochs executed                                 */
+    int32_t epochs;        /* epochs executed (with buckets, hc_sssp and
+                              hc_sssp_multi count their promotion epochs
+                              here too: epochs - push - pull of them)        */
     int32_t push_epochs;   /* top-down / push epochs                          */
     int32_t pull_epochs;   /* bottom-up / pull epochs                         */
     int32_t launches;      /* kernel launches issued for the loop             */
