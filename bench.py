@@ -337,13 +337,20 @@ def run_pagerank(hc, oracle, scale, degree, seed, check, tuned=None):
     return out
 
 
-def run_batched_sssp(hc, oracle, scale, degree, seed, check, tuned=None):
-    """64 SSSP sources of one Kronecker graph in one batched sweep vs the
-    same 64 sources run one after another (device time of each)."""
-    g = hc.HostGraph.generate("kron", scale, degree, weighted=True, seed=seed)
+def run_batched_sssp(hc, oracle, scale, degree, seed, check, tuned=None, g=None, dg=None,
+                     label=None):
+    """64 SSSP sources of one graph (a Kronecker graph generated here, or the
+    resident graph g / dg) in one batched sweep vs the same 64 sources run
+    one after another (device time of each)."""
+    if g is None:
+        g = hc.HostGraph.generate("kron", scale, degree, weighted=True, seed=seed)
+        label = f"kron{scale} ef{degree}"
     sources = g.pick_sources(64)
-    out = {"graph": f"kron{scale} ef{degree}", "n": g.num_nodes, "m": int(g.num_edges)}
-    with hc.DeviceGraph(g) as dg:
+    out = {"graph": label, "n": g.num_nodes, "m": int(g.num_edges)}
+    own = dg is None
+    if own:
+        dg = hc.DeviceGraph(g)
+    try:
         out["tuned_knobs"] = apply_tuned(tuned, g, ("multi", "sssp"), dg)
         buf = hc.DeviceArray(g.num_nodes * hc.MULTI_WIDTH, np.int32)
         hc.sssp_multi(dg, sources, out=buf, fetch=False)  # warm-up (capture)
@@ -368,12 +375,16 @@ def run_batched_sssp(hc, oracle, scale, degree, seed, check, tuned=None):
             out["columns_certified"] = 64
         buf.free()
         one.free()
+    finally:
+        if own:
+            dg.free()
     b_ms = r.stats["ms"]
     out.update({
         "batch_ms": round(b_ms, 3), "sequential_ms": round(seq_ms, 3),
         "batch_gteps": round(hc.gteps(64 * g.num_edges, b_ms), 3),
         "sequential_gteps": round(hc.gteps(64 * g.num_edges, seq_ms), 3),
         "speedup": round(seq_ms / b_ms, 2), "epochs": r.stats["epochs"],
+        "push_epochs": r.stats["push_epochs"], "pull_epochs": r.stats["pull_epochs"],
         "alg_gbs": round(r.stats["alg_bytes"] / (b_ms / 1e3) / 1e9, 1),
         "bit_exact_vs_single_and_oracle": exact})
     return out
@@ -901,6 +912,12 @@ def main():
         log("[batch] 64-source SSSP")
         batched = {"sssp64": run_batched_sssp(hc, oracle, args.batch_scale, args.degree,
                                               args.seed, args.check, args.tuned)}
+        if "uniform" in results:
+            # The serving's flat graph: push epochs and delta-stepping buckets.
+            ru = results["uniform"]
+            batched["sssp64_uniform"] = run_batched_sssp(
+                hc, oracle, args.scale, args.degree, args.seed, args.check, args.tuned,
+                g=ru["g"], dg=ru["dg"], label=f"uniform{args.scale} ef{args.degree}")
         log("[batch] 64-source BFS")
         gb = hc.HostGraph.generate("kron", args.batch_scale, args.degree, weighted=False,
                                    seed=args.seed)
