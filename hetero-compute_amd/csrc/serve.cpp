@@ -441,15 +441,18 @@ void worker(hc_server_s *s, int slot) {
         }
         const double t_done = now_ms();
         // Result checksums, on the slot's stream while the slot still
-        // holds the sweep.
+        // holds the sweep.  A failed checksum does not fail the queries:
+        // hc_server_checksum then computes it on demand.
         uint64_t sums[kWidth] = {};
-        const bool summed = status == HC_OK && s->opts.checksum;
-        if (summed) {
+        bool summed = false;
+        if (status == HC_OK && s->opts.checksum) {
             const int minus1 = kind == HC_QUERY_BFS ? 1 : 0;
-            status = width == 1 ? hc_checksum_i32(buf, (int32_t)g.host.num_nodes, 1, 0, 0, sums,
-                                                  stream)
-                                : hc_checksum_multi(buf, (int32_t)g.host.num_nodes,
-                                                    (int32_t)batch.size(), minus1, sums, stream);
+            summed = (width == 1 ? hc_checksum_i32(buf, (int32_t)g.host.num_nodes, 1, 0, 0,
+                                                   sums, stream)
+                                 : hc_checksum_multi(buf, (int32_t)g.host.num_nodes,
+                                                     (int32_t)batch.size(), minus1, sums,
+                                                     stream)) == HC_OK;
+            if (!summed) (void)hipGetLastError();
         }
 
         lk.lock();
